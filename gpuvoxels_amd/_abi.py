@@ -1,0 +1,157 @@
+"""ctypes mirror of include/vx3h.h (the engine's C ABI).
+
+The structures here must match the C header field for field; a CPU test checks
+their sizes against the compiled library's own sizeof() values.
+"""
+import ctypes as C
+
+MAX_CURVE = 16
+MAX_TOKENS = 256
+
+OK, EINVAL, ENOMEM, EDEVICE, EPARSE, EUNSUPPORTED, ENOTFOUND = range(7)
+MDL_LINEAR, MDL_LINEAR_FAIL, MDL_BILINEAR, MDL_DATA = range(4)
+DOF_TX, DOF_TY, DOF_TZ, DOF_RX, DOF_RY, DOF_RZ = 1, 2, 4, 8, 16, 32
+DOF_ALL = 63
+PATH_STREAM, PATH_RESIDENT, PATH_AUTO = range(3)
+PREC_F32, PREC_F64 = range(2)
+SIM_RUNNING, SIM_DONE, SIM_DIVERGED, SIM_EMPTY = range(4)
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [
+        ("mat_model", C.c_int),
+        ("elastic_mod", C.c_double),
+        ("plastic_mod", C.c_double),
+        ("yield_stress", C.c_double),
+        ("fail_stress", C.c_double),
+        ("density", C.c_double),
+        ("poissons_ratio", C.c_double),
+        ("cte", C.c_double),
+        ("u_static", C.c_double),
+        ("u_dynamic", C.c_double),
+        ("zeta_internal", C.c_double),
+        ("zeta_global", C.c_double),
+        ("zeta_collision", C.c_double),
+        ("n_data", C.c_int),
+        ("strain_data", C.c_double * MAX_CURVE),
+        ("stress_data", C.c_double * MAX_CURVE),
+        ("is_measured", C.c_int),
+        ("fixed", C.c_int),
+        ("thermal_on_after", C.c_double),
+        ("matid", C.c_int),
+    ]
+
+
+class External(C.Structure):
+    _fields_ = [
+        ("voxel", C.c_int),
+        ("dof_fixed", C.c_int),
+        ("translate", C.c_double * 3),
+        ("rotate", C.c_double * 3),
+        ("force", C.c_double * 3),
+        ("moment", C.c_double * 3),
+    ]
+
+
+class Token(C.Structure):
+    _fields_ = [("op", C.c_int), ("value", C.c_double)]
+
+
+class SimDesc(C.Structure):
+    _fields_ = [
+        ("voxel_size", C.c_double),
+        ("n_materials", C.c_int),
+        ("materials", C.POINTER(MaterialDesc)),
+        ("n_voxels", C.c_int),
+        ("ijk", C.POINTER(C.c_int)),
+        ("material", C.POINTER(C.c_int)),
+        ("phase", C.POINTER(C.c_double)),
+        ("n_externals", C.c_int),
+        ("externals", C.POINTER(External)),
+        ("gravity_mult", C.c_float),
+        ("floor_enabled", C.c_int),
+        ("temp_enabled", C.c_int),
+        ("vary_temp_enabled", C.c_int),
+        ("temp_base", C.c_double),
+        ("temp_amplitude", C.c_double),
+        ("temp_period", C.c_double),
+        ("enable_expansion", C.c_int),
+        ("dt_frac", C.c_double),
+        ("dt_fixed", C.c_double),
+        ("volume_effects", C.c_int),
+        ("collisions_enabled", C.c_int),
+        ("bounding_radius", C.c_double),
+        ("watch_distance", C.c_double),
+        ("n_stop_tokens", C.c_int),
+        ("stop_tokens", C.POINTER(Token)),
+        ("n_fitness_tokens", C.c_int),
+        ("fitness_tokens", C.POINTER(Token)),
+        ("max_steps", C.c_int),
+        ("save_position_of_all_voxels", C.c_int),
+    ]
+
+
+class BatchCfg(C.Structure):
+    _fields_ = [
+        ("max_device_bytes", C.c_size_t),
+        ("precision", C.c_int),
+        ("path", C.c_int),
+        ("block_size", C.c_int),
+        ("steps_per_launch", C.c_int),
+        ("collision_rebuild_interval", C.c_int),
+        ("use_graph", C.c_int),
+    ]
+
+
+class SimStatus(C.Structure):
+    _fields_ = [
+        ("slot", C.c_int),
+        ("state", C.c_int),
+        ("steps", C.c_int),
+        ("num_voxel", C.c_int),
+        ("num_measured_voxel", C.c_int),
+        ("current_time", C.c_double),
+        ("dt", C.c_double),
+        ("initial_com", C.c_double * 3),
+        ("current_com", C.c_double * 3),
+        ("fitness", C.c_double),
+        ("total_distance_of_all_voxels", C.c_double),
+        ("recent_angle", C.c_double),
+    ]
+
+
+class StateView(C.Structure):
+    _fields_ = [
+        ("pos", C.POINTER(C.c_double)),
+        ("orient", C.POINTER(C.c_double)),
+        ("lin_mom", C.POINTER(C.c_double)),
+        ("ang_mom", C.POINTER(C.c_double)),
+        ("vox_flags", C.POINTER(C.c_int)),
+        ("temp", C.POINTER(C.c_double)),
+        ("link_strain", C.POINTER(C.c_double)),
+        ("link_max_strain", C.POINTER(C.c_double)),
+        ("link_small_angle", C.POINTER(C.c_int)),
+    ]
+
+
+class TopologyView(C.Structure):
+    _fields_ = [
+        ("n_voxels", C.c_int),
+        ("n_links", C.c_int),
+        ("link_vneg", C.POINTER(C.c_int)),
+        ("link_vpos", C.POINTER(C.c_int)),
+        ("link_axis", C.POINTER(C.c_int)),
+        ("voxel_links", C.POINTER(C.c_int)),
+    ]
+
+
+class ReportEntry(C.Structure):
+    _fields_ = [
+        ("vxd_name", C.c_char_p),
+        ("status", C.POINTER(SimStatus)),
+        ("voxel_size", C.c_double),
+        ("save_positions", C.c_int),
+        ("init_pos", C.POINTER(C.c_double)),
+        ("pos", C.POINTER(C.c_double)),
+        ("mats", C.POINTER(C.c_int)),
+    ]

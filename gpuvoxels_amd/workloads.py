@@ -1,0 +1,116 @@
+"""Synthetic, seeded workloads (the benchmark and parity configurations).
+
+* ``cube``      6x6x6 solid cube, one actuated material (single-sim config).
+* ``robots``    uneven robots on a 12^3 grid: Bernoulli(0.5) occupancy, largest
+                face-connected component, one of four materials per voxel
+                (soft passive, stiff passive, actuated phase 0, actuated phase
+                half a cycle).  The component rule restates
+                Experiments/framework/voxelyze/helper.py:1-17 (scipy ``label`` with
+                its default face connectivity; the first label with the strictly
+                largest count wins) and the body layout of
+                Experiments/framework/voxelyze/workflow.py:45-60 (array [z, y, x],
+                x fastest in the structure order).
+* ``slab``      64x64x32 bilinear slab with a four-phase checkerboard.
+"""
+import numpy as np
+
+from . import _abi
+from .model import Material, Model, FITNESS_XY_DISTANCE
+
+
+def gravity_mult(grav_acc: float = -9.81) -> float:
+    """CVX_Sim::SetGravityAccel: Vx.setGravity(-grav/9.80665) with a float argument
+    (Voxelyze3/src/VXA/VX_Sim.cpp:485)."""
+    g = np.float32(grav_acc)
+    return float(np.float32(-float(g) / 9.80665))
+
+
+def _physics(m: Model, steps: int):
+    m.gravity_mult = gravity_mult(-9.81)
+    m.floor_enabled = 1
+    m.temp_enabled = 1
+    m.vary_temp_enabled = 1
+    m.temp_base = 25.0
+    m.temp_amplitude = 10.0       # CTE * A = 0.1
+    m.temp_period = 0.5           # 2 Hz
+    m.enable_expansion = 0
+    m.dt_frac = 0.9
+    m.volume_effects = 1
+    m.collisions_enabled = 0
+    m.max_steps = steps
+    m.fitness_tokens = list(FITNESS_XY_DISTANCE)
+
+
+def base_material(**kw) -> Material:
+    d = dict(elastic_mod=1e6, density=1000.0, poissons_ratio=0.35, cte=0.01, u_static=1.0, u_dynamic=0.5,
+             zeta_internal=1.0, zeta_collision=0.8, zeta_global=0.01, matid=1)
+    d.update(kw)
+    return Material(**d)
+
+
+def cube(n: int = 6, steps: int = 10000, voxel_size: float = 0.01) -> Model:
+    """Solid n^3 cube of one actuated material (E=1e6, nu=0.35, rho=1000, CTE 0.01)."""
+    m = Model(voxel_size=voxel_size, name="cube%d" % n)
+    m.materials = [base_material()]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    m.ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+    m.mat = np.zeros(n ** 3, np.int32)
+    m.phase = np.zeros(n ** 3)
+    _physics(m, steps)
+    return m
+
+
+def largest_component(body: np.ndarray) -> np.ndarray:
+    """Keeps the largest face-connected component of a boolean [z, y, x] body."""
+    from scipy.ndimage import label
+    labeled, total = label(body)
+    best_id, best_count = 0, 0
+    counts = np.bincount(labeled.ravel(), minlength=total + 1)
+    for i in range(1, total + 1):
+        if counts[i] > best_count:
+            best_count, best_id = counts[i], i
+    return labeled == best_id if best_id > 0 else np.zeros_like(body, dtype=bool)
+
+
+ROBOT_MATERIALS = [
+    dict(elastic_mod=1e5, cte=0.0, matid=1),   # soft passive
+    dict(elastic_mod=1e7, cte=0.0, matid=2),   # stiff passive
+    dict(elastic_mod=1e6, cte=0.01, matid=3),  # actuated, phase 0
+    dict(elastic_mod=1e6, cte=0.01, matid=4),  # actuated, phase half a cycle
+]
+ROBOT_PHASE = [0.0, 0.0, 0.0, 0.5]
+
+
+def robot(rng: np.random.Generator, grid: int = 12, p: float = 0.5, steps: int = 20000, voxel_size: float = 0.01,
+          name: str = "robot") -> Model:
+    occ = rng.random((grid, grid, grid)) < p
+    body = largest_component(occ)
+    mats = rng.integers(0, 4, size=body.shape)
+    zyx = np.argwhere(body)                       # lexicographic (z, y, x): x fastest
+    m = Model(voxel_size=voxel_size, name=name)
+    m.materials = [base_material(**kw) for kw in ROBOT_MATERIALS]
+    m.ijk = zyx[:, ::-1].astype(np.int32).copy()
+    m.mat = mats[body].astype(np.int32)
+    m.phase = np.array(ROBOT_PHASE)[m.mat]
+    _physics(m, steps)
+    return m
+
+
+def robots(n: int = 512, seed: int = 0, grid: int = 12, steps: int = 20000, voxel_size: float = 0.01):
+    rng = np.random.default_rng(seed)
+    return [robot(rng, grid=grid, steps=steps, voxel_size=voxel_size, name="robot_%04d" % i) for i in range(n)]
+
+
+def slab(nx: int = 64, ny: int = 64, nz: int = 32, steps: int = 5000, voxel_size: float = 0.01,
+         collisions: bool = True) -> Model:
+    """Bilinear slab (E=1e6 to yield strain 0.05, then 1e5) with a 4-phase checkerboard."""
+    m = Model(voxel_size=voxel_size, name="slab")
+    m.materials = [base_material(mat_model=_abi.MDL_BILINEAR, plastic_mod=1e5, yield_stress=5e4, fail_stress=1e7,
+                                 matid=k + 1) for k in range(4)]
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    m.ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+    m.mat = (((x // 4) % 2) * 2 + ((y // 4) % 2)).ravel().astype(np.int32)
+    m.phase = np.array([0.0, 0.25, 0.5, 0.75])[m.mat]
+    _physics(m, steps)
+    m.collisions_enabled = 1 if collisions else 0
+    return m

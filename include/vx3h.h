@@ -1,0 +1,235 @@
+/*
+ * vx3h.h -- C ABI of the MI355X Voxelyze3 physics engine.
+ *
+ * This is the seam between the C++ host layer (VXA/VXD loader, simulation
+ * manager, worker CLI, serving loop) and the HIP device engine.  Everything
+ * crossing it is plain C: opaque handles, POD structs, pointers and sizes.
+ * All functions return a vx3h_status; none aborts.  The last error message of
+ * the calling thread is available from vx3h_last_error().
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * gpuVoxels tree):
+ *   vx3h_vxa_parse        VX3_SimulationManager::readVXD
+ *                         (Voxelyze3/src/VX3/VX3_SimulationManager.cu:277-380)
+ *                         + CVX_Sim::ReadVXA/Import (Voxelyze3/src/VXA/VX_Sim.cpp:59-183)
+ *   vx3h_batch_create     VX3_SimulationManager ctor + device allocation
+ *                         (VX3_SimulationManager.cu:123-130,283)
+ *   vx3h_sim_admit        VX3_VoxelyzeKernel(CVX_Sim*) host ctor + cudaMemcpy
+ *                         (Voxelyze3/src/VX3/VX3_VoxelyzeKernel.cu:27-105,
+ *                          VX3_SimulationManager.cu:377)
+ *   vx3h_batch_step       CUDA_Simulation loop body calling
+ *                         VX3_VoxelyzeKernel::doTimeStep
+ *                         (VX3_SimulationManager.cu:62-69, VX3_VoxelyzeKernel.cu:237-359)
+ *   vx3h_batch_poll       VX3_SimulationManager::collectResults
+ *                         (VX3_SimulationManager.cu:416-458)
+ *   vx3h_sim_read_state   direct reads of the device voxel/link arrays done by
+ *                         collectResults (VX3_SimulationManager.cu:435-452)
+ *   vx3h_sim_retire       per-simulation cleanup (VX3_VoxelyzeKernel.cu:107-119)
+ *   vx3h_batch_destroy    VX3_SimulationManager dtor (VX3_SimulationManager.cu:131-136)
+ *   vx3h_report_xml       report writer of vx3_node_worker
+ *                         (Voxelyze3/src/Executables/vx3_node_worker.cu:98-141)
+ */
+#ifndef VX3H_H
+#define VX3H_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+typedef enum vx3h_status {
+    VX3H_OK = 0,
+    VX3H_EINVAL = 1,      /* bad argument or malformed model */
+    VX3H_ENOMEM = 2,      /* the model does not fit the batch's device memory budget */
+    VX3H_EDEVICE = 3,     /* HIP runtime error */
+    VX3H_EPARSE = 4,      /* VXA/VXD could not be parsed */
+    VX3H_EUNSUPPORTED = 5,/* VXA enables a feature this engine does not implement */
+    VX3H_ENOTFOUND = 6    /* slot or handle not found */
+} vx3h_status;
+
+const char* vx3h_last_error(void);
+vx3h_status vx3h_device_count(int* count);
+
+/* ---------------------------------------------------------------- model description
+ * The raw model a simulation is built from: the same information CVX_Sim::Import
+ * consumes (palette, lattice, structure, phase offsets, boundary conditions,
+ * environment) plus the VX3-only simulator keys read in readVXD.  Materials hold
+ * the VXA palette values before Voxelyze's setters clamp them; the engine applies
+ * the setters itself (CVX_Sim::CopyMat, Voxelyze3/src/VXA/VX_Sim.cpp:368-420).
+ */
+#define VX3H_MAX_CURVE 16      /* stress/strain data points per material (MDL_DATA) */
+#define VX3H_MAX_TOKENS 256    /* MathTree tokens per program */
+
+enum { VX3H_MDL_LINEAR = 0, VX3H_MDL_LINEAR_FAIL = 1, VX3H_MDL_BILINEAR = 2, VX3H_MDL_DATA = 3 };
+
+typedef struct vx3h_material_desc {
+    int    mat_model;           /* VX3H_MDL_*  (VXA/VX_Object.h:346) */
+    double elastic_mod;         /* Pa */
+    double plastic_mod;         /* Pa, bilinear */
+    double yield_stress;        /* Pa, bilinear */
+    double fail_stress;         /* Pa, LINEAR_FAIL / BILINEAR; 0 = none given */
+    double density;             /* kg/m^3 */
+    double poissons_ratio;
+    double cte;                 /* 1/degC */
+    double u_static, u_dynamic; /* floor friction coefficients */
+    double zeta_internal;       /* BondDampingZ */
+    double zeta_global;         /* SlowDampingZ */
+    double zeta_collision;      /* ColDampingZ */
+    int    n_data;              /* MDL_DATA points (strain_data/stress_data) */
+    double strain_data[VX3H_MAX_CURVE];
+    double stress_data[VX3H_MAX_CURVE];
+    int    is_measured;         /* counts toward center of mass and fitness */
+    int    fixed;               /* VX3 material-fixed voxels do not move */
+    double thermal_on_after;    /* TurnOnThermalExpansionAfterThisManySeconds */
+    int    matid;               /* palette id reported in the report's <mats> */
+} vx3h_material_desc;
+
+/* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
+enum { VX3H_DOF_TX = 1, VX3H_DOF_TY = 2, VX3H_DOF_TZ = 4, VX3H_DOF_RX = 8, VX3H_DOF_RY = 16, VX3H_DOF_RZ = 32 };
+
+typedef struct vx3h_external {
+    int    voxel;        /* index in the voxel list */
+    int    dof_fixed;    /* VX3H_DOF_* bits */
+    double translate[3]; /* prescribed displacement of fixed translational dofs (m) */
+    double rotate[3];    /* prescribed rotation vector of fixed rotational dofs (rad) */
+    double force[3];     /* N, global frame */
+    double moment[3];    /* N*m, global frame */
+} vx3h_external;
+
+typedef struct vx3h_token { int op; double value; } vx3h_token;   /* VX3_MathTreeToken */
+
+typedef struct vx3h_sim_desc {
+    double voxel_size;                      /* lattice dimension (m) */
+    int    n_materials;
+    const vx3h_material_desc* materials;
+    int    n_voxels;
+    const int*    ijk;       /* 3*n_voxels lattice indices, in insertion order */
+    const int*    material;  /* n_voxels indices into materials */
+    const double* phase;     /* n_voxels phase offsets in cycles, or NULL for all 0 */
+    int    n_externals;
+    const vx3h_external* externals;
+    /* environment */
+    float  gravity_mult;     /* gravity in g (CVoxelyze::grav); force = -m*9.80665*g */
+    int    floor_enabled;
+    int    temp_enabled;     /* Environment/Thermal/TempEnabled */
+    int    vary_temp_enabled;
+    double temp_base, temp_amplitude, temp_period;
+    int    enable_expansion; /* VX3: allow positive temperature (expansion) */
+    /* simulator */
+    double dt_frac;
+    double dt_fixed;         /* > 0: every step uses this dt (s) instead of dt_frac x recommended */
+    int    volume_effects;   /* VolumeEffectsEnabled; 0 zeroes every Poisson's ratio */
+    int    collisions_enabled;
+    double bounding_radius;  /* voxel units (default 0.75) */
+    double watch_distance;   /* voxel units (default 1.0) */
+    int    n_stop_tokens;    const vx3h_token* stop_tokens;     /* StopConditionFormula */
+    int    n_fitness_tokens; const vx3h_token* fitness_tokens;  /* FitnessFunction */
+    int    max_steps;        /* step budget; 0 = unlimited (the reference caps at 1e6) */
+    int    save_position_of_all_voxels;
+} vx3h_sim_desc;
+
+/* ---------------------------------------------------------------- VXA / VXD loader */
+typedef struct vx3h_vxa vx3h_vxa;   /* owns a parsed description */
+/* vxd_text may be NULL.  Children of <VXD> carrying replace="VXA.a.b" replace that
+ * path of the VXA tree before it is read (Voxelyze3/src/Utils/ctool.h:49-57). */
+vx3h_status vx3h_vxa_parse(const char* vxa_text, const char* vxd_text, vx3h_vxa** out);
+const vx3h_sim_desc* vx3h_vxa_desc(const vx3h_vxa* v);
+void        vx3h_vxa_free(vx3h_vxa* v);
+
+/* ---------------------------------------------------------------- batch engine */
+enum {
+    VX3H_PATH_STREAM = 0,    /* state in HBM; link pass + voxel pass per step over every resident sim */
+    VX3H_PATH_RESIDENT = 1,  /* one workgroup per sim, state held in LDS/registers across many steps */
+    VX3H_PATH_AUTO = 2       /* resident where the sim fits, stream otherwise */
+};
+enum { VX3H_PREC_F32 = 0, VX3H_PREC_F64 = 1 };
+
+typedef struct vx3h_batch_cfg {
+    size_t max_device_bytes;  /* admission budget for simulation state; 0 = 90% of free HBM */
+    int    precision;         /* VX3H_PREC_*: F32 is the product path; F64 pins parity */
+    int    path;              /* VX3H_PATH_* */
+    int    block_size;        /* stream kernels' workgroup size; 0 = tuned default */
+    int    steps_per_launch;  /* resident kernel steps per launch; 0 = tuned default */
+    int    collision_rebuild_interval; /* broadphase rebuild period in steps; 0 = on staleness */
+    int    use_graph;         /* capture the stream step sequence in a hipGraph */
+} vx3h_batch_cfg;
+
+typedef struct vx3h_batch vx3h_batch;
+
+enum { VX3H_SIM_RUNNING = 0, VX3H_SIM_DONE = 1, VX3H_SIM_DIVERGED = 2, VX3H_SIM_EMPTY = 3 };
+
+typedef struct vx3h_sim_status {
+    int    slot;
+    int    state;                 /* VX3H_SIM_* */
+    int    steps;                 /* CurStepCount */
+    int    num_voxel;
+    int    num_measured_voxel;
+    double current_time;          /* s (double accumulator, VX3_VoxelyzeKernel.cuh:54) */
+    double dt;                    /* step size used (DtFrac * recommended) */
+    double initial_com[3];        /* f64 center of mass of measured voxels */
+    double current_com[3];
+    double fitness;               /* FitnessFunction(current_com - initial_com, ...) */
+    double total_distance_of_all_voxels;
+    double recent_angle;
+} vx3h_sim_status;
+
+vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out);
+vx3h_status vx3h_batch_destroy(vx3h_batch* b);
+/* Builds the model (topology, materials, dt) on the host and copies it to the device. */
+vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, int* slot);
+/* Device bytes the model would take once admitted. */
+vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* desc, int precision, size_t* bytes);
+/* Advances every running sim by up to n_steps steps (each sim stops at its own
+ * step budget, stop condition or divergence).  Asynchronous on the batch stream. */
+vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps);
+vx3h_status vx3h_batch_sync(vx3h_batch* b);
+/* Status of every admitted sim (synchronizes). */
+vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int cap, int* n);
+vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot);
+
+/* Per-voxel / per-link state readback for parity checks (synchronizes).
+ * Any pointer may be NULL.  Arrays are in voxel/link list order, f64. */
+typedef struct vx3h_state_view {
+    double* pos;       /* 3*V */
+    double* orient;    /* 4*V (w,x,y,z) */
+    double* lin_mom;   /* 3*V */
+    double* ang_mom;   /* 3*V */
+    int*    vox_flags; /* V: bit 3 = floor static friction */
+    double* temp;      /* V */
+    double* link_strain;  /* L */
+    double* link_max_strain; /* L */
+    int*    link_small_angle; /* L */
+} vx3h_state_view;
+vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_view* view);
+
+typedef struct vx3h_topology_view {
+    int n_voxels, n_links;
+    int* link_vneg;   /* L */
+    int* link_vpos;   /* L */
+    int* link_axis;   /* L */
+    int* voxel_links; /* 6*V link index per direction X_POS..Z_NEG, -1 if none */
+} vx3h_topology_view;
+/* Host-side topology build only (no device); n_* are filled when arrays are NULL. */
+vx3h_status vx3h_build_topology(const vx3h_sim_desc* desc, vx3h_topology_view* view);
+
+/* ---------------------------------------------------------------- report */
+typedef struct vx3h_report_entry {
+    const char* vxd_name;        /* file name of the sim's VXD */
+    const vx3h_sim_status* status;
+    double voxel_size;
+    int    save_positions;       /* write init_pos/pos/mats */
+    const double* init_pos;      /* 3*V when save_positions */
+    const double* pos;           /* 3*V when save_positions */
+    const int* mats;             /* V when save_positions */
+} vx3h_report_entry;
+/* Writes the report XML into buf (NUL terminated); *len receives the size needed. */
+vx3h_status vx3h_report_xml(const char* input_dir_name, const vx3h_report_entry* entries, int n,
+                            char* buf, size_t cap, size_t* len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VX3H_H */
