@@ -1,0 +1,49 @@
+// vx3_batch.h -- device pointers of one batch, passed by value to every kernel.
+#ifndef VX3_BATCH_H
+#define VX3_BATCH_H
+#include "vx3_types.h"
+
+namespace vx3 {
+
+template <typename T> struct DevBatch {
+    // voxel state [Vcap]
+    T *px, *py, *pz;          // position
+    T *mx, *my, *mz;          // linear momentum
+    T *ow, *ox, *oy, *oz;     // orientation quaternion
+    T *ax, *ay, *az;          // angular momentum
+    int* vflags;              // VF_* incl. link-presence mask
+    float* tfac;              // 1 + temperature*CTE for the step about to run
+    double* phase;            // actuation phase offset (cycles)
+    int* vmat;                // index into vmatTab
+    int* vsim;                // owning sim slot, -1 if free
+    int* vext;                // index into extTab or -1
+    int* vijk;                // 3*Vcap lattice index (externals / reporting)
+    float *psx, *psy, *psz;   // Poisson strain of the voxel (CVX_Voxel::poissonsStrain)
+    T* F;                     // 36*Vcap link end loads: F[(d*6+c)*Vcap + v]
+    float* es;                // 6*Vcap axial strain of the link end in slot d: es[d*Vcap + v]
+    // link state [Lcap]
+    int *lneg, *lpos;
+    int* lmeta;               // axis | (link material << 2)
+    int* lsim;                // owning sim, -1 if free
+    int* lflags;              // LF_*
+    T *q2x, *q2y, *q2z;       // previous local pos2
+    T *a1x, *a1y, *a1z;       // previous angle1v
+    T *a2x, *a2y, *a2z;       // previous angle2v
+    float *strain, *maxStrain, *strainOffset;
+    // collision pairs (canonical order) and contact forces
+    int* cpair;               // 2*Ccap voxel indices (i < j)
+    float* cforce;            // 3*Ccap contact force on the first voxel
+    int* vcStart;             // per voxel: start into vcList (CSR of pairs touching the voxel)
+    int* vcList;              // signed pair references: +p+1 first voxel, -(p+1) second
+    // tables
+    const VoxMatDev* vmatTab;
+    const LinkMatDev* lmatTab;
+    const ExtDev* extTab;
+    SimCtl* sims;
+    const TokenDev* tokens;
+    int Vcap, Lcap, Ccap;
+    int vHi, lHi, nSims;      // high-water marks: kernels cover [0, vHi) / [0, lHi) / [0, nSims)
+};
+
+}  // namespace vx3
+#endif

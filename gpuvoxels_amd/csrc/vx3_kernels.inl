@@ -1,0 +1,385 @@
+// vx3_kernels.inl -- stream-path kernels, instantiated for one state precision
+// by the including translation unit (VX_REAL = float or double, VX_SUFFIX = f32/f64).
+//
+// One step of every running simulation of the batch (the body of
+// VX3_VoxelyzeKernel::doTimeStep, Voxelyze3/src/VX3/VX3_VoxelyzeKernel.cu:237-359,
+// with the CPU engine's per-step order, Voxelyze3/src/old/Voxelyze.cpp:251-284):
+//   k_poisson   voxel Poisson strain from last step's link end strains   (ν ≠ 0 only)
+//   k_links     beam element of every link, writes the two end loads into the
+//               (direction, voxel) force slots; flags divergence
+//   k_contacts  contact force of every collision pair (collisions only)
+//   k_voxels    gathers the six slots, integrates, writes next step's thermal factor
+//   k_control   per simulation: step count, divergence, periodic FP64 center of
+//               mass, time advance, stop condition
+#include "vx3_batch.h"
+#include "vx3_physics.h"
+
+namespace vx3 {
+namespace VX_NS {
+
+typedef VX_REAL real;
+
+__device__ __forceinline__ bool simRunning(const SimCtl* sims, int s) { return s >= 0 && sims[s].state == 0; }
+
+// CVX_Voxel::transverseArea / transverseStrainSum (VX_Voxel.cpp:345-373) from the cached Poisson strain
+__device__ __forceinline__ void transverse(const DevBatch<real>& b, int v, int axis, float size, bool nuZero, float& area, float& sum) {
+    if (nuZero) { area = size * size; sum = 0.0f; return; }
+    float sx = b.psx[v], sy = b.psy[v], sz = b.psz[v];
+    real s2 = (real)(size * size);
+    if (axis == 0) { area = (float)(s2 * ((real)1 + (real)sy) * ((real)1 + (real)sz)); sum = sy + sz; }
+    else if (axis == 1) { area = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sz)); sum = sx + sz; }
+    else { area = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sy)); sum = sx + sy; }
+}
+
+// Poisson strain of each voxel, CVX_Voxel::strain(true) (VX_Voxel.cpp:299-333), from the
+// end strains the link pass of the previous step left in the slots.
+__global__ void __launch_bounds__(256) k_poisson(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!simRunning(b.sims, s) || !(b.sims[s].flags & SC_POISSON)) return;
+    const int fl = b.vflags[v];
+    float st[3] = {0, 0, 0};
+    int n[3] = {0, 0, 0};
+    for (int d = 0; d < 6; d++) {
+        if (fl & (1 << (VF_LINKMASK_SHIFT + d))) {
+            st[d >> 1] += b.es[d * b.Vcap + v];
+            n[d >> 1]++;
+        }
+    }
+    const int e = b.vext[v];
+    const ExtDev* ext = e >= 0 ? &b.extTab[e] : nullptr;
+    bool tension[3];
+    for (int i = 0; i < 3; i++) {
+        if (n[i] == 2) st[i] *= 0.5f;
+        tension[i] = (n[i] == 2) || (ext && (n[i] == 1 && ((ext->dofFixed & (1 << i)) || ext->force[i] != 0)));
+    }
+    if (!(tension[0] && tension[1] && tension[2])) {
+        float add = 0;
+        for (int i = 0; i < 3; i++) if (tension[i]) add += st[i];
+        // pow of float arguments: evaluated in double and rounded once, which gives the
+        // correctly rounded single-precision result the host C library returns
+        const float pw = (float)pow((double)(1.0f + add), (double)(-b.vmatTab[b.vmat[v]].nu));
+        float value = pw - 1.0f;
+        for (int i = 0; i < 3; i++) if (!tension[i]) st[i] = value;
+    }
+    b.psx[v] = st[0]; b.psy[v] = st[1]; b.psz[v] = st[2];
+}
+
+__global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= b.lHi) return;
+    const int s = b.lsim[l];
+    if (!simRunning(b.sims, s)) return;
+    const SimCtl& sc = b.sims[s];
+    const int vn = b.lneg[l], vp = b.lpos[l];
+    const int meta = b.lmeta[l];
+    const int axis = meta & 3;
+    const LinkMatDev& mat = b.lmatTab[meta >> 2];
+    const VoxMatDev& mn = b.vmatTab[b.vmat[vn]];
+    const VoxMatDev& mp = b.vmatTab[b.vmat[vp]];
+    if (mn.fixed && mp.fixed) return;   // VX3_VoxelyzeKernel.cu:572
+    const real nom = (real)sc.voxSize;
+    // CVX_Link::updateRestLength: 0.5*(baseSize_neg + baseSize_pos)
+    const real restLength = (real)0.5 * (nom * (real)b.tfac[vn] + nom * (real)b.tfac[vp]);
+    float area, tss;
+    const float size = (float)sc.voxSize;
+    if (mat.nu != 0.0f) {
+        float an, sn, ap, sp;
+        transverse(b, vn, axis, size, mn.nu == 0.0f, an, sn);
+        transverse(b, vp, axis, size, mp.nu == 0.0f, ap, sp);
+        area = 0.5f * (an + ap);
+        tss = 0.5f * (sn + sp);
+    } else {
+        area = 0.5f * (size * size + size * size);
+        tss = 0.0f;
+    }
+    // damping multipliers 2*sqrt(m)*zeta/previousDt (VX_Voxel.h:131); material-fixed voxels never step
+    const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : sc.dt);
+    const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : sc.dt);
+    LinkState<real> st;
+    st.pos2 = V3<real>(b.q2x[l], b.q2y[l], b.q2z[l]);
+    st.a1v = V3<real>(b.a1x[l], b.a1y[l], b.a1z[l]);
+    st.a2v = V3<real>(b.a2x[l], b.a2y[l], b.a2z[l]);
+    st.strain = b.strain[l];
+    st.maxStrain = b.maxStrain[l];
+    st.strainOffset = b.strainOffset[l];
+    st.flags = b.lflags[l];
+    const V3<real> pN(b.px[vn], b.py[vn], b.pz[vn]), pP(b.px[vp], b.py[vp], b.pz[vp]);
+    const Q4<real> oN(b.ow[vn], b.ox[vn], b.oy[vn], b.oz[vn]), oP(b.ow[vp], b.ox[vp], b.oy[vp], b.oz[vp]);
+    V3<real> fN, mN, fP, mP;
+    linkForces<real>(mat, axis, pN, oN, pP, oP, restLength, area, tss, mat.nu != 0.0f, dmN, dmP, st, fN, mN, fP, mP);
+    b.q2x[l] = st.pos2.x; b.q2y[l] = st.pos2.y; b.q2z[l] = st.pos2.z;
+    b.a1x[l] = st.a1v.x; b.a1y[l] = st.a1v.y; b.a1z[l] = st.a1v.z;
+    b.a2x[l] = st.a2v.x; b.a2y[l] = st.a2v.y; b.a2z[l] = st.a2v.z;
+    b.strain[l] = st.strain;
+    b.maxStrain[l] = st.maxStrain;
+    b.strainOffset[l] = st.strainOffset;
+    b.lflags[l] = st.flags;
+    // slot of the negative end: the link lies in direction 2*axis (X_POS/Y_POS/Z_POS) from it
+    const int dn = 2 * axis, dp = 2 * axis + 1;
+    real* F = b.F;
+    const size_t V = (size_t)b.Vcap;
+    F[(dn * 6 + 0) * V + vn] = fN.x; F[(dn * 6 + 1) * V + vn] = fN.y; F[(dn * 6 + 2) * V + vn] = fN.z;
+    F[(dn * 6 + 3) * V + vn] = mN.x; F[(dn * 6 + 4) * V + vn] = mN.y; F[(dn * 6 + 5) * V + vn] = mN.z;
+    F[(dp * 6 + 0) * V + vp] = fP.x; F[(dp * 6 + 1) * V + vp] = fP.y; F[(dp * 6 + 2) * V + vp] = fP.z;
+    F[(dp * 6 + 3) * V + vp] = mP.x; F[(dp * 6 + 4) * V + vp] = mP.y; F[(dp * 6 + 5) * V + vp] = mP.z;
+    if (sc.flags & SC_POISSON) {
+        // CVX_Link::axialStrain(positiveEnd) (VX_Link.cpp:115-118)
+        const float r = mat.strainRatio;
+        b.es[dn * V + vn] = 2.0f * st.strain / (1.0f + r);
+        b.es[dp * V + vp] = 2.0f * st.strain * r / (1.0f + r);
+    }
+    if (st.strain > 100) atomicOr(&b.sims[s].flags, SC_DIVERGED_PENDING);   // Voxelyze.cpp:265
+}
+
+// CVX_Collision::updateContactForce (VX_Collision.cpp:41-56) for every watched pair.
+__global__ void __launch_bounds__(256) k_contacts(DevBatch<real> b, int nPairs) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nPairs) return;
+    const int i = b.cpair[2 * p], j = b.cpair[2 * p + 1];
+    const int s = b.vsim[i];
+    if (!simRunning(b.sims, s) || (b.sims[s].flags & SC_DIVERGED_PENDING)) return;
+    const VoxMatDev& m1 = b.vmatTab[b.vmat[i]];
+    const VoxMatDev& m2 = b.vmatTab[b.vmat[j]];
+    const float k = 2.0f / (1.0f / m1.penetrationStiffness + 1.0f / m2.penetrationStiffness);
+    const float c = 0.5f * (m1.zetaCollision * m1.c2xSqMxExS + m2.zetaCollision * m2.c2xSqMxExS);
+    const real nom = (real)b.sims[s].voxSize;
+    const float ox = (float)(b.px[j] - b.px[i]), oy = (float)(b.py[j] - b.py[i]), oz = (float)(b.pz[j] - b.pz[i]);
+    const real b1 = nom * (real)b.tfac[i], b2 = nom * (real)b.tfac[j];
+    const float nomDist = (float)(((b1 + b1 + b1) / (real)3.0f + (b2 + b2 + b2) / (real)3.0f) * (real)0.625f);
+    const float len = sqrtf(ox * ox + oy * oy + oz * oz);
+    const float rel = nomDist - len;
+    float fx = 0, fy = 0, fz = 0;
+    if (rel > 0) {
+        float ux = ox, uy = oy, uz = oz;
+        if (len > 0) { float inv = 1.0f / len; ux = inv * ox; uy = inv * oy; uz = inv * oz; }
+        const real mi1 = (real)m1.massInverse, mi2 = (real)m2.massInverse;
+        const real v1 = (b.mx[i] * mi1) * (real)ux + (b.my[i] * mi1) * (real)uy + (b.mz[i] * mi1) * (real)uz;
+        const real v2 = (b.mx[j] * mi2) * (real)ux + (b.my[j] * mi2) * (real)uy + (b.mz[j] * mi2) * (real)uz;
+        const float relVel = (float)(v1 - v2);
+        const float mag = k * rel + c * relVel;
+        fx = mag * ux; fy = mag * uy; fz = mag * uz;
+    }
+    b.cforce[3 * p] = fx; b.cforce[3 * p + 1] = fy; b.cforce[3 * p + 2] = fz;
+}
+
+__global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!simRunning(b.sims, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (sc.flags & SC_DIVERGED_PENDING) return;   // doTimeStep returns before the voxel loop
+    const VoxMatDev& m = b.vmatTab[b.vmat[v]];
+    const float tf = b.tfac[v];
+    if (!m.fixed) {
+        int fl = b.vflags[v];
+        V3<real> f(0, 0, 0), mo(0, 0, 0);
+        const size_t V = (size_t)b.Vcap;
+        const real* F = b.F;
+#pragma unroll
+        for (int d = 0; d < 6; d++) {
+            if (fl & (1 << (VF_LINKMASK_SHIFT + d))) {
+                f.add(V3<real>(F[(d * 6 + 0) * V + v], F[(d * 6 + 1) * V + v], F[(d * 6 + 2) * V + v]));
+                mo.add(V3<real>(F[(d * 6 + 3) * V + v], F[(d * 6 + 4) * V + v], F[(d * 6 + 5) * V + v]));
+            }
+        }
+        const int e = b.vext[v];
+        const ExtDev* ext = e >= 0 ? &b.extTab[e] : nullptr;
+        int ix = 0, iy = 0, iz = 0;
+        if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
+        int cb = 0, ce = 0;
+        if ((sc.flags & SC_COLLISIONS) && b.vcStart) { cb = b.vcStart[v]; ce = b.vcStart[v + 1]; }
+        V3<real> pos(b.px[v], b.py[v], b.pz[v]), lm(b.mx[v], b.my[v], b.mz[v]), am(b.ax[v], b.ay[v], b.az[v]);
+        Q4<real> o(b.ow[v], b.ox[v], b.oy[v], b.oz[v]);
+        voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, b.vcList, cb, ce, b.cforce, pos, lm, o, am, fl);
+        b.px[v] = pos.x; b.py[v] = pos.y; b.pz[v] = pos.z;
+        b.mx[v] = lm.x; b.my[v] = lm.y; b.mz[v] = lm.z;
+        b.ow[v] = o.w; b.ox[v] = o.x; b.oy[v] = o.y; b.oz[v] = o.z;
+        b.ax[v] = am.x; b.ay[v] = am.y; b.az[v] = am.z;
+        b.vflags[v] = fl;
+    }
+    // thermal factor for the next step: VX3 updateTemperature at t + dt (VX3_VoxelyzeKernel.cu:625-650)
+    if ((sc.flags & SC_VARY) && !m.fixed) {
+        const double tn = sc.t + (double)sc.dt;
+        if (!(m.thermalOnAfter > tn)) {
+            double T = sc.tempAmp * sin((double)(2 * 3.1415926f) * (tn / sc.tempPeriod + b.phase[v]));
+            if (!(sc.flags & SC_EXPANSION) && T > 0) T = 0;
+            const float temp = (float)T;
+            b.tfac[v] = 1 + temp * m.alphaCTE;
+        }
+    }
+}
+
+// ---- MathTree (queue evaluator of Utils/VX3_MathTree.h:50-192), device side, bounded programs
+#define VX3_DEV_MT_MAX 64
+__device__ double mathTreeEval(const TokenDev* buf, int n, double x, double y, double z, double hit, double t, double angle,
+                               double closeness, int ncp, int nvox) {
+    if (n <= 0) return 0.0;
+    double q[VX3_DEV_MT_MAX + 1];
+    int vc = 0, pc = 0;
+    for (int i = 0; i < n && i < VX3_DEV_MT_MAX; i++) {
+        const int op = buf[i].op;
+        const double val = buf[i].value;
+        double r = 0;
+        switch (op) {
+        case 0: return q[pc];
+        case 1: r = val; break;
+        case 2: r = 2.71828182845904523536; break;
+        case 3: r = 3.14159265358979323846; break;
+        case 4: {
+            int k = val < 0.5 ? 0 : (int)(val + 0.5);
+            r = k == 0 ? x : k == 1 ? y : k == 2 ? z : k == 3 ? hit : k == 4 ? t : k == 5 ? angle : k == 6 ? closeness
+                : k == 7 ? (double)ncp : k == 8 ? (double)nvox : 0.0;
+            break;
+        }
+        case 5: r = q[pc + 1] + q[pc]; pc += 2; break;
+        case 6: r = q[pc + 1] - q[pc]; pc += 2; break;
+        case 7: r = q[pc + 1] * q[pc]; pc += 2; break;
+        case 8: r = q[pc + 1] / q[pc]; pc += 2; break;
+        case 9: r = pow(q[pc + 1], q[pc]); pc += 2; break;
+        case 10: r = sqrt(q[pc]); pc++; break;
+        case 11: r = sin(q[pc]); pc++; break;
+        case 12: r = cos(q[pc]); pc++; break;
+        case 13: r = tan(q[pc]); pc++; break;
+        case 14: r = atan(q[pc]); pc++; break;
+        case 15: r = log(q[pc]); pc++; break;
+        case 16: r = rint(q[pc]); pc++; break;
+        case 17: r = fabs(q[pc]); pc++; break;
+        case 18: r = !q[pc]; pc++; break;
+        case 19: r = q[pc + 1] > q[pc]; pc += 2; break;
+        case 20: r = q[pc + 1] < q[pc]; pc += 2; break;
+        case 21: r = q[pc + 1] && q[pc]; pc += 2; break;
+        case 22: r = q[pc + 1] || q[pc]; pc += 2; break;
+        case 23: r = 0.5 * erfc(-q[pc] / sqrt(2.0)); pc++; break;
+        default: return -1;
+        }
+        if (pc > vc) return -1;
+        q[vc++] = r;
+    }
+    return q[pc < VX3_DEV_MT_MAX ? pc : 0];
+}
+
+// Block-wide deterministic FP64 center of mass of measured voxels (VX3_VoxelyzeKernel.cu:477-493).
+__device__ void blockCoM(const DevBatch<real>& b, const SimCtl& sc, double out[3]) {
+    __shared__ double red[4][256];
+    double sx = 0, sy = 0, sz = 0, sm = 0;
+    for (int v = sc.v0 + threadIdx.x; v < sc.v0 + sc.nv; v += blockDim.x) {
+        const VoxMatDev& m = b.vmatTab[b.vmat[v]];
+        if (!m.isMeasured) continue;
+        const double mass = m.mass;
+        sx += (double)b.px[v] * mass; sy += (double)b.py[v] * mass; sz += (double)b.pz[v] * mass;
+        sm += mass;
+    }
+    red[0][threadIdx.x] = sx; red[1][threadIdx.x] = sy; red[2][threadIdx.x] = sz; red[3][threadIdx.x] = sm;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    const double tot = red[3][0];
+    for (int k = 0; k < 3; k++) out[k] = tot == 0 ? 0.0 : (1.0 / tot) * red[k][0];
+    __syncthreads();
+}
+
+// Per-simulation bookkeeping after the voxel pass (doTimeStep tail, VX3_VoxelyzeKernel.cu:241,279,314-334,352,
+// and the stop test the driver loop makes before the next step, VX3_SimulationManager.cu:62-64).
+__global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
+    const int s = blockIdx.x;
+    if (s >= b.nSims) return;
+    SimCtl& sc = b.sims[s];
+    if (sc.state != 0) return;
+    const bool diverged = (sc.flags & SC_DIVERGED_PENDING) != 0;
+    const int steps = sc.steps + 1;
+    const bool sample = !diverged && sc.cycleStep > 0 && steps % sc.cycleStep == 0;
+    double com[3];
+    if (sample) blockCoM(b, sc, com);
+    if (threadIdx.x != 0) return;
+    sc.steps = steps;
+    if (diverged) {
+        sc.state = 2;
+        return;
+    }
+    if (sample) {
+        sc.angleSamples++;
+        for (int k = 0; k < 3; k++) { sc.comH0[k] = sc.comH1[k]; sc.comH1[k] = sc.com[k]; sc.com[k] = com[k]; }
+        const double* A = sc.comH0; const double* B = sc.comH1; const double* C = sc.com;
+        const bool bc = B[0] == C[0] && B[1] == C[1] && B[2] == C[2];
+        const bool ab = A[0] == B[0] && A[1] == B[1] && A[2] == B[2];
+        if (bc || ab || sc.angleSamples < 3) sc.recentAngle = 0;
+        else {
+            double d1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, d2[3] = {C[0] - B[0], C[1] - B[1], C[2] - B[2]};
+            double dot = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+            double l1 = sqrt((A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]));
+            double l2 = sqrt((B[0] - C[0]) * (B[0] - C[0]) + (B[1] - C[1]) * (B[1] - C[1]) + (B[2] - C[2]) * (B[2] - C[2]));
+            sc.recentAngle = acos(dot / (l1 * l2));
+        }
+    }
+    sc.t += (double)sc.dt;
+    if (sc.maxSteps > 0 && steps >= sc.maxSteps) { sc.state = 1; return; }
+    if (sc.stopLen > 0 &&
+        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], 0, sc.t, sc.recentAngle, 0, 0, sc.nv) > 0)
+        sc.state = 1;
+}
+
+// Recommended time step of each newly admitted simulation: max over its links of
+// k/min(m1, m2) (VX3_VoxelyzeKernel.cu:184-217), wave shuffles + LDS in FP64.
+__device__ __forceinline__ double waveMax(double x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__global__ void __launch_bounds__(256) k_dt(DevBatch<real> b, int s0, int s1) {
+    const int s = s0 + blockIdx.x;
+    if (s >= s1) return;
+    SimCtl& sc = b.sims[s];
+    if (sc.state != 0 || sc.optDt != 0.0) return;
+    __shared__ double part[8];
+    double mx = 0.0;
+    const float size = (float)sc.voxSize;
+    const real nom = (real)sc.voxSize;
+    for (int l = sc.l0 + threadIdx.x; l < sc.l0 + sc.nl; l += blockDim.x) {
+        const int vn = b.lneg[l], vp = b.lpos[l];
+        const LinkMatDev& mat = b.lmatTab[b.lmeta[l] >> 2];
+        const VoxMatDev& mn = b.vmatTab[b.vmat[vn]];
+        const VoxMatDev& mp = b.vmatTab[b.vmat[vp]];
+        float k;
+        if (mat.nu == 0.0f) k = mat.a1;
+        else {
+            const int axis = b.lmeta[l] & 3;
+            float an, sn, ap, sp;
+            transverse(b, vn, axis, size, mn.nu == 0.0f, an, sn);
+            transverse(b, vp, axis, size, mp.nu == 0.0f, ap, sp);
+            const float area = 0.5f * (an + ap);
+            const double rl = 0.5 * ((double)nom * (double)b.tfac[vn] + (double)nom * (double)b.tfac[vp]);
+            k = (float)(mat.eHat * area / ((b.strain[l] + 1) * rl));
+        }
+        const double m1 = mn.mass, m2 = mp.mass;
+        const double f2 = k / (m1 < m2 ? m1 : m2);
+        if (f2 > mx) mx = f2;
+    }
+    if (sc.nl == 0) {
+        for (int v = sc.v0 + threadIdx.x; v < sc.v0 + sc.nv; v += blockDim.x) {
+            const VoxMatDev& m = b.vmatTab[b.vmat[v]];
+            const double f2 = m.E * m.nomSize / m.mass;
+            if (f2 > mx) mx = f2;
+        }
+    }
+    mx = waveMax(mx);
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) part[wid] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = fmax(m, part[w]);
+        double opt = m <= 0.0f ? 0.0 : 1.0f / (6.283185f * sqrt(m));
+        if (opt < 1e-10) opt = 1e-10;
+        sc.optDt = opt;
+        if (!(sc.dt > 0)) sc.dt = (float)(sc.dtFrac * opt);   // doTimeStep: dt = DtFrac * OptimalDt (float)
+        sc.cycleStep = sc.tempPeriod / (double)sc.dt >= 1.0 ? (int)(sc.tempPeriod / (double)sc.dt) : 0;
+    }
+}
+
+}  // namespace VX_NS
+}  // namespace vx3

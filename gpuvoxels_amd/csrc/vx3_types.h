@@ -1,0 +1,93 @@
+// vx3_types.h -- POD tables shared by the host builder and the HIP kernels.
+//
+// Device layout (structure of arrays, one batch = many simulations):
+//   voxels  [0, Vcap): position, linear momentum, orientation, angular momentum in
+//           the state precision T (float on the product path, double for the
+//           parity-pinning path), plus flags, thermal size factor, material,
+//           owning simulation.  Each simulation owns one contiguous voxel range.
+//   links   [0, Lcap): endpoints, axis/material, beam state (previous local pose
+//           for damping, strains, small-angle flags).  Contiguous range per sim.
+//   forces  F[36][Vcap]: the link pass writes each link end's local force and
+//           moment into the slot (direction, voxel) of the voxel it acts on; the
+//           voxel pass reads its own six slots.  Slot d of voxel v is the link
+//           in direction d (X_POS..Z_NEG, Voxelyze3/src/old/types.h:13-20) -- the
+//           slot address is the gather index, so no CSR arrays or atomics are needed.
+#ifndef VX3_TYPES_H
+#define VX3_TYPES_H
+
+#include <stdint.h>
+
+#define VX3_MAX_CURVE 16
+
+// voxel flag bits (state bits follow Voxelyze3/src/old/types.h:32-38)
+enum {
+    VF_INTERIOR = 1 << 1,            // all six links present (the reference's "SURFACE" bit)
+    VF_FLOOR = 1 << 2,
+    VF_STATIC = 1 << 3,              // floor static friction
+    VF_COLL = 1 << 5,
+    VF_LINKMASK_SHIFT = 8,           // bits 8..13: link present in direction d
+};
+// link state bits
+enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2 };
+// per-sim control bits
+enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32 };
+
+struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
+    float mass, massInverse, sqrtMass, momentInertiaInverse;
+    float c2xSqMxExS, c2xSqIxExSxSxS;
+    float zetaInternal, zetaGlobal, zetaCollision;
+    float gravityForce;              // -mass*9.80665f*gravMult
+    float penetrationStiffness;      // (float)(2*E*nomSize)
+    float muStatic, muKinetic;
+    float alphaCTE;
+    float nu;
+    float E;
+    double nomSize;
+    double thermalOnAfter;
+    int fixed;                       // VX3 material-fixed
+    int isMeasured;
+};
+
+struct LinkMatDev {                 // CVX_MaterialLink derived constants (VX_MaterialLink.cpp:121-141)
+    float a1, a2, b1, b2, b3;
+    float sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp;
+    float E, nu, eHat;
+    float epsilonFail;               // -1 = none
+    float strainRatio;               // E(pos voxel)/E(neg voxel)
+    int linear;
+    int nData;
+    float strainData[VX3_MAX_CURVE];
+    float stressData[VX3_MAX_CURVE];
+};
+
+struct ExtDev {                     // CVX_External (VX_External.h:34-94)
+    int dofFixed;
+    float force[3], moment[3];
+    double translate[3];
+    double rotQ[4];                  // rotationQuat() (w,x,y,z)
+    int hasRotQ;
+    int ix, iy, iz;
+};
+
+struct SimCtl {                     // one running simulation (VX3_VoxelyzeKernel scalars)
+    double t;                        // currentTime (double accumulator)
+    double optDt;                    // OptimalDt
+    double tempAmp, tempPeriod;
+    double dtFrac;
+    double com[3], comH0[3], comH1[3], com0[3];
+    double recentAngle;
+    double voxSize;
+    float dt;                        // step size (float, as doTimeStep's parameter)
+    float tInit;                     // temperature of voxels not driven by the actuation
+    int state;                       // VX3H_SIM_*
+    int steps, maxSteps, cycleStep, angleSamples;
+    int flags;                       // SC_*
+    int v0, nv, l0, nl;
+    int stopOff, stopLen;            // MathTree program in the token pool
+    int nColl;                       // collision pairs owned (pair range below)
+    int c0;
+};
+
+struct TokenDev { int op; int pad; double value; };
+
+#endif

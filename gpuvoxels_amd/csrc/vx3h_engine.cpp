@@ -1,0 +1,672 @@
+// vx3h_engine.cpp -- the batch engine behind the C ABI (include/vx3h.h).
+//
+// A batch owns one HIP stream and structure-of-arrays device buffers for many
+// simulations.  Simulations occupy contiguous voxel and link ranges handed out
+// by first-fit free lists, so finished simulations can be retired and new ones
+// admitted between launches (continuous batching).  Every kernel covers the
+// high-water mark of the ranges and skips elements whose simulation is not
+// running.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vx3h.h"
+#include "vx3_launch.h"
+#include "vx3h_build.h"
+#include "vx3h_internal.h"
+
+namespace vx3 {
+thread_local std::string g_err;
+}
+using vx3::g_err;
+
+#define FAILV(code, msg)        \
+    do {                        \
+        g_err = (msg);          \
+        return (code);          \
+    } while (0)
+#define HIPCHK(x)                                                                 \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess) {                                                   \
+            g_err = std::string(#x) + " failed: " + hipGetErrorString(e_);        \
+            return VX3H_EDEVICE;                                                  \
+        }                                                                         \
+    } while (0)
+
+extern "C" const char* vx3h_last_error(void) { return g_err.c_str(); }
+
+extern "C" vx3h_status vx3h_device_count(int* count) {
+    if (!count) FAILV(VX3H_EINVAL, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return VX3H_OK;
+}
+
+namespace {
+
+using namespace vx3;
+
+// One device array family: `per` values of `elem` bytes per element, stored
+// either interleaved (index*per + k) or strided by capacity (k*cap + index).
+struct ArrSpec {
+    void** slot;
+    size_t elem;
+    int per;
+    bool strided;
+};
+
+struct Range { int start, len; };
+
+struct RangeAlloc {
+    std::map<int, int> freeRanges;   // start -> length
+    int hi = 0;
+    int alloc(int n, int cap, bool& grow) {
+        grow = false;
+        for (auto it = freeRanges.begin(); it != freeRanges.end(); ++it) {
+            if (it->second >= n) {
+                int s = it->first, len = it->second;
+                freeRanges.erase(it);
+                if (len > n) freeRanges[s + n] = len - n;
+                return s;
+            }
+        }
+        int s = hi;
+        hi += n;
+        if (hi > cap) grow = true;
+        return s;
+    }
+    void release(int s, int n) {
+        if (n <= 0) return;
+        freeRanges[s] = n;
+        auto it = freeRanges.find(s);
+        auto nx = std::next(it);
+        if (nx != freeRanges.end() && it->first + it->second == nx->first) { it->second += nx->second; freeRanges.erase(nx); }
+        if (it != freeRanges.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; freeRanges.erase(it); it = pv; }
+        }
+        if (it->first + it->second == hi) { hi = it->first; freeRanges.erase(it); }
+    }
+};
+
+struct SlotHost {
+    bool used = false;
+    int v0 = 0, nv = 0, l0 = 0, nl = 0;
+    std::vector<int> ijk;
+    std::vector<int> matid;
+    std::vector<int> measured;   // per voxel
+    std::vector<float> mass;     // per voxel
+    double voxSize = 0;
+    double com0[3] = {0, 0, 0};
+    std::vector<vx3h_token> fit;
+    size_t bytes = 0;
+};
+
+template <typename K> struct Dedup {
+    std::vector<K> items;
+    std::map<std::string, int> index;
+    int add(const K& k) {
+        std::string key((const char*)&k, sizeof(K));
+        auto it = index.find(key);
+        if (it != index.end()) return it->second;
+        int i = (int)items.size();
+        items.push_back(k);
+        index[key] = i;
+        return i;
+    }
+};
+
+}  // namespace
+
+struct vx3h_batch {
+    int device = 0;
+    vx3h_batch_cfg cfg{};
+    hipStream_t stream = nullptr;
+    size_t realSize = 4;
+    DevBatch<float> db{};   // pointer layout shared with DevBatch<double>
+    std::vector<ArrSpec> vArrs, lArrs;
+    RangeAlloc va, la;
+    std::vector<SlotHost> slots;
+    int simCap = 0;
+    Dedup<VoxMatDev> vmats;
+    Dedup<LinkMatDev> lmats;
+    std::vector<ExtDev> exts;
+    std::vector<TokenDev> tokens;
+    void* dVmat = nullptr; size_t capVmat = 0;
+    void* dLmat = nullptr; size_t capLmat = 0;
+    void* dExt = nullptr; size_t capExt = 0;
+    void* dTok = nullptr; size_t capTok = 0;
+    bool anyPoisson = false, anyColl = false;
+    int dtLo = 1 << 30, dtHi = -1;
+    size_t bytesUsed = 0, budget = 0;
+    // graph of `graphSteps` steps, valid for one layout version
+    hipGraphExec_t graphExec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graphSteps = 0;
+    long layout = 0, graphLayout = -1;
+};
+
+namespace {
+
+size_t voxelBytes(size_t rs) { return rs * (13 + 36) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 8; }
+size_t linkBytes(size_t rs) { return rs * 9 + 4 * 5 + 4 * 3; }
+
+void initSpecs(vx3h_batch* b) {
+    DevBatch<float>& d = b->db;
+    const size_t R = b->realSize;
+    auto V = [&](void* p, size_t e, int per, bool st) { b->vArrs.push_back({(void**)p, e, per, st}); };
+    auto L = [&](void* p, size_t e, int per, bool st) { b->lArrs.push_back({(void**)p, e, per, st}); };
+    V(&d.px, R, 1, false); V(&d.py, R, 1, false); V(&d.pz, R, 1, false);
+    V(&d.mx, R, 1, false); V(&d.my, R, 1, false); V(&d.mz, R, 1, false);
+    V(&d.ow, R, 1, false); V(&d.ox, R, 1, false); V(&d.oy, R, 1, false); V(&d.oz, R, 1, false);
+    V(&d.ax, R, 1, false); V(&d.ay, R, 1, false); V(&d.az, R, 1, false);
+    V(&d.vflags, 4, 1, false); V(&d.tfac, 4, 1, false); V(&d.phase, 8, 1, false);
+    V(&d.vmat, 4, 1, false); V(&d.vsim, 4, 1, false); V(&d.vext, 4, 1, false); V(&d.vijk, 4, 3, false);
+    V(&d.psx, 4, 1, false); V(&d.psy, 4, 1, false); V(&d.psz, 4, 1, false);
+    V(&d.F, R, 36, true); V(&d.es, 4, 6, true);
+    L(&d.lneg, 4, 1, false); L(&d.lpos, 4, 1, false); L(&d.lmeta, 4, 1, false); L(&d.lsim, 4, 1, false); L(&d.lflags, 4, 1, false);
+    L(&d.q2x, R, 1, false); L(&d.q2y, R, 1, false); L(&d.q2z, R, 1, false);
+    L(&d.a1x, R, 1, false); L(&d.a1y, R, 1, false); L(&d.a1z, R, 1, false);
+    L(&d.a2x, R, 1, false); L(&d.a2y, R, 1, false); L(&d.a2z, R, 1, false);
+    L(&d.strain, 4, 1, false); L(&d.maxStrain, 4, 1, false); L(&d.strainOffset, 4, 1, false);
+}
+
+vx3h_status growArrays(vx3h_batch* b, std::vector<ArrSpec>& arrs, int& cap, int need, int used, bool isVoxel) {
+    if (need <= cap) return VX3H_OK;
+    int ncap = std::max(need, std::max(2 * cap, 1024));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (ArrSpec& a : arrs) {
+        void* np = nullptr;
+        size_t bytes = (size_t)ncap * a.per * a.elem;
+        HIPCHK(hipMalloc(&np, bytes));
+        HIPCHK(hipMemsetAsync(np, 0, bytes, b->stream));
+        if (*a.slot && used > 0) {
+            if (a.strided)
+                HIPCHK(hipMemcpy2DAsync(np, (size_t)ncap * a.elem, *a.slot, (size_t)cap * a.elem, (size_t)used * a.elem, a.per,
+                                        hipMemcpyDeviceToDevice, b->stream));
+            else
+                HIPCHK(hipMemcpyAsync(np, *a.slot, (size_t)used * a.per * a.elem, hipMemcpyDeviceToDevice, b->stream));
+        }
+        HIPCHK(hipStreamSynchronize(b->stream));
+        if (*a.slot) HIPCHK(hipFree(*a.slot));
+        *a.slot = np;
+    }
+    // unused elements belong to no simulation
+    int* owner = isVoxel ? b->db.vsim : b->db.lsim;
+    if (ncap > used) HIPCHK(hipMemsetAsync(owner + used, 0xFF, (size_t)(ncap - used) * 4, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    cap = ncap;
+    b->layout++;
+    return VX3H_OK;
+}
+
+template <typename E> vx3h_status uploadTable(vx3h_batch* b, void*& dptr, size_t& cap, const std::vector<E>& v) {
+    if (v.empty()) return VX3H_OK;
+    if (v.size() > cap) {
+        size_t nc = std::max(v.size(), 2 * cap + 16);
+        void* np = nullptr;
+        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipMalloc(&np, nc * sizeof(E)));
+        if (dptr) HIPCHK(hipFree(dptr));
+        dptr = np;
+        cap = nc;
+        b->layout++;
+    }
+    HIPCHK(hipMemcpy(dptr, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
+    return VX3H_OK;
+}
+
+template <typename T> void putReal(std::vector<char>& out, const std::vector<double>& src) {
+    out.resize(src.size() * sizeof(T));
+    T* p = (T*)out.data();
+    for (size_t i = 0; i < src.size(); i++) p[i] = (T)src[i];
+}
+
+vx3h_status copyReal(vx3h_batch* b, void* dst, const std::vector<double>& src) {
+    std::vector<char> tmp;
+    if (b->realSize == 4) putReal<float>(tmp, src);
+    else putReal<double>(tmp, src);
+    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    return VX3H_OK;
+}
+
+vx3h_status readReal(vx3h_batch* b, const void* src, size_t n, std::vector<double>& out) {
+    out.resize(n);
+    if (b->realSize == 4) {
+        std::vector<float> t(n);
+        HIPCHK(hipMemcpy(t.data(), src, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) out[i] = t[i];
+    } else {
+        HIPCHK(hipMemcpy(out.data(), src, n * 8, hipMemcpyDeviceToHost));
+    }
+    return VX3H_OK;
+}
+
+char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx; }
+
+vx3h_status launchSteps(vx3h_batch* b, int n) {
+    StepOpts o;
+    o.poisson = b->anyPoisson;
+    o.contacts = false;
+    o.nPairs = 0;
+    o.block = b->cfg.block_size;
+    for (int i = 0; i < n; i++) {
+        if (b->realSize == 4) launchStep(b->db, b->stream, o);
+        else launchStep(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, o);
+    }
+    HIPCHK(hipGetLastError());
+    return VX3H_OK;
+}
+
+vx3h_status ensureGraph(vx3h_batch* b, int steps) {
+    if (b->graphExec && b->graphLayout == b->layout && b->graphSteps == steps) return VX3H_OK;
+    if (b->graphExec) { HIPCHK(hipGraphExecDestroy(b->graphExec)); b->graphExec = nullptr; }
+    if (b->graph) { HIPCHK(hipGraphDestroy(b->graph)); b->graph = nullptr; }
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    vx3h_status st = launchSteps(b, steps);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st != VX3H_OK) return st;
+    if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("graph capture failed: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->graphExec, g, nullptr, nullptr, 0));
+    b->graphLayout = b->layout;
+    b->graphSteps = steps;
+    return VX3H_OK;
+}
+
+}  // namespace
+
+extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out) {
+    if (!out) FAILV(VX3H_EINVAL, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) FAILV(VX3H_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= n) FAILV(VX3H_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    vx3h_batch* b = new vx3h_batch();
+    b->device = device;
+    if (cfg) b->cfg = *cfg;
+    b->realSize = b->cfg.precision == VX3H_PREC_F64 ? 8 : 4;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete b; FAILV(VX3H_EDEVICE, "stream creation failed"); }
+    memset(&b->db, 0, sizeof(b->db));
+    initSpecs(b);
+    size_t freeB = 0, totB = 0;
+    if (hipMemGetInfo(&freeB, &totB) != hipSuccess) freeB = 0;
+    b->budget = b->cfg.max_device_bytes ? b->cfg.max_device_bytes : (size_t)(0.9 * (double)freeB);
+    *out = b;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
+    if (!b) return VX3H_OK;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    if (b->graphExec) hipGraphExecDestroy(b->graphExec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (auto* arrs : {&b->vArrs, &b->lArrs})
+        for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
+    for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, (void*)b->db.sims}) if (p) hipFree(p);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* d, int precision, size_t* bytes) {
+    if (!d || !bytes) FAILV(VX3H_EINVAL, "null argument");
+    HostModel H;
+    if (!buildModel(d, H, g_err)) return VX3H_EINVAL;
+    size_t rs = precision == VX3H_PREC_F64 ? 8 : 4;
+    *bytes = (size_t)H.nv * voxelBytes(rs) + (size_t)H.nl * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, int* slot) {
+    if (!b || !desc || !slot) FAILV(VX3H_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    HostModel H;
+    if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
+    if (H.collisions) FAILV(VX3H_EUNSUPPORTED, "collisions are not enabled in this build of the stream path");
+    if (H.stopTok.size() > 64) FAILV(VX3H_EUNSUPPORTED, "stop condition longer than 64 tokens");
+    const size_t rs = b->realSize;
+    const size_t need = (size_t)H.nv * voxelBytes(rs) + (size_t)H.nl * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
+    if (b->bytesUsed + need > b->budget) FAILV(VX3H_ENOMEM, "simulation does not fit the batch's device memory budget");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    // slot
+    int s = -1;
+    for (int i = 0; i < (int)b->slots.size(); i++) if (!b->slots[i].used) { s = i; break; }
+    if (s < 0) { s = (int)b->slots.size(); b->slots.emplace_back(); }
+    if ((int)b->slots.size() > b->simCap) {
+        int nc = std::max((int)b->slots.size(), 2 * b->simCap + 8);
+        SimCtl* np = nullptr;
+        HIPCHK(hipMalloc(&np, nc * sizeof(SimCtl)));
+        HIPCHK(hipMemset(np, 0, nc * sizeof(SimCtl)));
+        if (b->db.sims) {
+            HIPCHK(hipMemcpy(np, b->db.sims, b->simCap * sizeof(SimCtl), hipMemcpyDeviceToDevice));
+            HIPCHK(hipFree(b->db.sims));
+        }
+        b->db.sims = np;
+        b->simCap = nc;
+        b->layout++;
+    }
+    // ranges
+    bool gv = false, gl = false;
+    int v0 = H.nv ? b->va.alloc(H.nv, b->db.Vcap, gv) : 0;
+    int l0 = H.nl ? b->la.alloc(H.nl, b->db.Lcap, gl) : 0;
+    if (gv) { vx3h_status st = growArrays(b, b->vArrs, b->db.Vcap, b->va.hi, v0, true); if (st) return st; }
+    if (gl) { vx3h_status st = growArrays(b, b->lArrs, b->db.Lcap, b->la.hi, l0, false); if (st) return st; }
+    // tables
+    std::vector<int> vmIdx(H.vmats.size()), lmIdx(H.lmats.size());
+    size_t nvm = b->vmats.items.size(), nlm = b->lmats.items.size();
+    for (size_t i = 0; i < H.vmats.size(); i++) vmIdx[i] = b->vmats.add(H.vmats[i]);
+    for (size_t i = 0; i < H.lmats.size(); i++) lmIdx[i] = b->lmats.add(H.lmats[i]);
+    if (b->vmats.items.size() != nvm || !b->dVmat) { vx3h_status st = uploadTable(b, b->dVmat, b->capVmat, b->vmats.items); if (st) return st; }
+    if (b->lmats.items.size() != nlm || !b->dLmat) { vx3h_status st = uploadTable(b, b->dLmat, b->capLmat, b->lmats.items); if (st) return st; }
+    int ext0 = (int)b->exts.size();
+    if (!H.exts.empty()) {
+        b->exts.insert(b->exts.end(), H.exts.begin(), H.exts.end());
+        vx3h_status st = uploadTable(b, b->dExt, b->capExt, b->exts);
+        if (st) return st;
+    }
+    int tok0 = (int)b->tokens.size();
+    for (const vx3h_token& t : H.stopTok) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
+    if (!H.stopTok.empty()) { vx3h_status st = uploadTable(b, b->dTok, b->capTok, b->tokens); if (st) return st; }
+    b->db.vmatTab = (const VoxMatDev*)b->dVmat;
+    b->db.lmatTab = (const LinkMatDev*)b->dLmat;
+    b->db.extTab = (const ExtDev*)b->dExt;
+    b->db.tokens = (const TokenDev*)b->dTok;
+
+    // voxel initial state (CVX_Voxel::reset + CVoxelyze::addVoxel, VX_Voxel.cpp:48-57, Voxelyze.cpp:439-461)
+    DevBatch<float>& d = b->db;
+    const int NV = H.nv, NL = H.nl;
+    if (NV) {
+        std::vector<double> px(NV), py(NV), pz(NV), zero(NV, 0.0), one(NV, 1.0);
+        for (int i = 0; i < NV; i++) {
+            px[i] = H.ijk[3 * i] * H.voxSize; py[i] = H.ijk[3 * i + 1] * H.voxSize; pz[i] = H.ijk[3 * i + 2] * H.voxSize;
+        }
+        vx3h_status st;
+        if ((st = copyReal(b, at(d.px, rs, v0), px))) return st;
+        if ((st = copyReal(b, at(d.py, rs, v0), py))) return st;
+        if ((st = copyReal(b, at(d.pz, rs, v0), pz))) return st;
+        for (void* p : {(void*)d.mx, (void*)d.my, (void*)d.mz, (void*)d.ox, (void*)d.oy, (void*)d.oz, (void*)d.ax, (void*)d.ay, (void*)d.az})
+            if ((st = copyReal(b, at(p, rs, v0), zero))) return st;
+        if ((st = copyReal(b, at(d.ow, rs, v0), one))) return st;
+        std::vector<int> vm(NV), vs(NV, s), ve(NV);
+        for (int i = 0; i < NV; i++) { vm[i] = vmIdx[H.vmat[i]]; ve[i] = H.vext[i] >= 0 ? ext0 + H.vext[i] : -1; }
+        HIPCHK(hipMemcpy(d.vflags + v0, H.vflags.data(), NV * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.tfac + v0, H.tfac0.data(), NV * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.phase + v0, H.phase.data(), NV * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.vmat + v0, vm.data(), NV * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.vext + v0, ve.data(), NV * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.vijk + 3 * (size_t)v0, H.ijk.data(), NV * 12, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(d.psx + v0, 0, NV * 4));
+        HIPCHK(hipMemset(d.psy + v0, 0, NV * 4));
+        HIPCHK(hipMemset(d.psz + v0, 0, NV * 4));
+        for (int k = 0; k < 6; k++) HIPCHK(hipMemset(d.es + (size_t)k * d.Vcap + v0, 0, NV * 4));
+        HIPCHK(hipMemcpy(d.vsim + v0, vs.data(), NV * 4, hipMemcpyHostToDevice));
+    }
+    if (NL) {
+        std::vector<int> ln(NL), lp(NL), lm(NL), ls(NL, s), lf(NL, LF_SMALL_ANGLE);
+        for (int i = 0; i < NL; i++) {
+            ln[i] = v0 + H.lneg[i]; lp[i] = v0 + H.lpos[i];
+            lm[i] = H.laxis[i] | (lmIdx[H.lmat[i]] << 2);
+        }
+        HIPCHK(hipMemcpy(d.lneg + l0, ln.data(), NL * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.lpos + l0, lp.data(), NL * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.lmeta + l0, lm.data(), NL * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d.lflags + l0, lf.data(), NL * 4, hipMemcpyHostToDevice));
+        std::vector<double> zero(NL, 0.0);
+        vx3h_status st;
+        for (void* p : {(void*)d.q2x, (void*)d.q2y, (void*)d.q2z, (void*)d.a1x, (void*)d.a1y, (void*)d.a1z, (void*)d.a2x, (void*)d.a2y, (void*)d.a2z})
+            if ((st = copyReal(b, at(p, rs, l0), zero))) return st;
+        HIPCHK(hipMemset(d.strain + l0, 0, NL * 4));
+        HIPCHK(hipMemset(d.maxStrain + l0, 0, NL * 4));
+        HIPCHK(hipMemset(d.strainOffset + l0, 0, NL * 4));
+        HIPCHK(hipMemcpy(d.lsim + l0, ls.data(), NL * 4, hipMemcpyHostToDevice));
+    }
+    // control block
+    SimCtl c;
+    memset(&c, 0, sizeof(c));
+    c.t = 0;
+    c.optDt = 0;
+    c.tempAmp = H.tempAmp;
+    c.tempPeriod = H.tempPeriod;
+    c.dtFrac = H.dtFrac;
+    for (int k = 0; k < 3; k++) { c.com[k] = H.com0[k]; c.com0[k] = H.com0[k]; }
+    c.voxSize = H.voxSize;
+    c.dt = H.dtFixed > 0 ? (float)H.dtFixed : 0.0f;
+    c.tInit = H.tInit;
+    c.state = VX3H_SIM_RUNNING;
+    c.maxSteps = H.maxSteps;
+    c.flags = H.flags;
+    c.v0 = v0; c.nv = NV; c.l0 = l0; c.nl = NL;
+    c.stopOff = tok0;
+    c.stopLen = (int)H.stopTok.size();
+    if (NV == 0) c.state = VX3H_SIM_DONE;   // "No links and no voxels" (VX3_SimulationManager.cu:15-19)
+    else if (!H.stopTok.empty() &&
+             evalMathTree(H.stopTok.data(), (int)H.stopTok.size(), c.com[0], c.com[1], c.com[2], 0, 0.0, 0, 0, 0, NV) > 0)
+        c.state = VX3H_SIM_DONE;
+    HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(c), hipMemcpyHostToDevice));
+
+    SlotHost& sh = b->slots[s];
+    sh = SlotHost();
+    sh.used = true;
+    sh.v0 = v0; sh.nv = NV; sh.l0 = l0; sh.nl = NL;
+    sh.ijk = H.ijk;
+    sh.voxSize = H.voxSize;
+    for (int k = 0; k < 3; k++) sh.com0[k] = H.com0[k];
+    sh.fit = H.fitTok;
+    sh.bytes = need;
+    sh.matid.resize(NV);
+    sh.measured.resize(NV);
+    sh.mass.resize(NV);
+    for (int i = 0; i < NV; i++) {
+        sh.matid[i] = desc->materials[H.vmat[i]].matid;
+        sh.measured[i] = H.vmats[H.vmat[i]].isMeasured;
+        sh.mass[i] = H.vmats[H.vmat[i]].mass;
+    }
+    b->bytesUsed += need;
+    if (H.anyPoisson) b->anyPoisson = true;
+    d.vHi = b->va.hi;
+    d.lHi = b->la.hi;
+    d.nSims = (int)b->slots.size();
+    b->layout++;
+    b->dtLo = std::min(b->dtLo, s);
+    b->dtHi = std::max(b->dtHi, s + 1);
+    *slot = s;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
+    if (!b || n_steps < 0) FAILV(VX3H_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    if (b->slots.empty() || n_steps == 0) return VX3H_OK;
+    if (b->dtHi > b->dtLo) {
+        if (b->realSize == 4) launchDt(b->db, b->stream, b->dtLo, b->dtHi);
+        else launchDt(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, b->dtLo, b->dtHi);
+        HIPCHK(hipGetLastError());
+        b->dtLo = 1 << 30;
+        b->dtHi = -1;
+    }
+    if (b->cfg.use_graph) {
+        const int G = std::min(n_steps, 32);
+        vx3h_status st = ensureGraph(b, G);
+        if (st) return st;
+        int reps = n_steps / G, rem = n_steps % G;
+        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(b->graphExec, b->stream));
+        if (rem) return launchSteps(b, rem);
+        return VX3H_OK;
+    }
+    return launchSteps(b, n_steps);
+}
+
+extern "C" vx3h_status vx3h_batch_sync(vx3h_batch* b) {
+    if (!b) FAILV(VX3H_EINVAL, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return VX3H_OK;
+}
+
+namespace vx3 {
+vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, std::vector<double>* posOut) {
+    const SlotHost& sh = b->slots[s];
+    memset(&o, 0, sizeof(o));
+    o.slot = s;
+    o.state = c.state;
+    o.steps = c.steps;
+    o.num_voxel = sh.nv;
+    o.current_time = c.t;
+    o.dt = c.dt;
+    o.recent_angle = c.recentAngle;
+    for (int k = 0; k < 3; k++) o.initial_com[k] = sh.com0[k];
+    std::vector<double> x, y, z;
+    vx3h_status st;
+    const size_t rs = b->realSize;
+    if ((st = readReal(b, at(b->db.px, rs, sh.v0), sh.nv, x))) return st;
+    if ((st = readReal(b, at(b->db.py, rs, sh.v0), sh.nv, y))) return st;
+    if ((st = readReal(b, at(b->db.pz, rs, sh.v0), sh.nv, z))) return st;
+    // updateCurrentCenterOfMass (VX3_VoxelyzeKernel.cu:477-493) + collectResults totals (VX3_SimulationManager.cu:442-452)
+    double tot = 0, sx = 0, sy = 0, sz = 0, dist = 0;
+    int nm = 0;
+    for (int i = 0; i < sh.nv; i++) {
+        if (!sh.measured[i]) continue;
+        double m = sh.mass[i];
+        sx += x[i] * m; sy += y[i] * m; sz += z[i] * m;
+        tot += m;
+        nm++;
+        double dx = x[i] - sh.ijk[3 * i] * sh.voxSize, dy = y[i] - sh.ijk[3 * i + 1] * sh.voxSize, dz = z[i] - sh.ijk[3 * i + 2] * sh.voxSize;
+        dist += sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    if (tot != 0) { double inv = 1.0 / tot; o.current_com[0] = inv * sx; o.current_com[1] = inv * sy; o.current_com[2] = inv * sz; }
+    o.num_measured_voxel = nm;
+    o.total_distance_of_all_voxels = dist;
+    double off[3] = {o.current_com[0] - sh.com0[0], o.current_com[1] - sh.com0[1], o.current_com[2] - sh.com0[2]};
+    // computeFitness (VX3_VoxelyzeKernel.cu:530-534)
+    o.fitness = evalMathTree(sh.fit.data(), (int)sh.fit.size(), off[0], off[1], off[2], 0, c.t, c.recentAngle, 0, 0, sh.nv);
+    if (posOut) {
+        posOut->resize(3 * sh.nv);
+        for (int i = 0; i < sh.nv; i++) { (*posOut)[3 * i] = x[i]; (*posOut)[3 * i + 1] = y[i]; (*posOut)[3 * i + 2] = z[i]; }
+    }
+    return VX3H_OK;
+}
+}  // namespace vx3
+
+extern "C" vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int cap, int* n) {
+    if (!b || !n) FAILV(VX3H_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    std::vector<SimCtl> ctl(b->slots.size());
+    if (!ctl.empty()) HIPCHK(hipMemcpy(ctl.data(), b->db.sims, ctl.size() * sizeof(SimCtl), hipMemcpyDeviceToHost));
+    int k = 0;
+    for (int s = 0; s < (int)b->slots.size(); s++) {
+        if (!b->slots[s].used) continue;
+        if (out && k < cap) {
+            vx3h_status st = pollOne(b, s, ctl[s], out[k], nullptr);
+            if (st) return st;
+        }
+        k++;
+    }
+    *n = k;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
+    if (!b) FAILV(VX3H_EINVAL, "null batch");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    SlotHost& sh = b->slots[slot];
+    if (sh.nv) HIPCHK(hipMemset(b->db.vsim + sh.v0, 0xFF, (size_t)sh.nv * 4));
+    if (sh.nl) HIPCHK(hipMemset(b->db.lsim + sh.l0, 0xFF, (size_t)sh.nl * 4));
+    SimCtl c;
+    memset(&c, 0, sizeof(c));
+    c.state = VX3H_SIM_EMPTY;
+    HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
+    b->va.release(sh.v0, sh.nv);
+    b->la.release(sh.l0, sh.nl);
+    b->bytesUsed -= sh.bytes;
+    sh = SlotHost();
+    b->db.vHi = b->va.hi;
+    b->db.lHi = b->la.hi;
+    b->layout++;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_view* view) {
+    if (!b || !view) FAILV(VX3H_EINVAL, "null argument");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const SlotHost& sh = b->slots[slot];
+    const DevBatch<float>& d = b->db;
+    const size_t rs = b->realSize;
+    std::vector<double> t0, t1, t2, t3;
+    vx3h_status st;
+    auto get3 = [&](void* a, void* bb, void* c, double* out) -> vx3h_status {
+        vx3h_status s2;
+        if ((s2 = readReal(b, at(a, rs, sh.v0), sh.nv, t0))) return s2;
+        if ((s2 = readReal(b, at(bb, rs, sh.v0), sh.nv, t1))) return s2;
+        if ((s2 = readReal(b, at(c, rs, sh.v0), sh.nv, t2))) return s2;
+        for (int i = 0; i < sh.nv; i++) { out[3 * i] = t0[i]; out[3 * i + 1] = t1[i]; out[3 * i + 2] = t2[i]; }
+        return VX3H_OK;
+    };
+    if (view->pos && (st = get3(d.px, d.py, d.pz, view->pos))) return st;
+    if (view->lin_mom && (st = get3(d.mx, d.my, d.mz, view->lin_mom))) return st;
+    if (view->ang_mom && (st = get3(d.ax, d.ay, d.az, view->ang_mom))) return st;
+    if (view->orient) {
+        if ((st = readReal(b, at(d.ow, rs, sh.v0), sh.nv, t3))) return st;
+        if ((st = readReal(b, at(d.ox, rs, sh.v0), sh.nv, t0))) return st;
+        if ((st = readReal(b, at(d.oy, rs, sh.v0), sh.nv, t1))) return st;
+        if ((st = readReal(b, at(d.oz, rs, sh.v0), sh.nv, t2))) return st;
+        for (int i = 0; i < sh.nv; i++) {
+            view->orient[4 * i] = t3[i]; view->orient[4 * i + 1] = t0[i]; view->orient[4 * i + 2] = t1[i]; view->orient[4 * i + 3] = t2[i];
+        }
+    }
+    if (view->vox_flags) HIPCHK(hipMemcpy(view->vox_flags, d.vflags + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
+    if (view->temp) {
+        std::vector<float> tf(sh.nv);
+        HIPCHK(hipMemcpy(tf.data(), d.tfac + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < sh.nv; i++) view->temp[i] = tf[i];   // thermal size factor 1 + T*CTE
+    }
+    if (sh.nl && (view->link_strain || view->link_max_strain)) {
+        std::vector<float> a(sh.nl);
+        if (view->link_strain) {
+            HIPCHK(hipMemcpy(a.data(), d.strain + sh.l0, sh.nl * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < sh.nl; i++) view->link_strain[i] = a[i];
+        }
+        if (view->link_max_strain) {
+            HIPCHK(hipMemcpy(a.data(), d.maxStrain + sh.l0, sh.nl * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < sh.nl; i++) view->link_max_strain[i] = a[i];
+        }
+    }
+    if (sh.nl && view->link_small_angle) {
+        std::vector<int> f(sh.nl);
+        HIPCHK(hipMemcpy(f.data(), d.lflags + sh.l0, sh.nl * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < sh.nl; i++) view->link_small_angle[i] = (f[i] & LF_SMALL_ANGLE) ? 1 : 0;
+    }
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_build_topology(const vx3h_sim_desc* desc, vx3h_topology_view* v) {
+    if (!desc || !v) FAILV(VX3H_EINVAL, "null argument");
+    HostModel H;
+    if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
+    v->n_voxels = H.nv;
+    v->n_links = H.nl;
+    for (int i = 0; i < H.nl; i++) {
+        if (v->link_vneg) v->link_vneg[i] = H.lneg[i];
+        if (v->link_vpos) v->link_vpos[i] = H.lpos[i];
+        if (v->link_axis) v->link_axis[i] = H.laxis[i];
+    }
+    if (v->voxel_links) memcpy(v->voxel_links, H.voxelLinks.data(), H.voxelLinks.size() * sizeof(int));
+    return VX3H_OK;
+}

@@ -1,0 +1,165 @@
+"""ctypes bindings of libvx3h.so (include/vx3h.h) and a small Batch wrapper.
+
+The library is the product: there is no Python or CPU fallback.  Loading fails
+loudly when gpuvoxels_amd/libvx3h.so has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvx3h.so")
+
+_lib = None
+
+
+class VX3Error(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("vx3h status %d: %s" % (status, msg))
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("gpuvoxels_amd/libvx3h.so is not built; run `python -m gpuvoxels_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    S = C.c_int
+    P = C.c_void_p
+    sig = {
+        "vx3h_last_error": (C.c_char_p, []),
+        "vx3h_device_count": (S, [C.POINTER(C.c_int)]),
+        "vx3h_batch_create": (S, [C.c_int, C.POINTER(_abi.BatchCfg), C.POINTER(P)]),
+        "vx3h_batch_destroy": (S, [P]),
+        "vx3h_sim_admit": (S, [P, C.POINTER(_abi.SimDesc), C.POINTER(C.c_int)]),
+        "vx3h_sim_footprint": (S, [C.POINTER(_abi.SimDesc), C.c_int, C.POINTER(C.c_size_t)]),
+        "vx3h_batch_step": (S, [P, C.c_int]),
+        "vx3h_batch_sync": (S, [P]),
+        "vx3h_batch_poll": (S, [P, C.POINTER(_abi.SimStatus), C.c_int, C.POINTER(C.c_int)]),
+        "vx3h_sim_retire": (S, [P, C.c_int]),
+        "vx3h_sim_read_state": (S, [P, C.c_int, C.POINTER(_abi.StateView)]),
+        "vx3h_build_topology": (S, [C.POINTER(_abi.SimDesc), C.POINTER(_abi.TopologyView)]),
+    }
+    optional = {
+        "vx3h_vxa_parse": (S, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
+        "vx3h_vxa_desc": (C.POINTER(_abi.SimDesc), [P]),
+        "vx3h_vxa_free": (None, [P]),
+        "vx3h_report_xml": (S, [C.c_char_p, C.POINTER(_abi.ReportEntry), C.c_int, C.c_char_p, C.c_size_t,
+                                C.POINTER(C.c_size_t)]),
+    }
+    for name, (res, args) in list(sig.items()) + list(optional.items()):
+        if not hasattr(L, name):
+            if name in sig:
+                raise RuntimeError("libvx3h.so lacks %s" % name)
+            continue
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status):
+    if status != _abi.OK:
+        raise VX3Error(status, lib().vx3h_last_error().decode())
+
+
+def device_count():
+    n = C.c_int(0)
+    check(lib().vx3h_device_count(C.byref(n)))
+    return n.value
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def topology(model):
+    d, keep = model.desc()
+    tv = _abi.TopologyView()
+    check(lib().vx3h_build_topology(C.byref(d), C.byref(tv)))
+    nv, nl = tv.n_voxels, tv.n_links
+    vneg, vpos, axis = np.zeros(nl, np.int32), np.zeros(nl, np.int32), np.zeros(nl, np.int32)
+    vl = np.zeros(6 * nv, np.int32)
+    tv.link_vneg, tv.link_vpos, tv.link_axis = _p(vneg, C.c_int), _p(vpos, C.c_int), _p(axis, C.c_int)
+    tv.voxel_links = _p(vl, C.c_int)
+    check(lib().vx3h_build_topology(C.byref(d), C.byref(tv)))
+    return vneg, vpos, axis, vl.reshape(-1, 6)
+
+
+class Batch:
+    """A device-resident batch of simulations (vx3h_batch)."""
+
+    def __init__(self, device=0, precision=_abi.PREC_F32, path=_abi.PATH_STREAM, max_device_bytes=0, block_size=0,
+                 steps_per_launch=0, use_graph=1, collision_rebuild_interval=0):
+        cfg = _abi.BatchCfg()
+        cfg.max_device_bytes = max_device_bytes
+        cfg.precision = precision
+        cfg.path = path
+        cfg.block_size = block_size
+        cfg.steps_per_launch = steps_per_launch
+        cfg.collision_rebuild_interval = collision_rebuild_interval
+        cfg.use_graph = use_graph
+        self.h = C.c_void_p()
+        check(lib().vx3h_batch_create(device, C.byref(cfg), C.byref(self.h)))
+        self.nv = {}
+        self.nl = {}
+
+    def close(self):
+        if self.h:
+            lib().vx3h_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, model):
+        d, keep = model.desc()
+        slot = C.c_int(-1)
+        check(lib().vx3h_sim_admit(self.h, C.byref(d), C.byref(slot)))
+        self.nv[slot.value] = model.n_voxels
+        return slot.value
+
+    def step(self, n):
+        check(lib().vx3h_batch_step(self.h, int(n)))
+
+    def sync(self):
+        check(lib().vx3h_batch_sync(self.h))
+
+    def poll(self):
+        n = C.c_int(0)
+        check(lib().vx3h_batch_poll(self.h, None, 0, C.byref(n)))
+        arr = (_abi.SimStatus * max(n.value, 1))()
+        check(lib().vx3h_batch_poll(self.h, arr, n.value, C.byref(n)))
+        return {arr[i].slot: arr[i] for i in range(n.value)}
+
+    def retire(self, slot):
+        check(lib().vx3h_sim_retire(self.h, slot))
+        self.nv.pop(slot, None)
+
+    def read_state(self, slot, nl=None):
+        nv = self.nv[slot]
+        pos, ori, lm, am = np.zeros(3 * nv), np.zeros(4 * nv), np.zeros(3 * nv), np.zeros(3 * nv)
+        fl, tf = np.zeros(nv, np.int32), np.zeros(nv)
+        v = _abi.StateView()
+        v.pos, v.orient, v.lin_mom, v.ang_mom = (_p(pos, C.c_double), _p(ori, C.c_double), _p(lm, C.c_double),
+                                                 _p(am, C.c_double))
+        v.vox_flags, v.temp = _p(fl, C.c_int), _p(tf, C.c_double)
+        out = {}
+        if nl:
+            st, ms, sa = np.zeros(nl), np.zeros(nl), np.zeros(nl, np.int32)
+            v.link_strain, v.link_max_strain, v.link_small_angle = (_p(st, C.c_double), _p(ms, C.c_double),
+                                                                    _p(sa, C.c_int))
+            out.update(strain=st, max_strain=ms, small_angle=sa)
+        check(lib().vx3h_sim_read_state(self.h, slot, C.byref(v)))
+        out.update(pos=pos.reshape(-1, 3), orient=ori.reshape(-1, 4), lin_mom=lm.reshape(-1, 3),
+                   ang_mom=am.reshape(-1, 3), flags=fl, tfac=tf)
+        return out

@@ -101,6 +101,14 @@ FITNESS_XY_DISTANCE = [(MT["mtVAR"], 1.0), (MT["mtVAR"], 1.0), (MT["mtVAR"], 0.0
                        (MT["mtEND"], 0.0)]
 
 
+# |(x, y, z)|: 3D CoM displacement, token order of
+# <mtSQRT><mtADD><mtADD><mtMUL>x x</mtMUL><mtMUL>y y</mtMUL></mtADD><mtMUL>z z</mtMUL></mtADD></mtSQRT>
+FITNESS_DISTANCE_3D = [(MT["mtVAR"], 1.0), (MT["mtVAR"], 1.0), (MT["mtVAR"], 0.0), (MT["mtVAR"], 0.0),
+                       (MT["mtVAR"], 2.0), (MT["mtVAR"], 2.0), (MT["mtMUL"], 0.0), (MT["mtMUL"], 0.0),
+                       (MT["mtMUL"], 0.0), (MT["mtADD"], 0.0), (MT["mtADD"], 0.0), (MT["mtSQRT"], 0.0),
+                       (MT["mtEND"], 0.0)]
+
+
 @dataclass
 class Model:
     voxel_size: float = 0.001

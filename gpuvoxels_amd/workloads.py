@@ -15,7 +15,7 @@
 import numpy as np
 
 from . import _abi
-from .model import Material, Model, FITNESS_XY_DISTANCE
+from .model import Material, Model, FITNESS_XY_DISTANCE, FITNESS_DISTANCE_3D
 
 
 def gravity_mult(grav_acc: float = -9.81) -> float:
@@ -57,6 +57,7 @@ def cube(n: int = 6, steps: int = 10000, voxel_size: float = 0.01) -> Model:
     m.mat = np.zeros(n ** 3, np.int32)
     m.phase = np.zeros(n ** 3)
     _physics(m, steps)
+    m.fitness_tokens = list(FITNESS_DISTANCE_3D)   # a symmetric cube does not walk: score its settling
     return m
 
 
