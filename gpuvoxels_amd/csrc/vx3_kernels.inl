@@ -19,7 +19,9 @@ namespace VX_NS {
 
 typedef VX_REAL real;
 
-__device__ __forceinline__ bool simRunning(const SimCtl* sims, int s) { return s >= 0 && sims[s].state == 0; }
+__device__ __forceinline__ bool simRunning(const SimCtl* sims, int s) {
+    return s >= 0 && sims[s].state == 0 && !(sims[s].flags & SC_RESIDENT);
+}
 
 // CVX_Voxel::transverseArea / transverseStrainSum (VX_Voxel.cpp:345-373) from the cached Poisson strain
 __device__ __forceinline__ void transverse(const DevBatch<real>& b, int v, int axis, float size, bool nuZero, float& area, float& sum) {
@@ -212,12 +214,12 @@ __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
     }
 }
 
-// ---- MathTree (queue evaluator of Utils/VX3_MathTree.h:50-192), device side, bounded programs
+// ---- MathTree (queue evaluator of Utils/VX3_MathTree.h:50-192), device side, bounded programs.
+// The value queue q lives in LDS (dynamically indexed), supplied by the caller.
 #define VX3_DEV_MT_MAX 64
-__device__ double mathTreeEval(const TokenDev* buf, int n, double x, double y, double z, double hit, double t, double angle,
-                               double closeness, int ncp, int nvox) {
+__device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, double x, double y, double z, double hit, double t,
+                                               double angle, double closeness, int ncp, int nvox, double* q) {
     if (n <= 0) return 0.0;
-    double q[VX3_DEV_MT_MAX + 1];
     int vc = 0, pc = 0;
     for (int i = 0; i < n && i < VX3_DEV_MT_MAX; i++) {
         const int op = buf[i].op;
@@ -287,10 +289,11 @@ __device__ void blockCoM(const DevBatch<real>& b, const SimCtl& sc, double out[3
 // Per-simulation bookkeeping after the voxel pass (doTimeStep tail, VX3_VoxelyzeKernel.cu:241,279,314-334,352,
 // and the stop test the driver loop makes before the next step, VX3_SimulationManager.cu:62-64).
 __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
+    __shared__ double sQ[VX3_DEV_MT_MAX + 1];
     const int s = blockIdx.x;
     if (s >= b.nSims) return;
     SimCtl& sc = b.sims[s];
-    if (sc.state != 0) return;
+    if (sc.state != 0 || (sc.flags & SC_RESIDENT)) return;
     const bool diverged = (sc.flags & SC_DIVERGED_PENDING) != 0;
     const int steps = sc.steps + 1;
     const bool sample = !diverged && sc.cycleStep > 0 && steps % sc.cycleStep == 0;
@@ -320,7 +323,7 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     sc.t += (double)sc.dt;
     if (sc.maxSteps > 0 && steps >= sc.maxSteps) { sc.state = 1; return; }
     if (sc.stopLen > 0 &&
-        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], 0, sc.t, sc.recentAngle, 0, 0, sc.nv) > 0)
+        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], 0, sc.t, sc.recentAngle, 0, 0, sc.nv, sQ) > 0)
         sc.state = 1;
 }
 

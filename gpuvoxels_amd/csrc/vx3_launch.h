@@ -11,8 +11,23 @@ struct StepOpts {
     int nPairs;        // collision pairs in cpair
     int block;         // workgroup size of the element passes
 };
-void launchStep(const DevBatch<float>& b, hipStream_t s, const StepOpts& o);
-void launchStep(const DevBatch<double>& b, hipStream_t s, const StepOpts& o);
+// Kernel passes of one step, in launch order; ev (optional) receives a start/stop
+// event pair per pass: ev[2*k], ev[2*k+1].
+enum { PASS_POISSON = 0, PASS_LINKS, PASS_CONTACTS, PASS_VOXELS, PASS_CONTROL, PASS_COUNT };
+void launchStep(const DevBatch<float>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);
+void launchStep(const DevBatch<double>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);
+// Resident multi-step kernel shapes: (block, voxels/thread, links/thread).
+struct ResidentShapeInfo { int block, vpt, lpt; size_t lds; };
+int residentShapes_f32(ResidentShapeInfo* out, int cap);
+int residentShapes_f64(ResidentShapeInfo* out, int cap);
+inline int residentShapes(bool f64, ResidentShapeInfo* out, int cap) {
+    return f64 ? residentShapes_f64(out, cap) : residentShapes_f32(out, cap);
+}
+// Launches the resident kernel of shape `shape` over the sims in simList (device array).
+hipError_t launchResident(const DevBatch<float>& b, hipStream_t s, int shape, const int* simList, int nSims, int nSteps,
+                          hipEvent_t* ev = nullptr);
+hipError_t launchResident(const DevBatch<double>& b, hipStream_t s, int shape, const int* simList, int nSims, int nSteps,
+                          hipEvent_t* ev = nullptr);
 void launchDt(const DevBatch<float>& b, hipStream_t s, int s0, int s1);
 void launchDt(const DevBatch<double>& b, hipStream_t s, int s0, int s1);
 }  // namespace vx3

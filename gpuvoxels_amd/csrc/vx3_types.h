@@ -30,7 +30,7 @@ enum {
 // link state bits
 enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2 };
 // per-sim control bits
-enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32 };
+enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32, SC_RESIDENT = 64 };
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
     float mass, massInverse, sqrtMass, momentInertiaInverse;

@@ -100,6 +100,7 @@ struct RangeAlloc {
 struct SlotHost {
     bool used = false;
     int v0 = 0, nv = 0, l0 = 0, nl = 0;
+    int shape = -1;              // resident kernel shape, -1 = stream path
     std::vector<int> ijk;
     std::vector<int> matid;
     std::vector<int> measured;   // per voxel
@@ -147,6 +148,14 @@ struct vx3h_batch {
     bool anyPoisson = false, anyColl = false;
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
+    // resident path: kernel shapes and, per shape, the sims it runs
+    std::vector<ResidentShapeInfo> shapes;
+    std::vector<int> shapePref;                  // shape indices in preference order
+    std::vector<std::vector<int>> resLists;
+    std::vector<int*> dResLists;
+    std::vector<size_t> capRes;
+    bool resDirty = false;
+    int nStreamSims = 0;
     // graph of `graphSteps` steps, valid for one layout version
     hipGraphExec_t graphExec = nullptr;
     hipGraph_t graph = nullptr;
@@ -283,6 +292,37 @@ vx3h_status ensureGraph(vx3h_batch* b, int steps) {
     return VX3H_OK;
 }
 
+vx3h_status syncResLists(vx3h_batch* b) {
+    if (!b->resDirty) return VX3H_OK;
+    for (size_t i = 0; i < b->resLists.size(); i++) {
+        const auto& L = b->resLists[i];
+        if (L.empty()) continue;
+        if (L.size() > b->capRes[i]) {
+            if (b->dResLists[i]) HIPCHK(hipFree(b->dResLists[i]));
+            b->capRes[i] = std::max(L.size(), 2 * b->capRes[i] + 64);
+            HIPCHK(hipMalloc(&b->dResLists[i], b->capRes[i] * sizeof(int)));
+        }
+        HIPCHK(hipMemcpy(b->dResLists[i], L.data(), L.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    b->resDirty = false;
+    return VX3H_OK;
+}
+
+vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
+    vx3h_status st = syncResLists(b);
+    if (st) return st;
+    for (size_t i = 0; i < b->resLists.size(); i++) {
+        const int n = (int)b->resLists[i].size();
+        if (!n) continue;
+        hipError_t e = b->realSize == 4
+            ? launchResident(b->db, b->stream, (int)i, b->dResLists[i], n, nSteps, ev ? ev + 2 * i : nullptr)
+            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, (int)i, b->dResLists[i], n, nSteps,
+                             ev ? ev + 2 * i : nullptr);
+        if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("resident launch failed: ") + hipGetErrorString(e));
+    }
+    return VX3H_OK;
+}
+
 }  // namespace
 
 extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out) {
@@ -300,6 +340,19 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
     if (e != hipSuccess) { delete b; FAILV(VX3H_EDEVICE, "stream creation failed"); }
     memset(&b->db, 0, sizeof(b->db));
     initSpecs(b);
+    {
+        ResidentShapeInfo tmp[16];
+        int ns = residentShapes(b->realSize == 8, tmp, 16);
+        b->shapes.assign(tmp, tmp + ns);
+        b->resLists.assign(ns, {});
+        b->dResLists.assign(ns, nullptr);
+        b->capRes.assign(ns, 0);
+        // default preference: smallest capacity first (more workgroups per CU), table order on ties
+        for (int i = 0; i < ns; i++) b->shapePref.push_back(i);
+        std::stable_sort(b->shapePref.begin(), b->shapePref.end(), [&](int x, int y) {
+            return b->shapes[x].block * b->shapes[x].vpt < b->shapes[y].block * b->shapes[y].vpt;
+        });
+    }
     size_t freeB = 0, totB = 0;
     if (hipMemGetInfo(&freeB, &totB) != hipSuccess) freeB = 0;
     b->budget = b->cfg.max_device_bytes ? b->cfg.max_device_bytes : (size_t)(0.9 * (double)freeB);
@@ -316,6 +369,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
     for (auto* arrs : {&b->vArrs, &b->lArrs})
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, (void*)b->db.sims}) if (p) hipFree(p);
+    for (int* p : b->dResLists) if (p) hipFree(p);
     hipStreamDestroy(b->stream);
     delete b;
     return VX3H_OK;
@@ -433,6 +487,15 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         HIPCHK(hipMemset(d.strainOffset + l0, 0, NL * 4));
         HIPCHK(hipMemcpy(d.lsim + l0, ls.data(), NL * 4, hipMemcpyHostToDevice));
     }
+    // execution path: resident when the sim fits one of the on-chip kernel shapes
+    int shape = -1;
+    if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0) {
+        for (int si : b->shapePref) {
+            const ResidentShapeInfo& r = b->shapes[si];
+            if (NV <= r.block * r.vpt && NL <= r.block * r.lpt) { shape = si; break; }
+        }
+        if (shape < 0 && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "simulation too large for the resident kernel");
+    }
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -447,7 +510,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     c.tInit = H.tInit;
     c.state = VX3H_SIM_RUNNING;
     c.maxSteps = H.maxSteps;
-    c.flags = H.flags;
+    c.flags = H.flags | (shape >= 0 ? SC_RESIDENT : 0);
     c.v0 = v0; c.nv = NV; c.l0 = l0; c.nl = NL;
     c.stopOff = tok0;
     c.stopLen = (int)H.stopTok.size();
@@ -466,6 +529,9 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     for (int k = 0; k < 3; k++) sh.com0[k] = H.com0[k];
     sh.fit = H.fitTok;
     sh.bytes = need;
+    sh.shape = shape;
+    if (shape >= 0) { b->resLists[shape].push_back(s); b->resDirty = true; }
+    else b->nStreamSims++;
     sh.matid.resize(NV);
     sh.measured.resize(NV);
     sh.mass.resize(NV);
@@ -497,16 +563,69 @@ extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
         b->dtLo = 1 << 30;
         b->dtHi = -1;
     }
-    if (b->cfg.use_graph) {
-        const int G = std::min(n_steps, 32);
-        vx3h_status st = ensureGraph(b, G);
+    const int chunk = b->cfg.steps_per_launch > 0 ? b->cfg.steps_per_launch : 1000;
+    for (int done = 0; done < n_steps;) {
+        const int c = std::min(chunk, n_steps - done);
+        vx3h_status st = launchResidentAll(b, c, nullptr);
         if (st) return st;
-        int reps = n_steps / G, rem = n_steps % G;
-        for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(b->graphExec, b->stream));
-        if (rem) return launchSteps(b, rem);
-        return VX3H_OK;
+        if (b->nStreamSims > 0) {
+            if (b->cfg.use_graph) {
+                const int G = std::min(c, 32);
+                if ((st = ensureGraph(b, G))) return st;
+                for (int r = 0; r < c / G; r++) HIPCHK(hipGraphLaunch(b->graphExec, b->stream));
+                if (c % G && (st = launchSteps(b, c % G))) return st;
+            } else if ((st = launchSteps(b, c))) return st;
+        }
+        done += c;
     }
-    return launchSteps(b, n_steps);
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, double* avg_ms) {
+    if (!b || !avg_ms || n_steps <= 0) FAILV(VX3H_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    for (int k = 0; k < VX3H_N_PASSES; k++) avg_ms[k] = 0;
+    if (b->slots.empty()) return VX3H_OK;
+    std::vector<hipEvent_t> ev(2 * PASS_COUNT * n_steps, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    StepOpts o;
+    o.poisson = b->anyPoisson;
+    o.contacts = false;
+    o.nPairs = 0;
+    o.block = b->cfg.block_size;
+    for (int i = 0; i < n_steps; i++) {
+        hipEvent_t* e = &ev[2 * PASS_COUNT * i];
+        if (b->realSize == 4) launchStep(b->db, b->stream, o, e);
+        else launchStep(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, o, e);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const bool ran[PASS_COUNT] = {o.poisson, b->db.lHi > 0, o.contacts && o.nPairs > 0, true, true};
+    for (int k = 0; k < PASS_COUNT; k++) {
+        if (!ran[k] || b->nStreamSims == 0) continue;
+        double tot = 0;
+        for (int i = 0; i < n_steps; i++) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[2 * PASS_COUNT * i + 2 * k], ev[2 * PASS_COUNT * i + 2 * k + 1]));
+            tot += ms;
+        }
+        avg_ms[k] = tot / n_steps;
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    // resident kernels: one launch of n_steps steps per shape in use; reports the launch durations' sum
+    std::vector<hipEvent_t> rev(2 * b->resLists.size(), nullptr);
+    for (auto& e : rev) HIPCHK(hipEventCreate(&e));
+    vx3h_status st = launchResidentAll(b, n_steps, rev.data());
+    if (st) return st;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < b->resLists.size(); i++) {
+        if (b->resLists[i].empty()) continue;
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, rev[2 * i], rev[2 * i + 1]));
+        avg_ms[5] += ms;
+    }
+    for (auto& e : rev) hipEventDestroy(e);
+    return VX3H_OK;
 }
 
 extern "C" vx3h_status vx3h_batch_sync(vx3h_batch* b) {
@@ -593,6 +712,11 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
     HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
     b->va.release(sh.v0, sh.nv);
     b->la.release(sh.l0, sh.nl);
+    if (sh.shape >= 0) {
+        auto& L = b->resLists[sh.shape];
+        L.erase(std::remove(L.begin(), L.end(), slot), L.end());
+        b->resDirty = true;
+    } else b->nStreamSims--;
     b->bytesUsed -= sh.bytes;
     sh = SlotHost();
     b->db.vHi = b->va.hi;

@@ -44,6 +44,7 @@ def lib():
         "vx3h_sim_retire": (S, [P, C.c_int]),
         "vx3h_sim_read_state": (S, [P, C.c_int, C.POINTER(_abi.StateView)]),
         "vx3h_build_topology": (S, [C.POINTER(_abi.SimDesc), C.POINTER(_abi.TopologyView)]),
+        "vx3h_batch_kernel_times": (S, [P, C.c_int, C.POINTER(C.c_double)]),
     }
     optional = {
         "vx3h_vxa_parse": (S, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
@@ -140,6 +141,15 @@ class Batch:
         arr = (_abi.SimStatus * max(n.value, 1))()
         check(lib().vx3h_batch_poll(self.h, arr, n.value, C.byref(n)))
         return {arr[i].slot: arr[i] for i in range(n.value)}
+
+    PASSES = ("poisson", "links", "contacts", "voxels", "control", "resident")
+
+    def kernel_times(self, n_steps):
+        """Mean per-pass kernel durations (ms) over n_steps steps, from HIP events
+        recorded on the batch's stream."""
+        out = (C.c_double * len(self.PASSES))()
+        check(lib().vx3h_batch_kernel_times(self.h, int(n_steps), out))
+        return {k: out[i] for i, k in enumerate(self.PASSES) if out[i] > 0}
 
     def retire(self, slot):
         check(lib().vx3h_sim_retire(self.h, slot))
