@@ -111,6 +111,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--path", choices=["auto", "stream", "resident"], default="auto")
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--block", type=int, default=0, help="preferred resident workgroup size (0 = engine default)")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--cpu-robots", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2000)
@@ -132,7 +133,7 @@ def main():
     del topo
     path = {"auto": _abi.PATH_AUTO, "stream": _abi.PATH_STREAM, "resident": _abi.PATH_RESIDENT}[args.path]
     prec = _abi.PREC_F64 if args.precision == "f64" else _abi.PREC_F32
-    b = engine.Batch(precision=prec, path=path, use_graph=1)
+    b = engine.Batch(precision=prec, path=path, use_graph=1, block_size=args.block)
     for m in models:
         b.admit(m)
     L = int(sum(len(engine.topology(m)[0]) for m in models))

@@ -41,6 +41,7 @@ template <typename T> struct DevBatch {
     const ExtDev* extTab;
     SimCtl* sims;
     const TokenDev* tokens;
+    const int* simMats;       // per sim: global voxel-material then link-material table indices it uses
     int Vcap, Lcap, Ccap;
     int vHi, lHi, nSims;      // high-water marks: kernels cover [0, vHi) / [0, lHi) / [0, nSims)
 };

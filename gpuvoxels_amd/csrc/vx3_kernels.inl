@@ -23,6 +23,30 @@ __device__ __forceinline__ bool simRunning(const SimCtl* sims, int s) {
     return s >= 0 && sims[s].state == 0 && !(sims[s].flags & SC_RESIDENT);
 }
 
+// Precision-dependent transcendentals.  The FP64 state path rounds like the host C
+// library (pow of float arguments evaluated in double and rounded once; sin in double);
+// the FP32 state path uses single-precision functions on a double-reduced argument.
+__device__ __forceinline__ float poissonValue(float base, float nu) {   // pow(base, -nu) - 1
+    if constexpr (sizeof(real) == 8) return (float)pow((double)base, (double)(-nu)) - 1.0f;
+    else return powf(base, -nu) - 1.0f;
+}
+// VX3 actuation temperature at time tn (VX3_VoxelyzeKernel.cu:625-650): A*sin(2*3.1415926f*(t/Tp + phase)),
+// contraction only unless expansion is enabled
+__device__ __forceinline__ float actuationTemp(double amp, double period, double phase, double tn, bool expansion) {
+    const double arg = (double)(2 * 3.1415926f) * (tn / period + phase);
+    if constexpr (sizeof(real) == 8) {
+        double T = amp * sin(arg);
+        if (!expansion && T > 0) T = 0;
+        return (float)T;
+    } else {
+        const double twoPi = 6.283185307179586;
+        const double r = arg - floor(arg * (1.0 / twoPi)) * twoPi;
+        float T = (float)amp * sinf((float)r);
+        if (!expansion && T > 0) T = 0;
+        return T;
+    }
+}
+
 // CVX_Voxel::transverseArea / transverseStrainSum (VX_Voxel.cpp:345-373) from the cached Poisson strain
 __device__ __forceinline__ void transverse(const DevBatch<real>& b, int v, int axis, float size, bool nuZero, float& area, float& sum) {
     if (nuZero) { area = size * size; sum = 0.0f; return; }
@@ -61,8 +85,7 @@ __global__ void __launch_bounds__(256) k_poisson(DevBatch<real> b) {
         for (int i = 0; i < 3; i++) if (tension[i]) add += st[i];
         // pow of float arguments: evaluated in double and rounded once, which gives the
         // correctly rounded single-precision result the host C library returns
-        const float pw = (float)pow((double)(1.0f + add), (double)(-b.vmatTab[b.vmat[v]].nu));
-        float value = pw - 1.0f;
+        const float value = poissonValue(1.0f + add, b.vmatTab[b.vmat[v]].nu);
         for (int i = 0; i < 3; i++) if (!tension[i]) st[i] = value;
     }
     b.psx[v] = st[0]; b.psy[v] = st[1]; b.psz[v] = st[2];
@@ -206,9 +229,7 @@ __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
     if ((sc.flags & SC_VARY) && !m.fixed) {
         const double tn = sc.t + (double)sc.dt;
         if (!(m.thermalOnAfter > tn)) {
-            double T = sc.tempAmp * sin((double)(2 * 3.1415926f) * (tn / sc.tempPeriod + b.phase[v]));
-            if (!(sc.flags & SC_EXPANSION) && T > 0) T = 0;
-            const float temp = (float)T;
+            const float temp = actuationTemp(sc.tempAmp, sc.tempPeriod, b.phase[v], tn, (sc.flags & SC_EXPANSION) != 0);
             b.tfac[v] = 1 + temp * m.alphaCTE;
         }
     }
@@ -217,6 +238,21 @@ __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
 // ---- MathTree (queue evaluator of Utils/VX3_MathTree.h:50-192), device side, bounded programs.
 // The value queue q lives in LDS (dynamically indexed), supplied by the caller.
 #define VX3_DEV_MT_MAX 64
+// Transcendental operators out of line, so that the evaluator inlined into the step
+// kernels stays small in registers.
+__device__ __noinline__ double mtFunc(int op, double a, double b) {
+    switch (op) {
+    case 9: return pow(a, b);
+    case 10: return sqrt(b);
+    case 11: return sin(b);
+    case 12: return cos(b);
+    case 13: return tan(b);
+    case 14: return atan(b);
+    case 15: return log(b);
+    case 23: return 0.5 * erfc(-b / sqrt(2.0));
+    default: return 0.0;
+    }
+}
 __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, double x, double y, double z, double hit, double t,
                                                double angle, double closeness, int ncp, int nvox, double* q) {
     if (n <= 0) return 0.0;
@@ -240,13 +276,13 @@ __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, doubl
         case 6: r = q[pc + 1] - q[pc]; pc += 2; break;
         case 7: r = q[pc + 1] * q[pc]; pc += 2; break;
         case 8: r = q[pc + 1] / q[pc]; pc += 2; break;
-        case 9: r = pow(q[pc + 1], q[pc]); pc += 2; break;
-        case 10: r = sqrt(q[pc]); pc++; break;
-        case 11: r = sin(q[pc]); pc++; break;
-        case 12: r = cos(q[pc]); pc++; break;
-        case 13: r = tan(q[pc]); pc++; break;
-        case 14: r = atan(q[pc]); pc++; break;
-        case 15: r = log(q[pc]); pc++; break;
+        case 9: r = mtFunc(9, q[pc + 1], q[pc]); pc += 2; break;
+        case 10: r = mtFunc(10, 0.0, q[pc]); pc++; break;
+        case 11: r = mtFunc(11, 0.0, q[pc]); pc++; break;
+        case 12: r = mtFunc(12, 0.0, q[pc]); pc++; break;
+        case 13: r = mtFunc(13, 0.0, q[pc]); pc++; break;
+        case 14: r = mtFunc(14, 0.0, q[pc]); pc++; break;
+        case 15: r = mtFunc(15, 0.0, q[pc]); pc++; break;
         case 16: r = rint(q[pc]); pc++; break;
         case 17: r = fabs(q[pc]); pc++; break;
         case 18: r = !q[pc]; pc++; break;
@@ -254,7 +290,7 @@ __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, doubl
         case 20: r = q[pc + 1] < q[pc]; pc += 2; break;
         case 21: r = q[pc + 1] && q[pc]; pc += 2; break;
         case 22: r = q[pc + 1] || q[pc]; pc += 2; break;
-        case 23: r = 0.5 * erfc(-q[pc] / sqrt(2.0)); pc++; break;
+        case 23: r = mtFunc(23, 0.0, q[pc]); pc++; break;
         default: return -1;
         }
         if (pc > vc) return -1;

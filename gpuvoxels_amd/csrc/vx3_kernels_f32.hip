@@ -26,67 +26,14 @@ void launchStep(const DevBatch<float>& b, hipStream_t s, const StepOpts& o, hipE
 #undef VX3_EV
 }
 
-int residentShapes_f32(ResidentShapeInfo* out, int cap) {
-    static const ResidentShapeInfo shapes[] = {
-        {256, 4, 6, k_f32::ResidentShape<256, 4, 6>::bytes},
-        {512, 2, 3, k_f32::ResidentShape<512, 2, 3>::bytes},
-        {256, 2, 3, k_f32::ResidentShape<256, 2, 3>::bytes},
-        {256, 1, 2, k_f32::ResidentShape<256, 1, 2>::bytes},
-    };
-    const int n = (int)(sizeof(shapes) / sizeof(shapes[0]));
-    for (int i = 0; i < n && i < cap; i++) out[i] = shapes[i];
-    return n;
-}
-
-hipError_t launchResident(const DevBatch<float>& b, hipStream_t s, int shape, const int* simList, int nSims, int nSteps, hipEvent_t* ev) {
-    if (nSims <= 0 || nSteps <= 0) return hipSuccess;
-    switch (shape) {
-    case 0: {
-        auto kern = k_f32::k_resident<256, 4, 6>;
-        const size_t lds = k_f32::ResidentShape<256, 4, 6>::bytes;
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        if (ev) hipEventRecord(ev[0], s);
-        hipLaunchKernelGGL(kern, dim3(nSims), dim3(256), lds, s, b, simList, nSteps);
-        if (ev) hipEventRecord(ev[1], s);
-        return hipGetLastError();
-    }
-    case 1: {
-        auto kern = k_f32::k_resident<512, 2, 3>;
-        const size_t lds = k_f32::ResidentShape<512, 2, 3>::bytes;
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        if (ev) hipEventRecord(ev[0], s);
-        hipLaunchKernelGGL(kern, dim3(nSims), dim3(512), lds, s, b, simList, nSteps);
-        if (ev) hipEventRecord(ev[1], s);
-        return hipGetLastError();
-    }
-    case 2: {
-        auto kern = k_f32::k_resident<256, 2, 3>;
-        const size_t lds = k_f32::ResidentShape<256, 2, 3>::bytes;
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        if (ev) hipEventRecord(ev[0], s);
-        hipLaunchKernelGGL(kern, dim3(nSims), dim3(256), lds, s, b, simList, nSteps);
-        if (ev) hipEventRecord(ev[1], s);
-        return hipGetLastError();
-    }
-    case 3: {
-        auto kern = k_f32::k_resident<256, 1, 2>;
-        const size_t lds = k_f32::ResidentShape<256, 1, 2>::bytes;
-        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        if (ev) hipEventRecord(ev[0], s);
-        hipLaunchKernelGGL(kern, dim3(nSims), dim3(256), lds, s, b, simList, nSteps);
-        if (ev) hipEventRecord(ev[1], s);
-        return hipGetLastError();
-    }
-    default: return hipErrorInvalidValue;
-    }
-}
-
 void launchDt(const DevBatch<float>& b, hipStream_t s, int s0, int s1) {
     if (s1 > s0) hipLaunchKernelGGL(k_f32::k_dt, dim3(s1 - s0), dim3(256), 0, s, b, s0, s1);
 }
 
 }  // namespace vx3
+
+// resident kernel shapes: (block, voxels per thread, links per axis per thread)
+#define VX3_RES_SHAPES(X) X(1024, 1, 1) X(512, 2, 1) X(256, 4, 2) X(256, 2, 1) X(256, 1, 1)
+#define VX3_RES_TABLE_FN residentShapes_f32
+#include "vx3_resident_launch.inl"
+

@@ -16,8 +16,11 @@ struct StepOpts {
 enum { PASS_POISSON = 0, PASS_LINKS, PASS_CONTACTS, PASS_VOXELS, PASS_CONTROL, PASS_COUNT };
 void launchStep(const DevBatch<float>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);
 void launchStep(const DevBatch<double>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);
-// Resident multi-step kernel shapes: (block, voxels/thread, links/thread).
-struct ResidentShapeInfo { int block, vpt, lpt; size_t lds; };
+// Resident multi-step kernel shapes: (block, voxels/thread, links per axis/thread), LDS bytes.
+// A sim fits a shape when V <= block*vpt, each axis has <= block*lpa links, and it uses at
+// most RES_MAX_VMATS voxel and RES_MAX_LMATS link material variants.
+struct ResidentShapeInfo { int block, vpt, lpa; size_t lds; };
+enum { RES_MAX_VMATS = 8, RES_MAX_LMATS = 16 };
 int residentShapes_f32(ResidentShapeInfo* out, int cap);
 int residentShapes_f64(ResidentShapeInfo* out, int cap);
 inline int residentShapes(bool f64, ResidentShapeInfo* out, int cap) {

@@ -1,14 +1,20 @@
 // vx3_resident.inl -- persistent multi-step kernel: one workgroup advances one
 // simulation for many steps with its working set on chip.
 //
-// Each thread owns up to LPT links and VPT voxels for the whole launch and keeps
-// their mutable state in registers.  What other threads need crosses LDS:
-//   pose   position + orientation + thermal factor (+ Poisson strain) per voxel,
-//          written by the voxel owner, read by the link owners;
-//   loads  the two local end loads of every link, written by the link owner and
-//          gathered by the voxel owners in direction order X_POS..Z_NEG, exactly
-//          as the stream path and CVX_Voxel::force() sum them.
-// A step is: link pass | barrier | voxel pass + per-simulation control | barrier.
+// Each thread owns VPT voxels and, for each of the three link axes, LPA links, and
+// keeps their mutable state in registers for the whole launch.  What other threads
+// need crosses LDS:
+//   pose    position + orientation + thermal factor + Poisson strain per voxel,
+//           written by the voxel owner, read by the link owners;
+//   loads   the two local end loads of the links of ONE axis at a time: the link
+//           pass runs axis by axis (X, Y, Z) and after each axis the voxel owners add
+//           that axis' two directions to their running sums.  Summing X_POS, X_NEG,
+//           Y_POS, ... in this order is exactly CVX_Voxel::force()/moment()
+//           (Voxelyze3/src/old/VX_Voxel.cpp:232-269), and the load buffer holds one
+//           axis instead of all links;
+//   tables  the simulation's voxel and link material constants.
+// A step is: 3 x (link pass of one axis | barrier | partial gather | barrier),
+// voxel integration + per-simulation control, barrier.
 // Global memory is read once at launch start and written once at the end in the
 // stream path's layout, so the two paths can hand simulations to each other.
 //
@@ -17,32 +23,49 @@
 namespace vx3 {
 namespace VX_NS {
 
-template <int BS, int VPT, int LPT>
+#ifndef VX3_RES_VMATS
+#define VX3_RES_VMATS 8      // voxel materials a resident sim may use
+#define VX3_RES_LMATS 16     // link material variants a resident sim may use
+#endif
+
+template <int BS, int VPT, int LPA>
 struct ResidentShape {
-    static constexpr int VMAX = BS * VPT;
-    static constexpr int LMAX = BS * LPT;
-    // LDS bytes: pose (7 real + tfac + 3 poisson floats per voxel), loads (12 real per link),
-    // end strains (2 floats per link)
-    static constexpr size_t bytes = (size_t)VMAX * (7 * sizeof(real) + 4 * sizeof(float)) +
-                                    (size_t)LMAX * (12 * sizeof(real) + 2 * sizeof(float));
+    static constexpr int VMAX = BS * VPT;   // voxels per sim
+    static constexpr int LAX = BS * LPA;    // links per axis per sim
+    static constexpr size_t tables = VX3_RES_VMATS * sizeof(VoxMatDev) + VX3_RES_LMATS * sizeof(LinkMatDev);
+    static constexpr size_t bytes = tables
+                                  + (size_t)VMAX * 7 * sizeof(real)      // pose
+                                  + (size_t)LAX * 12 * sizeof(real)      // loads of one axis
+                                  + (size_t)VMAX * 4 * sizeof(float)     // thermal factor, Poisson strain
+                                  + (size_t)LAX * 6 * sizeof(float);     // end strains (axis, end, slot)
+    // setup tables alias the load buffer: voxel -> slot per direction, and (axis, slot) -> link
+    static_assert((size_t)VMAX * 6 * 2 + (size_t)LAX * 3 * 2 <= (size_t)LAX * 12 * sizeof(real), "setup alias");
 };
 
-template <int BS, int VPT, int LPT>
+__device__ __forceinline__ void ldsCopyWords(int* dst, const int* src, int words, int tid, int bs) {
+    for (int i = tid; i < words; i += bs) dst[i] = src[i];
+}
+
+template <int BS, int VPT, int LPA>
 __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* simList, int nSteps) {
-    typedef ResidentShape<BS, VPT, LPT> Shape;
-    constexpr int VMAX = Shape::VMAX, LMAX = Shape::LMAX;
+    typedef ResidentShape<BS, VPT, LPA> Shape;
+    constexpr int VMAX = Shape::VMAX, LAX = Shape::LAX;
     extern __shared__ __align__(16) unsigned char smem[];
-    real* sPx = (real*)smem;
+    VoxMatDev* sVM = (VoxMatDev*)smem;
+    LinkMatDev* sLM = (LinkMatDev*)(smem + VX3_RES_VMATS * sizeof(VoxMatDev));
+    real* sPx = (real*)(smem + Shape::tables);
     real* sPy = sPx + VMAX; real* sPz = sPy + VMAX;
     real* sOw = sPz + VMAX; real* sOx = sOw + VMAX; real* sOy = sOx + VMAX; real* sOz = sOy + VMAX;
-    real* sF = sOz + VMAX;                      // [12][LMAX]
-    float* sTf = (float*)(sF + 12 * LMAX);
+    real* sF = sOz + VMAX;                      // [12][LAX]
+    float* sTf = (float*)(sF + 12 * LAX);
     float* sPsx = sTf + VMAX; float* sPsy = sPsx + VMAX; float* sPsz = sPsy + VMAX;
-    float* sEs = sPsz + VMAX;                   // [2][LMAX]
-    unsigned short* sVL = (unsigned short*)sF;  // [6][VMAX] link table, only during setup
-    __shared__ int sRun, sDiv;
+    float* sEs = sPsz + VMAX;                   // [(axis*2 + end)][LAX]
+    unsigned short* sVL = (unsigned short*)sF;  // [6][VMAX] slot of the link in direction d (setup only)
+    unsigned short* sSL = sVL + 6 * VMAX;       // [3][LAX]  link of (axis, slot) (setup only)
+    __shared__ int sRun, sDiv, sCnt[3];
     __shared__ double sQ[VX3_DEV_MT_MAX + 1];
     __shared__ double sRed[4][BS / 64];
+    __shared__ double sCom[3], sComH0[3], sComH1[3], sAngle;   // center-of-mass history, owned by thread 0
 
     const int s = simList[blockIdx.x];
     SimCtl& scg = b.sims[s];
@@ -60,46 +83,69 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     const double tempAmp = scg.tempAmp, tempPeriod = scg.tempPeriod;
     const int cycleStep = scg.cycleStep, maxSteps = scg.maxSteps;
     const int stopLen = scg.stopLen, stopOff = scg.stopOff;
+    const int mat0 = scg.mat0, nvm = scg.nvm, nlm = scg.nlm;
     const size_t VC = (size_t)b.Vcap;
 
-    // ---- setup: voxel link table, link state and constants into registers
+    // ---- setup: material tables, link slots, voxel -> slot table
+    for (int i = 0; i < nvm; i++)
+        ldsCopyWords((int*)&sVM[i], (const int*)&b.vmatTab[b.simMats[mat0 + i]], sizeof(VoxMatDev) / 4, tid, BS);
+    for (int i = 0; i < nlm; i++)
+        ldsCopyWords((int*)&sLM[i], (const int*)&b.lmatTab[b.simMats[mat0 + nvm + i]], sizeof(LinkMatDev) / 4, tid, BS);
     for (int i = tid; i < 6 * VMAX; i += BS) sVL[i] = 0xFFFF;
+    if (tid < 3) sCnt[tid] = 0;
     __syncthreads();
-    int lNeg[LPT], lPos[LPT], lMeta[LPT], lMatN[LPT], lMatP[LPT];
-    LinkState<real> ls[LPT];
+    for (int l = tid; l < nl; l += BS) {
+        const int g = l0 + l;
+        const int ax = b.lmeta[g] & 3;
+        const int slot = atomicAdd(&sCnt[ax], 1);   // admission guarantees slot < LAX
+        sSL[ax * LAX + slot] = (unsigned short)l;
+        sVL[(2 * ax) * VMAX + (b.lneg[g] - v0)] = (unsigned short)slot;
+        sVL[(2 * ax + 1) * VMAX + (b.lpos[g] - v0)] = (unsigned short)slot;
+    }
+    __syncthreads();
+    auto localVmat = [&](int gidx) {
+        int r = 0;
+        for (int i = 0; i < nvm; i++) if (b.simMats[mat0 + i] == gidx) r = i;
+        return r;
+    };
+    auto localLmat = [&](int gidx) {
+        int r = 0;
+        for (int i = 0; i < nlm; i++) if (b.simMats[mat0 + nvm + i] == gidx) r = i;
+        return r;
+    };
+    // links: (axis a, k) -> slot tid + k*BS of axis a
+    int lEnds[3][LPA];    // vneg | vpos << 16 (sim-local), -1 = no link
+    int lMats[3][LPA];    // link mat | neg voxel mat << 8 | pos voxel mat << 16 (local)
+    int lIdx[3][LPA];     // sim-local link index
+    LinkState<real> ls[3][LPA];
 #pragma unroll
-    for (int k = 0; k < LPT; k++) {
-        const int l = tid + k * BS;
-        lNeg[k] = -1;
-        if (l < nl) {
-            const int g = l0 + l;
-            lNeg[k] = b.lneg[g] - v0;
-            lPos[k] = b.lpos[g] - v0;
-            lMeta[k] = b.lmeta[g];
-            lMatN[k] = b.vmat[v0 + lNeg[k]];
-            lMatP[k] = b.vmat[v0 + lPos[k]];
-            ls[k].pos2 = V3<real>(b.q2x[g], b.q2y[g], b.q2z[g]);
-            ls[k].a1v = V3<real>(b.a1x[g], b.a1y[g], b.a1z[g]);
-            ls[k].a2v = V3<real>(b.a2x[g], b.a2y[g], b.a2z[g]);
-            ls[k].strain = b.strain[g];
-            ls[k].maxStrain = b.maxStrain[g];
-            ls[k].strainOffset = b.strainOffset[g];
-            ls[k].flags = b.lflags[g];
-            const int ax = lMeta[k] & 3;
-            sVL[(2 * ax) * VMAX + lNeg[k]] = (unsigned short)l;
-            sVL[(2 * ax + 1) * VMAX + lPos[k]] = (unsigned short)l;
-            if (poisson) {
-                sEs[l] = b.es[(2 * ax) * VC + v0 + lNeg[k]];
-                sEs[LMAX + l] = b.es[(2 * ax + 1) * VC + v0 + lPos[k]];
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int k = 0; k < LPA; k++) {
+            const int slot = tid + k * BS;
+            lEnds[a][k] = -1;
+            if (slot < sCnt[a]) {
+                const int l = sSL[a * LAX + slot];
+                const int g = l0 + l;
+                const int vn = b.lneg[g] - v0, vq = b.lpos[g] - v0;
+                lIdx[a][k] = l;
+                lEnds[a][k] = vn | (vq << 16);
+                lMats[a][k] = localLmat(b.lmeta[g] >> 2) | (localVmat(b.vmat[v0 + vn]) << 8) | (localVmat(b.vmat[v0 + vq]) << 16);
+                ls[a][k].pos2 = V3<real>(b.q2x[g], b.q2y[g], b.q2z[g]);
+                ls[a][k].a1v = V3<real>(b.a1x[g], b.a1y[g], b.a1z[g]);
+                ls[a][k].a2v = V3<real>(b.a2x[g], b.a2y[g], b.a2z[g]);
+                ls[a][k].strain = b.strain[g];
+                ls[a][k].maxStrain = b.maxStrain[g];
+                ls[a][k].strainOffset = b.strainOffset[g];
+                ls[a][k].flags = b.lflags[g];
             }
         }
     }
-    __syncthreads();
     V3<real> vp[VPT], vm[VPT], va[VPT];
     Q4<real> vo[VPT];
     int vf[VPT], vmat[VPT], vext[VPT];
     double vph[VPT];
-    unsigned short vl[VPT][6];
+    int vl[VPT][3];       // slots of directions (2i, 2i+1) packed lo | hi << 16, 0xFFFF = none
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         const int v = tid + k * BS;
@@ -111,14 +157,14 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             va[k] = V3<real>(b.ax[g], b.ay[g], b.az[g]);
             vo[k] = Q4<real>(b.ow[g], b.ox[g], b.oy[g], b.oz[g]);
             vf[k] = b.vflags[g];
-            vmat[k] = b.vmat[g];
+            vmat[k] = localVmat(b.vmat[g]);
             vext[k] = b.vext[g];
             vph[k] = b.phase[g];
 #pragma unroll
-            for (int d = 0; d < 6; d++) vl[k][d] = sVL[d * VMAX + v];
+            for (int i = 0; i < 3; i++) vl[k][i] = (int)sVL[(2 * i) * VMAX + v] | ((int)sVL[(2 * i + 1) * VMAX + v] << 16);
         }
     }
-    __syncthreads();   // sVL aliases sF: done with it
+    __syncthreads();   // setup tables alias the load buffer: done with them
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         const int v = tid + k * BS;
@@ -130,69 +176,109 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             sPsx[v] = b.psx[g]; sPsy[v] = b.psy[g]; sPsz[v] = b.psz[g];
         }
     }
+    if (poisson) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int k = 0; k < LPA; k++) {
+                if (lEnds[a][k] < 0) continue;
+                const int slot = tid + k * BS;
+                const int vn = lEnds[a][k] & 0xFFFF, vq = lEnds[a][k] >> 16;
+                sEs[(2 * a) * LAX + slot] = b.es[(2 * a) * VC + v0 + vn];
+                sEs[(2 * a + 1) * LAX + slot] = b.es[(2 * a + 1) * VC + v0 + vq];
+            }
+        }
+    }
     double t = scg.t;
     int steps = scg.steps;
     int angleSamples = scg.angleSamples;
-    double com[3] = {scg.com[0], scg.com[1], scg.com[2]};
-    double comH0[3] = {scg.comH0[0], scg.comH0[1], scg.comH0[2]};
-    double comH1[3] = {scg.comH1[0], scg.comH1[1], scg.comH1[2]};
-    double recentAngle = scg.recentAngle;
     int state = 0;
-    if (tid == 0) { sRun = 1; sDiv = 0; }
+    if (tid == 0) {
+        sRun = 1; sDiv = 0;
+        for (int q = 0; q < 3; q++) { sCom[q] = scg.com[q]; sComH0[q] = scg.comH0[q]; sComH1[q] = scg.comH1[q]; }
+        sAngle = scg.recentAngle;
+    }
     __syncthreads();
 
     for (int it = 0; it < nSteps; it++) {
-        // ---- link pass
+        // keep per-step LDS address arithmetic inside the loop (register pressure)
 #pragma unroll
-        for (int k = 0; k < LPT; k++) {
-            if (lNeg[k] < 0) continue;
-            const int l = tid + k * BS;
-            const int vn = lNeg[k], vq = lPos[k];
-            const int axis = lMeta[k] & 3;
-            const LinkMatDev& mat = b.lmatTab[lMeta[k] >> 2];
-            const VoxMatDev& mn = b.vmatTab[lMatN[k]];
-            const VoxMatDev& mp = b.vmatTab[lMatP[k]];
-            if (mn.fixed && mp.fixed) continue;
-            const real restLength = (real)0.5 * (nom * (real)sTf[vn] + nom * (real)sTf[vq]);
-            float area, tss;
-            if (mat.nu != 0.0f) {
-                float an, sn, ap, sp;
-                // CVX_Voxel::transverseArea / transverseStrainSum from the pose cache
-                auto tr = [&](int v, bool nuZero, float& ar, float& su) {
-                    if (nuZero) { ar = size * size; su = 0.0f; return; }
-                    const float sx = sPsx[v], sy = sPsy[v], sz = sPsz[v];
-                    const real s2 = (real)(size * size);
-                    if (axis == 0) { ar = (float)(s2 * ((real)1 + (real)sy) * ((real)1 + (real)sz)); su = sy + sz; }
-                    else if (axis == 1) { ar = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sz)); su = sx + sz; }
-                    else { ar = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sy)); su = sx + sy; }
-                };
-                tr(vn, mn.nu == 0.0f, an, sn);
-                tr(vq, mp.nu == 0.0f, ap, sp);
-                area = 0.5f * (an + ap);
-                tss = 0.5f * (sn + sp);
-            } else {
-                area = 0.5f * (size * size + size * size);
-                tss = 0.0f;
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int k = 0; k < LPA; k++) { asm volatile("" : "+v"(lEnds[a][k])); asm volatile("" : "+v"(lMats[a][k])); }
+#pragma unroll
+        for (int k = 0; k < VPT; k++) { asm volatile("" : "+v"(vl[k][0])); asm volatile("" : "+v"(vl[k][1])); asm volatile("" : "+v"(vl[k][2])); asm volatile("" : "+v"(vmat[k])); }
+        V3<real> fs[VPT], ms[VPT];
+#pragma unroll
+        for (int k = 0; k < VPT; k++) { fs[k] = V3<real>(0, 0, 0); ms[k] = V3<real>(0, 0, 0); }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            // ---- link pass of axis a
+#pragma unroll
+            for (int k = 0; k < LPA; k++) {
+                if (lEnds[a][k] < 0) continue;
+                const int slot = tid + k * BS;
+                const int vn = lEnds[a][k] & 0xFFFF, vq = lEnds[a][k] >> 16;
+                const LinkMatDev& mat = sLM[lMats[a][k] & 0xFF];
+                const VoxMatDev& mn = sVM[(lMats[a][k] >> 8) & 0xFF];
+                const VoxMatDev& mp = sVM[lMats[a][k] >> 16];
+                if (mn.fixed && mp.fixed) continue;   // its slots are never read: both ends are fixed
+                const real restLength = (real)0.5 * (nom * (real)sTf[vn] + nom * (real)sTf[vq]);
+                float area, tss;
+                if (mat.nu != 0.0f) {
+                    float an, sn, ap, sp;
+                    // CVX_Voxel::transverseArea / transverseStrainSum from the pose cache
+                    auto tr = [&](int v, bool nuZero, float& ar, float& su) {
+                        if (nuZero) { ar = size * size; su = 0.0f; return; }
+                        const float sx = sPsx[v], sy = sPsy[v], sz = sPsz[v];
+                        const real s2 = (real)(size * size);
+                        if (a == 0) { ar = (float)(s2 * ((real)1 + (real)sy) * ((real)1 + (real)sz)); su = sy + sz; }
+                        else if (a == 1) { ar = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sz)); su = sx + sz; }
+                        else { ar = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sy)); su = sx + sy; }
+                    };
+                    tr(vn, mn.nu == 0.0f, an, sn);
+                    tr(vq, mp.nu == 0.0f, ap, sp);
+                    area = 0.5f * (an + ap);
+                    tss = 0.5f * (sn + sp);
+                } else {
+                    area = 0.5f * (size * size + size * size);
+                    tss = 0.0f;
+                }
+                const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : dtf);
+                const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : dtf);
+                const V3<real> pN(sPx[vn], sPy[vn], sPz[vn]), pP(sPx[vq], sPy[vq], sPz[vq]);
+                const Q4<real> oN(sOw[vn], sOx[vn], sOy[vn], sOz[vn]), oP(sOw[vq], sOx[vq], sOy[vq], sOz[vq]);
+                V3<real> fN, mN, fP, mP;
+                linkForces<real>(mat, a, pN, oN, pP, oP, restLength, area, tss, mat.nu != 0.0f, dmN, dmP, ls[a][k], fN, mN, fP, mP);
+                sF[0 * LAX + slot] = fN.x; sF[1 * LAX + slot] = fN.y; sF[2 * LAX + slot] = fN.z;
+                sF[3 * LAX + slot] = mN.x; sF[4 * LAX + slot] = mN.y; sF[5 * LAX + slot] = mN.z;
+                sF[6 * LAX + slot] = fP.x; sF[7 * LAX + slot] = fP.y; sF[8 * LAX + slot] = fP.z;
+                sF[9 * LAX + slot] = mP.x; sF[10 * LAX + slot] = mP.y; sF[11 * LAX + slot] = mP.z;
+                if (poisson) {
+                    // CVX_Link::axialStrain(positiveEnd) (VX_Link.cpp:115-118)
+                    const float r = mat.strainRatio;
+                    sEs[(2 * a) * LAX + slot] = 2.0f * ls[a][k].strain / (1.0f + r);
+                    sEs[(2 * a + 1) * LAX + slot] = 2.0f * ls[a][k].strain * r / (1.0f + r);
+                }
+                if (ls[a][k].strain > 100) sDiv = 1;
             }
-            const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : dtf);
-            const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : dtf);
-            const V3<real> pN(sPx[vn], sPy[vn], sPz[vn]), pP(sPx[vq], sPy[vq], sPz[vq]);
-            const Q4<real> oN(sOw[vn], sOx[vn], sOy[vn], sOz[vn]), oP(sOw[vq], sOx[vq], sOy[vq], sOz[vq]);
-            V3<real> fN, mN, fP, mP;
-            linkForces<real>(mat, axis, pN, oN, pP, oP, restLength, area, tss, mat.nu != 0.0f, dmN, dmP, ls[k], fN, mN, fP, mP);
-            sF[0 * LMAX + l] = fN.x; sF[1 * LMAX + l] = fN.y; sF[2 * LMAX + l] = fN.z;
-            sF[3 * LMAX + l] = mN.x; sF[4 * LMAX + l] = mN.y; sF[5 * LMAX + l] = mN.z;
-            sF[6 * LMAX + l] = fP.x; sF[7 * LMAX + l] = fP.y; sF[8 * LMAX + l] = fP.z;
-            sF[9 * LMAX + l] = mP.x; sF[10 * LMAX + l] = mP.y; sF[11 * LMAX + l] = mP.z;
-            if (poisson) {
-                const float r = mat.strainRatio;
-                sEs[l] = 2.0f * ls[k].strain / (1.0f + r);
-                sEs[LMAX + l] = 2.0f * ls[k].strain * r / (1.0f + r);
+            __syncthreads();
+            // ---- the voxels add this axis' two directions: 2a (negative end) then 2a+1 (positive end)
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                if (vmat[k] < 0) continue;
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int slot = e ? ((vl[k][a] >> 16) & 0xFFFF) : (vl[k][a] & 0xFFFF);
+                    if (slot == 0xFFFF) continue;
+                    const int c = 6 * e;
+                    fs[k].add(V3<real>(sF[(c + 0) * LAX + slot], sF[(c + 1) * LAX + slot], sF[(c + 2) * LAX + slot]));
+                    ms[k].add(V3<real>(sF[(c + 3) * LAX + slot], sF[(c + 4) * LAX + slot], sF[(c + 5) * LAX + slot]));
+                }
             }
-            if (ls[k].strain > 100) sDiv = 1;
+            if (a < 2) __syncthreads();   // the next axis overwrites the load buffer
         }
-        __syncthreads();
-        const bool diverged = sDiv != 0;
+        const bool diverged = sDiv != 0;   // written before the last link-pass barrier
         steps++;
         const bool sample = !diverged && cycleStep > 0 && steps % cycleStep == 0;
         double psx = 0, psy = 0, psz = 0, psm = 0;
@@ -203,28 +289,17 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             for (int k = 0; k < VPT; k++) {
                 if (vmat[k] < 0) continue;
                 const int v = tid + k * BS;
-                const VoxMatDev& m = b.vmatTab[vmat[k]];
+                const VoxMatDev& m = sVM[vmat[k]];
                 if (!m.fixed) {
-                    V3<real> f(0, 0, 0), mo(0, 0, 0);
-#pragma unroll
-                    for (int d = 0; d < 6; d++) {
-                        const int l = vl[k][d];
-                        if (l == 0xFFFF) continue;
-                        const int e = (d & 1) ? 6 : 0;   // odd direction: this voxel is the positive end
-                        f.add(V3<real>(sF[(e + 0) * LMAX + l], sF[(e + 1) * LMAX + l], sF[(e + 2) * LMAX + l]));
-                        mo.add(V3<real>(sF[(e + 3) * LMAX + l], sF[(e + 4) * LMAX + l], sF[(e + 5) * LMAX + l]));
-                    }
                     const ExtDev* ext = vext[k] >= 0 ? &b.extTab[vext[k]] : nullptr;
                     int ix = 0, iy = 0, iz = 0;
                     if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
-                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], f, mo, nullptr, 0, 0, nullptr, vp[k], vm[k], vo[k], va[k], vf[k]);
+                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], nullptr, 0, 0, nullptr, vp[k], vm[k], vo[k], va[k], vf[k]);
                     sPx[v] = vp[k].x; sPy[v] = vp[k].y; sPz[v] = vp[k].z;
                     sOw[v] = vo[k].w; sOx[v] = vo[k].x; sOy[v] = vo[k].y; sOz[v] = vo[k].z;
                 }
                 if (vary && !m.fixed && !(m.thermalOnAfter > tn)) {
-                    double T = tempAmp * sin((double)(2 * 3.1415926f) * (tn / tempPeriod + vph[k]));
-                    if (!expansion && T > 0) T = 0;
-                    const float temp = (float)T;
+                    const float temp = actuationTemp(tempAmp, tempPeriod, vph[k], tn, expansion);
                     sTf[v] = 1 + temp * m.alphaCTE;
                 }
                 if (poisson) {
@@ -232,11 +307,14 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     float st[3] = {0, 0, 0};
                     int n[3] = {0, 0, 0};
 #pragma unroll
-                    for (int d = 0; d < 6; d++) {
-                        const int l = vl[k][d];
-                        if (l == 0xFFFF) continue;
-                        st[d >> 1] += sEs[((d & 1) ? LMAX : 0) + l];
-                        n[d >> 1]++;
+                    for (int i = 0; i < 3; i++) {
+#pragma unroll
+                        for (int e = 0; e < 2; e++) {
+                            const int slot = e ? ((vl[k][i] >> 16) & 0xFFFF) : (vl[k][i] & 0xFFFF);
+                            if (slot == 0xFFFF) continue;
+                            st[i] += sEs[(2 * i + e) * LAX + slot];
+                            n[i]++;
+                        }
                     }
                     const ExtDev* ext = vext[k] >= 0 ? &b.extTab[vext[k]] : nullptr;
                     bool tension[3];
@@ -249,8 +327,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                         float add = 0;
 #pragma unroll
                         for (int i = 0; i < 3; i++) if (tension[i]) add += st[i];
-                        const float pw = (float)pow((double)(1.0f + add), (double)(-m.nu));
-                        const float value = pw - 1.0f;
+                        const float value = poissonValue(1.0f + add, m.nu);
 #pragma unroll
                         for (int i = 0; i < 3; i++) if (!tension[i]) st[i] = value;
                     }
@@ -271,51 +348,55 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             }
             if ((tid & 63) == 0) { sRed[0][tid >> 6] = psx; sRed[1][tid >> 6] = psy; sRed[2][tid >> 6] = psz; sRed[3][tid >> 6] = psm; }
             __syncthreads();
-            double tot[4] = {0, 0, 0, 0};
-            for (int w = 0; w < BS / 64; w++) {
-#pragma unroll
-                for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
-            }
             angleSamples++;
-#pragma unroll
-            for (int q = 0; q < 3; q++) { comH0[q] = comH1[q]; comH1[q] = com[q]; com[q] = tot[3] == 0 ? 0.0 : (1.0 / tot[3]) * tot[q]; }
-            const double* A = comH0; const double* B = comH1; const double* C = com;
-            const bool bc = B[0] == C[0] && B[1] == C[1] && B[2] == C[2];
-            const bool ab = A[0] == B[0] && A[1] == B[1] && A[2] == B[2];
-            if (bc || ab || angleSamples < 3) recentAngle = 0;
-            else {
-                double d1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, d2[3] = {C[0] - B[0], C[1] - B[1], C[2] - B[2]};
-                double dot = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
-                double l1 = sqrt((A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]));
-                double l2 = sqrt((B[0] - C[0]) * (B[0] - C[0]) + (B[1] - C[1]) * (B[1] - C[1]) + (B[2] - C[2]) * (B[2] - C[2]));
-                recentAngle = acos(dot / (l1 * l2));
+            if (tid == 0) {
+                double tot[4] = {0, 0, 0, 0};
+                for (int w = 0; w < BS / 64; w++) {
+                    for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
+                }
+                double* A = sComH0; double* B = sComH1; double* C = sCom;
+                for (int q = 0; q < 3; q++) { A[q] = B[q]; B[q] = C[q]; C[q] = tot[3] == 0 ? 0.0 : (1.0 / tot[3]) * tot[q]; }
+                const bool bc = B[0] == C[0] && B[1] == C[1] && B[2] == C[2];
+                const bool ab = A[0] == B[0] && A[1] == B[1] && A[2] == B[2];
+                if (bc || ab || angleSamples < 3) sAngle = 0;
+                else {
+                    double d1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, d2[3] = {C[0] - B[0], C[1] - B[1], C[2] - B[2]};
+                    double dot = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2];
+                    double l1 = sqrt((A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2]));
+                    double l2 = sqrt((B[0] - C[0]) * (B[0] - C[0]) + (B[1] - C[1]) * (B[1] - C[1]) + (B[2] - C[2]) * (B[2] - C[2]));
+                    sAngle = acos(dot / (l1 * l2));
+                }
             }
         }
         // ---- per-simulation control (uniform across the workgroup)
         if (diverged) { state = 2; break; }
         t = tn;
         bool stop = maxSteps > 0 && steps >= maxSteps;
-        if (!stop && stopLen > 0 && tid == 0) sRun = mathTreeEval(b.tokens + stopOff, stopLen, com[0], com[1], com[2], 0, t, recentAngle, 0, 0, nv, sQ) > 0 ? 0 : 1;
-        __syncthreads();
+        if (!stop && stopLen > 0 && tid == 0) sRun = mathTreeEval(b.tokens + stopOff, stopLen, sCom[0], sCom[1], sCom[2], 0, t, sAngle, 0, 0, nv, sQ) > 0 ? 0 : 1;
+        __syncthreads();   // next step's link pass reads the new poses
         if (stop || !sRun) { state = 1; break; }
     }
 
     // ---- write back in the stream layout
 #pragma unroll
-    for (int k = 0; k < LPT; k++) {
-        if (lNeg[k] < 0) continue;
-        const int l = tid + k * BS, g = l0 + l;
-        b.q2x[g] = ls[k].pos2.x; b.q2y[g] = ls[k].pos2.y; b.q2z[g] = ls[k].pos2.z;
-        b.a1x[g] = ls[k].a1v.x; b.a1y[g] = ls[k].a1v.y; b.a1z[g] = ls[k].a1v.z;
-        b.a2x[g] = ls[k].a2v.x; b.a2y[g] = ls[k].a2v.y; b.a2z[g] = ls[k].a2v.z;
-        b.strain[g] = ls[k].strain;
-        b.maxStrain[g] = ls[k].maxStrain;
-        b.strainOffset[g] = ls[k].strainOffset;
-        b.lflags[g] = ls[k].flags;
-        if (poisson) {
-            const int ax = lMeta[k] & 3;
-            b.es[(2 * ax) * VC + v0 + lNeg[k]] = sEs[l];
-            b.es[(2 * ax + 1) * VC + v0 + lPos[k]] = sEs[LMAX + l];
+    for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int k = 0; k < LPA; k++) {
+            if (lEnds[a][k] < 0) continue;
+            const int g = l0 + lIdx[a][k];
+            b.q2x[g] = ls[a][k].pos2.x; b.q2y[g] = ls[a][k].pos2.y; b.q2z[g] = ls[a][k].pos2.z;
+            b.a1x[g] = ls[a][k].a1v.x; b.a1y[g] = ls[a][k].a1v.y; b.a1z[g] = ls[a][k].a1v.z;
+            b.a2x[g] = ls[a][k].a2v.x; b.a2y[g] = ls[a][k].a2v.y; b.a2z[g] = ls[a][k].a2v.z;
+            b.strain[g] = ls[a][k].strain;
+            b.maxStrain[g] = ls[a][k].maxStrain;
+            b.strainOffset[g] = ls[a][k].strainOffset;
+            b.lflags[g] = ls[a][k].flags;
+            if (poisson) {
+                const int slot = tid + k * BS;
+                const int vn = lEnds[a][k] & 0xFFFF, vq = lEnds[a][k] >> 16;
+                b.es[(2 * a) * VC + v0 + vn] = sEs[(2 * a) * LAX + slot];
+                b.es[(2 * a + 1) * VC + v0 + vq] = sEs[(2 * a + 1) * LAX + slot];
+            }
         }
     }
 #pragma unroll
@@ -334,9 +415,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         scg.t = t;
         scg.steps = steps;
         scg.angleSamples = angleSamples;
-#pragma unroll
-        for (int q = 0; q < 3; q++) { scg.com[q] = com[q]; scg.comH0[q] = comH0[q]; scg.comH1[q] = comH1[q]; }
-        scg.recentAngle = recentAngle;
+        for (int q = 0; q < 3; q++) { scg.com[q] = sCom[q]; scg.comH0[q] = sComH0[q]; scg.comH1[q] = sComH1[q]; }
+        scg.recentAngle = sAngle;
         scg.state = state;
     }
 }

@@ -86,6 +86,7 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int stopOff, stopLen;            // MathTree program in the token pool
     int nColl;                       // collision pairs owned (pair range below)
     int c0;
+    int mat0, nvm, nlm;              // the sim's materials: simMats[mat0 .. +nvm) voxel, then nlm link table indices
 };
 
 struct TokenDev { int op; int pad; double value; };

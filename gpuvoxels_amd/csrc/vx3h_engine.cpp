@@ -141,10 +141,12 @@ struct vx3h_batch {
     Dedup<LinkMatDev> lmats;
     std::vector<ExtDev> exts;
     std::vector<TokenDev> tokens;
+    std::vector<int> simMats;                    // per sim: global material table indices it uses
     void* dVmat = nullptr; size_t capVmat = 0;
     void* dLmat = nullptr; size_t capLmat = 0;
     void* dExt = nullptr; size_t capExt = 0;
     void* dTok = nullptr; size_t capTok = 0;
+    void* dSimMats = nullptr; size_t capSimMats = 0;
     bool anyPoisson = false, anyColl = false;
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
@@ -266,7 +268,7 @@ vx3h_status launchSteps(vx3h_batch* b, int n) {
     o.poisson = b->anyPoisson;
     o.contacts = false;
     o.nPairs = 0;
-    o.block = b->cfg.block_size;
+    o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
         else launchStep(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, o);
@@ -347,11 +349,16 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
         b->resLists.assign(ns, {});
         b->dResLists.assign(ns, nullptr);
         b->capRes.assign(ns, 0);
-        // default preference: smallest capacity first (more workgroups per CU), table order on ties
+        // default preference: smallest voxel capacity first (more workgroups per CU), table order on ties
         for (int i = 0; i < ns; i++) b->shapePref.push_back(i);
         std::stable_sort(b->shapePref.begin(), b->shapePref.end(), [&](int x, int y) {
             return b->shapes[x].block * b->shapes[x].vpt < b->shapes[y].block * b->shapes[y].vpt;
         });
+        // a configured block size moves the shapes of that block size to the front
+        if (b->cfg.block_size > 0)
+            std::stable_sort(b->shapePref.begin(), b->shapePref.end(), [&](int x, int y) {
+                return (b->shapes[x].block == b->cfg.block_size) > (b->shapes[y].block == b->cfg.block_size);
+            });
     }
     size_t freeB = 0, totB = 0;
     if (hipMemGetInfo(&freeB, &totB) != hipSuccess) freeB = 0;
@@ -368,7 +375,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
     if (b->graph) hipGraphDestroy(b->graph);
     for (auto* arrs : {&b->vArrs, &b->lArrs})
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
-    for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, (void*)b->db.sims}) if (p) hipFree(p);
+    for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
     hipStreamDestroy(b->stream);
     delete b;
@@ -431,6 +438,11 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         vx3h_status st = uploadTable(b, b->dExt, b->capExt, b->exts);
         if (st) return st;
     }
+    int mat0 = (int)b->simMats.size();
+    b->simMats.insert(b->simMats.end(), vmIdx.begin(), vmIdx.end());
+    b->simMats.insert(b->simMats.end(), lmIdx.begin(), lmIdx.end());
+    { vx3h_status st = uploadTable(b, b->dSimMats, b->capSimMats, b->simMats); if (st) return st; }
+    b->db.simMats = (const int*)b->dSimMats;
     int tok0 = (int)b->tokens.size();
     for (const vx3h_token& t : H.stopTok) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
     if (!H.stopTok.empty()) { vx3h_status st = uploadTable(b, b->dTok, b->capTok, b->tokens); if (st) return st; }
@@ -489,10 +501,14 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     }
     // execution path: resident when the sim fits one of the on-chip kernel shapes
     int shape = -1;
-    if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0) {
+    int perAxis[3] = {0, 0, 0};
+    for (int i = 0; i < NL; i++) perAxis[H.laxis[i]]++;
+    const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS;
+    if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
-            if (NV <= r.block * r.vpt && NL <= r.block * r.lpt) { shape = si; break; }
+            const int lax = r.block * r.lpa;
+            if (NV <= r.block * r.vpt && perAxis[0] <= lax && perAxis[1] <= lax && perAxis[2] <= lax) { shape = si; break; }
         }
         if (shape < 0 && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "simulation too large for the resident kernel");
     }
@@ -513,6 +529,9 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     c.flags = H.flags | (shape >= 0 ? SC_RESIDENT : 0);
     c.v0 = v0; c.nv = NV; c.l0 = l0; c.nl = NL;
     c.stopOff = tok0;
+    c.mat0 = mat0;
+    c.nvm = (int)vmIdx.size();
+    c.nlm = (int)lmIdx.size();
     c.stopLen = (int)H.stopTok.size();
     if (NV == 0) c.state = VX3H_SIM_DONE;   // "No links and no voxels" (VX3_SimulationManager.cu:15-19)
     else if (!H.stopTok.empty() &&
@@ -592,7 +611,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     o.poisson = b->anyPoisson;
     o.contacts = false;
     o.nPairs = 0;
-    o.block = b->cfg.block_size;
+    o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];
         if (b->realSize == 4) launchStep(b->db, b->stream, o, e);

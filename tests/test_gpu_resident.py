@@ -48,10 +48,29 @@ def test_resident_f64_equals_stream_f64_on_robots():
     br.close()
 
 
-def test_resident_f32_cube_within_tolerance():
+@pytest.mark.parametrize("block", [512, 256])
+def test_resident_f64_shapes_equal_stream_on_full_robots(block):
+    # 730-908-voxel robots: the (256, 4, 2) and (512, 2, 1) FP64 shapes
+    rs = workloads.robots(3, seed=5, steps=0)
+    bs = engine.Batch(precision=_abi.PREC_F64, path=_abi.PATH_STREAM)
+    br = engine.Batch(precision=_abi.PREC_F64, path=_abi.PATH_RESIDENT, steps_per_launch=250, block_size=block)
+    ss = [bs.admit(r) for r in rs]
+    sr = [br.admit(r) for r in rs]
+    bs.step(500)
+    br.step(500)
+    for a, c in zip(ss, sr):
+        ga, gc = bs.read_state(a), br.read_state(c)
+        assert np.max(np.abs(ga["pos"] - gc["pos"])) < 1e-14
+        assert np.array_equal(ga["flags"], gc["flags"])
+    bs.close()
+    br.close()
+
+
+@pytest.mark.parametrize("block", [1024, 512, 256])
+def test_resident_f32_cube_within_tolerance(block):
     m = workloads.cube(6, steps=0)
     o = OSim(m)
-    b = engine.Batch(precision=_abi.PREC_F32, path=_abi.PATH_RESIDENT, steps_per_launch=500)
+    b = engine.Batch(precision=_abi.PREC_F32, path=_abi.PATH_RESIDENT, steps_per_launch=500, block_size=block)
     s = b.admit(m)
     o.step(1000)
     b.step(1000)
