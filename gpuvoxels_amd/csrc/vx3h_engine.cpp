@@ -163,6 +163,11 @@ struct vx3h_batch {
     hipGraph_t graph = nullptr;
     int graphSteps = 0;
     long layout = 0, graphLayout = -1;
+    // launch timing (vx3h_batch_set_timing)
+    bool timing = false;
+    std::vector<hipEvent_t> evPool;              // free events
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evUsed;
+    long long timedSteps = 0;
 };
 
 namespace {
@@ -310,16 +315,29 @@ vx3h_status syncResLists(vx3h_batch* b) {
     return VX3H_OK;
 }
 
+vx3h_status takeEvent(vx3h_batch* b, hipEvent_t& e) {
+    if (b->evPool.empty()) { HIPCHK(hipEventCreate(&e)); }
+    else { e = b->evPool.back(); b->evPool.pop_back(); }
+    return VX3H_OK;
+}
+
 vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
     vx3h_status st = syncResLists(b);
     if (st) return st;
     for (size_t i = 0; i < b->resLists.size(); i++) {
         const int n = (int)b->resLists[i].size();
         if (!n) continue;
+        hipEvent_t pair[2];
+        hipEvent_t* evp = ev ? ev + 2 * i : nullptr;
+        if (!ev && b->timing) {
+            if ((st = takeEvent(b, pair[0])) || (st = takeEvent(b, pair[1]))) return st;
+            b->evUsed.push_back({pair[0], pair[1]});
+            b->timedSteps += nSteps;
+            evp = pair;
+        }
         hipError_t e = b->realSize == 4
-            ? launchResident(b->db, b->stream, (int)i, b->dResLists[i], n, nSteps, ev ? ev + 2 * i : nullptr)
-            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, (int)i, b->dResLists[i], n, nSteps,
-                             ev ? ev + 2 * i : nullptr);
+            ? launchResident(b->db, b->stream, (int)i, b->dResLists[i], n, nSteps, evp)
+            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, (int)i, b->dResLists[i], n, nSteps, evp);
         if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("resident launch failed: ") + hipGetErrorString(e));
     }
     return VX3H_OK;
@@ -377,6 +395,8 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
+    for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
+    for (hipEvent_t e : b->evPool) hipEventDestroy(e);
     hipStreamDestroy(b->stream);
     delete b;
     return VX3H_OK;
@@ -644,6 +664,68 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
         avg_ms[5] += ms;
     }
     for (auto& e : rev) hipEventDestroy(e);
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_batch_set_timing(vx3h_batch* b, int enable) {
+    if (!b) FAILV(VX3H_EINVAL, "null batch");
+    b->timing = enable != 0;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_batch_timing(vx3h_batch* b, double* total_ms, int* launches, long long* steps) {
+    if (!b || !total_ms || !launches || !steps) FAILV(VX3H_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    double tot = 0;
+    for (auto& pr : b->evUsed) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        tot += ms;
+        b->evPool.push_back(pr.first);
+        b->evPool.push_back(pr.second);
+    }
+    *total_ms = tot;
+    *launches = (int)b->evUsed.size();
+    *steps = b->timedSteps;
+    b->evUsed.clear();
+    b->timedSteps = 0;
+    return VX3H_OK;
+}
+
+// Plain streaming copy: the reference point for HBM traffic counters and for the
+// bandwidth a simple kernel reaches on this device.
+__global__ void __launch_bounds__(256) k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+extern "C" vx3h_status vx3h_calib_copy(int device, size_t bytes, int reps, double* ms_per_copy) {
+    if (bytes < 16 || reps <= 0 || !ms_per_copy) FAILV(VX3H_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(device));
+    const size_t n = bytes / 16;
+    void *a = nullptr, *c = nullptr;
+    HIPCHK(hipMalloc(&a, n * 16));
+    if (hipMalloc(&c, n * 16) != hipSuccess) { hipFree(a); FAILV(VX3H_ENOMEM, "calibration buffers"); }
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(hipMemsetAsync(a, 0, n * 16, st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const int grid = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_calib_copy, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)c, n);   // warm
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_calib_copy, dim3(grid), dim3(256), 0, st, (const float4*)a, (float4*)c, n);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = ms / reps;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipStreamDestroy(st);
+    hipFree(a);
+    hipFree(c);
     return VX3H_OK;
 }
 

@@ -45,6 +45,9 @@ def lib():
         "vx3h_sim_read_state": (S, [P, C.c_int, C.POINTER(_abi.StateView)]),
         "vx3h_build_topology": (S, [C.POINTER(_abi.SimDesc), C.POINTER(_abi.TopologyView)]),
         "vx3h_batch_kernel_times": (S, [P, C.c_int, C.POINTER(C.c_double)]),
+        "vx3h_batch_set_timing": (S, [P, C.c_int]),
+        "vx3h_calib_copy": (S, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+        "vx3h_batch_timing": (S, [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     }
     optional = {
         "vx3h_vxa_parse": (S, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
@@ -74,6 +77,13 @@ def device_count():
     n = C.c_int(0)
     check(lib().vx3h_device_count(C.byref(n)))
     return n.value
+
+
+def calib_copy(nbytes, reps=1, device=0):
+    """Mean ms of a plain device copy of nbytes (reads nbytes, writes nbytes)."""
+    ms = C.c_double(0)
+    check(lib().vx3h_calib_copy(device, int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def _p(a, ct):
@@ -150,6 +160,15 @@ class Batch:
         out = (C.c_double * len(self.PASSES))()
         check(lib().vx3h_batch_kernel_times(self.h, int(n_steps), out))
         return {k: out[i] for i, k in enumerate(self.PASSES) if out[i] > 0}
+
+    def set_timing(self, enable=True):
+        check(lib().vx3h_batch_set_timing(self.h, int(bool(enable))))
+
+    def timing(self):
+        """(summed ms, launches, steps) of the resident launches since the last call."""
+        ms, n, st = C.c_double(0), C.c_int(0), C.c_longlong(0)
+        check(lib().vx3h_batch_timing(self.h, C.byref(ms), C.byref(n), C.byref(st)))
+        return ms.value, n.value, st.value
 
     def retire(self, slot):
         check(lib().vx3h_sim_retire(self.h, slot))

@@ -224,6 +224,18 @@ vx3h_status vx3h_build_topology(const vx3h_sim_desc* desc, vx3h_topology_view* v
 #define VX3H_N_PASSES 6
 vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, double* avg_ms);
 
+/* Launch timing of ordinary stepping (no counterpart in the reference): while
+ * enabled, vx3h_batch_step records a HIP event pair on the batch's stream around
+ * every resident multi-step launch.  vx3h_batch_timing synchronizes the stream and
+ * returns, since the last call, the summed duration in ms, the number of launches
+ * and the number of steps those launches were asked to run; it then resets them. */
+vx3h_status vx3h_batch_set_timing(vx3h_batch* b, int enable);
+/* Plain device-to-device copy kernel of `bytes` bytes, run reps times (no
+ * counterpart in the reference): the known-traffic reference for HBM counters and
+ * the bandwidth a simple streaming kernel reaches; returns the mean ms per copy. */
+vx3h_status vx3h_calib_copy(int device, size_t bytes, int reps, double* ms_per_copy);
+vx3h_status vx3h_batch_timing(vx3h_batch* b, double* total_ms, int* launches, long long* steps);
+
 /* ---------------------------------------------------------------- report */
 typedef struct vx3h_report_entry {
     const char* vxd_name;        /* file name of the sim's VXD */
