@@ -155,3 +155,14 @@ class ReportEntry(C.Structure):
         ("pos", C.POINTER(C.c_double)),
         ("mats", C.POINTER(C.c_int)),
     ]
+
+
+class VxaInfo(C.Structure):
+    _fields_ = [
+        ("enable_attach", C.c_int), ("enable_detach", C.c_int),
+        ("enable_cilia", C.c_int), ("enable_signals", C.c_int),
+        ("secondary_experiment", C.c_int),
+        ("record_step_size", C.c_int),
+        ("max_dist_close_pairs", C.c_double),
+        ("has_force_field", C.c_int),
+    ]

@@ -49,13 +49,15 @@ def lib():
         "vx3h_calib_copy": (S, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
         "vx3h_batch_timing": (S, [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     }
-    optional = {
+    sig.update({
         "vx3h_vxa_parse": (S, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
         "vx3h_vxa_desc": (C.POINTER(_abi.SimDesc), [P]),
+        "vx3h_vxa_get_info": (S, [P, C.POINTER(_abi.VxaInfo), C.POINTER(C.c_char_p)]),
         "vx3h_vxa_free": (None, [P]),
         "vx3h_report_xml": (S, [C.c_char_p, C.POINTER(_abi.ReportEntry), C.c_int, C.c_char_p, C.c_size_t,
                                 C.POINTER(C.c_size_t)]),
-    }
+    })
+    optional = {}
     for name, (res, args) in list(sig.items()) + list(optional.items()):
         if not hasattr(L, name):
             if name in sig:
@@ -77,6 +79,66 @@ def device_count():
     n = C.c_int(0)
     check(lib().vx3h_device_count(C.byref(n)))
     return n.value
+
+
+class Vxa:
+    """A VXA (optionally patched by a VXD) read by the engine's loader (vx3h_vxa_parse).
+    `desc` points into memory the object owns."""
+
+    def __init__(self, vxa_text, vxd_text=None):
+        self.h = C.c_void_p()
+        check(lib().vx3h_vxa_parse(vxa_text.encode() if isinstance(vxa_text, str) else vxa_text,
+                                   None if vxd_text is None else
+                                   (vxd_text.encode() if isinstance(vxd_text, str) else vxd_text), C.byref(self.h)))
+        self.desc = lib().vx3h_vxa_desc(self.h).contents
+        self.desc._owner = self   # the description points into memory this object frees
+        info = _abi.VxaInfo()
+        raw = C.c_char_p()
+        check(lib().vx3h_vxa_get_info(self.h, C.byref(info), C.byref(raw)))
+        self.info = info
+        self.raw_print = raw.value.decode() if raw.value else ""
+
+    @property
+    def n_voxels(self):
+        return self.desc.n_voxels
+
+    def close(self):
+        if self.h:
+            lib().vx3h_vxa_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def report_xml(input_dir_name, entries):
+    """entries: list of (vxd_name, SimStatus, voxel_size, positions or None) where
+    positions = (init_pos (V,3), pos (V,3), mats (V,)) when SavePositionOfAllVoxels."""
+    arr = (_abi.ReportEntry * max(len(entries), 1))()
+    keep = []
+    for i, (name, st, vs, posi) in enumerate(entries):
+        nm = name.encode()
+        keep.append(nm)
+        arr[i].vxd_name = nm
+        s = _abi.SimStatus()
+        C.memmove(C.byref(s), C.byref(st), C.sizeof(s))
+        keep.append(s)
+        arr[i].status = C.pointer(s)
+        arr[i].voxel_size = vs
+        if posi is not None:
+            a, b, m = (np.ascontiguousarray(posi[0], np.float64), np.ascontiguousarray(posi[1], np.float64),
+                       np.ascontiguousarray(posi[2], np.int32))
+            keep += [a, b, m]
+            arr[i].save_positions = 1
+            arr[i].init_pos, arr[i].pos, arr[i].mats = _p(a, C.c_double), _p(b, C.c_double), _p(m, C.c_int)
+    n = C.c_size_t(0)
+    check(lib().vx3h_report_xml(input_dir_name.encode(), arr, len(entries), None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value)
+    check(lib().vx3h_report_xml(input_dir_name.encode(), arr, len(entries), buf, n.value, C.byref(n)))
+    return buf.value.decode()
 
 
 def calib_copy(nbytes, reps=1, device=0):
@@ -133,7 +195,11 @@ class Batch:
             pass
 
     def admit(self, model):
-        d, keep = model.desc()
+        """Admits a Model or a Vxa; returns the slot."""
+        if isinstance(model, Vxa):
+            d = model.desc
+        else:
+            d, keep = model.desc()
         slot = C.c_int(-1)
         check(lib().vx3h_sim_admit(self.h, C.byref(d), C.byref(slot)))
         self.nv[slot.value] = model.n_voxels

@@ -118,6 +118,7 @@ class Model:
     phase: Optional[np.ndarray] = None        # (V,) float64 (cycles)
     externals: List[Ext] = field(default_factory=list)
     gravity_mult: float = 0.0
+    gravity_acc: Optional[float] = None       # GravAcc the multiplier came from (VXA writer), if known
     floor_enabled: int = 0
     temp_enabled: int = 0
     vary_temp_enabled: int = 0

@@ -27,6 +27,7 @@ def gravity_mult(grav_acc: float = -9.81) -> float:
 
 def _physics(m: Model, steps: int):
     m.gravity_mult = gravity_mult(-9.81)
+    m.gravity_acc = -9.81
     m.floor_enabled = 1
     m.temp_enabled = 1
     m.vary_temp_enabled = 1

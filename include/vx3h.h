@@ -138,6 +138,19 @@ typedef struct vx3h_vxa vx3h_vxa;   /* owns a parsed description */
 vx3h_status vx3h_vxa_parse(const char* vxa_text, const char* vxd_text, vx3h_vxa** out);
 const vx3h_sim_desc* vx3h_vxa_desc(const vx3h_vxa* v);
 void        vx3h_vxa_free(vx3h_vxa* v);
+/* VX3 simulator keys the loader read that the physics description does not carry
+ * (VX3_SimulationManager.cu:326-367): features this engine does not run, history
+ * recording, and the RawPrint text the reference echoes.  raw_print may be NULL;
+ * the string stays valid until vx3h_vxa_free. */
+typedef struct vx3h_vxa_info {
+    int    enable_attach, enable_detach;   /* AttachDetach.EnableAttach / EnableDetach */
+    int    enable_cilia, enable_signals;   /* EnableCilia / EnableSignals */
+    int    secondary_experiment;           /* SecondaryExperiment */
+    int    record_step_size;               /* RecordHistory.RecordStepSize */
+    double max_dist_close_pairs;           /* MaxDistInVoxelLengthsToCountAsPair */
+    int    has_force_field;                /* a ForceField element is present */
+} vx3h_vxa_info;
+vx3h_status vx3h_vxa_get_info(const vx3h_vxa* v, vx3h_vxa_info* info, const char** raw_print);
 
 /* ---------------------------------------------------------------- batch engine */
 enum {
@@ -230,11 +243,12 @@ vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, double* avg_ms);
  * returns, since the last call, the summed duration in ms, the number of launches
  * and the number of steps those launches were asked to run; it then resets them. */
 vx3h_status vx3h_batch_set_timing(vx3h_batch* b, int enable);
+vx3h_status vx3h_batch_timing(vx3h_batch* b, double* total_ms, int* launches, long long* steps);
+
 /* Plain device-to-device copy kernel of `bytes` bytes, run reps times (no
  * counterpart in the reference): the known-traffic reference for HBM counters and
  * the bandwidth a simple streaming kernel reaches; returns the mean ms per copy. */
 vx3h_status vx3h_calib_copy(int device, size_t bytes, int reps, double* ms_per_copy);
-vx3h_status vx3h_batch_timing(vx3h_batch* b, double* total_ms, int* launches, long long* steps);
 
 /* ---------------------------------------------------------------- report */
 typedef struct vx3h_report_entry {

@@ -181,8 +181,19 @@ vxo_sim* ref_load_vxa(const char* text) {
     h->sim->ReadVXA(&XML, &msg);
     h->sim->Import(NULL, NULL, &msg);
     h->vx = &h->sim->Vx;
-    for (int i = 0; i < h->vx->materialCount(); i++) { h->measured.push_back(h->vx->material(i)->isMeasured); h->fixedm.push_back(0); h->thermalOn.push_back(0); }
+    for (int i = 0; i < h->vx->materialCount(); i++) {
+        CVX_Material* m = h->vx->material(i);
+        h->measured.push_back(m->isMeasured);
+        h->fixedm.push_back(m->fixed ? 1 : 0);
+        h->thermalOn.push_back(m->TurnOnThermalExpansionAfterThisManySeconds);
+    }
     for (int i = 0; i < h->vx->voxelCount(); i++) h->phase.push_back(h->vx->voxel(i)->phaseOffset);
+    // VX3 driver settings the kernel constructor takes from the CPU model
+    // (VX3_VoxelyzeKernel.cu:27-105): thermal variation, amplitude, period, DtFrac
+    h->vary = h->env->IsTempVaryEnabled();
+    h->TempAmplitude = h->env->GetTempAmplitude();
+    h->TempPeriod = h->env->GetTempPeriod();
+    h->DtFrac = h->sim->DtFrac;
     indexVoxels(h);
     return h;
 }

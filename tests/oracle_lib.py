@@ -108,6 +108,15 @@ class OSim:
         self.nv, self.nl = nv.value, nl.value
 
     @classmethod
+    def from_desc(cls, desc, kind="oracle"):
+        """A simulation built from a vx3h_sim_desc (kept alive by the caller)."""
+        lib = load(kind)
+        h = lib.vxo_create(C.byref(desc))
+        if not h:
+            raise ValueError(lib.vxo_last_error().decode())
+        return cls(None, kind, handle=h)
+
+    @classmethod
     def from_vxa(cls, text):
         lib = load("ref")
         h = lib.ref_load_vxa(text.encode())
