@@ -29,7 +29,6 @@ UNITS = [
     # (source, compiler, extra flags)
     ("vx3_kernels_f32.hip", "hip", []),
     ("vx3_kernels_f64.hip", "hip", ["-ffp-contract=off"]),
-    ("vx3_resident.hip", "hip", []),
     ("vx3_collide.hip", "hip", []),
     ("vx3h_engine.cpp", "hip", ["-x", "hip"]),
     ("vx3h_build.cpp", "cxx", []),
@@ -66,7 +65,8 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     worker_src = os.path.join(CSRC, "vx3_node_worker.cpp")
     if os.path.exists(worker_src):
-        _run(["g++"] + CXX_FLAGS + ["-o", WORKER, worker_src, "-L" + HERE, "-lvx3h", "-Wl,-rpath,$ORIGIN"])
+        _run(["g++"] + CXX_FLAGS + ["-o", WORKER, worker_src, os.path.join(CSRC, "vx3_manager.cpp"), "-L" + HERE, "-lvx3h",
+                                    "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print("built", LIB)
     return LIB
