@@ -14,7 +14,7 @@ DOF_TX, DOF_TY, DOF_TZ, DOF_RX, DOF_RY, DOF_RZ = 1, 2, 4, 8, 16, 32
 DOF_ALL = 63
 PATH_STREAM, PATH_RESIDENT, PATH_AUTO = range(3)
 PREC_F32, PREC_F64 = range(2)
-SIM_RUNNING, SIM_DONE, SIM_DIVERGED, SIM_EMPTY = range(4)
+SIM_RUNNING, SIM_DONE, SIM_DIVERGED, SIM_EMPTY, SIM_FAILED = range(5)
 
 
 class MaterialDesc(C.Structure):

@@ -30,11 +30,16 @@ template <typename T> struct DevBatch {
     T *a1x, *a1y, *a1z;       // previous angle1v
     T *a2x, *a2y, *a2z;       // previous angle2v
     float *strain, *maxStrain, *strainOffset;
-    // collision pairs (canonical order) and contact forces
-    int* cpair;               // 2*Ccap voxel indices (i < j)
-    float* cforce;            // 3*Ccap contact force on the first voxel
-    int* vcStart;             // per voxel: start into vcList (CSR of pairs touching the voxel)
-    int* vcList;              // signed pair references: +p+1 first voxel, -(p+1) second
+    // collisions: per voxel the watched partners in ascending index order (both
+    // directions), the order CVX_Voxel::force walks its colWatch list
+    float* lastWatch;         // 3*Vcap position at the last rebuild (CVX_Voxel::lastColWatchPosition)
+    int *nbStart, *nbCount;   // per voxel range in nbList
+    int *nearStart, *nearCount;  // per voxel range in nearList: surface voxels within the link-hop radius
+    int* nearList;            // sim-local indices, ascending
+    int* nbList;              // global voxel indices
+    int* hashCnt;             // per bucket fill count
+    int* hashEnt;             // VX3_COL_BUCKET voxel indices per bucket
+    int* hashSim;             // owning sim of each bucket, -1 if free
     // tables
     const VoxMatDev* vmatTab;
     const LinkMatDev* lmatTab;
@@ -42,7 +47,8 @@ template <typename T> struct DevBatch {
     SimCtl* sims;
     const TokenDev* tokens;
     const int* simMats;       // per sim: global voxel-material then link-material table indices it uses
-    int Vcap, Lcap, Ccap;
+    int Vcap, Lcap, Ccap;     // Ccap: nbList capacity
+    int hashTot;              // buckets in use (high-water mark)
     int vHi, lHi, nSims;      // high-water marks: kernels cover [0, vHi) / [0, lHi) / [0, nSims)
 };
 

@@ -158,36 +158,225 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     if (st.strain > 100) atomicOr(&b.sims[s].flags, SC_DIVERGED_PENDING);   // Voxelyze.cpp:265
 }
 
-// CVX_Collision::updateContactForce (VX_Collision.cpp:41-56) for every watched pair.
-__global__ void __launch_bounds__(256) k_contacts(DevBatch<real> b, int nPairs) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nPairs) return;
-    const int i = b.cpair[2 * p], j = b.cpair[2 * p + 1];
-    const int s = b.vsim[i];
-    if (!simRunning(b.sims, s) || (b.sims[s].flags & SC_DIVERGED_PENDING)) return;
-    const VoxMatDev& m1 = b.vmatTab[b.vmat[i]];
-    const VoxMatDev& m2 = b.vmatTab[b.vmat[j]];
-    const float k = 2.0f / (1.0f / m1.penetrationStiffness + 1.0f / m2.penetrationStiffness);
-    const float c = 0.5f * (m1.zetaCollision * m1.c2xSqMxExS + m2.zetaCollision * m2.c2xSqMxExS);
-    const real nom = (real)b.sims[s].voxSize;
-    const float ox = (float)(b.px[j] - b.px[i]), oy = (float)(b.py[j] - b.py[i]), oz = (float)(b.pz[j] - b.pz[i]);
-    const real b1 = nom * (real)b.tfac[i], b2 = nom * (real)b.tfac[j];
-    const float nomDist = (float)(((b1 + b1 + b1) / (real)3.0f + (b2 + b2 + b2) / (real)3.0f) * (real)0.625f);
-    const float len = sqrtf(ox * ox + oy * oy + oz * oz);
-    const float rel = nomDist - len;
-    float fx = 0, fy = 0, fz = 0;
-    if (rel > 0) {
-        float ux = ox, uy = oy, uz = oz;
-        if (len > 0) { float inv = 1.0f / len; ux = inv * ox; uy = inv * oy; uz = inv * oz; }
-        const real mi1 = (real)m1.massInverse, mi2 = (real)m2.massInverse;
-        const real v1 = (b.mx[i] * mi1) * (real)ux + (b.my[i] * mi1) * (real)uy + (b.mz[i] * mi1) * (real)uz;
-        const real v2 = (b.mx[j] * mi2) * (real)ux + (b.my[j] * mi2) * (real)uy + (b.mz[j] * mi2) * (real)uz;
-        const float relVel = (float)(v1 - v2);
-        const float mag = k * rel + c * relVel;
-        fx = mag * ux; fy = mag * uy; fz = mag * uz;
-    }
-    b.cforce[3 * p] = fx; b.cforce[3 * p + 1] = fy; b.cforce[3 * p + 2] = fz;
+// ---- collisions (CVoxelyze::updateCollisions / regenerateCollisions, Voxelyze.cpp:671-751)
+//
+// Instead of the CPU engine's all-pairs regeneration, every surface voxel of a
+// simulation being rebuilt enters a spatial hash (cell edge = watch radius) and
+// finds its partners in the 27 neighbouring cells.  The partner criterion is the
+// CPU one: both voxels on the surface, centre distance^2 <= watchRadius^2, and the
+// partner not "nearby" (within the link-hop radius, CVX_Voxel::generateNearby).
+// Each voxel keeps its partners in ascending index order -- the order in which
+// CVX_Voxel::force walks the colWatch list built by the pair loop -- and computes
+// the contact force of each pair itself, in the voxel pass.
+
+__device__ __forceinline__ bool colVoxel(const DevBatch<real>& b, int v, int& s) {
+    s = b.vsim[v];
+    if (!simRunning(b.sims, s)) return false;
+    const int f = b.sims[s].flags;
+    if (!(f & SC_COLLISIONS) || (f & SC_DIVERGED_PENDING)) return false;
+    return !(b.vflags[v] & VF_INTERIOR);
 }
+
+__device__ __forceinline__ void colCell(const DevBatch<real>& b, int v, float cell, int c[3]) {
+    c[0] = (int)floor((double)b.px[v] / (double)cell);
+    c[1] = (int)floor((double)b.py[v] / (double)cell);
+    c[2] = (int)floor((double)b.pz[v] / (double)cell);
+}
+__device__ __forceinline__ unsigned colHash(int x, int y, int z) {
+    return ((unsigned)x * 73856093u) ^ ((unsigned)y * 19349663u) ^ ((unsigned)z * 83492791u);
+}
+
+// the CPU pair criterion between two surface voxels of one simulation
+__device__ __forceinline__ bool colWatched(const DevBatch<real>& b, const SimCtl& sc, int v, int u) {
+    const real dx = b.px[v] - b.px[u], dy = b.py[v] - b.py[u], dz = b.pz[v] - b.pz[u];
+    if (dx * dx + dy * dy + dz * dz > (real)sc.colThreshSq) return false;
+    // nearby: within the link-hop radius; graph distance >= lattice L1 distance, so only
+    // lattice-close voxels can be nearby
+    const int li = abs(b.vijk[3 * v] - b.vijk[3 * u]) + abs(b.vijk[3 * v + 1] - b.vijk[3 * u + 1]) +
+                   abs(b.vijk[3 * v + 2] - b.vijk[3 * u + 2]);
+    if (li > sc.colNearHops) return true;
+    const int local = u - sc.v0;
+    int lo = b.nearStart[v], hi = lo + b.nearCount[v];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int x = b.nearList[mid];
+        if (x == local) return false;
+        if (x < local) lo = mid + 1; else hi = mid;
+    }
+    return true;
+}
+
+// decides, per simulation, whether its lists are rebuilt this step: the first step, every
+// colInterval steps, or once a surface voxel moved more than recalcDist since the last rebuild
+__global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    int s;
+    if (!colVoxel(b, v, s)) return;
+    SimCtl& sc = b.sims[s];
+    if (sc.flags & SC_REBUILD) return;
+    if (sc.colInterval > 0 && sc.steps % sc.colInterval == 0) { atomicOr(&sc.flags, SC_REBUILD); return; }
+    const real dx = b.px[v] - (real)b.lastWatch[3 * v], dy = b.py[v] - (real)b.lastWatch[3 * v + 1],
+               dz = b.pz[v] - (real)b.lastWatch[3 * v + 2];
+    if (dx * dx + dy * dy + dz * dz > (real)sc.colRecalcSq) atomicOr(&sc.flags, SC_REBUILD);
+}
+
+__global__ void __launch_bounds__(256) k_col_clear(DevBatch<real> b) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= b.hashTot) return;
+    const int s = b.hashSim[h];
+    if (s >= 0 && simRunning(b.sims, s) && (b.sims[s].flags & SC_REBUILD)) b.hashCnt[h] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_col_insert(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    int s;
+    if (!colVoxel(b, v, s)) return;
+    SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_REBUILD)) return;
+    b.lastWatch[3 * v] = (float)b.px[v];
+    b.lastWatch[3 * v + 1] = (float)b.py[v];
+    b.lastWatch[3 * v + 2] = (float)b.pz[v];
+    int c[3];
+    colCell(b, v, sc.colCell, c);
+    const int h = sc.h0 + (int)(colHash(c[0], c[1], c[2]) & ((1u << sc.hBits) - 1));
+    const int slot = atomicAdd(&b.hashCnt[h], 1);
+    if (slot < VX3_COL_BUCKET) b.hashEnt[(size_t)h * VX3_COL_BUCKET + slot] = v;
+    else atomicOr(&sc.flags, SC_COL_OVERFLOW);
+}
+
+// visits every watched partner u of surface voxel v once (any order)
+template <typename F>
+__device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const SimCtl& sc, int v, F f) {
+    int c[3];
+    colCell(b, v, sc.colCell, c);
+    const unsigned mask = (1u << sc.hBits) - 1;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
+                const int h = sc.h0 + (int)(colHash(cx, cy, cz) & mask);
+                // a bucket shared by several of the 27 cells is read once, for the first of them
+                bool seen = false;
+                for (int ez = -1; ez <= dz && !seen; ez++)
+                    for (int ey = -1; ey <= 1 && !seen; ey++)
+                        for (int ex = -1; ex <= 1 && !seen; ex++) {
+                            if (ez == dz && (ey > dy || (ey == dy && ex >= dx))) continue;
+                            if (sc.h0 + (int)(colHash(c[0] + ex, c[1] + ey, c[2] + ez) & mask) == h) seen = true;
+                        }
+                if (seen) continue;
+                const int n = min(b.hashCnt[h], VX3_COL_BUCKET);
+                for (int k = 0; k < n; k++) {
+                    const int u = b.hashEnt[(size_t)h * VX3_COL_BUCKET + k];
+                    if (u != v && colWatched(b, sc, v, u)) f(u);
+                }
+            }
+}
+
+__global__ void __launch_bounds__(256) k_col_count(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    int s;
+    if (!colVoxel(b, v, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_REBUILD)) return;
+    int n = 0;
+    colForPartners(b, sc, v, [&](int) { n++; });
+    b.nbCount[v] = n;
+}
+
+// per rebuilt simulation: exclusive scan of the partner counts over its voxels
+__global__ void __launch_bounds__(1024) k_col_scan(DevBatch<real> b) {
+    __shared__ int part[1024 / 64];
+    __shared__ int carry;
+    const int s = blockIdx.x;
+    if (s >= b.nSims) return;
+    SimCtl& sc = b.sims[s];
+    if (sc.state != 0 || (sc.flags & SC_RESIDENT) || !(sc.flags & SC_COLLISIONS) || !(sc.flags & SC_REBUILD) ||
+        (sc.flags & SC_DIVERGED_PENDING))
+        return;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int base = 0; base < sc.nv; base += 1024) {
+        const int v = sc.v0 + base + threadIdx.x;
+        const bool in = base + (int)threadIdx.x < sc.nv;
+        const int x = in && !(b.vflags[v] & VF_INTERIOR) ? b.nbCount[v] : 0;
+        int incl = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) part[wid] = incl;
+        __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wid; w++) before += part[w];
+        if (in) b.nbStart[v] = sc.nb0 + before + incl - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (carry > sc.nbCap) atomicOr(&sc.flags, SC_COL_OVERFLOW);
+        sc.rebuilds++;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_col_emit(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    int s;
+    if (!colVoxel(b, v, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_REBUILD) || (sc.flags & SC_COL_OVERFLOW)) return;
+    int* out = b.nbList + b.nbStart[v];
+    int n = 0;
+    colForPartners(b, sc, v, [&](int u) { out[n++] = u; });
+    for (int i = 1; i < n; i++) {   // ascending partner order
+        const int x = out[i];
+        int j = i - 1;
+        while (j >= 0 && out[j] > x) { out[j + 1] = out[j]; j--; }
+        out[j + 1] = x;
+    }
+}
+
+// contact force on voxel a from its pair with c, from the pair's first voxel
+// min(a, c): CVX_Collision::updateContactForce (VX_Collision.cpp:41-56) and
+// contactForce(pVoxel) (VX_Collision.h:44)
+struct Contacts {
+    const DevBatch<real>* b;
+    int v, start, count;
+    real nom;
+    template <typename T> __device__ __forceinline__ void subtract(V3<T>& force) const {
+        for (int k = 0; k < count; k++) {
+            const int u = b->nbList[start + k];
+            const int i = v < u ? v : u, j = v < u ? u : v;
+            const VoxMatDev& m1 = b->vmatTab[b->vmat[i]];
+            const VoxMatDev& m2 = b->vmatTab[b->vmat[j]];
+            const float kp = 2.0f / (1.0f / m1.penetrationStiffness + 1.0f / m2.penetrationStiffness);
+            const float cd = 0.5f * (m1.zetaCollision * m1.c2xSqMxExS + m2.zetaCollision * m2.c2xSqMxExS);
+            const float ox = (float)(b->px[j] - b->px[i]), oy = (float)(b->py[j] - b->py[i]), oz = (float)(b->pz[j] - b->pz[i]);
+            const real b1 = nom * (real)b->tfac[i], b2 = nom * (real)b->tfac[j];
+            const float nomDist = (float)(((b1 + b1 + b1) / (real)3.0f + (b2 + b2 + b2) / (real)3.0f) * (real)0.625f);
+            const float len = sqrtf(ox * ox + oy * oy + oz * oz);
+            const float rel = nomDist - len;
+            float fx = 0, fy = 0, fz = 0;
+            if (rel > 0) {
+                float ux = ox, uy = oy, uz = oz;
+                if (len > 0) { const float inv = 1.0f / len; ux = inv * ox; uy = inv * oy; uz = inv * oz; }
+                const real mi1 = (real)m1.massInverse, mi2 = (real)m2.massInverse;
+                const real v1 = (b->mx[i] * mi1) * (real)ux + (b->my[i] * mi1) * (real)uy + (b->mz[i] * mi1) * (real)uz;
+                const real v2 = (b->mx[j] * mi2) * (real)ux + (b->my[j] * mi2) * (real)uy + (b->mz[j] * mi2) * (real)uz;
+                const float relVel = (float)(v1 - v2);
+                const float mag = kp * rel + cd * relVel;
+                fx = ux * mag; fy = uy * mag; fz = uz * mag;
+            }
+            if (v != i) { fx = -fx; fy = -fy; fz = -fz; }
+            force.x -= (T)fx; force.y -= (T)fy; force.z -= (T)fz;
+        }
+    }
+};
 
 __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -195,7 +384,7 @@ __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
     const int s = b.vsim[v];
     if (!simRunning(b.sims, s)) return;
     const SimCtl& sc = b.sims[s];
-    if (sc.flags & SC_DIVERGED_PENDING) return;   // doTimeStep returns before the voxel loop
+    if (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW)) return;   // doTimeStep returns before the voxel loop
     const VoxMatDev& m = b.vmatTab[b.vmat[v]];
     const float tf = b.tfac[v];
     if (!m.fixed) {
@@ -214,11 +403,14 @@ __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
         const ExtDev* ext = e >= 0 ? &b.extTab[e] : nullptr;
         int ix = 0, iy = 0, iz = 0;
         if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
-        int cb = 0, ce = 0;
-        if ((sc.flags & SC_COLLISIONS) && b.vcStart) { cb = b.vcStart[v]; ce = b.vcStart[v + 1]; }
         V3<real> pos(b.px[v], b.py[v], b.pz[v]), lm(b.mx[v], b.my[v], b.mz[v]), am(b.ax[v], b.ay[v], b.az[v]);
         Q4<real> o(b.ow[v], b.ox[v], b.oy[v], b.oz[v]);
-        voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, b.vcList, cb, ce, b.cforce, pos, lm, o, am, fl);
+        if ((sc.flags & SC_COLLISIONS) && !(fl & VF_INTERIOR)) {
+            Contacts con{&b, v, b.nbStart[v], b.nbCount[v], (real)sc.voxSize};
+            voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, con, pos, lm, o, am, fl);
+        } else {
+            voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, NoContacts(), pos, lm, o, am, fl);
+        }
         b.px[v] = pos.x; b.py[v] = pos.y; b.pz[v] = pos.z;
         b.mx[v] = lm.x; b.my[v] = lm.y; b.mz[v] = lm.z;
         b.ow[v] = o.w; b.ox[v] = o.x; b.oy[v] = o.y; b.oz[v] = o.z;
@@ -337,6 +529,11 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     if (sample) blockCoM(b, sc, com);
     if (threadIdx.x != 0) return;
     sc.steps = steps;
+    sc.flags &= ~SC_REBUILD;
+    if (sc.flags & SC_COL_OVERFLOW) {
+        sc.state = 4;
+        return;
+    }
     if (diverged) {
         sc.state = 2;
         return;

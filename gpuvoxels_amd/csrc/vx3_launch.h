@@ -7,8 +7,7 @@
 namespace vx3 {
 struct StepOpts {
     bool poisson;      // any resident sim has a non-zero Poisson's ratio
-    bool contacts;     // run the contact-force pass
-    int nPairs;        // collision pairs in cpair
+    bool contacts;     // some simulation has collisions: run the collision passes
     int block;         // workgroup size of the element passes
 };
 // Kernel passes of one step, in launch order; ev (optional) receives a start/stop

@@ -252,16 +252,19 @@ __device__ __forceinline__ void linkForces(const LinkMatDev& mat, int axis, cons
     mPos = toAxisOriginal(axis, mPos);
 }
 
+// No collision contacts (the resident path, or a simulation without collisions).
+struct NoContacts {
+    template <typename T> __device__ __forceinline__ void subtract(V3<T>&) const {}
+};
+
 // Voxel translation + rotation update: CVX_Voxel::timeStep after force() and
-// moment() have summed the link loads (fLocal/mLocal, voxel frame).  Contact
-// forces of the collision pairs touching this voxel are subtracted one by one in
-// pair order, as CVX_Voxel::force walks its colWatch list (VX_Voxel.cpp:245-249):
-// clist[cb..ce) holds +(p+1) when the voxel is the pair's first voxel, -(p+1) otherwise.
-template <typename T>
+// moment() have summed the link loads (fLocal/mLocal, voxel frame).  contacts
+// subtracts the contact force of each watched pair from the global force, one by
+// one in the voxel's colWatch order (VX_Voxel.cpp:245-249).
+template <typename T, typename Contacts>
 __device__ __forceinline__ void voxelStep(const VoxMatDev& m, const ExtDev* ext, float dt, T nomSize, int ix, int iy, int iz,
-                                          float tfac, const V3<T>& fLocal, const V3<T>& mLocal, const int* clist, int cb,
-                                          int ce, const float* cforce, V3<T>& pos, V3<T>& linMom, Q4<T>& orient,
-                                          V3<T>& angMom, int& flags) {
+                                          float tfac, const V3<T>& fLocal, const V3<T>& mLocal, const Contacts& contacts,
+                                          V3<T>& pos, V3<T>& linMom, Q4<T>& orient, V3<T>& angMom, int& flags) {
     if (ext && (ext->dofFixed & 0x3F) == 0x3F) {
         pos = V3<T>((T)(ix * (double)nomSize + ext->translate[0]), (T)(iy * (double)nomSize + ext->translate[1]),
                     (T)(iz * (double)nomSize + ext->translate[2]));
@@ -274,13 +277,7 @@ __device__ __forceinline__ void voxelStep(const VoxMatDev& m, const ExtDev* ext,
     if (ext) curForce.add(V3<T>((T)ext->force[0], (T)ext->force[1], (T)ext->force[2]));
     curForce.sub(linMom.mul((T)m.massInverse).mul((T)(m.zetaGlobal * m.c2xSqMxExS)));
     curForce.z += (T)m.gravityForce;
-    for (int c = cb; c < ce; c++) {
-        int r = clist[c];
-        int p = (r > 0 ? r : -r) - 1;
-        float fx = cforce[3 * p], fy = cforce[3 * p + 1], fz = cforce[3 * p + 2];
-        if (r < 0) { fx = -fx; fy = -fy; fz = -fz; }
-        curForce.x -= (T)fx; curForce.y -= (T)fy; curForce.z -= (T)fz;
-    }
+    contacts.subtract(curForce);
     V3<T> fricForce = curForce;
     const bool floorOn = (flags & VF_FLOOR) != 0;
     // floorPenetration (VX_Voxel.h:124): baseSizeAverage()/2 - nomSize/2 - pos.z

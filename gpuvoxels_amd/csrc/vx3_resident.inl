@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     const ExtDev* ext = vext[k] >= 0 ? &b.extTab[vext[k]] : nullptr;
                     int ix = 0, iy = 0, iz = 0;
                     if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
-                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], nullptr, 0, 0, nullptr, vp[k], vm[k], vo[k], va[k], vf[k]);
+                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], NoContacts(), vp[k], vm[k], vo[k], va[k], vf[k]);
                     sPx[v] = vp[k].x; sPy[v] = vp[k].y; sPz[v] = vp[k].z;
                     sOw[v] = vo[k].w; sOx[v] = vo[k].x; sOy[v] = vo[k].y; sOz[v] = vo[k].z;
                 }

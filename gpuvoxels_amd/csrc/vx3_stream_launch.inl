@@ -1,0 +1,40 @@
+// vx3_stream_launch.inl -- host launchers of one step of the stream path and of the
+// recommended-dt pass, for the precision of the including unit (VX_REAL, VX_NS).
+
+namespace vx3 {
+
+namespace {
+void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs) {
+    const int gv = (b.vHi + bs - 1) / bs;
+    hipLaunchKernelGGL(VX_NS::k_col_stale, dim3(gv), dim3(bs), 0, s, b);
+    if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_clear, dim3((b.hashTot + 255) / 256), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_insert, dim3(gv), dim3(bs), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_count, dim3(gv), dim3(bs), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_scan, dim3(b.nSims), dim3(1024), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_emit, dim3(gv), dim3(bs), 0, s, b);
+}
+}  // namespace
+
+void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev) {
+    const int bs = o.block > 0 ? o.block : 256;
+    if (b.vHi <= 0) return;
+    const int gv = (b.vHi + bs - 1) / bs, gl = (b.lHi + bs - 1) / bs;
+#define VX3_EV(k, launch)                                   \
+    do {                                                    \
+        if (ev) hipEventRecord(ev[2 * (k)], s);             \
+        launch;                                             \
+        if (ev) hipEventRecord(ev[2 * (k) + 1], s);         \
+    } while (0)
+    if (o.poisson) VX3_EV(PASS_POISSON, hipLaunchKernelGGL(VX_NS::k_poisson, dim3(gv), dim3(bs), 0, s, b));
+    if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
+    if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs));
+    VX3_EV(PASS_VOXELS, hipLaunchKernelGGL(VX_NS::k_voxels, dim3(gv), dim3(bs), 0, s, b));
+    VX3_EV(PASS_CONTROL, hipLaunchKernelGGL(VX_NS::k_control, dim3(b.nSims), dim3(256), 0, s, b));
+#undef VX3_EV
+}
+
+void launchDt(const DevBatch<VX_REAL>& b, hipStream_t s, int s0, int s1) {
+    if (s1 > s0) hipLaunchKernelGGL(VX_NS::k_dt, dim3(s1 - s0), dim3(256), 0, s, b, s0, s1);
+}
+
+}  // namespace vx3

@@ -30,7 +30,10 @@ enum {
 // link state bits
 enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2 };
 // per-sim control bits
-enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32, SC_RESIDENT = 64 };
+enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32, SC_RESIDENT = 64,
+       SC_REBUILD = 128,        // collision neighbour lists are rebuilt this step
+       SC_COL_OVERFLOW = 256 }; // a collision table overflowed: the simulation fails
+#define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
     float mass, massInverse, sqrtMass, momentInertiaInverse;
@@ -87,6 +90,15 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int nColl;                       // collision pairs owned (pair range below)
     int c0;
     int mat0, nvm, nlm;              // the sim's materials: simMats[mat0 .. +nvm) voxel, then nlm link table indices
+    // collisions (CVoxelyze::updateCollisions, Voxelyze.cpp:671-711)
+    float colThreshSq;               // watchRadiusMm^2 (pairs closer than this are watched)
+    float colRecalcSq;               // recalcDist^2 (a surface voxel moving further triggers a rebuild)
+    float colCell;                   // spatial-hash cell edge = watchRadiusMm
+    int colInterval;                 // > 0: also rebuild every colInterval steps
+    int colNearHops;                 // link-hop radius of the excluded "nearby" voxels (int(2*watchRadiusVx))
+    int nb0, nbCap;                  // neighbour-list range in nbList
+    int h0, hBits;                   // hash-table range: (1 << hBits) buckets from bucket h0
+    int rebuilds;                    // neighbour-list rebuilds so far
 };
 
 struct TokenDev { int op; int pad; double value; };

@@ -109,6 +109,15 @@ struct SlotHost {
     double com0[3] = {0, 0, 0};
     std::vector<vx3h_token> fit;
     size_t bytes = 0;
+    // collision pools: neighbour lists, hash buckets, nearby lists
+    int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0;
+};
+
+// Device arrays sharing one index space of first-fit ranges (collision pools).
+struct DevPool {
+    std::vector<std::pair<void**, size_t>> arrs;   // slot, bytes per element
+    int cap = 0;
+    RangeAlloc ra;
 };
 
 template <typename K> struct Dedup {
@@ -147,6 +156,7 @@ struct vx3h_batch {
     void* dExt = nullptr; size_t capExt = 0;
     void* dTok = nullptr; size_t capTok = 0;
     void* dSimMats = nullptr; size_t capSimMats = 0;
+    DevPool nbPool, hashPool, nearPool;
     bool anyPoisson = false, anyColl = false;
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
@@ -188,6 +198,11 @@ void initSpecs(vx3h_batch* b) {
     V(&d.vmat, 4, 1, false); V(&d.vsim, 4, 1, false); V(&d.vext, 4, 1, false); V(&d.vijk, 4, 3, false);
     V(&d.psx, 4, 1, false); V(&d.psy, 4, 1, false); V(&d.psz, 4, 1, false);
     V(&d.F, R, 36, true); V(&d.es, 4, 6, true);
+    V(&d.lastWatch, 4, 3, false); V(&d.nbStart, 4, 1, false); V(&d.nbCount, 4, 1, false);
+    V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
+    b->nbPool.arrs = {{(void**)&d.nbList, 4}};
+    b->hashPool.arrs = {{(void**)&d.hashCnt, 4}, {(void**)&d.hashEnt, 4 * VX3_COL_BUCKET}, {(void**)&d.hashSim, 4}};
+    b->nearPool.arrs = {{(void**)&d.nearList, 4}};
     L(&d.lneg, 4, 1, false); L(&d.lpos, 4, 1, false); L(&d.lmeta, 4, 1, false); L(&d.lsim, 4, 1, false); L(&d.lflags, 4, 1, false);
     L(&d.q2x, R, 1, false); L(&d.q2y, R, 1, false); L(&d.q2z, R, 1, false);
     L(&d.a1x, R, 1, false); L(&d.a1y, R, 1, false); L(&d.a1z, R, 1, false);
@@ -220,6 +235,25 @@ vx3h_status growArrays(vx3h_batch* b, std::vector<ArrSpec>& arrs, int& cap, int 
     if (ncap > used) HIPCHK(hipMemsetAsync(owner + used, 0xFF, (size_t)(ncap - used) * 4, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     cap = ncap;
+    b->layout++;
+    return VX3H_OK;
+}
+
+vx3h_status poolAlloc(vx3h_batch* b, DevPool& p, int n, int& start) {
+    bool grow = false;
+    start = p.ra.alloc(n, p.cap, grow);
+    if (!grow) return VX3H_OK;
+    const int ncap = std::max(p.ra.hi, std::max(2 * p.cap, 4096));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto& a : p.arrs) {
+        void* np = nullptr;
+        HIPCHK(hipMalloc(&np, (size_t)ncap * a.second));
+        HIPCHK(hipMemset(np, 0xFF, (size_t)ncap * a.second));   // hashSim: no owner
+        if (*a.first && p.cap) HIPCHK(hipMemcpy(np, *a.first, (size_t)p.cap * a.second, hipMemcpyDeviceToDevice));
+        if (*a.first) HIPCHK(hipFree(*a.first));
+        *a.first = np;
+    }
+    p.cap = ncap;
     b->layout++;
     return VX3H_OK;
 }
@@ -271,8 +305,7 @@ char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx;
 vx3h_status launchSteps(vx3h_batch* b, int n) {
     StepOpts o;
     o.poisson = b->anyPoisson;
-    o.contacts = false;
-    o.nPairs = 0;
+    o.contacts = b->anyColl;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
@@ -395,6 +428,8 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
+    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool})
+        for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
     for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
     for (hipEvent_t e : b->evPool) hipEventDestroy(e);
     hipStreamDestroy(b->stream);
@@ -416,7 +451,6 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     HIPCHK(hipSetDevice(b->device));
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
-    if (H.collisions) FAILV(VX3H_EUNSUPPORTED, "collisions are not enabled in this build of the stream path");
     if (H.stopTok.size() > 64) FAILV(VX3H_EUNSUPPORTED, "stop condition longer than 64 tokens");
     const size_t rs = b->realSize;
     const size_t need = (size_t)H.nv * voxelBytes(rs) + (size_t)H.nl * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
@@ -557,6 +591,73 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     else if (!H.stopTok.empty() &&
              evalMathTree(H.stopTok.data(), (int)H.stopTok.size(), c.com[0], c.com[1], c.com[2], 0, 0.0, 0, 0, 0, NV) > 0)
         c.state = VX3H_SIM_DONE;
+    // collisions: thresholds of CVoxelyze::updateCollisions (Voxelyze.cpp:673-676, float arithmetic),
+    // nearby lists of CVX_Voxel::generateNearby (Voxelyze.cpp:394-417) and device pool ranges
+    int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0;
+    if (H.collisions && NV > 0) {
+        const float br = (float)H.boundingRadius, wd = (float)H.watchDistance;
+        const float wrv = 2 * br + wd;
+        const float wrMm = (float)(H.voxSize * wrv);
+        const float recalc = (float)(H.voxSize * wd / 2);
+        c.colThreshSq = wrMm * wrMm;
+        c.colRecalcSq = recalc * recalc;
+        c.colCell = wrMm;
+        c.colInterval = b->cfg.collision_rebuild_interval;
+        c.colNearHops = (int)(wrv * 2);
+        const int hops = c.colNearHops;
+        std::vector<int> nearStart(NV, 0), nearCount(NV, 0), nearList, dist(NV, -1), q, tmp;
+        int nSurf = 0;
+        for (int v = 0; v < NV; v++) {
+            if (H.vflags[v] & VF_INTERIOR) continue;
+            nSurf++;
+            q.assign(1, v);
+            dist[v] = 0;
+            for (size_t qi = 0; qi < q.size(); qi++) {
+                const int u = q[qi];
+                if (dist[u] >= hops) continue;
+                for (int d = 0; d < 6; d++) {
+                    const int l = H.voxelLinks[6 * u + d];
+                    if (l < 0) continue;
+                    const int w = H.lneg[l] == u ? H.lpos[l] : H.lneg[l];
+                    if (dist[w] < 0) { dist[w] = dist[u] + 1; q.push_back(w); }
+                }
+            }
+            tmp.clear();
+            for (int u : q) {
+                if (u != v && !(H.vflags[u] & VF_INTERIOR)) tmp.push_back(u);
+                dist[u] = -1;
+            }
+            std::sort(tmp.begin(), tmp.end());
+            nearStart[v] = (int)nearList.size();
+            nearCount[v] = (int)tmp.size();
+            nearList.insert(nearList.end(), tmp.begin(), tmp.end());
+        }
+        nbN = std::max(64, 64 * nSurf);
+        int hb = 6;
+        while ((1 << hb) < 2 * nSurf) hb++;
+        hN = 1 << hb;
+        nearN = std::max(1, (int)nearList.size());
+        vx3h_status st;
+        if ((st = poolAlloc(b, b->nbPool, nbN, nb0)) || (st = poolAlloc(b, b->hashPool, hN, h0)) ||
+            (st = poolAlloc(b, b->nearPool, nearN, near0)))
+            return st;
+        for (int& x : nearStart) x += near0;
+        if (!nearList.empty()) HIPCHK(hipMemcpy(b->db.nearList + near0, nearList.data(), nearList.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->db.nearStart + v0, nearStart.data(), NV * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->db.nearCount + v0, nearCount.data(), NV * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(b->db.nbCount + v0, 0, NV * 4));
+        HIPCHK(hipMemset(b->db.nbStart + v0, 0, NV * 4));
+        std::vector<int> own(hN, s);
+        HIPCHK(hipMemcpy(b->db.hashSim + h0, own.data(), hN * 4, hipMemcpyHostToDevice));
+        c.nb0 = nb0;
+        c.nbCap = nbN;
+        c.h0 = h0;
+        c.hBits = hb;
+        c.flags |= SC_REBUILD;
+        b->anyColl = true;
+        b->db.Ccap = b->nbPool.cap;
+        b->db.hashTot = b->hashPool.ra.hi;
+    }
     HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(c), hipMemcpyHostToDevice));
 
     SlotHost& sh = b->slots[s];
@@ -569,6 +670,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.fit = H.fitTok;
     sh.bytes = need;
     sh.shape = shape;
+    sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN;
     if (shape >= 0) { b->resLists[shape].push_back(s); b->resDirty = true; }
     else b->nStreamSims++;
     sh.matid.resize(NV);
@@ -629,8 +731,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     StepOpts o;
     o.poisson = b->anyPoisson;
-    o.contacts = false;
-    o.nPairs = 0;
+    o.contacts = b->anyColl;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];
@@ -639,7 +740,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
-    const bool ran[PASS_COUNT] = {o.poisson, b->db.lHi > 0, o.contacts && o.nPairs > 0, true, true};
+    const bool ran[PASS_COUNT] = {o.poisson, b->db.lHi > 0, o.contacts, true, true};
     for (int k = 0; k < PASS_COUNT; k++) {
         if (!ran[k] || b->nStreamSims == 0) continue;
         double tot = 0;
@@ -813,6 +914,13 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
     HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
     b->va.release(sh.v0, sh.nv);
     b->la.release(sh.l0, sh.nl);
+    if (sh.hN) {
+        HIPCHK(hipMemset(b->db.hashSim + sh.h0, 0xFF, (size_t)sh.hN * 4));
+        b->hashPool.ra.release(sh.h0, sh.hN);
+        b->nbPool.ra.release(sh.nb0, sh.nbN);
+        b->nearPool.ra.release(sh.near0, sh.nearN);
+        b->db.hashTot = b->hashPool.ra.hi;
+    }
     if (sh.shape >= 0) {
         auto& L = b->resLists[sh.shape];
         L.erase(std::remove(L.begin(), L.end(), slot), L.end());
@@ -823,6 +931,36 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
     b->db.vHi = b->va.hi;
     b->db.lHi = b->la.hi;
     b->layout++;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_sim_collision_pairs(vx3h_batch* b, int slot, int* pairs, int cap, int* n) {
+    if (!b || !n || cap < 0) FAILV(VX3H_EINVAL, "bad argument");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const SlotHost& sh = b->slots[slot];
+    *n = 0;
+    if (!sh.nbN || !sh.nv) return VX3H_OK;
+    std::vector<int> st(sh.nv), cnt(sh.nv), fl(sh.nv);
+    HIPCHK(hipMemcpy(st.data(), b->db.nbStart + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), b->db.nbCount + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fl.data(), b->db.vflags + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
+    std::vector<int> list(sh.nbN);
+    HIPCHK(hipMemcpy(list.data(), b->db.nbList + sh.nb0, (size_t)sh.nbN * 4, hipMemcpyDeviceToHost));
+    int k = 0;
+    for (int v = 0; v < sh.nv; v++) {
+        if (fl[v] & VF_INTERIOR) continue;
+        for (int i = 0; i < cnt[v]; i++) {
+            const int idx = st[v] - sh.nb0 + i;
+            if (idx < 0 || idx >= sh.nbN) FAILV(VX3H_EDEVICE, "collision list out of range");
+            const int u = list[idx] - sh.v0;
+            if (u <= v) continue;
+            if (pairs && k < cap) { pairs[2 * k] = v; pairs[2 * k + 1] = u; }
+            k++;
+        }
+    }
+    *n = k;
     return VX3H_OK;
 }
 

@@ -47,6 +47,7 @@ def lib():
         "vx3h_batch_kernel_times": (S, [P, C.c_int, C.POINTER(C.c_double)]),
         "vx3h_batch_set_timing": (S, [P, C.c_int]),
         "vx3h_calib_copy": (S, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+        "vx3h_sim_collision_pairs": (S, [P, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
         "vx3h_batch_timing": (S, [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     }
     sig.update({
@@ -235,6 +236,14 @@ class Batch:
         ms, n, st = C.c_double(0), C.c_int(0), C.c_longlong(0)
         check(lib().vx3h_batch_timing(self.h, C.byref(ms), C.byref(n), C.byref(st)))
         return ms.value, n.value, st.value
+
+    def collision_pairs(self, slot):
+        """(n, 2) sim-local voxel pairs (i < j) the simulation currently watches, ascending."""
+        n = C.c_int(0)
+        check(lib().vx3h_sim_collision_pairs(self.h, slot, None, 0, C.byref(n)))
+        out = np.zeros(2 * max(n.value, 1), np.int32)
+        check(lib().vx3h_sim_collision_pairs(self.h, slot, _p(out, C.c_int), n.value, C.byref(n)))
+        return out[: 2 * n.value].reshape(-1, 2)
 
     def retire(self, slot):
         check(lib().vx3h_sim_retire(self.h, slot))

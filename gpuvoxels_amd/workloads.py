@@ -116,3 +116,21 @@ def slab(nx: int = 64, ny: int = 64, nz: int = 32, steps: int = 5000, voxel_size
     _physics(m, steps)
     m.collisions_enabled = 1 if collisions else 0
     return m
+
+
+def drop(n: int = 3, gap: int = 1, shift: int = 1, steps: int = 2000, voxel_size: float = 0.01) -> Model:
+    """Two detached n^3 cubes in one simulation: the upper one, `gap` voxels above and
+    `shift` voxels across, falls onto the lower one (voxel-voxel contacts)."""
+    m = Model(voxel_size=voxel_size, name="drop")
+    m.materials = [base_material(cte=0.0)]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    lower = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    upper = lower + np.array([shift, shift, n + gap])
+    cells = np.concatenate([lower, upper])
+    order = np.lexsort((cells[:, 0], cells[:, 1], cells[:, 2]))    # structure-index order
+    m.ijk = cells[order].astype(np.int32)
+    m.mat = np.zeros(len(cells), np.int32)
+    _physics(m, steps)
+    m.vary_temp_enabled = 0
+    m.collisions_enabled = 1
+    return m

@@ -172,7 +172,8 @@ typedef struct vx3h_batch_cfg {
 
 typedef struct vx3h_batch vx3h_batch;
 
-enum { VX3H_SIM_RUNNING = 0, VX3H_SIM_DONE = 1, VX3H_SIM_DIVERGED = 2, VX3H_SIM_EMPTY = 3 };
+enum { VX3H_SIM_RUNNING = 0, VX3H_SIM_DONE = 1, VX3H_SIM_DIVERGED = 2, VX3H_SIM_EMPTY = 3,
+       VX3H_SIM_FAILED = 4   /* a collision table overflowed its capacity; the results are not valid */ };
 
 typedef struct vx3h_sim_status {
     int    slot;
@@ -249,6 +250,12 @@ vx3h_status vx3h_batch_timing(vx3h_batch* b, double* total_ms, int* launches, lo
  * counterpart in the reference): the known-traffic reference for HBM counters and
  * the bandwidth a simple streaming kernel reaches; returns the mean ms per copy. */
 vx3h_status vx3h_calib_copy(int device, size_t bytes, int reps, double* ms_per_copy);
+
+/* Collision pairs a simulation currently watches, as sim-local voxel index pairs
+ * (i < j) in ascending order -- CVoxelyze's collision list after its last
+ * regeneration (Voxelyze3/src/old/Voxelyze.cpp:726-751).  pairs holds 2*cap ints and
+ * may be NULL to query *n. */
+vx3h_status vx3h_sim_collision_pairs(vx3h_batch* b, int slot, int* pairs, int cap, int* n);
 
 /* ---------------------------------------------------------------- report */
 typedef struct vx3h_report_entry {
