@@ -150,6 +150,50 @@ def gpu_fitness(models, precision, steps):
     return out
 
 
+def serving_run(precision, seconds, rate, seed=4):
+    """Poisson arrivals of robots whose voxel counts are log-uniform in [100, 50000]
+    (the 12^3 robot generator on scaled grids) and horizons uniform in [2000, 20000]
+    steps, served by the continuous-batching loop; per-request latency is submit to
+    result.  Returns (summary dict, request models)."""
+    import numpy as np
+    from gpuvoxels_amd import engine, workloads
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    while t < seconds:
+        t += rng.exponential(1.0 / rate)
+        target = float(np.exp(rng.uniform(np.log(100), np.log(50000))))
+        grid = max(5, int(round((target / 0.475) ** (1.0 / 3.0))))
+        m = workloads.robot(rng, grid=grid, steps=0, name="req")
+        reqs.append((t, m, int(rng.integers(2000, 20001))))
+    sv = engine.Server(precision=precision, quantum_steps=500)
+    lat, vsteps, i, tickets = [], 0.0, 0, {}
+    t0 = sv.clock()
+    first = last = None
+    while True:
+        now = sv.clock() - t0
+        while i < len(reqs) and reqs[i][0] <= now:
+            tickets[sv.submit(reqs[i][1], reqs[i][2])] = i
+            i += 1
+        active = sv.round()
+        for r in sv.results():
+            lat.append(r.t_done - r.t_submit)
+            vsteps += r.voxel_steps
+            first = r.t_submit if first is None else min(first, r.t_submit)
+            last = r.t_done if last is None else max(last, r.t_done)
+        if i == len(reqs) and active == 0:
+            break
+        if active == 0 and i < len(reqs):
+            time.sleep(max(0.0, min(0.01, reqs[i][0] - (sv.clock() - t0))))
+    sv.close()
+    lat = np.array(lat)
+    span = last - first
+    return {"requests": len(reqs), "rate_per_s": rate, "arrival_seconds": seconds,
+            "voxels_min_max": [int(min(m.n_voxels for _, m, _ in reqs)), int(max(m.n_voxels for _, m, _ in reqs))],
+            "value": vsteps / span, "unit": "voxel-steps/s", "seconds": span,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "dtype": "f64" if precision == 1 else "f32"}, reqs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +212,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=6.0, help="arrival window of the serving run (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=40.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (two launches: warm-up + timed), a kernel of "
                          "known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -257,6 +303,15 @@ def main():
         out["plain_path"] = {"value": pv, "unit": "voxel-steps/s", "steps": args.plain_steps, "robots": len(models),
                              "dtype": args.precision, "batched_speedup": value / pv}
         log("plain path %.3e voxel-steps/s (%.2f s) -> batched speedup %.1fx" % (pv, pdt, value / pv))
+    if args.serving_seconds > 0:
+        srv, reqs = serving_run(prec, args.serving_seconds, args.serving_rate)
+        # the plain path on the same requests: each alone, one step per launch sequence
+        sub = [m for _, m, _ in reqs[:8]]
+        pv, _ = plain_path(sub, prec, 20)
+        srv["plain_path"] = {"value": pv, "unit": "voxel-steps/s", "requests": len(sub), "steps": 20}
+        srv["speedup_vs_plain"] = srv["value"] / pv
+        out["serving"] = srv
+        log("serving: %s" % srv)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import ref_available

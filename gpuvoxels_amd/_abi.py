@@ -166,3 +166,12 @@ class VxaInfo(C.Structure):
         ("max_dist_close_pairs", C.c_double),
         ("has_force_field", C.c_int),
     ]
+
+
+class ServerCfg(C.Structure):
+    _fields_ = [("batch", BatchCfg), ("quantum_steps", C.c_int), ("max_sims", C.c_int)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("ticket", C.c_longlong), ("status", SimStatus), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("voxel_steps", C.c_double)]

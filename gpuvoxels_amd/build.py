@@ -33,6 +33,7 @@ UNITS = [
     ("vx3h_engine.cpp", "hip", ["-x", "hip"]),
     ("vx3h_build.cpp", "cxx", []),
     ("vx3h_vxa.cpp", "cxx", []),
+    ("vx3h_server.cpp", "cxx", []),
 ]
 
 

@@ -1,0 +1,198 @@
+// vx3h_server.cpp -- continuous-batching request loop over one device batch.
+//
+// The reference runs a fixed set of VXD files per process: it reads them all, starts
+// one device thread per simulation and waits for the slowest
+// (VX3_SimulationManager.cu:138-156; vx3_node_worker.cu:63-96).  Here requests
+// arrive at any time: each is deep-copied into a queue, admitted in arrival order
+// once its footprint (vx3h_sim_footprint) fits the budget left by the running
+// ones, stepped with everything else on the device, and retired when it finishes,
+// freeing its ranges for the next admissions.  Admission, retirement and result
+// reads happen between rounds of quantum_steps steps.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vx3h.h"
+#include "vx3h_internal.h"
+
+using vx3::g_err;
+
+namespace {
+
+// an owned copy of a vx3h_sim_desc and every array it points to
+struct DescCopy {
+    vx3h_sim_desc d;
+    std::vector<vx3h_material_desc> mats;
+    std::vector<int> ijk, mat;
+    std::vector<double> phase;
+    std::vector<vx3h_external> exts;
+    std::vector<vx3h_token> stop, fit;
+    void take(const vx3h_sim_desc& s) {
+        d = s;
+        mats.assign(s.materials, s.materials + s.n_materials);
+        ijk.assign(s.ijk, s.ijk + 3 * (size_t)s.n_voxels);
+        mat.assign(s.material, s.material + s.n_voxels);
+        if (s.phase) phase.assign(s.phase, s.phase + s.n_voxels);
+        if (s.n_externals) exts.assign(s.externals, s.externals + s.n_externals);
+        if (s.n_stop_tokens) stop.assign(s.stop_tokens, s.stop_tokens + s.n_stop_tokens);
+        if (s.n_fitness_tokens) fit.assign(s.fitness_tokens, s.fitness_tokens + s.n_fitness_tokens);
+        d.materials = mats.data();
+        d.ijk = ijk.data();
+        d.material = mat.data();
+        d.phase = s.phase ? phase.data() : nullptr;
+        d.externals = exts.empty() ? nullptr : exts.data();
+        d.stop_tokens = stop.empty() ? nullptr : stop.data();
+        d.fitness_tokens = fit.empty() ? nullptr : fit.data();
+    }
+};
+
+struct Request {
+    long long ticket;
+    DescCopy desc;
+    size_t bytes = 0;
+    double tSubmit = 0, tAdmit = 0;
+    int slot = -1;
+};
+
+}  // namespace
+
+struct vx3h_server {
+    vx3h_server_cfg cfg{};
+    vx3h_batch* batch = nullptr;
+    size_t budget = 0, used = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::mutex mu;                      // guards queue and nextTicket
+    std::deque<Request*> queue;
+    long long nextTicket = 1;
+    std::map<int, Request*> running;    // slot -> request
+    std::vector<vx3h_server_result> done;
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+extern "C" double vx3h_server_clock(const vx3h_server* sv) { return sv ? sv->now() : 0.0; }
+
+extern "C" vx3h_status vx3h_server_create(int device, const vx3h_server_cfg* cfg, vx3h_server** out) {
+    if (!cfg || !out) { g_err = "null argument"; return VX3H_EINVAL; }
+    *out = nullptr;
+    vx3h_server* sv = new vx3h_server();
+    sv->cfg = *cfg;
+    if (sv->cfg.quantum_steps <= 0) sv->cfg.quantum_steps = 1000;
+    vx3h_status st = vx3h_batch_create(device, &sv->cfg.batch, &sv->batch);
+    if (st != VX3H_OK) { delete sv; return st; }
+    sv->budget = sv->cfg.batch.max_device_bytes;
+    sv->t0 = std::chrono::steady_clock::now();
+    *out = sv;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_server_destroy(vx3h_server* sv) {
+    if (!sv) return VX3H_OK;
+    vx3h_batch_destroy(sv->batch);
+    for (Request* r : sv->queue) delete r;
+    for (auto& kv : sv->running) delete kv.second;
+    delete sv;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_server_submit(vx3h_server* sv, const vx3h_sim_desc* desc, int max_steps, long long* ticket) {
+    if (!sv || !desc) { g_err = "null argument"; return VX3H_EINVAL; }
+    Request* r = new Request();
+    r->desc.take(*desc);
+    if (max_steps > 0) r->desc.d.max_steps = max_steps;
+    vx3h_status st = vx3h_sim_footprint(&r->desc.d, sv->cfg.batch.precision, &r->bytes);
+    if (st != VX3H_OK) { delete r; return st; }
+    if (sv->budget && r->bytes > sv->budget) { delete r; g_err = "request larger than the server's device budget"; return VX3H_ENOMEM; }
+    std::lock_guard<std::mutex> lk(sv->mu);
+    r->ticket = sv->nextTicket++;
+    r->tSubmit = sv->now();
+    sv->queue.push_back(r);
+    if (ticket) *ticket = r->ticket;
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
+    if (!sv) { g_err = "null server"; return VX3H_EINVAL; }
+    // admit in arrival order while the footprint fits
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (sv->queue.empty()) break;
+            Request* h = sv->queue.front();
+            if (sv->cfg.max_sims > 0 && (int)sv->running.size() >= sv->cfg.max_sims) break;
+            if (sv->budget && sv->used + h->bytes > sv->budget) break;
+            sv->queue.pop_front();
+            r = h;
+        }
+        int slot = -1;
+        vx3h_status st = vx3h_sim_admit(sv->batch, &r->desc.d, &slot);
+        if (st == VX3H_ENOMEM && !sv->running.empty()) {   // the engine's own budget: wait for retirements
+            std::lock_guard<std::mutex> lk(sv->mu);
+            sv->queue.push_front(r);
+            break;
+        }
+        if (st != VX3H_OK) {
+            // a request that cannot run finishes at once, with its error as a failed state
+            vx3h_server_result res{};
+            res.ticket = r->ticket;
+            res.status.state = VX3H_SIM_FAILED;
+            res.status.slot = -1;
+            res.t_submit = r->tSubmit;
+            res.t_admit = res.t_done = sv->now();
+            sv->done.push_back(res);
+            delete r;
+            continue;
+        }
+        r->slot = slot;
+        r->tAdmit = sv->now();
+        sv->used += r->bytes;
+        sv->running[slot] = r;
+    }
+    if (!sv->running.empty()) {
+        vx3h_status st = vx3h_batch_step(sv->batch, sv->cfg.quantum_steps);
+        if (st != VX3H_OK) return st;
+        int n = 0;
+        if ((st = vx3h_batch_poll(sv->batch, nullptr, 0, &n)) != VX3H_OK) return st;
+        std::vector<vx3h_sim_status> sts(n > 0 ? n : 1);
+        if ((st = vx3h_batch_poll(sv->batch, sts.data(), n, &n)) != VX3H_OK) return st;
+        const double t = sv->now();
+        for (int i = 0; i < n; i++) {
+            if (sts[i].state == VX3H_SIM_RUNNING) continue;
+            auto it = sv->running.find(sts[i].slot);
+            if (it == sv->running.end()) continue;
+            Request* r = it->second;
+            vx3h_server_result res{};
+            res.ticket = r->ticket;
+            res.status = sts[i];
+            res.t_submit = r->tSubmit;
+            res.t_admit = r->tAdmit;
+            res.t_done = t;
+            res.voxel_steps = (double)sts[i].num_voxel * sts[i].steps;
+            sv->done.push_back(res);
+            if ((st = vx3h_sim_retire(sv->batch, r->slot)) != VX3H_OK) return st;
+            sv->used -= r->bytes;
+            sv->running.erase(it);
+            delete r;
+        }
+    }
+    if (active) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        *active = (int)(sv->running.size() + sv->queue.size());
+    }
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_server_results(vx3h_server* sv, vx3h_server_result* out, int cap, int* n) {
+    if (!sv || !n) { g_err = "null argument"; return VX3H_EINVAL; }
+    if (!out || cap <= 0) { *n = (int)sv->done.size(); return VX3H_OK; }
+    const int k = std::min(cap, (int)sv->done.size());
+    for (int i = 0; i < k; i++) out[i] = sv->done[i];
+    sv->done.erase(sv->done.begin(), sv->done.begin() + k);
+    *n = k;
+    return VX3H_OK;
+}

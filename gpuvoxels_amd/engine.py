@@ -48,6 +48,12 @@ def lib():
         "vx3h_batch_set_timing": (S, [P, C.c_int]),
         "vx3h_calib_copy": (S, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
         "vx3h_sim_collision_pairs": (S, [P, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
+        "vx3h_server_create": (S, [C.c_int, C.POINTER(_abi.ServerCfg), C.POINTER(P)]),
+        "vx3h_server_destroy": (S, [P]),
+        "vx3h_server_submit": (S, [P, C.POINTER(_abi.SimDesc), C.c_int, C.POINTER(C.c_longlong)]),
+        "vx3h_server_round": (S, [P, C.POINTER(C.c_int)]),
+        "vx3h_server_results": (S, [P, C.POINTER(_abi.ServerResult), C.c_int, C.POINTER(C.c_int)]),
+        "vx3h_server_clock": (C.c_double, [P]),
         "vx3h_batch_timing": (S, [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     }
     sig.update({
@@ -267,3 +273,59 @@ class Batch:
         out.update(pos=pos.reshape(-1, 3), orient=ori.reshape(-1, 4), lin_mom=lm.reshape(-1, 3),
                    ang_mom=am.reshape(-1, 3), flags=fl, tfac=tf)
         return out
+
+
+class Server:
+    """Continuous-batching request loop (vx3h_server_*): submit models as they arrive,
+    run rounds, collect results with submit/admit/finish times."""
+
+    def __init__(self, device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO, max_device_bytes=0, quantum_steps=1000,
+                 max_sims=0, steps_per_launch=0):
+        cfg = _abi.ServerCfg()
+        cfg.batch.precision = precision
+        cfg.batch.path = path
+        cfg.batch.max_device_bytes = max_device_bytes
+        cfg.batch.steps_per_launch = steps_per_launch
+        cfg.batch.use_graph = 1
+        cfg.quantum_steps = quantum_steps
+        cfg.max_sims = max_sims
+        self.h = C.c_void_p()
+        check(lib().vx3h_server_create(device, C.byref(cfg), C.byref(self.h)))
+
+    def submit(self, model, max_steps=0):
+        """Queues a Model or Vxa (the server copies it); returns the ticket."""
+        if isinstance(model, Vxa):
+            d = model.desc
+        else:
+            d, keep = model.desc()
+        t = C.c_longlong(0)
+        check(lib().vx3h_server_submit(self.h, C.byref(d), int(max_steps), C.byref(t)))
+        return t.value
+
+    def round(self):
+        n = C.c_int(0)
+        check(lib().vx3h_server_round(self.h, C.byref(n)))
+        return n.value
+
+    def results(self):
+        n = C.c_int(0)
+        check(lib().vx3h_server_results(self.h, None, 0, C.byref(n)))
+        if not n.value:
+            return []
+        arr = (_abi.ServerResult * n.value)()
+        check(lib().vx3h_server_results(self.h, arr, n.value, C.byref(n)))
+        return [arr[i] for i in range(n.value)]
+
+    def clock(self):
+        return lib().vx3h_server_clock(self.h)
+
+    def close(self):
+        if self.h:
+            lib().vx3h_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -257,6 +257,38 @@ vx3h_status vx3h_calib_copy(int device, size_t bytes, int reps, double* ms_per_c
  * may be NULL to query *n. */
 vx3h_status vx3h_sim_collision_pairs(vx3h_batch* b, int slot, int* pairs, int cap, int* n);
 
+/* ---------------------------------------------------------------- serving
+ * Continuous batching over one batch (replaces the file-at-a-time worker loop of
+ * Voxelyze3/src/Executables/vx3_node_worker.cu:63-96 and
+ * VX3_SimulationManager::start, VX3_SimulationManager.cu:138-156): requests are
+ * queued as they arrive (any thread), admitted first-come-first-served whenever
+ * their device footprint fits the budget, stepped together, and retired as they
+ * finish, between rounds of `quantum_steps` steps.  vx3h_server_round runs one
+ * scheduling round; results report per-request submit/admit/finish times. */
+typedef struct vx3h_server vx3h_server;
+typedef struct vx3h_server_cfg {
+    vx3h_batch_cfg batch;     /* engine configuration (its max_device_bytes is the admission budget) */
+    int    quantum_steps;     /* steps per round; 0 = 1000 */
+    int    max_sims;          /* concurrent simulations; 0 = unbounded */
+} vx3h_server_cfg;
+typedef struct vx3h_server_result {
+    long long ticket;
+    vx3h_sim_status status;   /* final status (slot is the slot it ran in) */
+    double t_submit, t_admit, t_done;   /* seconds on the server's monotonic clock */
+    double voxel_steps;       /* voxels x steps simulated */
+} vx3h_server_result;
+vx3h_status vx3h_server_create(int device, const vx3h_server_cfg* cfg, vx3h_server** out);
+vx3h_status vx3h_server_destroy(vx3h_server* sv);
+/* Copies desc; max_steps > 0 overrides the description's step budget.  Thread-safe. */
+vx3h_status vx3h_server_submit(vx3h_server* sv, const vx3h_sim_desc* desc, int max_steps, long long* ticket);
+/* One round: admit what fits, step every running simulation quantum_steps, retire the
+ * finished ones.  *active receives running + queued requests after the round. */
+vx3h_status vx3h_server_round(vx3h_server* sv, int* active);
+/* Finished requests since the last call (cap may be 0 to query *n). */
+vx3h_status vx3h_server_results(vx3h_server* sv, vx3h_server_result* out, int cap, int* n);
+/* Seconds since the server was created, on the clock of t_submit/t_admit/t_done. */
+double vx3h_server_clock(const vx3h_server* sv);
+
 /* ---------------------------------------------------------------- report */
 typedef struct vx3h_report_entry {
     const char* vxd_name;        /* file name of the sim's VXD */
