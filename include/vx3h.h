@@ -162,7 +162,8 @@ enum { VX3H_PREC_F32 = 0, VX3H_PREC_F64 = 1 };
 
 typedef struct vx3h_batch_cfg {
     size_t max_device_bytes;  /* admission budget for simulation state; 0 = 90% of free HBM */
-    int    precision;         /* VX3H_PREC_*: F32 is the product path; F64 pins parity */
+    int    precision;         /* VX3H_PREC_*: F64 state meets the reference's trajectories over the
+                                 full horizon; F32 state is faster and meets them over short horizons */
     int    path;              /* VX3H_PATH_* */
     int    block_size;        /* stream kernels' workgroup size; 0 = tuned default */
     int    steps_per_launch;  /* resident kernel steps per launch; 0 = tuned default */

@@ -1,0 +1,179 @@
+"""Known-answer tests that pin the CPU oracle (oracle/vx_oracle.cpp) to the
+reference's own unit tests and to closed forms:
+
+* VoxCAD/src/VX2/test/tVoxelyze.h:142-198   singleBondFixedFree: F/k and torsion/bending
+* VoxCAD/src/VX2/test/tVoxelyze.h:415-443   freqency: axial period 2*pi/sqrt(k/m)
+* VoxCAD/src/VX2/test/tVoxelyze.h:1034-1114 staticFriction: no sliding below mu_s*m*g
+* VoxCAD/src/VX2/test/tVX_MaterialLink.h:3-37  harmonic-mean link modulus 20/11
+* VoxCAD/src/VX2/test/tVX_MaterialVoxel.h:4-12 mass and moment of inertia of a voxel
+* thermal expansion of a free bar: length change n*s*CTE*T
+"""
+import numpy as np
+import pytest
+
+from gpuvoxels_amd.model import Ext, Material, Model
+from oracle_lib import OSim
+
+TX, TY, TZ = 1, 2, 4
+RX, RY, RZ = 8, 16, 32
+ALL = 63
+OFFSETS = {"X_POS": (1, 0, 0), "X_NEG": (-1, 0, 0), "Y_POS": (0, 1, 0), "Y_NEG": (0, -1, 0),
+           "Z_POS": (0, 0, 1), "Z_NEG": (0, 0, -1)}
+
+
+def two_voxels(direction, free_dofs_fixed, force=(0, 0, 0), moment=(0, 0, 0)):
+    """test2Vox of tVoxelyze.h:64-115: voxel 1 fixed at the origin, voxel 2 beside it in
+    `direction`, loaded and constrained in all dofs but one; E = 1e6, rho = 1e3,
+    s = 1 mm, internal damping 1, global damping 0.2."""
+    off = OFFSETS[direction]
+    m = Model(voxel_size=0.001)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, zeta_internal=1.0, zeta_global=0.2)]
+    cells = [(0, 0, 0), off]
+    shift = [-min(0, off[k]) for k in range(3)]   # lattice indices must be non-negative
+    ijk = np.array([[c[k] + shift[k] for k in range(3)] for c in cells], np.int32)
+    order = np.lexsort((ijk[:, 0], ijk[:, 1], ijk[:, 2]))
+    m.ijk = ijk[order]
+    m.mat = np.zeros(2, np.int32)
+    first = int(np.where(order == 0)[0][0])
+    second = 1 - first
+    m.externals = [Ext(first, dof_fixed=ALL), Ext(second, dof_fixed=free_dofs_fixed, force=force, moment=moment)]
+    return m, second, off
+
+
+def settle(m, second, off, what, expected, max_steps=1000):
+    """Number of steps until the watched quantity stays within 1e-5 relative of the
+    expected value for 10 consecutive steps (the test2Vox convergence rule)."""
+    o = OSim(m)
+    ts = o.recommended_dt()
+    run = 0
+    for k in range(max_steps):
+        o.step_raw(ts)
+        st = o.state()
+        if what < 3:
+            ref = m.ijk[second] * m.voxel_size
+            data = st["pos"][second][what] - ref[what]
+        else:
+            q = st["orient"][second]
+            # rotation vector of the (w, x, y, z) quaternion, small angles
+            ang = 2 * np.arccos(np.clip(q[0], -1, 1))
+            s = np.sqrt(max(1 - q[0] * q[0], 0.0))
+            rv = np.zeros(3) if s < 1e-12 else ang * q[1:] / s
+            data = rv[what - 3]
+        if abs(data - expected) < abs(expected) * 1e-5 or np.float32(data) == np.float32(expected):
+            run += 1
+        else:
+            run = 0
+        if run == 10:
+            return k
+    return max_steps
+
+
+@pytest.mark.parametrize("direction,force,free,axis,expected,limit", [
+    ("X_POS", (1e-3, 0, 0), ALL & ~TX, 0, 1e-6, 150),
+    ("X_POS", (-1e-3, 0, 0), ALL & ~TX, 0, -1e-6, 150),
+    ("X_POS", (0, 1e-3, 0), ALL & ~TY, 1, 1e-6, 150),
+    ("X_POS", (0, 0, -1e-3), ALL & ~TZ, 2, -1e-6, 150),
+    ("Y_NEG", (0, 1e-3, 0), ALL & ~TY, 1, 1e-6, 150),
+    ("Z_POS", (1e-3, 0, 0), ALL & ~TX, 0, 1e-6, 150),
+])
+def test_single_bond_force(direction, force, free, axis, expected, limit):
+    m, second, off = two_voxels(direction, free, force=force)
+    assert settle(m, second, off, axis, expected) < limit
+
+
+@pytest.mark.parametrize("direction,moment,free,axis,expected,limit", [
+    ("X_POS", (1e-9, 0, 0), ALL & ~RX, 3, 1.2e-5, 200),    # torsion
+    ("X_POS", (0, 1e-9, 0), ALL & ~RY, 4, 3e-6, 200),      # bending
+    ("Y_POS", (0, 1e-9, 0), ALL & ~RY, 4, 1.2e-5, 200),
+    ("Z_POS", (1e-9, 0, 0), ALL & ~RX, 3, 3e-6, 200),
+])
+def test_single_bond_moment(direction, moment, free, axis, expected, limit):
+    m, second, off = two_voxels(direction, free, moment=moment)
+    assert settle(m, second, off, axis, expected) < limit
+
+
+def calc_period(ts, data):   # zero crossings, tVoxelyze.h:43-62
+    zc = []
+    for i in range(1, len(data)):
+        if (data[i - 1] <= 0 < data[i]) or (data[i - 1] >= 0 > data[i]):
+            perc = data[i - 1] / (data[i - 1] - data[i])
+            zc.append(ts * (i - 1 + perc))
+    return np.mean(np.diff(zc)) * 2
+
+
+def test_axial_frequency():
+    m = Model(voxel_size=0.001)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, zeta_internal=0.0, zeta_global=0.0)]
+    m.ijk = np.array([[0, 0, 0], [1, 0, 0]], np.int32)
+    m.mat = np.zeros(2, np.int32)
+    m.externals = [Ext(0, dof_fixed=ALL), Ext(1, dof_fixed=ALL & ~TX, force=(1e-3, 0, 0))]
+    o = OSim(m)
+    ts = o.recommended_dt() / 10
+    data = []
+    for _ in range(1000):
+        o.step_raw(ts)
+        data.append(o.state()["pos"][1][0] - 0.001001)
+    expected = 2 * 3.1415926 / np.sqrt(1e9)
+    assert calc_period(ts, data) == pytest.approx(expected, abs=expected / 1000)
+
+
+def test_static_friction_holds_below_threshold():
+    vs, rho = 0.001, 1e3
+    m = Model(voxel_size=vs)
+    m.materials = [Material(elastic_mod=1e6, density=rho, u_static=1.0, u_dynamic=0.1, zeta_global=1.0)]
+    m.ijk = np.array([[0, 0, 0]], np.int32)
+    m.mat = np.zeros(1, np.int32)
+    m.floor_enabled = 1
+    m.gravity_mult = 1.0
+    o = OSim(m)
+    ts = o.recommended_dt()
+    for _ in range(50):
+        o.step_raw(ts)
+    normal = np.float32(rho * vs ** 3 * 9.80665)
+    x0 = o.state()["pos"][0][0]
+    o.set_force(0, (0.9 * normal, 0, 0))     # just below mu_s * m * g
+    for _ in range(200):
+        o.step_raw(ts)
+    assert o.state()["pos"][0][0] == x0
+    o.set_force(0, (1.1 * normal, 0, 0))     # above: it slides
+    for _ in range(200):
+        o.step_raw(ts)
+    assert o.state()["pos"][0][0] > x0
+
+
+def test_link_material_harmonic_mean():
+    m = Model(voxel_size=0.001)
+    m.materials = [Material(elastic_mod=1e6, density=1e3), Material(elastic_mod=1e7, density=1e3)]
+    m.ijk = np.array([[0, 0, 0], [1, 0, 0]], np.int32)
+    m.mat = np.array([0, 1], np.int32)
+    idx, consts = OSim(m).link_material_consts()
+    assert consts[idx[0], 10] == pytest.approx(20.0 / 11.0 * 1e6, rel=1e-6)   # E of the link
+
+
+def test_voxel_material_mass_and_inertia():
+    s, rho = 0.002, 100.0   # tVX_MaterialVoxel.h: mass 8e-7, I = mass*s^2/6
+    m = Model(voxel_size=s)
+    m.materials = [Material(elastic_mod=1e6, density=rho)]
+    m.ijk = np.array([[0, 0, 0]], np.int32)
+    m.mat = np.zeros(1, np.int32)
+    c = OSim(m).voxel_material_consts()[0]
+    assert c[0] == pytest.approx(8e-7, rel=1e-6)
+    assert c[3] == pytest.approx(8e-7 * s * s / 6, rel=1e-6)
+
+
+def test_free_bar_thermal_expansion():
+    n, s, cte, temp = 6, 0.01, 0.01, 5.0
+    m = Model(voxel_size=s)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, cte=cte, zeta_internal=1.0, zeta_global=0.5)]
+    m.ijk = np.array([[i, 0, 0] for i in range(n)], np.int32)
+    m.mat = np.zeros(n, np.int32)
+    m.temp_enabled = 1
+    m.temp_base = 25.0
+    m.temp_amplitude = temp     # no variation: every voxel sits at TempBase + A, i.e. T = A
+    o = OSim(m)
+    for _ in range(20000):
+        if o.step(1) == 0:
+            break
+    p = o.state()["pos"]
+    # each of the n-1 links rests at s*(1 + CTE*T)
+    assert p[-1][0] - p[0][0] == pytest.approx((n - 1) * s * (1 + cte * temp), rel=1e-5)
