@@ -177,3 +177,93 @@ def test_free_bar_thermal_expansion():
     p = o.state()["pos"]
     # each of the n-1 links rests at s*(1 + CTE*T)
     assert p[-1][0] - p[0][0] == pytest.approx((n - 1) * s * (1 + cte * temp), rel=1e-5)
+
+
+# ---- VX3 driver semantics (Voxelyze3/src/VX3): closed forms and invariants
+
+def row_of_voxels(phases, expansion=0, amplitude=10.0, period=0.1):
+    m = Model(voxel_size=0.01)
+    # E = 1e3 keeps the fixed 1 ms step below the stable 1.6 ms
+    m.materials = [Material(elastic_mod=1e3, density=1e3, cte=0.01)]
+    m.ijk = np.array([[i, 0, 0] for i in range(len(phases))], np.int32)
+    m.mat = np.zeros(len(phases), np.int32)
+    m.phase = np.array(phases, np.float64)
+    m.temp_enabled = m.vary_temp_enabled = 1
+    m.temp_amplitude, m.temp_period = amplitude, period
+    m.enable_expansion = expansion
+    m.dt_fixed = 0.001
+    return m
+
+
+@pytest.mark.parametrize("expansion", [0, 1])
+def test_phase_temperature_closed_form(expansion):
+    """T_v = A sin(2 pi (t/T_p + phi_v)) with phi in cycles, clamped to <= 0 unless
+    expansion is enabled (VX3_VoxelyzeKernel.cu:625-650): at t = 0 the phases
+    0, 1/4, 1/2, 3/4 give 0, A, 0, -A."""
+    a = 10.0
+    o = OSim(row_of_voxels([0.0, 0.25, 0.5, 0.75], expansion, a))
+    o.step(1)                       # temperature is set from t = 0 before the first step
+    t = o.state()["temp"]
+    want = np.array([0.0, a if expansion else 0.0, 0.0, -a])
+    assert np.allclose(t, want, atol=1e-5 * a)
+
+
+def test_phase_is_a_time_shift():
+    """A voxel at phase 1/4 and t = 0 has the temperature a voxel at phase 0 has at
+    t = T_p/4 (25 steps of 1 ms with T_p = 0.1 s)."""
+    o = OSim(row_of_voxels([0.0, 0.25], 1))
+    o.step(1)
+    t_shift = o.state()["temp"][1]
+    o.step(25)
+    assert o.state()["temp"][0] == pytest.approx(t_shift, rel=1e-4)
+
+
+def test_recommended_dt_closed_form():
+    """dt = DtFrac / (2 pi sqrt(k/m)), k = E s for a nu = 0 link of length s,
+    m = rho s^3 (Voxelyze.cpp:286-311, VX3_VoxelyzeKernel.cu:184-217,245-253)."""
+    s, e, rho = 0.01, 1e6, 1e3
+    m = Model(voxel_size=s)
+    m.materials = [Material(elastic_mod=e, density=rho)]
+    m.ijk = np.array([[0, 0, 0], [1, 0, 0]], np.int32)
+    m.mat = np.zeros(2, np.int32)
+    m.dt_frac = 0.9
+    o = OSim(m)
+    o.step(1)
+    want = 0.9 / (2 * np.pi * np.sqrt(e * s / (rho * s ** 3)))
+    assert o.status().dt == pytest.approx(want, rel=1e-6)
+
+
+def test_center_of_mass_measured_voxels_only():
+    """The CoM is mass weighted over measured voxels (VX3_VoxelyzeKernel.cu:477-493):
+    a symmetric 4^3 cube centres at 1.5 s; an unmeasured extra column is ignored."""
+    s = 0.01
+    m = Model(voxel_size=s)
+    m.materials = [Material(elastic_mod=1e6, density=1e3), Material(elastic_mod=1e6, density=5e3, is_measured=0)]
+    cube = [(i, j, k) for k in range(4) for j in range(4) for i in range(4)]
+    extra = [(4, 0, k) for k in range(4)]
+    m.ijk = np.array(cube + extra, np.int32)
+    m.mat = np.array([0] * len(cube) + [1] * len(extra), np.int32)
+    o = OSim(m)
+    o.step(1)
+    assert np.allclose(list(o.status().initial_com), [1.5 * s] * 3, rtol=0, atol=1e-12)
+
+
+def test_fitness_and_stop_condition():
+    """Fitness is the MathTree of the CoM displacement (here its 3D length); the
+    stop formula t > 0.01 ends the run at the first step count n with n*dt > 0.01,
+    dt being the float-rounded 1 ms (VX3_SimulationManager.cu:62-115)."""
+    from gpuvoxels_amd.model import FITNESS_DISTANCE_3D, MT
+    m = row_of_voxels([0.0, 0.5, 0.25], 0)
+    m.fitness_tokens = FITNESS_DISTANCE_3D
+    m.stop_tokens = [(MT["mtCONST"], 0.01), (MT["mtVAR"], 4.0), (MT["mtGREATERTHAN"], 0.0), (MT["mtEND"], 0.0)]
+    m.gravity_mult = 1.0
+    m.floor_enabled = 1
+    o = OSim(m)
+    o.step(1000)
+    st = o.status()
+    dt = float(np.float32(0.001))
+    n = next(k for k in range(1, 100) if k * dt > 0.01)
+    assert st.state == 1 and st.steps == n
+    d = np.array(list(st.current_com)) - np.array(list(st.initial_com))
+    assert d[2] < -1e-4       # it fell onto the floor
+    assert st.fitness == pytest.approx(np.linalg.norm(d), rel=1e-12)
