@@ -2,8 +2,9 @@
 //
 // Device layout (structure of arrays, one batch = many simulations):
 //   voxels  [0, Vcap): position, linear momentum, orientation, angular momentum in
-//           the state precision T (float on the product path, double for the
-//           parity-pinning path), plus flags, thermal size factor, material,
+//           the state precision T (double: the reference's trajectories over the
+//           full horizon; float: faster, same trajectories over short horizons),
+//           plus flags, thermal size factor, material,
 //           owning simulation.  Each simulation owns one contiguous voxel range.
 //   links   [0, Lcap): endpoints, axis/material, beam state (previous local pose
 //           for damping, strains, small-angle flags).  Contiguous range per sim.
@@ -87,8 +88,6 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int flags;                       // SC_*
     int v0, nv, l0, nl;
     int stopOff, stopLen;            // MathTree program in the token pool
-    int nColl;                       // collision pairs owned (pair range below)
-    int c0;
     int mat0, nvm, nlm;              // the sim's materials: simMats[mat0 .. +nvm) voxel, then nlm link table indices
     // collisions (CVoxelyze::updateCollisions, Voxelyze.cpp:671-711)
     float colThreshSq;               // watchRadiusMm^2 (pairs closer than this are watched)
