@@ -18,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(os.path.dirname(HERE), "_build")
 LIB = os.path.join(HERE, "libvx3h.so")
 WORKER = os.path.join(HERE, "vx3_node_worker")
+LAUNCHER = os.path.join(HERE, "Voxelyze3")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -29,7 +30,6 @@ UNITS = [
     # (source, compiler, extra flags)
     ("vx3_kernels_f32.hip", "hip", []),
     ("vx3_kernels_f64.hip", "hip", ["-ffp-contract=off"]),
-    ("vx3_collide.hip", "hip", []),
     ("vx3h_engine.cpp", "hip", ["-x", "hip"]),
     ("vx3h_build.cpp", "cxx", []),
     ("vx3h_vxa.cpp", "cxx", []),
@@ -68,6 +68,9 @@ def build(verbose=False):
     if os.path.exists(worker_src):
         _run(["g++"] + CXX_FLAGS + ["-o", WORKER, worker_src, os.path.join(CSRC, "vx3_manager.cpp"), "-L" + HERE, "-lvx3h",
                                     "-Wl,-rpath,$ORIGIN"])
+    launcher_src = os.path.join(CSRC, "vx3_launcher.cpp")
+    if os.path.exists(launcher_src):
+        _run(["g++"] + CXX_FLAGS + ["-o", LAUNCHER, launcher_src])
     if verbose:
         print("built", LIB)
     return LIB

@@ -45,15 +45,30 @@ std::string trim(const std::string& s) {
     return a == std::string::npos ? "" : s.substr(a, b - a + 1);
 }
 
+// the five predefined XML entities decoded
+std::string unescape(const std::string& s) {
+    static const char* ent[5][2] = {{"&amp;", "&"}, {"&lt;", "<"}, {"&gt;", ">"}, {"&quot;", "\""}, {"&apos;", "'"}};
+    std::string o;
+    for (size_t i = 0; i < s.size();) {
+        bool hit = false;
+        if (s[i] == '&')
+            for (auto& e : ent)
+                if (s.compare(i, strlen(e[0]), e[0]) == 0) { o += e[1]; i += strlen(e[0]); hit = true; break; }
+        if (!hit) o += s[i++];
+    }
+    return o;
+}
+
 // text of every <tag>...</tag> in s (flat; the task file has no nesting beyond <vxd><f>)
-std::vector<std::string> elements(const std::string& s, const std::string& tag) {
+std::vector<std::string> elements(const std::string& s, const std::string& tag, bool decode = true) {
     std::vector<std::string> out;
     const std::string open = "<" + tag + ">", close = "</" + tag + ">";
     size_t p = 0;
     while ((p = s.find(open, p)) != std::string::npos) {
         size_t q = s.find(close, p + open.size());
         if (q == std::string::npos) break;
-        out.push_back(trim(s.substr(p + open.size(), q - p - open.size())));
+        const std::string body = trim(s.substr(p + open.size(), q - p - open.size()));
+        out.push_back(decode ? unescape(body) : body);
         p = q + close.size();
     }
     return out;
@@ -92,7 +107,7 @@ int main(int argc, char** argv) {
     ss << tf.rdbuf();
     const std::string task = ss.str();
     std::vector<std::string> vxa = elements(task, "vxa"), dir = elements(task, "input_dir");
-    std::vector<std::string> vxdBlock = elements(task, "vxd");
+    std::vector<std::string> vxdBlock = elements(task, "vxd", false);
     std::vector<std::string> files;
     for (const std::string& blk : vxdBlock) for (const std::string& f : elements(blk, "f")) files.push_back(f);
     if (vxa.empty() || dir.empty()) { printf("Error: task file lacks <vxa> or <input_dir>.\n"); return 1; }
