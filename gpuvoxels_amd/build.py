@@ -3,8 +3,9 @@
     python -m gpuvoxels_amd.build
 
 HIP translation units are compiled by hipcc for gfx950 with packed FP32
-instructions disabled; the FP64 kernel unit is compiled without floating-point
-contraction so that it follows the CPU engine's rounding.  Host-only units are
+instructions disabled; the kernel units without machine loop-invariant code motion
+(KERNEL_FLAGS), the FP64 kernel unit also without floating-point contraction so
+that it follows the CPU engine's rounding.  Host-only units are
 compiled by g++.
 """
 import os
@@ -25,11 +26,17 @@ ARCH = "gfx950"
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function"]
+# Kernel units: no machine-level loop-invariant code motion.  The multi-step kernel
+# calls double-precision sin/cos/acos inside its step loop; hoisting their 64-bit
+# polynomial coefficients out of the loop pins them in registers for the whole
+# launch and forces spills to scratch (e.g. FP64 512x2x1: 336 B/lane of scratch
+# with, none without).
+KERNEL_FLAGS = ["-mllvm", "-disable-machine-licm"]
 
 UNITS = [
     # (source, compiler, extra flags)
-    ("vx3_kernels_f32.hip", "hip", []),
-    ("vx3_kernels_f64.hip", "hip", ["-ffp-contract=off"]),
+    ("vx3_kernels_f32.hip", "hip", KERNEL_FLAGS),
+    ("vx3_kernels_f64.hip", "hip", KERNEL_FLAGS + ["-ffp-contract=off"]),
     ("vx3h_engine.cpp", "hip", ["-x", "hip"]),
     ("vx3h_build.cpp", "cxx", []),
     ("vx3h_vxa.cpp", "cxx", []),

@@ -7,8 +7,9 @@
 
 #include "vx3_stream_launch.inl"
 
-// resident kernel shapes: (block, voxels per thread, links per axis per thread)
-#define VX3_RES_SHAPES(X) X(256, 4, 2) X(512, 2, 1) X(256, 2, 1) X(256, 1, 1)
+// resident kernel shapes: (block, voxels per thread, links per axis per thread); on
+// equal voxel capacity the first listed is preferred (512x2x1: two waves per SIMD)
+#define VX3_RES_SHAPES(X) X(512, 2, 1) X(256, 4, 2) X(256, 2, 1) X(256, 1, 1)
 #define VX3_RES_TABLE_FN residentShapes_f64
 #include "vx3_resident_launch.inl"
 

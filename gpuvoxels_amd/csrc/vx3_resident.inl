@@ -141,8 +141,9 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             }
         }
     }
-    V3<real> vp[VPT], vm[VPT], va[VPT];
-    Q4<real> vo[VPT];
+    // position and orientation live only in the LDS pose cache (their single copy);
+    // momenta stay in registers
+    V3<real> vm[VPT], va[VPT];
     int vf[VPT], vmat[VPT], vext[VPT];
     double vph[VPT];
     int vl[VPT][3];       // slots of directions (2i, 2i+1) packed lo | hi << 16, 0xFFFF = none
@@ -152,10 +153,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         vmat[k] = -1;
         if (v < nv) {
             const int g = v0 + v;
-            vp[k] = V3<real>(b.px[g], b.py[g], b.pz[g]);
             vm[k] = V3<real>(b.mx[g], b.my[g], b.mz[g]);
             va[k] = V3<real>(b.ax[g], b.ay[g], b.az[g]);
-            vo[k] = Q4<real>(b.ow[g], b.ox[g], b.oy[g], b.oz[g]);
             vf[k] = b.vflags[g];
             vmat[k] = localVmat(b.vmat[g]);
             vext[k] = b.vext[g];
@@ -170,8 +169,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         const int v = tid + k * BS;
         if (v < nv) {
             const int g = v0 + v;
-            sPx[v] = vp[k].x; sPy[v] = vp[k].y; sPz[v] = vp[k].z;
-            sOw[v] = vo[k].w; sOx[v] = vo[k].x; sOy[v] = vo[k].y; sOz[v] = vo[k].z;
+            sPx[v] = b.px[g]; sPy[v] = b.py[g]; sPz[v] = b.pz[g];
+            sOw[v] = b.ow[g]; sOx[v] = b.ox[g]; sOy[v] = b.oy[g]; sOz[v] = b.oz[g];
             sTf[v] = b.tfac[g];
             sPsx[v] = b.psx[g]; sPsy[v] = b.psy[g]; sPsz[v] = b.psz[g];
         }
@@ -290,13 +289,15 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                 if (vmat[k] < 0) continue;
                 const int v = tid + k * BS;
                 const VoxMatDev& m = sVM[vmat[k]];
+                V3<real> vp(sPx[v], sPy[v], sPz[v]);
                 if (!m.fixed) {
                     const ExtDev* ext = vext[k] >= 0 ? &b.extTab[vext[k]] : nullptr;
                     int ix = 0, iy = 0, iz = 0;
                     if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
-                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], NoContacts(), vp[k], vm[k], vo[k], va[k], vf[k]);
-                    sPx[v] = vp[k].x; sPy[v] = vp[k].y; sPz[v] = vp[k].z;
-                    sOw[v] = vo[k].w; sOx[v] = vo[k].x; sOy[v] = vo[k].y; sOz[v] = vo[k].z;
+                    Q4<real> vo(sOw[v], sOx[v], sOy[v], sOz[v]);
+                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], NoContacts(), vp, vm[k], vo, va[k], vf[k]);
+                    sPx[v] = vp.x; sPy[v] = vp.y; sPz[v] = vp.z;
+                    sOw[v] = vo.w; sOx[v] = vo.x; sOy[v] = vo.y; sOz[v] = vo.z;
                 }
                 if (vary && !m.fixed && !(m.thermalOnAfter > tn)) {
                     const float temp = actuationTemp(tempAmp, tempPeriod, vph[k], tn, expansion);
@@ -335,7 +336,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                 }
                 if (sample && m.isMeasured) {
                     const double mass = m.mass;
-                    psx += (double)vp[k].x * mass; psy += (double)vp[k].y * mass; psz += (double)vp[k].z * mass;
+                    psx += (double)vp.x * mass; psy += (double)vp.y * mass; psz += (double)vp.z * mass;
                     psm += mass;
                 }
             }
@@ -403,10 +404,10 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     for (int k = 0; k < VPT; k++) {
         if (vmat[k] < 0) continue;
         const int v = tid + k * BS, g = v0 + v;
-        b.px[g] = vp[k].x; b.py[g] = vp[k].y; b.pz[g] = vp[k].z;
+        b.px[g] = sPx[v]; b.py[g] = sPy[v]; b.pz[g] = sPz[v];
         b.mx[g] = vm[k].x; b.my[g] = vm[k].y; b.mz[g] = vm[k].z;
         b.ax[g] = va[k].x; b.ay[g] = va[k].y; b.az[g] = va[k].z;
-        b.ow[g] = vo[k].w; b.ox[g] = vo[k].x; b.oy[g] = vo[k].y; b.oz[g] = vo[k].z;
+        b.ow[g] = sOw[v]; b.ox[g] = sOx[v]; b.oy[g] = sOy[v]; b.oz[g] = sOz[v];
         b.vflags[g] = vf[k];
         b.tfac[g] = sTf[v];
         b.psx[g] = sPsx[v]; b.psy[g] = sPsy[v]; b.psz[g] = sPsz[v];
