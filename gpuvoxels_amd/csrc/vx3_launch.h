@@ -17,9 +17,10 @@ void launchStep(const DevBatch<float>& b, hipStream_t s, const StepOpts& o, hipE
 void launchStep(const DevBatch<double>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);
 // Resident multi-step kernel shapes: (block, voxels/thread, links per axis/thread), LDS bytes.
 // A sim fits a shape when V <= block*vpt, each axis has <= block*lpa links, and it uses at
-// most RES_MAX_VMATS voxel and RES_MAX_LMATS link material variants.
+// most RES_MAX_VMATS voxel and RES_MAX_LMATS link material variants and RES_MAX_PHASES
+// distinct actuation phase offsets.
 struct ResidentShapeInfo { int block, vpt, lpa; size_t lds; };
-enum { RES_MAX_VMATS = 8, RES_MAX_LMATS = 16 };
+enum { RES_MAX_VMATS = 8, RES_MAX_LMATS = 16, RES_MAX_PHASES = 64 };
 int residentShapes_f32(ResidentShapeInfo* out, int cap);
 int residentShapes_f64(ResidentShapeInfo* out, int cap);
 inline int residentShapes(bool f64, ResidentShapeInfo* out, int cap) {

@@ -59,12 +59,13 @@ template <typename T> struct Q4 {
                   w * f.y - x * f.z + y * f.w + z * f.x, w * f.z + x * f.y - y * f.x + z * f.w);
     }
     __device__ Q4 conj() const { return Q4(w, -x, -y, -z); }
-    // Quat3D::ToRotationVector (Quat3D.h:121-126)
-    __device__ V3<T> toRotVec() const {
+    // Quat3D::ToRotationVector (Quat3D.h:121-126).  halfAngle >= 0 is acos(w) when
+    // the caller knows it (the quaternion came from fromAngleToPosX).
+    __device__ V3<T> toRotVec(T halfAngle = (T)-1) const {
         if (w >= (T)1.0 || w <= (T)-1.0) return V3<T>(0, 0, 0);
         T sl = (T)1.0 - w * w;
         if (sl < (T)2.4e-3) { T f = msqrt(((T)2 - (T)2 * w) / sl); return V3<T>(f * ((T)2.0 * x), f * ((T)2.0 * y), f * ((T)2.0 * z)); }
-        T a = macos(w), inv = (T)1.0 / msqrt(sl);
+        T a = halfAngle >= 0 ? halfAngle : macos(w), inv = (T)1.0 / msqrt(sl);
         return V3<T>(inv * (a * ((T)2.0 * x)), inv * (a * ((T)2.0 * y)), inv * (a * ((T)2.0 * z)));
     }
     // Quat3D::FromRotationVector (Quat3D.h:128-143)
@@ -77,24 +78,29 @@ template <typename T> struct Q4 {
         q.x = th.x * s; q.y = th.y * s; q.z = th.z * s;
         return q;
     }
-    // Quat3D::FromAngleToPosX (Quat3D.h:145-170)
-    __device__ void fromAngleToPosX(const V3<T>& r) {
-        if (r.x == 0 && r.y == 0 && r.z == 0) return;
+    // Quat3D::FromAngleToPosX (Quat3D.h:145-170).  Returns theta/2 when the general
+    // branch ran (for toRotVec below), -1 otherwise.  The half-angle cosine and sine
+    // come from cos(theta) = n.x by the half-angle identities, cos(theta/2) =
+    // sqrt((1 + n.x)/2) and sin(theta/2) = sqrt((1 - n.x)/2) (theta in [0, pi]),
+    // instead of cos/sin of acos(n.x): equal to within rounding, and two square
+    // roots in place of two trigonometric calls.
+    __device__ T fromAngleToPosX(const V3<T>& r) {
+        if (r.x == 0 && r.y == 0 && r.z == 0) return (T)-1;
         T yx = r.y / r.x, zx = r.z / r.x;
         const T SA = (T)1.732e-2;
         if (yx < SA && yx > -SA && zx < SA && zx > -SA) {
             x = 0; y = (T)0.5 * zx; z = (T)-0.5 * yx;
             w = (T)1 + (T)0.5 * (-y * y - z * z);
-            return;
+            return (T)-1;
         }
         V3<T> n = r;
         { T l = msqrt(n.x * n.x + n.y * n.y + n.z * n.z); if (l > 0) { T li = (T)1.0 / l; n.x *= li; n.y *= li; n.z *= li; } }
         T theta = macos(n.x);
-        if (theta > (T)(3.14159265358979 - 1e-7)) { w = 0; x = 0; y = 1; z = 0; return; }
+        if (theta > (T)(3.14159265358979 - 1e-7)) { w = 0; x = 0; y = 1; z = 0; return (T)-1; }
         const T ami = (T)1.0 / msqrt(n.z * n.z + n.y * n.y);
-        const T a = (T)0.5 * theta;
-        const T s = msin(a);
-        w = mcos(a); x = 0; y = n.z * ami * s; z = -n.y * ami * s;
+        const T s = msqrt(((T)1 - n.x) * (T)0.5);
+        w = msqrt(((T)1 + n.x) * (T)0.5); x = 0; y = n.z * ami * s; z = -n.y * ami * s;
+        return (T)0.5 * theta;
     }
     __device__ V3<T> rot(const V3<T>& f) const {   // RotateVec3D
         T fx = f.x, fy = f.y, fz = f.z;
@@ -175,6 +181,7 @@ __device__ __forceinline__ void linkForces(const LinkMatDev& mat, int axis, cons
     float smallTurn = (float)((mabs(pos2.z) + mabs(pos2.y)) / pos2.x);
     float extendPerc = (float)(mabs((T)1 - pos2.x / restLength));
     bool small = (st.flags & LF_SMALL_ANGLE) != 0;
+    T halfTheta = (T)-1;
     if (!small && smallTurn < 0.05f && extendPerc < 0.50f) {
         small = true;
         st.flags &= ~LF_LOCAL_VELOCITY_VALID;
@@ -187,11 +194,11 @@ __device__ __forceinline__ void linkForces(const LinkMatDev& mat, int axis, cons
         pos2.x -= restLength;
     } else {
         st.flags &= ~LF_SMALL_ANGLE;
-        angle1.fromAngleToPosX(pos2);
+        halfTheta = angle1.fromAngleToPosX(pos2);
         angle2 = angle1 * angle2;
         pos2 = V3<T>(pos2.len() - restLength, 0, 0);
     }
-    const V3<T> a1v = angle1.toRotVec();
+    const V3<T> a1v = angle1.toRotVec(halfTheta);
     const V3<T> a2v = angle2.toRotVec();
     st.pos2 = pos2; st.a1v = a1v; st.a2v = a2v;
 

@@ -23,6 +23,7 @@
 namespace vx3 {
 namespace VX_NS {
 
+#define VX3_RES_PHASES 64   // distinct actuation phases evaluated once per step instead of per voxel
 #ifndef VX3_RES_VMATS
 #define VX3_RES_VMATS 8      // voxel materials a resident sim may use
 #define VX3_RES_LMATS 16     // link material variants a resident sim may use
@@ -62,10 +63,13 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     float* sEs = sPsz + VMAX;                   // [(axis*2 + end)][LAX]
     unsigned short* sVL = (unsigned short*)sF;  // [6][VMAX] slot of the link in direction d (setup only)
     unsigned short* sSL = sVL + 6 * VMAX;       // [3][LAX]  link of (axis, slot) (setup only)
-    __shared__ int sRun, sDiv, sCnt[3];
+    __shared__ int sRun, sDiv, sCnt[3], sExp[3], sExpAt[3];
     __shared__ double sQ[VX3_DEV_MT_MAX + 1];
     __shared__ double sRed[4][BS / 64];
     __shared__ double sCom[3], sComH0[3], sComH1[3], sAngle;   // center-of-mass history, owned by thread 0
+    __shared__ double sPhase[VX3_RES_PHASES];   // distinct actuation phase offsets of the sim
+    __shared__ float sPhT[VX3_RES_PHASES];      // their temperatures for the step being run
+    __shared__ int sNph;                        // table size; -1: more than VX3_RES_PHASES (admission prevents it)
 
     const int s = simList[blockIdx.x];
     SimCtl& scg = b.sims[s];
@@ -92,16 +96,33 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     for (int i = 0; i < nlm; i++)
         ldsCopyWords((int*)&sLM[i], (const int*)&b.lmatTab[b.simMats[mat0 + nvm + i]], sizeof(LinkMatDev) / 4, tid, BS);
     for (int i = tid; i < 6 * VMAX; i += BS) sVL[i] = 0xFFFF;
-    if (tid < 3) sCnt[tid] = 0;
+    if (tid < 3) { sCnt[tid] = 0; sExp[tid] = 0; sExpAt[tid] = 0; }
+    __syncthreads();
+    // Links that took the expensive branches last step (large-angle frame, or a
+    // relative rotation past toRotVec's series range, |a2v| >= ~0.098 rad) get the
+    // lowest slots of their axis, so they share as few wavefronts as possible: a
+    // wavefront runs a branch if any of its lanes takes it.  Which thread computes a
+    // link does not change its result.
+    auto expensive = [&](int g) {
+        const real x = b.a2x[g], y = b.a2y[g], z = b.a2z[g];
+        return !(b.lflags[g] & LF_SMALL_ANGLE) || x * x + y * y + z * z > (real)0.009;
+    };
+    for (int l = tid; l < nl; l += BS) {
+        const int g = l0 + l;
+        if (expensive(g)) atomicAdd(&sExp[b.lmeta[g] & 3], 1);
+    }
     __syncthreads();
     for (int l = tid; l < nl; l += BS) {
         const int g = l0 + l;
         const int ax = b.lmeta[g] & 3;
-        const int slot = atomicAdd(&sCnt[ax], 1);   // admission guarantees slot < LAX
+        // expensive links take [0, sExp), the others [sExp, count); admission guarantees count <= LAX
+        const int slot = expensive(g) ? atomicAdd(&sExpAt[ax], 1) : sExp[ax] + atomicAdd(&sCnt[ax], 1);
         sSL[ax * LAX + slot] = (unsigned short)l;
         sVL[(2 * ax) * VMAX + (b.lneg[g] - v0)] = (unsigned short)slot;
         sVL[(2 * ax + 1) * VMAX + (b.lpos[g] - v0)] = (unsigned short)slot;
     }
+    __syncthreads();
+    if (tid < 3) sCnt[tid] += sExp[tid];   // links of axis tid
     __syncthreads();
     auto localVmat = [&](int gidx) {
         int r = 0;
@@ -143,9 +164,44 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     }
     // position and orientation live only in the LDS pose cache (their single copy);
     // momenta stay in registers
+    // Actuation phases: the temperature depends on the voxel only through its phase
+    // offset, and simulations use few distinct offsets.  Wave 0 collects them
+    // (lane i keeps entry i), so each step evaluates sin once per distinct offset;
+    // the values are those the per-voxel formula gives.
+    if (tid < 64) {
+        // compared as bit patterns: every value, NaN included, equals itself
+        long long tab = 0;
+        int n = 0;
+        bool over = false;
+        for (int base = 0; base < nv && !over; base += 64) {
+            const int v = base + tid;
+            const long long ph = v < nv ? __double_as_longlong(b.phase[v0 + v]) : 0;
+            bool pending = v < nv;
+            while (!over && __any(pending)) {
+                const int src = __ffsll((unsigned long long)__ballot(pending)) - 1;   // lowest pending lane
+                const long long cand = __shfl(ph, src, 64);
+                if (!__any(tid < n && tab == cand)) {
+                    if (n == VX3_RES_PHASES) over = true;
+                    else {
+                        if (tid == n) tab = cand;
+                        n++;
+                    }
+                }
+                if (ph == cand) pending = false;   // clears at least lane src
+            }
+        }
+        if (tid < n) sPhase[tid] = __longlong_as_double(tab);
+        if (tid == 0) sNph = over ? -1 : n;
+    }
+    __syncthreads();
+    const int nph = sNph;
+    if (nph < 0) {
+        if (tid == 0) scg.state = 4;   // failed
+        return;
+    }
     V3<real> vm[VPT], va[VPT];
     int vf[VPT], vmat[VPT], vext[VPT];
-    double vph[VPT];
+    int vpi[VPT];         // index of the voxel's phase offset in sPhase
     int vl[VPT][3];       // slots of directions (2i, 2i+1) packed lo | hi << 16, 0xFFFF = none
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
@@ -158,7 +214,10 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             vf[k] = b.vflags[g];
             vmat[k] = localVmat(b.vmat[g]);
             vext[k] = b.vext[g];
-            vph[k] = b.phase[g];
+            const long long ph = __double_as_longlong(b.phase[g]);
+            vpi[k] = 0;
+            for (int i = 0; i < nph; i++)
+                if (__double_as_longlong(sPhase[i]) == ph) vpi[k] = i;
 #pragma unroll
             for (int i = 0; i < 3; i++) vl[k][i] = (int)sVL[(2 * i) * VMAX + v] | ((int)sVL[(2 * i + 1) * VMAX + v] << 16);
         }
@@ -207,6 +266,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             for (int k = 0; k < LPA; k++) { asm volatile("" : "+v"(lEnds[a][k])); asm volatile("" : "+v"(lMats[a][k])); }
 #pragma unroll
         for (int k = 0; k < VPT; k++) { asm volatile("" : "+v"(vl[k][0])); asm volatile("" : "+v"(vl[k][1])); asm volatile("" : "+v"(vl[k][2])); asm volatile("" : "+v"(vmat[k])); }
+        // temperatures of this step's distinct phases (read in the voxel pass, after the link barriers)
+        if (vary && tid < nph) sPhT[tid] = actuationTemp(tempAmp, tempPeriod, sPhase[tid], t + dt, expansion);
         V3<real> fs[VPT], ms[VPT];
 #pragma unroll
         for (int k = 0; k < VPT; k++) { fs[k] = V3<real>(0, 0, 0); ms[k] = V3<real>(0, 0, 0); }
@@ -300,8 +361,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     sOw[v] = vo.w; sOx[v] = vo.x; sOy[v] = vo.y; sOz[v] = vo.z;
                 }
                 if (vary && !m.fixed && !(m.thermalOnAfter > tn)) {
-                    const float temp = actuationTemp(tempAmp, tempPeriod, vph[k], tn, expansion);
-                    sTf[v] = 1 + temp * m.alphaCTE;
+                    sTf[v] = 1 + sPhT[vpi[k]] * m.alphaCTE;
                 }
                 if (poisson) {
                     // CVX_Voxel::strain(true) for the next step from this step's end strains

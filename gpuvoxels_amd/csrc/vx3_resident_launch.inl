@@ -4,6 +4,9 @@
 
 namespace vx3 {
 
+static_assert(VX3_RES_VMATS == RES_MAX_VMATS && VX3_RES_LMATS == RES_MAX_LMATS && VX3_RES_PHASES == RES_MAX_PHASES,
+              "resident kernel table sizes and admission limits agree");
+
 int VX3_RES_TABLE_FN(ResidentShapeInfo* out, int cap) {
 #define VX3_ROW(B, V, L) {B, V, L, VX_NS::ResidentShape<B, V, L>::bytes},
     static const ResidentShapeInfo shapes[] = {VX3_RES_SHAPES(VX3_ROW)};

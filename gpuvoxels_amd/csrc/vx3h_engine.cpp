@@ -557,7 +557,13 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     int shape = -1;
     int perAxis[3] = {0, 0, 0};
     for (int i = 0; i < NL; i++) perAxis[H.laxis[i]]++;
-    const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS;
+    // the resident kernel evaluates the actuation once per distinct phase offset (bit pattern)
+    std::vector<uint64_t> phases(NV);
+    for (int i = 0; i < NV; i++) memcpy(&phases[i], &H.phase[i], 8);
+    std::sort(phases.begin(), phases.end());
+    const int nPhases = (int)(std::unique(phases.begin(), phases.end()) - phases.begin());
+    const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
+                       nPhases <= RES_MAX_PHASES;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
