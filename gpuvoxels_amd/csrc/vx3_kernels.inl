@@ -13,6 +13,7 @@
 //               mass, time advance, stop condition
 #include "vx3_batch.h"
 #include "vx3_physics.h"
+#include "vx3_powf.h"
 
 namespace vx3 {
 namespace VX_NS {
@@ -27,8 +28,11 @@ __device__ __forceinline__ bool simRunning(const SimCtl* sims, int s) {
 // library (pow of float arguments evaluated in double and rounded once; sin in double);
 // the FP32 state path uses single-precision functions on a double-reduced argument.
 __device__ __forceinline__ float poissonValue(float base, float nu) {   // pow(base, -nu) - 1
-    if constexpr (sizeof(real) == 8) return (float)pow((double)base, (double)(-nu)) - 1.0f;
-    else return powf(base, -nu) - 1.0f;
+    if constexpr (sizeof(real) == 8) {
+        float p;
+        if (powNearOne(base, -nu, p)) return p - 1.0f;   // same float as the line below, cheaper
+        return (float)pow((double)base, (double)(-nu)) - 1.0f;
+    } else return powf(base, -nu) - 1.0f;
 }
 // VX3 actuation temperature at time tn (VX3_VoxelyzeKernel.cu:625-650): A*sin(2*3.1415926f*(t/Tp + phase)),
 // contraction only unless expansion is enabled
