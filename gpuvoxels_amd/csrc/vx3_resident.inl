@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     __shared__ double sPhase[VX3_RES_PHASES];   // distinct actuation phase offsets of the sim
     __shared__ float sPhT[VX3_RES_PHASES];      // their temperatures for the step being run
     __shared__ int sNph;                        // table size; -1: more than VX3_RES_PHASES (admission prevents it)
+    __shared__ float sDm[VX3_RES_VMATS];        // CVX_Voxel::dampingMultiplier per voxel material (dt is fixed)
 
     const int s = simList[blockIdx.x];
     SimCtl& scg = b.sims[s];
@@ -123,6 +124,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     }
     __syncthreads();
     if (tid < 3) sCnt[tid] += sExp[tid];   // links of axis tid
+    // 2*sqrt(m)*zeta/previousDt (VX_Voxel.h:131); every step of a launch uses the sim's one dt
+    if (tid < nvm) sDm[tid] = 2 * sVM[tid].sqrtMass * sVM[tid].zetaInternal / (sVM[tid].fixed ? 0.0f : dtf);
     __syncthreads();
     auto localVmat = [&](int gidx) {
         int r = 0;
@@ -304,8 +307,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     area = 0.5f * (size * size + size * size);
                     tss = 0.0f;
                 }
-                const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : dtf);
-                const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : dtf);
+                const float dmN = sDm[(lMats[a][k] >> 8) & 0xFF];
+                const float dmP = sDm[lMats[a][k] >> 16];
                 const V3<real> pN(sPx[vn], sPy[vn], sPz[vn]), pP(sPx[vq], sPy[vq], sPz[vq]);
                 const Q4<real> oN(sOw[vn], sOx[vn], sOy[vn], sOz[vn]), oP(sOw[vq], sOx[vq], sOy[vq], sOz[vq]);
                 V3<real> fN, mN, fP, mP;
