@@ -14,6 +14,10 @@ friction stick/slip amplifies rounding differences, and only FP64 state keeps
 every robot's fitness within 0.5% of the CPU engine over the 20,000-step
 horizon.  The FP32 state path is timed beside it (`f32`).
 
+Unless --block/--steps-per-launch are given, the batch configuration (preferred
+workgroup size, steps per resident launch) is the autotuner's choice
+(vx3h_autotune) on this same workload, reported under `autotune`.
+
 Printed: one JSON line with voxel-steps/s (value), the roofline of the dominant
 kernel (HIP events on the engine's stream), the plain one-simulation-per-launch
 HIP path on the same robots, and the CPU engine timed on this host's cores on a
@@ -203,8 +207,15 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f64")
     ap.add_argument("--path", choices=["auto", "stream", "resident"], default="auto")
-    ap.add_argument("--block", type=int, default=0, help="preferred resident workgroup size (0 = engine default)")
-    ap.add_argument("--steps-per-launch", type=int, default=1000, help="steps of one resident kernel launch")
+    ap.add_argument("--block", type=int, default=None,
+                    help="preferred workgroup size (0 = engine order); default: chosen by the autotuner")
+    ap.add_argument("--steps-per-launch", type=int, default=None,
+                    help="steps of one resident kernel launch; default: chosen by the autotuner")
+    ap.add_argument("--no-autotune", action="store_true", help="block 0 and 1000 steps per launch unless given")
+    ap.add_argument("--tune-steps", type=int, default=2000, help="timed steps per autotuner candidate")
+    ap.add_argument("--tune-warmup", type=int, default=4000,
+                    help="untimed steps per autotuner candidate (the robots' motion, and with it the share of "
+                         "large-angle links, builds up over the first few thousand steps)")
     ap.add_argument("--kernel-steps", type=int, default=200, help="extra per-pass timed steps (stream-path sims only)")
     ap.add_argument("--f32-steps", type=int, default=20000, help="timed steps of the FP32 side run (0 = skip)")
     ap.add_argument("--plain-steps", type=int, default=50)
@@ -244,8 +255,26 @@ def main():
     f64 = args.precision == "f64"
     prec = _abi.PREC_F64 if f64 else _abi.PREC_F32
 
-    dt, kt, all_running, (res_n, res_steps) = run_batch(models, prec, path, args.block, args.steps, args.warmup,
-                                                         args.kernel_steps, args.steps_per_launch)
+    def tuned(p):
+        """(block, steps per launch, autotuner record) for precision p: the given values,
+        else the autotuner's choice on this workload (vx3h_autotune), else the defaults."""
+        if (args.block is not None and args.steps_per_launch is not None) or args.no_autotune:
+            return (args.block or 0), (args.steps_per_launch or 1000), None
+        t0 = time.perf_counter()
+        cfg, trials = engine.autotune(models, precision=p, path=path, warmup_steps=args.tune_warmup,
+                                      trial_steps=args.tune_steps)
+        rec = {"seconds": time.perf_counter() - t0, "trial_steps": args.tune_steps, "warmup_steps": args.tune_warmup,
+               "chosen": {"block_size": cfg["block_size"], "steps_per_launch": cfg["steps_per_launch"]},
+               "trials": [{"block_size": b_, "steps_per_launch": s_, "voxel_steps_per_s": v_}
+                          for b_, s_, _, v_ in trials]}
+        log("autotune (%s): %s in %.1f s" % ("f64" if p == _abi.PREC_F64 else "f32", rec["chosen"], rec["seconds"]))
+        block = args.block if args.block is not None else cfg["block_size"]
+        spl_ = args.steps_per_launch if args.steps_per_launch is not None else cfg["steps_per_launch"]
+        return block, spl_, rec
+
+    block, spl_cfg, tune_rec = tuned(prec)
+    dt, kt, all_running, (res_n, res_steps) = run_batch(models, prec, path, block, args.steps, args.warmup,
+                                                         args.kernel_steps, spl_cfg)
     value = V * args.steps / dt
     log("%s: timed %d steps in %.3f s -> %.3e voxel-steps/s; kernel ms %s" % (args.precision, args.steps, dt, value, kt))
     dom = max(kt, key=kt.get)
@@ -289,14 +318,16 @@ def main():
                                "component, 4 materials, seed %d, gravity+floor friction+2 Hz actuation+Poisson, "
                                "20,000-step horizon" % (len(models), args.seed),
                    "robots": len(models), "voxels": V, "links": L, "path": args.path,
-                   "all_running": all_running},
+                   "block_size": block, "steps_per_launch": spl_cfg, "all_running": all_running},
         "roofline": roof,
     }
+    if tune_rec:
+        out["autotune"] = tune_rec
     if args.f32_steps and f64:
-        dt32, kt32, _, _ = run_batch(models, _abi.PREC_F32, path, args.block, args.f32_steps, args.warmup, 0,
-                                     args.steps_per_launch)
+        block32, spl32, _ = tuned(_abi.PREC_F32)
+        dt32, kt32, _, _ = run_batch(models, _abi.PREC_F32, path, block32, args.f32_steps, args.warmup, 0, spl32)
         out["f32"] = {"value": V * args.f32_steps / dt32, "unit": "voxel-steps/s", "steps": args.f32_steps,
-                      "ms_per_step": dt32 / args.f32_steps * 1e3}
+                      "ms_per_step": dt32 / args.f32_steps * 1e3, "block_size": block32, "steps_per_launch": spl32}
         log("f32: %.3e voxel-steps/s" % out["f32"]["value"])
     if not args.no_plain:
         pv, pdt = plain_path(models, prec, args.plain_steps)

@@ -175,3 +175,8 @@ class ServerCfg(C.Structure):
 class ServerResult(C.Structure):
     _fields_ = [("ticket", C.c_longlong), ("status", SimStatus), ("t_submit", C.c_double), ("t_admit", C.c_double),
                 ("t_done", C.c_double), ("voxel_steps", C.c_double)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("block_size", C.c_int), ("steps_per_launch", C.c_int), ("collision_rebuild_interval", C.c_int),
+                ("voxel_steps_per_s", C.c_double)]

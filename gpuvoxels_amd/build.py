@@ -41,6 +41,7 @@ UNITS = [
     ("vx3h_build.cpp", "cxx", []),
     ("vx3h_vxa.cpp", "cxx", []),
     ("vx3h_server.cpp", "cxx", []),
+    ("vx3h_tune.cpp", "cxx", []),
 ]
 
 

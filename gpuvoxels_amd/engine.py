@@ -55,6 +55,9 @@ def lib():
         "vx3h_server_results": (S, [P, C.POINTER(_abi.ServerResult), C.c_int, C.POINTER(C.c_int)]),
         "vx3h_server_clock": (C.c_double, [P]),
         "vx3h_batch_timing": (S, [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+        "vx3h_autotune": (S, [C.c_int, C.POINTER(C.POINTER(_abi.SimDesc)), C.c_int, C.POINTER(_abi.BatchCfg), C.c_int,
+                              C.c_int, C.c_int, C.POINTER(_abi.BatchCfg), C.POINTER(_abi.TuneTrial), C.c_int,
+                              C.POINTER(C.c_int)]),
     }
     sig.update({
         "vx3h_vxa_parse": (S, [C.c_char_p, C.c_char_p, C.POINTER(P)]),
@@ -170,6 +173,35 @@ def topology(model):
     tv.voxel_links = _p(vl, C.c_int)
     check(lib().vx3h_build_topology(C.byref(d), C.byref(tv)))
     return vneg, vpos, axis, vl.reshape(-1, 6)
+
+
+def autotune(models, precision=_abi.PREC_F64, path=_abi.PATH_AUTO, warmup_steps=200, trial_steps=1000,
+             tune_rebuild_interval=False, device=0, use_graph=1):
+    """vx3h_autotune over Models or Vxas: returns (best configuration as a dict of
+    Batch keyword arguments, [(block_size, steps_per_launch, rebuild_interval,
+    voxel-steps/s), ...] for every candidate measured)."""
+    keep, ptrs = [], []
+    for m in models:
+        if isinstance(m, Vxa):
+            ptrs.append(C.pointer(m.desc))
+        else:
+            d, k = m.desc()
+            keep.append((d, k))
+            ptrs.append(C.pointer(d))
+    arr = (C.POINTER(_abi.SimDesc) * len(ptrs))(*ptrs)
+    base = _abi.BatchCfg()
+    base.precision, base.path, base.use_graph = precision, path, use_graph
+    best = _abi.BatchCfg()
+    cap = 64
+    trials = (_abi.TuneTrial * cap)()
+    n = C.c_int(0)
+    check(lib().vx3h_autotune(device, arr, len(ptrs), C.byref(base), int(warmup_steps), int(trial_steps),
+                              int(bool(tune_rebuild_interval)), C.byref(best), trials, cap, C.byref(n)))
+    cfg = dict(precision=best.precision, path=best.path, block_size=best.block_size,
+               steps_per_launch=best.steps_per_launch, collision_rebuild_interval=best.collision_rebuild_interval,
+               use_graph=best.use_graph)
+    return cfg, [(t.block_size, t.steps_per_launch, t.collision_rebuild_interval, t.voxel_steps_per_s)
+                 for t in trials[: min(n.value, cap)]]
 
 
 class Batch:

@@ -258,6 +258,24 @@ vx3h_status vx3h_calib_copy(int device, size_t bytes, int reps, double* ms_per_c
  * may be NULL to query *n. */
 vx3h_status vx3h_sim_collision_pairs(vx3h_batch* b, int slot, int* pairs, int cap, int* n);
 
+/* ---------------------------------------------------------------- autotuner
+ * No counterpart in the reference, whose launch shapes are fixed in the source
+ * (VX3_VoxelyzeKernel.cu:259-269,566-623).  Runs the given simulations from their
+ * initial state once per candidate configuration, in a fresh batch on `device`:
+ * warmup_steps untimed, then trial_steps timed; *best receives the fastest (in
+ * voxel-steps/s) of base with block_size in {0, 128, 256, 512 (, 1024 for F32)},
+ * steps_per_launch in {250, 500, 1000, 2000} and, only when tune_rebuild_interval is
+ * set and some simulation has collisions, collision_rebuild_interval in
+ * {0, 4, 8, 16}.  The first two never change a result; extra collision rebuilds do.
+ * trials (may be NULL) receives up to cap measured candidates, *n_trials their count. */
+typedef struct vx3h_tune_trial {
+    int    block_size, steps_per_launch, collision_rebuild_interval;
+    double voxel_steps_per_s;
+} vx3h_tune_trial;
+vx3h_status vx3h_autotune(int device, const vx3h_sim_desc* const* descs, int n_descs, const vx3h_batch_cfg* base,
+                          int warmup_steps, int trial_steps, int tune_rebuild_interval, vx3h_batch_cfg* best,
+                          vx3h_tune_trial* trials, int cap, int* n_trials);
+
 /* ---------------------------------------------------------------- serving
  * Continuous batching over one batch (replaces the file-at-a-time worker loop of
  * Voxelyze3/src/Executables/vx3_node_worker.cu:63-96 and

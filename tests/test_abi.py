@@ -20,7 +20,7 @@ STRUCTS = {
     "vx3h_sim_desc": _abi.SimDesc, "vx3h_batch_cfg": _abi.BatchCfg, "vx3h_sim_status": _abi.SimStatus,
     "vx3h_state_view": _abi.StateView, "vx3h_topology_view": _abi.TopologyView,
     "vx3h_report_entry": _abi.ReportEntry, "vx3h_vxa_info": _abi.VxaInfo, "vx3h_server_cfg": _abi.ServerCfg,
-    "vx3h_server_result": _abi.ServerResult,
+    "vx3h_server_result": _abi.ServerResult, "vx3h_tune_trial": _abi.TuneTrial,
 }
 
 
