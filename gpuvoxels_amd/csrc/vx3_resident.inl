@@ -206,6 +206,11 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     int vf[VPT], vmat[VPT], vext[VPT];
     int vpi[VPT];         // index of the voxel's phase offset in sPhase
     int vl[VPT][3];       // slots of directions (2i, 2i+1) packed lo | hi << 16, 0xFFFF = none
+    // CVX_Voxel::strain's per-axis case, fixed for the launch (topology and externals do
+    // not change): bit i = axis i has links on both sides (their strains are averaged),
+    // bit 3+i = axis i is in tension, i.e. has two links or one link and a fixed or
+    // forced degree of freedom on that axis (VX_Voxel.cpp:299-333)
+    int vpm[VPT];
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         const int v = tid + k * BS;
@@ -223,6 +228,14 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                 if (__double_as_longlong(sPhase[i]) == ph) vpi[k] = i;
 #pragma unroll
             for (int i = 0; i < 3; i++) vl[k][i] = (int)sVL[(2 * i) * VMAX + v] | ((int)sVL[(2 * i + 1) * VMAX + v] << 16);
+            const ExtDev* ext = vext[k] >= 0 ? &b.extTab[vext[k]] : nullptr;
+            vpm[k] = 0;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int n = ((vl[k][i] & 0xFFFF) != 0xFFFF) + (((vl[k][i] >> 16) & 0xFFFF) != 0xFFFF);
+                const bool tension = (n == 2) || (ext && n == 1 && ((ext->dofFixed & (1 << i)) || ext->force[i] != 0));
+                vpm[k] |= (n == 2 ? 1 << i : 0) | (tension ? 8 << i : 0);
+            }
         }
     }
     __syncthreads();   // setup tables alias the load buffer: done with them
@@ -367,33 +380,26 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     sTf[v] = 1 + sPhT[vpi[k]] * m.alphaCTE;
                 }
                 if (poisson) {
-                    // CVX_Voxel::strain(true) for the next step from this step's end strains
-                    float st[3] = {0, 0, 0};
-                    int n[3] = {0, 0, 0};
+                    // CVX_Voxel::strain(true) for the next step from this step's end strains:
+                    // per axis the sum of the present ends in direction order, halved when
+                    // both are present; the axes not in tension take the Poisson value
+                    float st[3];
 #pragma unroll
                     for (int i = 0; i < 3; i++) {
-#pragma unroll
-                        for (int e = 0; e < 2; e++) {
-                            const int slot = e ? ((vl[k][i] >> 16) & 0xFFFF) : (vl[k][i] & 0xFFFF);
-                            if (slot == 0xFFFF) continue;
-                            st[i] += sEs[(2 * i + e) * LAX + slot];
-                            n[i]++;
-                        }
+                        const int s0 = vl[k][i] & 0xFFFF, s1 = (vl[k][i] >> 16) & 0xFFFF;
+                        const float x0 = sEs[(2 * i) * LAX + (s0 != 0xFFFF ? s0 : 0)];
+                        const float x1 = sEs[(2 * i + 1) * LAX + (s1 != 0xFFFF ? s1 : 0)];
+                        float sum = s0 != 0xFFFF ? 0.0f + x0 : 0.0f;
+                        if (s1 != 0xFFFF) sum += x1;
+                        st[i] = (vpm[k] & (1 << i)) ? sum * 0.5f : sum;
                     }
-                    const ExtDev* ext = vext[k] >= 0 ? &b.extTab[vext[k]] : nullptr;
-                    bool tension[3];
-#pragma unroll
-                    for (int i = 0; i < 3; i++) {
-                        if (n[i] == 2) st[i] *= 0.5f;
-                        tension[i] = (n[i] == 2) || (ext && (n[i] == 1 && ((ext->dofFixed & (1 << i)) || ext->force[i] != 0)));
-                    }
-                    if (!(tension[0] && tension[1] && tension[2])) {
+                    if ((vpm[k] & 56) != 56) {
                         float add = 0;
 #pragma unroll
-                        for (int i = 0; i < 3; i++) if (tension[i]) add += st[i];
+                        for (int i = 0; i < 3; i++) if (vpm[k] & (8 << i)) add += st[i];
                         const float value = poissonValue(1.0f + add, m.nu);
 #pragma unroll
-                        for (int i = 0; i < 3; i++) if (!tension[i]) st[i] = value;
+                        for (int i = 0; i < 3; i++) if (!(vpm[k] & (8 << i))) st[i] = value;
                     }
                     sPsx[v] = st[0]; sPsy[v] = st[1]; sPsz[v] = st[2];
                 }
