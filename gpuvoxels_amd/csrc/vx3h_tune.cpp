@@ -2,10 +2,10 @@
 //
 // The reference has no tuner: its launch shapes are fixed in the source
 // (VX3_VoxelyzeKernel.cu:259-269, 566-623).  Here the engine has choices that do not
-// change any result -- which resident kernel shape a simulation runs in (the
-// preferred workgroup size), how many steps one resident launch runs, the element
-// passes' workgroup size -- and one that does, the collision rebuild period (extra
-// rebuilds beyond the staleness rule), which is tuned only when the caller allows it.
+// change any result: which resident kernel shape a simulation runs in (the preferred
+// workgroup size), how many steps one resident launch runs, the element passes'
+// workgroup size, and, on request, the collision rebuild period (extra rebuilds
+// beyond the staleness rule add only watched pairs that cannot touch in between).
 // Every candidate runs the same simulations from their initial state in a fresh
 // batch: warm-up steps, then timed steps between two synchronizations; the fastest
 // in voxel-steps per second wins.

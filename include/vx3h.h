@@ -266,8 +266,11 @@ vx3h_status vx3h_sim_collision_pairs(vx3h_batch* b, int slot, int* pairs, int ca
  * voxel-steps/s) of base with block_size in {0, 128, 256, 512 (, 1024 for F32)},
  * steps_per_launch in {250, 500, 1000, 2000} and, only when tune_rebuild_interval is
  * set and some simulation has collisions, collision_rebuild_interval in
- * {0, 4, 8, 16}.  The first two never change a result; extra collision rebuilds do.
- * trials (may be NULL) receives up to cap measured candidates, *n_trials their count. */
+ * {0, 4, 8, 16}.  None of them changes a result: the first two only decide which
+ * threads compute what and when a launch ends; extra collision rebuilds only add
+ * watched pairs that cannot touch before the staleness rule would rebuild anyway
+ * (they contribute zero force).  trials (may be NULL) receives up to cap measured
+ * candidates, *n_trials their count. */
 typedef struct vx3h_tune_trial {
     int    block_size, steps_per_launch, collision_rebuild_interval;
     double voxel_steps_per_s;
