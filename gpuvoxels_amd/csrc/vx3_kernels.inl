@@ -4,11 +4,11 @@
 // One step of every running simulation of the batch (the body of
 // VX3_VoxelyzeKernel::doTimeStep, Voxelyze3/src/VX3/VX3_VoxelyzeKernel.cu:237-359,
 // with the CPU engine's per-step order, Voxelyze3/src/old/Voxelyze.cpp:251-284):
-//   k_poisson   voxel Poisson strain from last step's link end strains   (ν ≠ 0 only)
 //   k_links     beam element of every link, writes the two end loads into the
 //               (direction, voxel) force slots; flags divergence
-//   k_contacts  contact force of every collision pair (collisions only)
-//   k_voxels    gathers the six slots, integrates, writes next step's thermal factor
+//   k_col_*     collision neighbour lists, rebuilt when stale (collisions only)
+//   k_voxels    gathers the six slots, adds contacts, integrates, writes next step's
+//               thermal factor and Poisson strain (ν ≠ 0 only)
 //   k_control   per simulation: step count, divergence, periodic FP64 center of
 //               mass, time advance, stop condition
 #include "vx3_batch.h"
@@ -61,13 +61,11 @@ __device__ __forceinline__ void transverse(const DevBatch<real>& b, int v, int a
     else { area = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sy)); sum = sx + sy; }
 }
 
-// Poisson strain of each voxel, CVX_Voxel::strain(true) (VX_Voxel.cpp:299-333), from the
-// end strains the link pass of the previous step left in the slots.
-__global__ void __launch_bounds__(256) k_poisson(DevBatch<real> b) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= b.vHi) return;
-    const int s = b.vsim[v];
-    if (!simRunning(b.sims, s) || !(b.sims[s].flags & SC_POISSON)) return;
+// Poisson strain of voxel v for the next step, CVX_Voxel::strain(true)
+// (VX_Voxel.cpp:299-333), from the end strains this step's link pass left in the
+// slots (the value the next step's link pass would compute lazily from the same
+// strains).  Called at the end of the voxel pass.
+__device__ __forceinline__ void poissonStrain(const DevBatch<real>& b, int v) {
     const int fl = b.vflags[v];
     float st[3] = {0, 0, 0};
     int n[3] = {0, 0, 0};
@@ -429,6 +427,7 @@ __global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
             b.tfac[v] = 1 + temp * m.alphaCTE;
         }
     }
+    if (sc.flags & SC_POISSON) poissonStrain(b, v);
 }
 
 // ---- MathTree (queue evaluator of Utils/VX3_MathTree.h:50-192), device side, bounded programs.

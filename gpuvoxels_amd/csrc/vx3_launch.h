@@ -6,12 +6,12 @@
 
 namespace vx3 {
 struct StepOpts {
-    bool poisson;      // any resident sim has a non-zero Poisson's ratio
     bool contacts;     // some simulation has collisions: run the collision passes
     int block;         // workgroup size of the element passes
 };
 // Kernel passes of one step, in launch order; ev (optional) receives a start/stop
-// event pair per pass: ev[2*k], ev[2*k+1].
+// event pair per pass: ev[2*k], ev[2*k+1].  PASS_POISSON no longer runs on its own
+// (the voxel pass computes the Poisson strain); its index keeps the ABI's numbering.
 enum { PASS_POISSON = 0, PASS_LINKS, PASS_CONTACTS, PASS_VOXELS, PASS_CONTROL, PASS_COUNT };
 void launchStep(const DevBatch<float>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);
 void launchStep(const DevBatch<double>& b, hipStream_t s, const StepOpts& o, hipEvent_t* ev = nullptr);

@@ -25,7 +25,6 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
         launch;                                             \
         if (ev) hipEventRecord(ev[2 * (k) + 1], s);         \
     } while (0)
-    if (o.poisson) VX3_EV(PASS_POISSON, hipLaunchKernelGGL(VX_NS::k_poisson, dim3(gv), dim3(bs), 0, s, b));
     if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
     if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs));
     VX3_EV(PASS_VOXELS, hipLaunchKernelGGL(VX_NS::k_voxels, dim3(gv), dim3(bs), 0, s, b));

@@ -157,7 +157,7 @@ struct vx3h_batch {
     void* dTok = nullptr; size_t capTok = 0;
     void* dSimMats = nullptr; size_t capSimMats = 0;
     DevPool nbPool, hashPool, nearPool;
-    bool anyPoisson = false, anyColl = false;
+    bool anyColl = false;
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
     // resident path: kernel shapes and, per shape, the sims it runs
@@ -304,7 +304,6 @@ char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx;
 
 vx3h_status launchSteps(vx3h_batch* b, int n) {
     StepOpts o;
-    o.poisson = b->anyPoisson;
     o.contacts = b->anyColl;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
@@ -688,7 +687,6 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         sh.mass[i] = H.vmats[H.vmat[i]].mass;
     }
     b->bytesUsed += need;
-    if (H.anyPoisson) b->anyPoisson = true;
     d.vHi = b->va.hi;
     d.lHi = b->la.hi;
     d.nSims = (int)b->slots.size();
@@ -736,7 +734,6 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     std::vector<hipEvent_t> ev(2 * PASS_COUNT * n_steps, nullptr);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     StepOpts o;
-    o.poisson = b->anyPoisson;
     o.contacts = b->anyColl;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
@@ -746,7 +743,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(b->stream));
-    const bool ran[PASS_COUNT] = {o.poisson, b->db.lHi > 0, o.contacts, true, true};
+    const bool ran[PASS_COUNT] = {false, b->db.lHi > 0, o.contacts, true, true};   // Poisson: part of the voxel pass
     for (int k = 0; k < PASS_COUNT; k++) {
         if (!ran[k] || b->nStreamSims == 0) continue;
         double tot = 0;

@@ -234,8 +234,9 @@ vx3h_status vx3h_build_topology(const vx3h_sim_desc* desc, vx3h_topology_view* v
  * commented-out clock() in VX3_VoxelyzeKernel.cu:238-239,353-357): runs n_steps
  * steps without graph capture, with HIP events recorded on the batch's stream
  * around every kernel pass, and returns each pass's mean duration in ms
- * (avg_ms[k], k = 0 Poisson, 1 links, 2 contacts, 3 voxels, 4 control,
- * 5 resident multi-step kernel; 0 when the pass did not run). */
+ * (avg_ms[k], k = 0 Poisson (now part of the voxel pass: always 0), 1 links,
+ * 2 contacts, 3 voxels, 4 control, 5 resident multi-step kernel; 0 when the pass
+ * did not run). */
 #define VX3H_N_PASSES 6
 vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, double* avg_ms);
 
