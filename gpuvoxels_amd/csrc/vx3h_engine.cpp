@@ -168,6 +168,13 @@ struct vx3h_batch {
     std::vector<size_t> capRes;
     bool resDirty = false;
     int nStreamSims = 0;
+    // one stream per resident shape: each shape's launches run beside the stream-path
+    // steps of the main stream (disjoint simulations) and beside the other shapes;
+    // fork/join events order them with the main stream around every chunk of steps
+    std::vector<hipStream_t> resStreams;
+    std::vector<hipEvent_t> joinEv;
+    std::vector<bool> joinPending;
+    hipEvent_t forkEv = nullptr;
     // graph of `graphSteps` steps, valid for one layout version
     hipGraphExec_t graphExec = nullptr;
     hipGraph_t graph = nullptr;
@@ -353,12 +360,27 @@ vx3h_status takeEvent(vx3h_batch* b, hipEvent_t& e) {
     return VX3H_OK;
 }
 
+// Launches every resident shape's kernel for nSteps steps, each on its own stream
+// after the main stream's work so far; joinResident makes the main stream wait for
+// them.  Between the two the caller enqueues the stream-path steps on the main stream.
 vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
     vx3h_status st = syncResLists(b);
     if (st) return st;
+    bool forked = false;
     for (size_t i = 0; i < b->resLists.size(); i++) {
         const int n = (int)b->resLists[i].size();
         if (!n) continue;
+        if (!forked) {
+            if (!b->forkEv) HIPCHK(hipEventCreateWithFlags(&b->forkEv, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(b->forkEv, b->stream));
+            forked = true;
+        }
+        if (!b->resStreams[i]) {
+            HIPCHK(hipStreamCreateWithFlags(&b->resStreams[i], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&b->joinEv[i], hipEventDisableTiming));
+        }
+        HIPCHK(hipStreamWaitEvent(b->resStreams[i], b->forkEv, 0));
+        b->joinPending[i] = true;
         hipEvent_t pair[2];
         hipEvent_t* evp = ev ? ev + 2 * i : nullptr;
         if (!ev && b->timing) {
@@ -367,10 +389,21 @@ vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
             b->timedSteps += nSteps;
             evp = pair;
         }
+        hipStream_t rs = b->resStreams[i];
         hipError_t e = b->realSize == 4
-            ? launchResident(b->db, b->stream, (int)i, b->dResLists[i], n, nSteps, evp)
-            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, (int)i, b->dResLists[i], n, nSteps, evp);
+            ? launchResident(b->db, rs, (int)i, b->dResLists[i], n, nSteps, evp)
+            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), rs, (int)i, b->dResLists[i], n, nSteps, evp);
         if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("resident launch failed: ") + hipGetErrorString(e));
+    }
+    return VX3H_OK;
+}
+
+vx3h_status joinResident(vx3h_batch* b) {
+    for (size_t i = 0; i < b->resStreams.size(); i++) {
+        if (!b->joinPending[i]) continue;
+        HIPCHK(hipEventRecord(b->joinEv[i], b->resStreams[i]));
+        HIPCHK(hipStreamWaitEvent(b->stream, b->joinEv[i], 0));
+        b->joinPending[i] = false;
     }
     return VX3H_OK;
 }
@@ -399,6 +432,9 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
         b->resLists.assign(ns, {});
         b->dResLists.assign(ns, nullptr);
         b->capRes.assign(ns, 0);
+        b->resStreams.assign(ns, nullptr);
+        b->joinEv.assign(ns, nullptr);
+        b->joinPending.assign(ns, false);
         // default preference: smallest voxel capacity first (more workgroups per CU), table order on ties
         for (int i = 0; i < ns; i++) b->shapePref.push_back(i);
         std::stable_sort(b->shapePref.begin(), b->shapePref.end(), [&](int x, int y) {
@@ -421,6 +457,11 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
     if (!b) return VX3H_OK;
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
+    for (size_t i = 0; i < b->resStreams.size(); i++) {
+        if (b->resStreams[i]) { hipStreamSynchronize(b->resStreams[i]); hipStreamDestroy(b->resStreams[i]); }
+        if (b->joinEv[i]) hipEventDestroy(b->joinEv[i]);
+    }
+    if (b->forkEv) hipEventDestroy(b->forkEv);
     if (b->graphExec) hipGraphExecDestroy(b->graphExec);
     if (b->graph) hipGraphDestroy(b->graph);
     for (auto* arrs : {&b->vArrs, &b->lArrs})
@@ -712,15 +753,19 @@ extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
     for (int done = 0; done < n_steps;) {
         const int c = std::min(chunk, n_steps - done);
         vx3h_status st = launchResidentAll(b, c, nullptr);
-        if (st) return st;
-        if (b->nStreamSims > 0) {
+        if (!st && b->nStreamSims > 0) {
             if (b->cfg.use_graph) {
                 const int G = std::min(c, 32);
-                if ((st = ensureGraph(b, G))) return st;
-                for (int r = 0; r < c / G; r++) HIPCHK(hipGraphLaunch(b->graphExec, b->stream));
-                if (c % G && (st = launchSteps(b, c % G))) return st;
-            } else if ((st = launchSteps(b, c))) return st;
+                st = ensureGraph(b, G);
+                for (int r = 0; !st && r < c / G; r++)
+                    if (hipGraphLaunch(b->graphExec, b->stream) != hipSuccess) { g_err = "graph launch failed"; st = VX3H_EDEVICE; }
+                if (!st && c % G) st = launchSteps(b, c % G);
+            } else st = launchSteps(b, c);
         }
+        // the main stream waits for the resident launches, also after a failure
+        const vx3h_status js = joinResident(b);
+        if (st) return st;
+        if (js) return js;
         done += c;
     }
     return VX3H_OK;
@@ -759,7 +804,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     std::vector<hipEvent_t> rev(2 * b->resLists.size(), nullptr);
     for (auto& e : rev) HIPCHK(hipEventCreate(&e));
     vx3h_status st = launchResidentAll(b, n_steps, rev.data());
-    if (st) return st;
+    if (st || (st = joinResident(b))) return st;
     HIPCHK(hipStreamSynchronize(b->stream));
     for (size_t i = 0; i < b->resLists.size(); i++) {
         if (b->resLists[i].empty()) continue;
