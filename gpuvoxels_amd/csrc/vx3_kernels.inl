@@ -380,7 +380,9 @@ struct Contacts {
     }
 };
 
-__global__ void __launch_bounds__(256) k_voxels(DevBatch<real> b) {
+// at most 128 VGPRs (four waves per SIMD): the voxel pass is a gather over the force
+// slots, latency-bound at the three waves its unconstrained allocation allows
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_voxels(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
