@@ -33,6 +33,11 @@ template <typename T> struct DevBatch {
     // collisions: per voxel the watched partners in ascending index order (both
     // directions), the order CVX_Voxel::force walks its colWatch list
     float* lastWatch;         // 3*Vcap position at the last rebuild (CVX_Voxel::lastColWatchPosition)
+    // contact snapshot: position, linear momentum and thermal factor of every voxel of a
+    // collision sim as they are before the voxel pass, which reads its partners' values
+    // from here while it updates the live arrays in place
+    T *cpx, *cpy, *cpz, *cmx, *cmy, *cmz;
+    float* ctf;
     int *nbStart, *nbCount;   // per voxel range in nbList
     int *nearStart, *nearCount;  // per voxel range in nearList: surface voxels within the link-hop radius
     int* nearList;            // sim-local indices, ascending

@@ -325,6 +325,19 @@ __global__ void __launch_bounds__(1024) k_col_scan(DevBatch<real> b) {
     }
 }
 
+// Contact snapshot (see DevBatch::cpx): the reference computes every contact force
+// from the state before any voxel moves this step (CVoxelyze::updateCollisions ahead of
+// the voxel loop, Voxelyze.cpp:263-279).
+__global__ void __launch_bounds__(256) k_col_snap(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!simRunning(b.sims, s) || !(b.sims[s].flags & SC_COLLISIONS) || (b.vflags[v] & VF_INTERIOR)) return;
+    b.cpx[v] = b.px[v]; b.cpy[v] = b.py[v]; b.cpz[v] = b.pz[v];
+    b.cmx[v] = b.mx[v]; b.cmy[v] = b.my[v]; b.cmz[v] = b.mz[v];
+    b.ctf[v] = b.tfac[v];
+}
+
 __global__ void __launch_bounds__(256) k_col_emit(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
@@ -358,8 +371,9 @@ struct Contacts {
             const VoxMatDev& m2 = b->vmatTab[b->vmat[j]];
             const float kp = 2.0f / (1.0f / m1.penetrationStiffness + 1.0f / m2.penetrationStiffness);
             const float cd = 0.5f * (m1.zetaCollision * m1.c2xSqMxExS + m2.zetaCollision * m2.c2xSqMxExS);
-            const float ox = (float)(b->px[j] - b->px[i]), oy = (float)(b->py[j] - b->py[i]), oz = (float)(b->pz[j] - b->pz[i]);
-            const real b1 = nom * (real)b->tfac[i], b2 = nom * (real)b->tfac[j];
+            // both voxels' state from the contact snapshot (pre-step values)
+            const float ox = (float)(b->cpx[j] - b->cpx[i]), oy = (float)(b->cpy[j] - b->cpy[i]), oz = (float)(b->cpz[j] - b->cpz[i]);
+            const real b1 = nom * (real)b->ctf[i], b2 = nom * (real)b->ctf[j];
             const float nomDist = (float)(((b1 + b1 + b1) / (real)3.0f + (b2 + b2 + b2) / (real)3.0f) * (real)0.625f);
             const float len = sqrtf(ox * ox + oy * oy + oz * oz);
             const float rel = nomDist - len;
@@ -368,8 +382,8 @@ struct Contacts {
                 float ux = ox, uy = oy, uz = oz;
                 if (len > 0) { const float inv = 1.0f / len; ux = inv * ox; uy = inv * oy; uz = inv * oz; }
                 const real mi1 = (real)m1.massInverse, mi2 = (real)m2.massInverse;
-                const real v1 = (b->mx[i] * mi1) * (real)ux + (b->my[i] * mi1) * (real)uy + (b->mz[i] * mi1) * (real)uz;
-                const real v2 = (b->mx[j] * mi2) * (real)ux + (b->my[j] * mi2) * (real)uy + (b->mz[j] * mi2) * (real)uz;
+                const real v1 = (b->cmx[i] * mi1) * (real)ux + (b->cmy[i] * mi1) * (real)uy + (b->cmz[i] * mi1) * (real)uz;
+                const real v2 = (b->cmx[j] * mi2) * (real)ux + (b->cmy[j] * mi2) * (real)uy + (b->cmz[j] * mi2) * (real)uz;
                 const float relVel = (float)(v1 - v2);
                 const float mag = kp * rel + cd * relVel;
                 fx = ux * mag; fy = uy * mag; fz = uz * mag;

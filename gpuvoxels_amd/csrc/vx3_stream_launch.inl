@@ -12,6 +12,7 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs) {
     hipLaunchKernelGGL(VX_NS::k_col_count, dim3(gv), dim3(bs), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_scan, dim3(b.nSims), dim3(1024), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_emit, dim3(gv), dim3(bs), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_snap, dim3(gv), dim3(bs), 0, s, b);
 }
 }  // namespace
 

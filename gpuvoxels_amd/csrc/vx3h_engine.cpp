@@ -189,7 +189,7 @@ struct vx3h_batch {
 
 namespace {
 
-size_t voxelBytes(size_t rs) { return rs * (13 + 36) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 8; }
+size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8; }
 size_t linkBytes(size_t rs) { return rs * 9 + 4 * 5 + 4 * 3; }
 
 void initSpecs(vx3h_batch* b) {
@@ -206,6 +206,8 @@ void initSpecs(vx3h_batch* b) {
     V(&d.psx, 4, 1, false); V(&d.psy, 4, 1, false); V(&d.psz, 4, 1, false);
     V(&d.F, R, 36, true); V(&d.es, 4, 6, true);
     V(&d.lastWatch, 4, 3, false); V(&d.nbStart, 4, 1, false); V(&d.nbCount, 4, 1, false);
+    V(&d.cpx, R, 1, false); V(&d.cpy, R, 1, false); V(&d.cpz, R, 1, false);
+    V(&d.cmx, R, 1, false); V(&d.cmy, R, 1, false); V(&d.cmz, R, 1, false); V(&d.ctf, 4, 1, false);
     V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
     b->nbPool.arrs = {{(void**)&d.nbList, 4}};
     b->hashPool.arrs = {{(void**)&d.hashCnt, 4}, {(void**)&d.hashEnt, 4 * VX3_COL_BUCKET}, {(void**)&d.hashSim, 4}};

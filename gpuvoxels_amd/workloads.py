@@ -134,3 +134,36 @@ def drop(n: int = 3, gap: int = 1, shift: int = 1, steps: int = 2000, voxel_size
     m.vary_temp_enabled = 0
     m.collisions_enabled = 1
     return m
+
+
+def balls(n_side: int = 4, layers: int = 2, radius: int = 4, gap: float = 1.5, steps: int = 3000,
+          voxel_size: float = 0.01, seed: int = 0) -> Model:
+    """Many voxel balls in one simulation (the many-body collision configuration):
+    n_side x n_side balls of the given radius per layer, centres `2*radius + 1 + gap`
+    voxels apart, each layer shifted by half a spacing and raised by one spacing, so
+    upper balls fall into the gaps of the lower layer (ball-ball and floor contacts).
+    Ball voxels: lattice cells within `radius` of the centre (257 for radius 4)."""
+    m = Model(voxel_size=voxel_size, name="balls")
+    m.materials = [base_material(cte=0.0, zeta_global=1e-4)]   # light air drag: the balls fall freely
+    r = radius
+    d = np.arange(-r, r + 1)
+    z, y, x = np.meshgrid(d, d, d, indexing="ij")
+    inside = x * x + y * y + z * z <= r * r
+    ball = np.stack([x[inside], y[inside], z[inside]], axis=1)
+    spacing = 2 * r + 1 + gap
+    cells = []
+    for layer in range(layers):
+        off = 0.5 * spacing * (layer % 2)
+        for i in range(n_side):
+            for j in range(n_side):
+                c = np.array([r + off + i * spacing, r + off + j * spacing, r + layer * spacing])
+                cells.append(ball + np.round(c).astype(np.int64))
+    cells = np.concatenate(cells)
+    order = np.lexsort((cells[:, 0], cells[:, 1], cells[:, 2]))    # structure-index order
+    m.ijk = cells[order].astype(np.int32)
+    assert len(np.unique(m.ijk, axis=0)) == len(m.ijk), "balls overlap"
+    m.mat = np.zeros(len(cells), np.int32)
+    _physics(m, steps)
+    m.vary_temp_enabled = 0
+    m.collisions_enabled = 1
+    return m

@@ -60,3 +60,28 @@ def test_drop_f32_within_tolerance():
     m = workloads.drop(3, gap=1, shift=1, steps=0)
     op, gp, os_, gs, ost, gst = run_both(m, 1000, 1, precision=_abi.PREC_F32)[-1]
     assert np.max(np.abs(os_["pos"] - gs["pos"])) / (6 * m.voxel_size) < 1e-4
+
+
+def test_many_balls_pairs_and_trajectory_f64():
+    """18 balls of 257 voxels in one simulation (two offset layers): the upper layer
+    falls into the lower one; pair lists equal the oracle's at every checkpoint and
+    some cross-ball pair comes within the contact envelope."""
+    m = workloads.balls(3, 2, 4, 1.5, steps=0)
+    ball = np.zeros(m.n_voxels, np.int64)
+    # ball id of each voxel: nearest of the generating centres, recovered by clustering the lattice cells
+    from scipy.ndimage import label
+    grid = np.zeros(m.ijk.max(axis=0) + 1, bool)
+    grid[tuple(m.ijk.T)] = True
+    lab, n = label(grid)
+    assert n == 18
+    ball[:] = lab[tuple(m.ijk.T)]
+    contact = False
+    for op, gp, os_, gs, ost, gst in run_both(m, 4800, 6):
+        assert gst.state == ost.state == _abi.SIM_RUNNING
+        assert np.array_equal(op, gp)
+        assert np.max(np.abs(os_["pos"] - gs["pos"])) < 1e-9
+        if len(gp):
+            d = np.linalg.norm(gs["pos"][gp[:, 0]] - gs["pos"][gp[:, 1]], axis=1)
+            cross = ball[gp[:, 0]] != ball[gp[:, 1]]
+            contact |= bool(np.any(d[cross] < 1.25 * m.voxel_size))
+    assert contact, "no two balls touched"
