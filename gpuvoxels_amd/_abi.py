@@ -39,6 +39,7 @@ class MaterialDesc(C.Structure):
         ("fixed", C.c_int),
         ("thermal_on_after", C.c_double),
         ("matid", C.c_int),
+        ("color", C.c_int * 4),
     ]
 
 
@@ -117,6 +118,7 @@ class SimStatus(C.Structure):
         ("fitness", C.c_double),
         ("total_distance_of_all_voxels", C.c_double),
         ("recent_angle", C.c_double),
+        ("recommended_dt", C.c_double),
     ]
 
 
@@ -165,6 +167,7 @@ class VxaInfo(C.Structure):
         ("record_step_size", C.c_int),
         ("max_dist_close_pairs", C.c_double),
         ("has_force_field", C.c_int),
+        ("record_voxel", C.c_int), ("record_link", C.c_int),
     ]
 
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <sstream>
 #include <stdexcept>
@@ -63,8 +64,8 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
     cfg.path = opt.path;
     vx3h_batch* b = nullptr;
     check(vx3h_batch_create(device, &cfg, &b), "batch creation");
-    struct Job { std::string name; int slot; vx3h_vxa* v; };
-    std::vector<Job> jobs;
+    struct Job { std::string name; int slot; vx3h_vxa* v; vx3h_vxa_info info; };
+    std::vector<Job> jobs, history;
     try {
         for (const std::string& f : files) {
             const std::string vxd = readFile(joinPath(input_dir, f));
@@ -84,11 +85,14 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
                 if (!opt.allow_unsupported) { vx3h_vxa_free(v); throw std::runtime_error(f + ": not supported by this engine:" + missing); }
                 fprintf(stderr, "warning: %s: ignoring%s\n", f.c_str(), missing.c_str());
             }
-            if (info.record_step_size > 0) fprintf(stderr, "warning: %s: history recording is not produced\n", f.c_str());
+            if (info.record_step_size > 0) {   // recorded simulations run alone, after the batch
+                history.push_back({baseName(f), -1, v, info});
+                continue;
+            }
             int slot = -1;
             vx3h_status st = vx3h_sim_admit(b, vx3h_vxa_desc(v), &slot);
             if (st != VX3H_OK) { vx3h_vxa_free(v); check(st, f); }
-            jobs.push_back({baseName(f), slot, v});
+            jobs.push_back({baseName(f), slot, v, info});
         }
         // run until every simulation is done or diverged
         for (;;) {
@@ -101,11 +105,11 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             if (!running) break;
             check(vx3h_batch_step(b, opt.steps_per_poll), "step");
         }
-        int n = 0;
-        check(vx3h_batch_poll(b, nullptr, 0, &n), "poll");
-        std::vector<vx3h_sim_status> sts(std::max(n, 1));
-        check(vx3h_batch_poll(b, sts.data(), n, &n), "poll");
-        for (const Job& j : jobs) {
+        auto collect = [&](vx3h_batch* bt, const Job& j) {
+            int n = 0;
+            check(vx3h_batch_poll(bt, nullptr, 0, &n), "poll");
+            std::vector<vx3h_sim_status> sts(std::max(n, 1));
+            check(vx3h_batch_poll(bt, sts.data(), n, &n), "poll");
             const vx3h_sim_status* s = nullptr;
             for (int i = 0; i < n; i++) if (sts[i].slot == j.slot) s = &sts[i];
             if (!s) throw std::runtime_error(j.name + ": no status");
@@ -134,20 +138,79 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
                 }
                 vx3h_state_view view{};
                 view.pos = r.voxel_position.data();
-                check(vx3h_sim_read_state(b, j.slot, &view), j.name);
+                check(vx3h_sim_read_state(bt, j.slot, &view), j.name);
             }
             if (opt.verbose)
                 printf("%d) Simulation %s ends: Time: %f%s.\n", device, j.name.c_str(), r.currentTime,
                        r.state == VX3H_SIM_DIVERGED ? " (diverged)" : "");
             h_results.push_back(std::move(r));
+        };
+        for (const Job& j : jobs) collect(b, j);
+        vx3h_batch_destroy(b);
+        b = nullptr;
+        for (Job& j : history) {
+            check(vx3h_batch_create(device, &cfg, &b), "batch creation");
+            check(vx3h_sim_admit(b, vx3h_vxa_desc(j.v), &j.slot), j.name);
+            runHistory(b, j.slot, j.info, vx3h_vxa_desc(j.v)->dt_frac);
+            collect(b, j);
+            vx3h_batch_destroy(b);
+            b = nullptr;
         }
     } catch (...) {
         for (Job& j : jobs) vx3h_vxa_free(j.v);
-        vx3h_batch_destroy(b);
+        for (Job& j : history) vx3h_vxa_free(j.v);
+        if (b) vx3h_batch_destroy(b);
         throw;
     }
     for (Job& j : jobs) vx3h_vxa_free(j.v);
-    vx3h_batch_destroy(b);
+    for (Job& j : history) vx3h_vxa_free(j.v);
+}
+
+// The playback stream of one simulation (VX3_SimulationManager.cu:40-108): the
+// settings header, the real_stepsize line, then one record after every step j (the
+// loop index, steps done - 1) with j % real_stepsize == 0, up to the stop condition,
+// a divergence or 1000000 steps.
+void SimulationManager::runHistory(vx3h_batch* b, int slot, const vx3h_vxa_info& info, double dtFrac) {
+    FILE* out = opt.history_out ? opt.history_out : stdout;
+    std::string text;
+    auto fetch = [&](auto fn, auto... args) {
+        size_t len = 0;
+        check(fn(b, slot, args..., nullptr, 0, &len), "history");
+        text.assign(len, '\0');
+        check(fn(b, slot, args..., &text[0], len, &len), "history");
+        text.resize(strlen(text.c_str()));
+        fputs(text.c_str(), out);
+    };
+    auto status = [&]() {
+        int n = 0;
+        check(vx3h_batch_poll(b, nullptr, 0, &n), "poll");
+        std::vector<vx3h_sim_status> sts(std::max(n, 1));
+        check(vx3h_batch_poll(b, sts.data(), n, &n), "poll");
+        for (int i = 0; i < n; i++) if (sts[i].slot == slot) return sts[i];
+        throw std::runtime_error("history: no status");
+    };
+    fetch(vx3h_sim_history_header);
+    // the engine computes the recommended step on the device with the first step, so
+    // the first step runs before the real_stepsize line is printed (the text is the same)
+    check(vx3h_batch_step(b, 1), "step");
+    vx3h_sim_status s = status();
+    const int real_stepsize =
+        s.recommended_dt > 0 ? int(info.record_step_size / (10000 * s.recommended_dt * dtFrac)) + 1 : 1;
+    fprintf(out, "real_stepsize: %d ; recommendedTimeStep %f; d_v3->DtFrac %f . \n", real_stepsize, s.recommended_dt, dtFrac);
+    const long long maxSteps = 1000000;
+    long long done = s.steps;
+    if (done == 1 && s.state != VX3H_SIM_DIVERGED) fetch(vx3h_sim_history_record, 0, info.record_voxel, info.record_link);
+    while (s.state == VX3H_SIM_RUNNING && done >= 1 && done < maxSteps) {
+        // the next recorded step count: 1 + a multiple of real_stepsize
+        const long long next = std::min(maxSteps, done + real_stepsize - (done - 1) % real_stepsize);
+        check(vx3h_batch_step(b, (int)(next - done)), "step");
+        s = status();
+        if (s.steps <= done) break;
+        done = s.steps;
+        if (s.state == VX3H_SIM_DIVERGED) break;
+        if ((done - 1) % real_stepsize == 0) fetch(vx3h_sim_history_record, (int)(done - 1), info.record_voxel, info.record_link);
+    }
+    fflush(out);
 }
 
 }  // namespace vx3

@@ -9,6 +9,7 @@
 #ifndef VX3_MANAGER_H
 #define VX3_MANAGER_H
 
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,7 @@ struct ManagerOptions {
     int steps_per_poll = 2000;          // steps between host polls of the batch
     bool allow_unsupported = false;     // run VXAs that enable features this engine ignores
     bool verbose = true;
+    FILE* history_out = nullptr;        // playback stream of RecordHistory simulations [stdout]
 };
 
 class SimulationManager {
@@ -58,6 +60,7 @@ private:
     int num_of_devices;
     ManagerOptions opt;
     void runDevice(int device, const std::vector<std::string>& files);
+    void runHistory(vx3h_batch* b, int slot, const vx3h_vxa_info& info, double dtFrac);
 };
 
 }  // namespace vx3

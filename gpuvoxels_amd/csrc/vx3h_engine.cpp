@@ -9,6 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdarg>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -111,6 +114,12 @@ struct SlotHost {
     size_t bytes = 0;
     // collision pools: neighbour lists, hash buckets, nearby lists
     int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0;
+    // history records: topology and link constants of the sim, palette in order
+    std::vector<int> vlinks;     // 6 per voxel, sim-local link index or -1
+    std::vector<char> surface;   // per voxel: not all six links present
+    std::vector<int> lneg, lpos; // per link, sim-local voxel indices
+    std::vector<float> lratio, lfail;   // per link: strainRatio, epsilonFail (-1 = none)
+    std::vector<std::array<int, 5>> palette;   // matid, r, g, b, a
 };
 
 // Device arrays sharing one index space of first-fit ranges (collision pools).
@@ -721,6 +730,18 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN;
     if (shape >= 0) { b->resLists[shape].push_back(s); b->resDirty = true; }
     else b->nStreamSims++;
+    sh.vlinks = H.voxelLinks;
+    sh.surface.resize(NV);
+    for (int i = 0; i < NV; i++) sh.surface[i] = (H.vflags[i] & VF_INTERIOR) ? 0 : 1;
+    sh.lneg = H.lneg;
+    sh.lpos = H.lpos;
+    sh.lratio.resize(NL);
+    sh.lfail.resize(NL);
+    for (int i = 0; i < NL; i++) { sh.lratio[i] = H.lmats[H.lmat[i]].strainRatio; sh.lfail[i] = H.lmats[H.lmat[i]].epsilonFail; }
+    for (int i = 0; i < desc->n_materials; i++) {
+        const vx3h_material_desc& md = desc->materials[i];
+        sh.palette.push_back({md.matid, md.color[0], md.color[1], md.color[2], md.color[3]});
+    }
     sh.matid.resize(NV);
     sh.measured.resize(NV);
     sh.mass.resize(NV);
@@ -897,6 +918,7 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
     o.num_voxel = sh.nv;
     o.current_time = c.t;
     o.dt = c.dt;
+    o.recommended_dt = c.optDt;
     o.recent_angle = c.recentAngle;
     for (int k = 0; k < 3; k++) o.initial_com[k] = sh.com0[k];
     std::vector<double> x, y, z;
@@ -1067,6 +1089,121 @@ extern "C" vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_v
         for (int i = 0; i < sh.nl; i++) view->link_small_angle[i] = (f[i] & LF_SMALL_ANGLE) ? 1 : 0;
     }
     return VX3H_OK;
+}
+
+namespace {
+
+void appendf(std::string& out, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void appendf(std::string& out, const char* fmt, ...) {
+    char tmp[256];
+    va_list ap;
+    va_start(ap, fmt);
+    const int n = vsnprintf(tmp, sizeof(tmp), fmt, ap);
+    va_end(ap);
+    if (n < (int)sizeof(tmp)) { out.append(tmp, n > 0 ? n : 0); return; }
+    std::vector<char> big(n + 1);
+    va_start(ap, fmt);
+    vsnprintf(big.data(), big.size(), fmt, ap);
+    va_end(ap);
+    out.append(big.data(), n);
+}
+
+vx3h_status putText(const std::string& t, char* buf, size_t cap, size_t* len) {
+    if (len) *len = t.size() + 1;
+    if (buf && cap) {
+        const size_t k = std::min(cap - 1, t.size());
+        memcpy(buf, t.data(), k);
+        buf[k] = 0;
+    }
+    return VX3H_OK;
+}
+
+}  // namespace
+
+// VX3_SimulationManager.cu:40-49
+extern "C" vx3h_status vx3h_sim_history_header(vx3h_batch* b, int slot, char* buf, size_t cap, size_t* len) {
+    if (!b) FAILV(VX3H_EINVAL, "null batch");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    const SlotHost& sh = b->slots[slot];
+    std::string t;
+    appendf(t, "\n{{{setting}}}<rescale>0.001</rescale>\n");
+    for (const auto& m : sh.palette)
+        appendf(t, "{{{setting}}}<matcolor><id>%d</id><r>%.2f</r><g>%.2f</g><b>%.2f</b><a>%.2f</a></matcolor>\n", m[0],
+                m[1] / 255., m[2] / 255., m[3] / 255., m[4] / 255.);
+    appendf(t, "\n{{{setting}}}<voxel_size>%f</voxel_size>\n", sh.voxSize);
+    return putText(t, buf, cap, len);
+}
+
+// VX3_SimulationManager.cu:70-112 with CVX_Voxel::cornerOffset (VX3_Voxel.cu:147-159) and
+// Quat3D::AngleDegrees (Utils/VX3_Quat3D.h:301-306) evaluated on the host from the
+// device state
+extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step_label, int record_voxel, int record_link,
+                                               char* buf, size_t cap, size_t* len) {
+    if (!b) FAILV(VX3H_EINVAL, "null batch");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    const SlotHost& sh = b->slots[slot];
+    const int NV = sh.nv, NL = sh.nl;
+    std::vector<double> pos(3 * (size_t)NV), ori(4 * (size_t)NV), tf(NV), strain(NL), maxStrain(NL);
+    vx3h_state_view view{};
+    view.pos = pos.data();
+    view.orient = ori.data();
+    view.temp = tf.data();
+    view.link_strain = strain.data();
+    view.link_max_strain = maxStrain.data();
+    vx3h_status st = vx3h_sim_read_state(b, slot, &view);
+    if (st) return st;
+    int n = 0;
+    if ((st = vx3h_batch_poll(b, nullptr, 0, &n))) return st;
+    SimCtl c;
+    HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(SimCtl), hipMemcpyDeviceToHost));
+    const double vs = 1 / 0.001;
+    std::string t;
+    if (record_voxel) {
+        appendf(t, "<<<Step%d Time:%f>>>", step_label, c.t);
+        for (int v = 0; v < NV; v++) {
+            if (!sh.surface[v]) continue;
+            const double* p = &pos[3 * (size_t)v];
+            const double* q = &ori[4 * (size_t)v];
+            appendf(t, "%.1f,%.1f,%.1f,", p[0] * vs, p[1] * vs, p[2] * vs);
+            const double w = q[0] > 1 ? 1 : q[0];
+            appendf(t, "%.1f,%.2f,%.2f,%.2f,", 2.0 * acos(w) * 57.29577951308232, q[1], q[2], q[3]);
+            // baseSize(): nominal size times the float thermal factor 1 + T*CTE
+            const double half = 0.5 * (sh.voxSize * (double)(float)tf[v]);
+            double corner[2][3];
+            for (int cI = 0; cI < 2; cI++) {   // NNN, then PPP
+                const bool posLink = cI == 1;
+                for (int i = 0; i < 3; i++) {
+                    const int l = sh.vlinks[6 * (size_t)v + 2 * i + (posLink ? 0 : 1)];
+                    double sv;
+                    if (l >= 0 && !(sh.lfail[l] != -1.0f && (float)maxStrain[l] > sh.lfail[l])) {
+                        const float s = (float)strain[l], r = sh.lratio[l];
+                        const float ax = posLink ? 2.0f * s * r / (1.0f + r) : 2.0f * s / (1.0f + r);   // axialStrain
+                        sv = (1 + ax) * (posLink ? 1 : -1);
+                    } else sv = posLink ? 1.0 : -1.0;
+                    corner[cI][i] = (double)(float)(half * sv);
+                }
+            }
+            appendf(t, "%.1f,%.1f,%.1f,%.1f,%.1f,%.1f,", corner[0][0] * vs, corner[0][1] * vs, corner[0][2] * vs,
+                    corner[1][0] * vs, corner[1][1] * vs, corner[1][2] * vs);
+            appendf(t, "%d,", sh.matid[v]);
+            appendf(t, "%.1f,", 0.0);   // localSignal: signals are not simulated
+            appendf(t, ";");
+        }
+        appendf(t, "<<<>>>");
+    }
+    if (record_link) {
+        appendf(t, "|[[[%d]]]", step_label);
+        for (int l = 0; l < NL; l++) {
+            const double* a = &pos[3 * (size_t)sh.lpos[l]];
+            const double* z = &pos[3 * (size_t)sh.lneg[l]];
+            appendf(t, "%.4f,%.4f,%.4f,", a[0], a[1], a[2]);
+            appendf(t, "%.4f,%.4f,%.4f,", z[0], z[1], z[2]);
+            appendf(t, ";");
+        }
+        appendf(t, "[[[]]]");
+    }
+    appendf(t, "\n");
+    return putText(t, buf, cap, len);
 }
 
 extern "C" vx3h_status vx3h_build_topology(const vx3h_sim_desc* desc, vx3h_topology_view* v) {

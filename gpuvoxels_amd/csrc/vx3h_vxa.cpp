@@ -333,6 +333,7 @@ struct PalMat {                 // CVXC_Material (VX_Object.cpp:1373-1460), SING
     double thermalOnAfter = 0;
     int fixed = 0;
     double E = 0, plasticMod = 0, yieldStress = 0, failStress = 0, density = 0, nu = 0, cte = 0, uS = 0, uD = 0;
+    float rgba[4] = {0.5f, 0.5f, 0.5f, 1.0f};   // Display colour (CVXC_Material::ClearMaterial defaults)
 };
 
 bool readRegion(Rip& r, Region& g) {
@@ -573,14 +574,30 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
             r.up();
         }
         if (r.find("Palette")) {
+            // one material object is read over and over (ReadPaletteXML, VX_Object.cpp:477-488):
+            // a Display or Mechanical block that a material lacks keeps the values of the
+            // material before it
+            PalMat m;
             while (r.find("Material")) {
-                PalMat m;
+                m.strain.clear();
+                m.stress.clear();
                 int mt;
                 if (r.load("MatType", mt)) m.type = mt;
-                else if (r.find("Structure")) { r.up(); m.type = 1; }
-                else if (r.find("RandIndex1")) { r.up(); m.type = 3; }
-                else if (r.find("VXC") || r.find("DMF")) { r.up(); m.type = 2; }
-                else m.type = 0;
+                else {   // the reference's detective work, including its fall-through to SINGLE
+                    if (r.find("Structure")) { r.up(); m.type = 1; }
+                    if (r.find("RandIndex1")) { r.up(); m.type = 3; }
+                    else if (r.find("VXC") || r.find("DMF")) { r.up(); m.type = 2; }
+                    else m.type = 0;
+                }
+                std::string name;
+                r.load("Name", name);   // moves the cursor as the reference's read does
+                if (r.find("Display")) {
+                    if (!r.load("Red", m.rgba[0])) m.rgba[0] = 0.5f;
+                    if (!r.load("Green", m.rgba[1])) m.rgba[1] = 0.5f;
+                    if (!r.load("Blue", m.rgba[2])) m.rgba[2] = 0.5f;
+                    if (!r.load("Alpha", m.rgba[3])) m.rgba[3] = 1.0f;
+                    r.up();
+                }
                 if (m.type == 0 && r.find("Mechanical")) {
                     if (!r.load("MatModel", m.model)) m.model = 0;
                     if (r.find("SSData")) {
@@ -604,15 +621,15 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                     if (!r.load("isMeasured", m.isMeasured)) m.isMeasured = 1;
                     if (!r.load("TurnOnThermalExpansionAfterThisManySeconds", m.thermalOnAfter)) m.thermalOnAfter = 0;
                     if (!r.load("Fixed", m.fixed)) m.fixed = 0;
-                    r.load("Elastic_Mod", m.E);
-                    r.load("Plastic_Mod", m.plasticMod);
-                    r.load("Yield_Stress", m.yieldStress);
-                    r.load("Fail_Stress", m.failStress);
-                    r.load("Density", m.density);
-                    r.load("Poissons_Ratio", m.nu);
-                    r.load("CTE", m.cte);
-                    r.load("uStatic", m.uS);
-                    r.load("uDynamic", m.uD);
+                    if (!r.load("Elastic_Mod", m.E)) m.E = 0;
+                    if (!r.load("Plastic_Mod", m.plasticMod)) m.plasticMod = 0;
+                    if (!r.load("Yield_Stress", m.yieldStress)) m.yieldStress = 0;
+                    if (!r.load("Fail_Stress", m.failStress)) m.failStress = 0;
+                    if (!r.load("Density", m.density)) m.density = 0;
+                    if (!r.load("Poissons_Ratio", m.nu)) m.nu = 0;
+                    if (!r.load("CTE", m.cte)) m.cte = 0;
+                    if (!r.load("uStatic", m.uS)) m.uS = 0;
+                    if (!r.load("uDynamic", m.uD)) m.uD = 0;
                     r.up();
                 }
                 pal.push_back(m);
@@ -702,6 +719,10 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
         d.fixed = p.fixed != 0;
         d.thermal_on_after = p.thermalOnAfter;
         d.matid = (int)i + 1;   // palette index; 0 is the erase material
+        for (int k = 0; k < 4; k++) {   // GetRedi() etc. (VX_Object.h:419-425), then setRed's clamp
+            const int c = (int)(p.rgba[k] * 255);
+            d.color[k] = c > 255 ? 255 : (c < 0 ? 0 : c);
+        }
         palToMat[i + 1] = (int)o.mats.size();
         o.mats.push_back(d);
     }
@@ -805,6 +826,8 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     in.record_step_size = ptGet<int>(doc, S + "RecordHistory.RecordStepSize", 0);
     in.max_dist_close_pairs = ptGet<double>(doc, S + "MaxDistInVoxelLengthsToCountAsPair", 0.0);
     in.has_force_field = ptFind(doc, S + "ForceField") != nullptr;
+    in.record_voxel = ptGet<int>(doc, S + "RecordHistory.RecordVoxel", 1);
+    in.record_link = ptGet<int>(doc, S + "RecordHistory.RecordLink", 0);
     const XNode* rp = ptFind(doc, doc.name + ".RawPrint");
     if (rp) o.rawPrint = rp->data;
 

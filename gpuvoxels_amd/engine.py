@@ -55,6 +55,9 @@ def lib():
         "vx3h_server_results": (S, [P, C.POINTER(_abi.ServerResult), C.c_int, C.POINTER(C.c_int)]),
         "vx3h_server_clock": (C.c_double, [P]),
         "vx3h_batch_timing": (S, [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
+        "vx3h_sim_history_header": (S, [P, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "vx3h_sim_history_record": (S, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]),
         "vx3h_autotune": (S, [C.c_int, C.POINTER(C.POINTER(_abi.SimDesc)), C.c_int, C.POINTER(_abi.BatchCfg), C.c_int,
                               C.c_int, C.c_int, C.POINTER(_abi.BatchCfg), C.POINTER(_abi.TuneTrial), C.c_int,
                               C.POINTER(C.c_int)]),
@@ -286,6 +289,21 @@ class Batch:
     def retire(self, slot):
         check(lib().vx3h_sim_retire(self.h, slot))
         self.nv.pop(slot, None)
+
+    def _text(self, fn, *args):
+        n = C.c_size_t(0)
+        check(fn(*args, None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(fn(*args, buf, n.value, C.byref(n)))
+        return buf.value.decode()
+
+    def history_header(self, slot):
+        """The history stream's settings block (rescale, material colours, voxel size)."""
+        return self._text(lib().vx3h_sim_history_header, self.h, slot)
+
+    def history_record(self, slot, step_label, record_voxel=1, record_link=0):
+        """One history line of the simulation's current state."""
+        return self._text(lib().vx3h_sim_history_record, self.h, slot, step_label, record_voxel, record_link)
 
     def read_state(self, slot, nl=None):
         nv = self.nv[slot]

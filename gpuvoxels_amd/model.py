@@ -37,6 +37,7 @@ class Material:
     fixed: int = 0
     thermal_on_after: float = 0.0
     matid: int = 0
+    color: tuple = (0.5, 0.5, 0.5, 1.0)   # Display Red, Green, Blue, Alpha in [0, 1]
 
     def to_c(self) -> _abi.MaterialDesc:
         m = _abi.MaterialDesc()
@@ -61,6 +62,8 @@ class Material:
         m.fixed = self.fixed
         m.thermal_on_after = self.thermal_on_after
         m.matid = self.matid
+        for k in range(4):   # CVXC_Material::GetRedi etc. then CVX_Material::setRed's clamp
+            m.color[k] = min(255, max(0, int(np.float32(self.color[k]) * np.float32(255))))
         return m
 
 

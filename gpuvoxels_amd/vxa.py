@@ -136,9 +136,11 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
             mech.append("<SSData><NumDataPts>%d</NumDataPts><StrainData>%s</StrainData><StressData>%s</StressData></SSData>"
                         % (len(mt.strain_data), "".join("<Strain>%s</Strain>" % _num(e) for e in mt.strain_data),
                            "".join("<Stress>%s</Stress>" % _num(s) for s in mt.stress_data)))
+        r, g, b, a = (_num(c) for c in mt.color)
         out.append('<Material ID="%d"><MatType>0</MatType><Name>%s</Name>'
-                   "<Display><Red>0.5</Red><Green>0.5</Green><Blue>0.5</Blue><Alpha>1</Alpha></Display>"
-                   "<Mechanical>%s</Mechanical></Material>" % (i + 1, escape("mat%d" % (i + 1)), "".join(mech)))
+                   "<Display><Red>%s</Red><Green>%s</Green><Blue>%s</Blue><Alpha>%s</Alpha></Display>"
+                   "<Mechanical>%s</Mechanical></Material>" % (i + 1, escape("mat%d" % (i + 1)), r, g, b, a,
+                                                             "".join(mech)))
     out.append("</Palette>")
     out.append('<Structure Compression="ASCII_READABLE"><X_Voxels>%d</X_Voxels><Y_Voxels>%d</Y_Voxels><Z_Voxels>%d</Z_Voxels>'
                % (XV, YV, ZV))

@@ -85,6 +85,8 @@ typedef struct vx3h_material_desc {
     int    fixed;               /* VX3 material-fixed voxels do not move */
     double thermal_on_after;    /* TurnOnThermalExpansionAfterThisManySeconds */
     int    matid;               /* palette id reported in the report's <mats> */
+    int    color[4];            /* Display Red/Green/Blue/Alpha as 0-255 (int)(x*255), clamped
+                                   (VXA/VX_Object.h:419-425, old/VX_Material.cpp:245-283) */
 } vx3h_material_desc;
 
 /* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
@@ -149,6 +151,7 @@ typedef struct vx3h_vxa_info {
     int    record_step_size;               /* RecordHistory.RecordStepSize */
     double max_dist_close_pairs;           /* MaxDistInVoxelLengthsToCountAsPair */
     int    has_force_field;                /* a ForceField element is present */
+    int    record_voxel, record_link;      /* RecordHistory.RecordVoxel [1] / RecordLink [0] */
 } vx3h_vxa_info;
 vx3h_status vx3h_vxa_get_info(const vx3h_vxa* v, vx3h_vxa_info* info, const char** raw_print);
 
@@ -189,6 +192,8 @@ typedef struct vx3h_sim_status {
     double fitness;               /* FitnessFunction(current_com - initial_com, ...) */
     double total_distance_of_all_voxels;
     double recent_angle;
+    double recommended_dt;        /* optimal step before DtFrac (recommendedTimeStep, VX3_VoxelyzeKernel.cu:184-217);
+                                     0 until the first step */
 } vx3h_sim_status;
 
 vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out);
@@ -311,6 +316,21 @@ vx3h_status vx3h_server_round(vx3h_server* sv, int* active);
 vx3h_status vx3h_server_results(vx3h_server* sv, vx3h_server_result* out, int cap, int* n);
 /* Seconds since the server was created, on the clock of t_submit/t_admit/t_done. */
 double vx3h_server_clock(const vx3h_server* sv);
+
+/* ---------------------------------------------------------------- history
+ * The VoxCAD playback stream the reference prints while a simulation with
+ * RecordHistory.RecordStepSize > 0 runs (VX3_SimulationManager.cu:40-114).  Text is
+ * written NUL terminated into buf; *len receives the size needed (buf may be NULL).
+ * The header holds the {{{setting}}} lines (rescale, material colours of the
+ * simulation's palette in order, voxel size).  A record is one line,
+ * "<<<Step%d Time:%f>>>" + one entry per surface voxel (position and corner
+ * offsets in mm, orientation angle and axis, material id, signal) + "<<<>>>", and,
+ * when record_link is set, "|[[[%d]]]" + the two end positions of every link +
+ * "[[[]]]"; step_label is the reference's loop index (steps done - 1).  Reads the
+ * device state (synchronizes). */
+vx3h_status vx3h_sim_history_header(vx3h_batch* b, int slot, char* buf, size_t cap, size_t* len);
+vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step_label, int record_voxel, int record_link,
+                                    char* buf, size_t cap, size_t* len);
 
 /* ---------------------------------------------------------------- report */
 typedef struct vx3h_report_entry {

@@ -213,6 +213,17 @@ void ref_vxa_voxels(vxo_sim* h, int* ijk, int* mat, double* phase) {
         phase[i] = v->phaseOffset;
     }
 }
+/* Colour channels [0,255] and matid of each material the importer made, in palette
+ * order (CVX_Sim::CopyMat, VX_Sim.cpp:368-389); returns the material count. */
+int ref_vxa_colors(vxo_sim* h, int* rgba, int* matid, int cap) {
+    const int n = h->vx->materialCount();
+    for (int i = 0; i < n && i < cap; i++) {
+        CVX_Material* m = h->vx->material(i);
+        rgba[4 * i] = m->red(); rgba[4 * i + 1] = m->green(); rgba[4 * i + 2] = m->blue(); rgba[4 * i + 3] = m->alpha();
+        matid[i] = m->matid;
+    }
+    return n;
+}
 void ref_vxa_externals(vxo_sim* h, int* dof, double* trans, double* rot, float* force, float* moment) {
     for (int i = 0; i < h->vx->voxelCount(); i++) {
         CVX_Voxel* v = h->vx->voxel(i);
@@ -377,6 +388,7 @@ void vxo_vx3_status(vxo_sim* h, vx3h_sim_status* st) {
     st->num_voxel = h->vx->voxelCount();
     st->current_time = h->currentTime;
     st->dt = h->lastDt;
+    st->recommended_dt = h->OptimalDt;
     st->initial_com[0] = h->initialCoM.x; st->initial_com[1] = h->initialCoM.y; st->initial_com[2] = h->initialCoM.z;
     st->current_com[0] = h->currentCoM.x; st->current_com[1] = h->currentCoM.y; st->current_com[2] = h->currentCoM.z;
     st->fitness = vxo_mathtree_eval(h->fitTok.data(), (int)h->fitTok.size(), off.x, off.y, off.z, 0, h->currentTime, h->recentAngle, 0, 0,

@@ -1462,6 +1462,7 @@ void vxo_vx3_status(vxo_sim* h, vx3h_sim_status* st) {
     st->num_voxel = (int)s->vx.voxelsList.size();
     st->current_time = s->currentTime;
     st->dt = s->lastDt;
+    st->recommended_dt = s->OptimalDt;
     st->initial_com[0] = s->initialCoM.x; st->initial_com[1] = s->initialCoM.y; st->initial_com[2] = s->initialCoM.z;
     st->current_com[0] = s->currentCoM.x; st->current_com[1] = s->currentCoM.y; st->current_com[2] = s->currentCoM.z;
     st->fitness = mathtree_eval(s->fitTok.data(), (int)s->fitTok.size(), off.x, off.y, off.z, 0, s->currentTime, s->recentAngle, 0, 0,

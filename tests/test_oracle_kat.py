@@ -267,3 +267,50 @@ def test_fitness_and_stop_condition():
     d = np.array(list(st.current_com)) - np.array(list(st.initial_com))
     assert d[2] < -1e-4       # it fell onto the floor
     assert st.fitness == pytest.approx(np.linalg.norm(d), rel=1e-12)
+
+
+def test_corner_offsets_split_strain_between_halves():
+    """CVX_Voxel::cornerOffset (VX_Voxel.cpp / VX3_Voxel.cu:147-159), the NNN and PPP
+    corners the history stream prints: a stretched link between a soft (E1) and a stiff
+    (E2) voxel splits its strain as two springs in series, each half straining in
+    inverse proportion to its modulus (2*s*E_other/(E1+E2)); faces without a link stay
+    at half the voxel size; the two halves together extend by the whole link."""
+    size, e1, e2 = 0.001, 1e6, 4e6
+    m = Model(voxel_size=size)
+    m.materials = [Material(elastic_mod=e1, density=1e3, zeta_internal=1.0, zeta_global=0.2),
+                   Material(elastic_mod=e2, density=1e3, zeta_internal=1.0, zeta_global=0.2)]
+    m.ijk = np.array([[0, 0, 0], [1, 0, 0]], np.int32)
+    m.mat = np.array([0, 1], np.int32)
+    m.externals = [Ext(0, dof_fixed=ALL), Ext(1, dof_fixed=ALL & ~TX, force=(0.5, 0, 0))]
+    o = OSim(m)
+    ts = o.recommended_dt()
+    for _ in range(300):
+        o.step_raw(ts)
+    s = float(o.link_state()["strain"][0])
+    assert s > 1e-5
+    h = 0.5 * size
+    n0, p0 = o.corner_offset(0, 0), o.corner_offset(0, 7)
+    n1, p1 = o.corner_offset(1, 0), o.corner_offset(1, 7)
+    half0 = 2 * s * e2 / (e1 + e2)      # the soft voxel's half
+    half1 = 2 * s * e1 / (e1 + e2)
+    assert p0[0] == pytest.approx(h * (1 + half0), rel=1e-6)
+    assert n1[0] == pytest.approx(-h * (1 + half1), rel=1e-6)
+    assert n0[0] == np.float32(-h) and p1[0] == np.float32(h)
+    for c in (n0, n1):
+        assert c[1] == np.float32(-h) and c[2] == np.float32(-h)
+    for c in (p0, p1):
+        assert c[1] == np.float32(h) and c[2] == np.float32(h)
+    assert (p0[0] - h) + (-n1[0] - h) == pytest.approx(size * s, rel=1e-5)
+
+
+def test_corner_offsets_follow_thermal_size():
+    """Without links the corners sit at half the thermal size, 0.5*s*(1 + T*CTE)."""
+    m = Model(voxel_size=0.01)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, cte=0.01)]
+    m.ijk = np.array([[0, 0, 0]], np.int32)
+    m.mat = np.zeros(1, np.int32)
+    o = OSim(m)
+    o.lib.vxo_set_voxel_temperature(o.h, 0, np.float32(12.5).item())
+    h = 0.5 * 0.01 * float(np.float32(1) + np.float32(12.5) * np.float32(0.01))
+    assert np.array_equal(o.corner_offset(0, 7), np.float32([h, h, h]))
+    assert np.array_equal(o.corner_offset(0, 0), np.float32([-h, -h, -h]))

@@ -24,6 +24,9 @@ def same_state(a, b):
     la, lb = a.link_state(), b.link_state()
     for k in ("strain", "max_strain", "strain_offset", "small_angle", "bool_states", "rest_length", "forces"):
         assert np.array_equal(la[k], lb[k]), k
+    for v in range(a.nv):        # CVX_Voxel::cornerOffset of the NNN and PPP corners (the history stream's)
+        for corner in (0, 7):
+            assert np.array_equal(a.corner_offset(v, corner), b.corner_offset(v, corner)), (v, corner)
 
 
 def run_both(model, steps, chunk):
@@ -41,7 +44,8 @@ def run_both(model, steps, chunk):
         done += chunk
         same_state(a, b)
         sa, sb = a.status(), b.status()
-        assert (sa.steps, sa.state, sa.current_time, sa.fitness) == (sb.steps, sb.state, sb.current_time, sb.fitness)
+        assert (sa.steps, sa.state, sa.current_time, sa.fitness, sa.recommended_dt) == \
+            (sb.steps, sb.state, sb.current_time, sb.fitness, sb.recommended_dt)
     return a, b
 
 
@@ -84,3 +88,17 @@ def test_plastic_material():
     m.temp_amplitude = 40.0
     a, _ = run_both(m, 600, 200)
     assert a.link_state()["strain_offset"].max() > 0     # links did yield
+    assert a.link_state()["max_strain"].max() > 0
+
+
+def test_failed_links_corner_offsets():
+    # links past their failure strain stop counting in cornerOffset
+    m = workloads.cube(3, steps=0)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, poissons_ratio=0.35, cte=0.05, u_static=1.0, u_dynamic=0.5,
+                            zeta_internal=1.0, zeta_global=0.0, zeta_collision=0.8, mat_model=_abi.MDL_BILINEAR,
+                            plastic_mod=2e5, yield_stress=20.0, fail_stress=22.0)]
+    m.mat[:] = 0
+    m.temp_amplitude = 40.0
+    a, _ = run_both(m, 400, 100)
+    fail = a.link_state()["max_strain"] > 20.0 / 1e6 + (22.0 - 20.0) / 2e5   # failure strain 3e-5
+    assert fail.any() and not fail.all()

@@ -38,6 +38,8 @@ def loader_vectors(text, steps=200):
         for k in range(3):
             tr[3 * e.voxel + k], ro[3 * e.voxel + k] = e.translate[k], e.rotate[k]
             fo[3 * e.voxel + k], mo[3 * e.voxel + k] = e.force[k], e.moment[k]
+    color = np.array([list(d.materials[i].color) for i in range(d.n_materials)], np.int32).reshape(-1, 4)
+    mid = np.array([d.materials[i].matid for i in range(d.n_materials)], np.int32)
     o = OSim.from_desc(d)
     vmc = o.voxel_material_consts()
     lidx, lmc = o.link_material_consts()
@@ -46,13 +48,13 @@ def loader_vectors(text, steps=200):
     return v, dict(nv=nv, nl=o.nl, ijk=ijk, mat=mat, phase=ph, ext_dof=dof, ext_trans=tr, ext_rot=ro, ext_force=fo,
                    ext_moment=mo, env_grav=np.float32(d.gravity_mult), env_floor=int(d.floor_enabled),
                    env_dtfrac=d.dt_frac, env_temp_amp=d.temp_amplitude, env_temp_period=d.temp_period, vmat_consts=vmc,
-                   lmat_index=lidx, lmat_consts=lmc, state=st)
+                   lmat_index=lidx, lmat_consts=lmc, mat_color=color, mat_id=mid, state=st)
 
 
 def check_against(ref, got):
     assert int(ref["nv"]) == got["nv"] and int(ref["nl"]) == got["nl"]
     for k in ("ijk", "mat", "phase", "ext_dof", "ext_trans", "ext_rot", "ext_force", "ext_moment", "vmat_consts",
-              "lmat_index", "lmat_consts"):
+              "lmat_index", "lmat_consts", "mat_color", "mat_id"):
         assert np.array_equal(np.asarray(ref[k]), np.asarray(got[k])), k
     assert np.float32(ref["env_grav"]) == got["env_grav"]
     assert int(ref["env_floor"]) == got["env_floor"]
@@ -138,6 +140,25 @@ def test_vxd_replaces_paths():
     d2 = engine.Vxa(base, '<VXD><Thermal><TempBase>99</TempBase></Thermal>'
                           '<X replace="Other.Simulator">1</X></VXD>').desc
     assert d2.temp_base == m.temp_base
+
+
+def test_record_history_keys():
+    # VX3_SimulationManager.cu:326-367 reads RecordHistory.RecordStepSize [0],
+    # RecordVoxel [1] and RecordLink [0]
+    base = vxa.model_to_vxa(workloads.cube(3, steps=0))
+    info = engine.Vxa(base).info
+    assert (info.record_step_size, info.record_voxel, info.record_link) == (0, 1, 0)
+    info = engine.Vxa(base, vxa.vxd({"VXA.Simulator.RecordHistory": "<RecordStepSize>100</RecordStepSize>"
+                                                                    "<RecordLink>1</RecordLink>"})).info
+    assert (info.record_step_size, info.record_voxel, info.record_link) == (100, 1, 1)
+    info = engine.Vxa(base, vxa.vxd({"VXA.Simulator.RecordHistory": "<RecordStepSize>7</RecordStepSize>"
+                                                                    "<RecordVoxel>0</RecordVoxel>"})).info
+    assert (info.record_step_size, info.record_voxel, info.record_link) == (7, 0, 0)
+    # the leaf form the experiment workflow writes (workflow.py:90-97) into a base without RecordHistory
+    assert "RecordHistory" not in base
+    info = engine.Vxa(base, '<VXD><RecordStepSize replace="VXA.Simulator.RecordHistory.RecordStepSize">100'
+                            '</RecordStepSize></VXD>').info
+    assert (info.record_step_size, info.record_voxel, info.record_link) == (100, 1, 0)
 
 
 def _with_fitness(base, xml):

@@ -69,6 +69,23 @@ def _robot_with_stop():
     return vxa.model_to_vxa(m)
 
 
+def _display_colors():
+    # Display colours: in range, past 1 (clamped to 255), negative (clamped to 0),
+    # a partial Display block (missing channels read 0.5) and none at all, and a
+    # material without a Mechanical block (both blocks keep the previous material's values)
+    m = workloads.robots(1, seed=11, grid=8, steps=0)[0]
+    m.name = "colors"
+    for mt, c in zip(m.materials, [(1.0, 0.2, 0.35, 0.8), (1.7, -0.25, 0.999, 0.0), (0.5, 0.5, 0.5, 1.0),
+                                   (0.1, 0.3, 0.7, 0.9)]):
+        mt.color = c
+    text = vxa.model_to_vxa(m)
+    mats = re.findall(r'<Material ID="\d+">.*?</Material>', text, re.S)
+    text = text.replace(mats[2], re.sub(r"<Display>.*?</Display>", "", mats[2]))
+    text = text.replace(mats[3], re.sub(r"<Green>[^<]*</Green><Blue>[^<]*</Blue>", "",
+                                        re.sub(r"<Mechanical>.*?</Mechanical>", "", mats[3], flags=re.S)))
+    return text
+
+
 CASES = {
     "robot": lambda: vxa.model_to_vxa(workloads.robots(2, seed=3, steps=0)[1]),
     "cube6": lambda: vxa.model_to_vxa(workloads.cube(6, steps=0)),
@@ -78,6 +95,7 @@ CASES = {
     "robot_base64": _base64_structure,
     "legacy_thermal": _legacy_thermal_and_defaults,
     "robot_stop": _robot_with_stop,
+    "colors": _display_colors,
 }
 
 
