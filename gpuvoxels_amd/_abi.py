@@ -14,6 +14,8 @@ DOF_TX, DOF_TY, DOF_TZ, DOF_RX, DOF_RY, DOF_RZ = 1, 2, 4, 8, 16, 32
 DOF_ALL = 63
 PATH_STREAM, PATH_RESIDENT, PATH_AUTO = range(3)
 PREC_F32, PREC_F64 = range(2)
+(SC_NONE, SC_MAX_TIME_STEPS, SC_MAX_SIM_TIME, SC_TEMP_CYCLES, SC_CONST_MAXENERGY, SC_MIN_KE,
+ SC_MIN_MAXMOVE) = range(7)
 SIM_RUNNING, SIM_DONE, SIM_DIVERGED, SIM_EMPTY, SIM_FAILED = range(5)
 
 
@@ -89,6 +91,9 @@ class SimDesc(C.Structure):
         ("fitness_tokens", C.POINTER(Token)),
         ("max_steps", C.c_int),
         ("save_position_of_all_voxels", C.c_int),
+        ("stop_condition_type", C.c_int),
+        ("stop_condition_value", C.c_double),
+        ("equilibrium_mode", C.c_int),
     ]
 
 
@@ -168,6 +173,8 @@ class VxaInfo(C.Structure):
         ("max_dist_close_pairs", C.c_double),
         ("has_force_field", C.c_int),
         ("record_voxel", C.c_int), ("record_link", C.c_int),
+        ("stop_condition_type", C.c_int), ("stop_condition_value", C.c_double),
+        ("equilibrium_mode", C.c_int),
     ]
 
 

@@ -50,6 +50,7 @@ template <typename T> struct DevBatch {
     const LinkMatDev* lmatTab;
     const ExtDev* extTab;
     SimCtl* sims;
+    double* hist;             // per sim 3*VX3_HIST: kinetic energy, total energy, max move histories
     const TokenDev* tokens;
     const int* simMats;       // per sim: global voxel-material then link-material table indices it uses
     int Vcap, Lcap, Ccap;     // Ccap: nbList capacity

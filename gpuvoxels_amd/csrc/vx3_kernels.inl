@@ -533,6 +533,119 @@ __device__ void blockCoM(const DevBatch<real>& b, const SimCtl& sc, double out[3
     __syncthreads();
 }
 
+// VoxCAD's per-step statistics for its stop conditions and equilibrium mode
+// (CVX_Sim::TimeStep, UpdateStats, KineticEDecreasing; VoxCAD/src/VX1/VX_Sim.cpp:862-891,
+// 1055-1160, 1833-1837): with equilibrium mode, all motion is zeroed when the kinetic
+// energy of the last three steps is below that of the three before; then the total
+// kinetic energy, the total strain energy and the largest voxel speed times dt are
+// pushed into 500-step histories, newest first, -1 where the statistic is not needed.
+// Sums run in double over float terms, as in the reference; the order of the sums is
+// the block's, so they can differ from a sequential sum in the last bits.
+__device__ void voxcadStats(const DevBatch<real>& b, SimCtl& sc, int s) {
+    __shared__ double red[3][256];
+    __shared__ int zero;
+    double* H = b.hist + (size_t)s * 3 * VX3_HIST;   // [0, 500) kinetic, then total, then max move
+    const bool kin = sc.scType == 4 || sc.scType == 5 || sc.eqMode;
+    const bool strainE = sc.scType == 4;
+    const bool vel = sc.scType == 6 || (sc.flags & SC_COLLISIONS);
+    if (threadIdx.x == 0) {
+        int z = 0;
+        if (sc.eqMode) {
+            double k[6];
+            for (int i = 0; i < 6; i++) k[i] = H[(sc.histHead + i) % VX3_HIST];
+            z = k[0] + k[1] + k[2] < k[3] + k[4] + k[5] &&
+                !(k[0] == 0 || k[1] == 0 || k[2] == 0 || k[3] == 0 || k[4] == 0 || k[5] == 0);
+        }
+        zero = z;
+    }
+    __syncthreads();
+    double ke = 0, v2 = 0, se = 0;
+    for (int v = sc.v0 + threadIdx.x; v < sc.v0 + sc.nv; v += blockDim.x) {
+        if (zero) {   // CVX_Voxel::haltMotion
+            b.mx[v] = 0; b.my[v] = 0; b.mz[v] = 0;
+            b.ax[v] = 0; b.ay[v] = 0; b.az[v] = 0;
+        }
+        const VoxMatDev& m = b.vmatTab[b.vmat[v]];
+        const double lx = b.mx[v], ly = b.my[v], lz = b.mz[v];
+        const double ax = b.ax[v], ay = b.ay[v], az = b.az[v];
+        // CVX_Voxel::kineticEnergy (old/VX_Voxel.h:104)
+        ke += (float)(0.5 * (m.massInverse * (lx * lx + ly * ly + lz * lz) + m.momentInertiaInverse * (ax * ax + ay * ay + az * az)));
+        // velocity().Length2() (old/VX_Voxel.h:100)
+        const double vx = lx * m.massInverse, vy = ly * m.massInverse, vz = lz * m.massInverse;
+        v2 = fmax(v2, vx * vx + vy * vy + vz * vz);
+    }
+    if (strainE) {
+        const size_t V = (size_t)b.Vcap;
+        for (int l = sc.l0 + threadIdx.x; l < sc.l0 + sc.nl; l += blockDim.x) {
+            // CVX_Link::strainEnergy (old/VX_Link.cpp:243-249) from the loads the link pass
+            // left in the slots of its two voxels
+            const int meta = b.lmeta[l], axis = meta & 3;
+            const LinkMatDev& mat = b.lmatTab[meta >> 2];
+            const int vn = b.lneg[l], vp = b.lpos[l], dn = 2 * axis, dp = 2 * axis + 1;
+            const double fNx = b.F[(dn * 6 + 0) * V + vn];
+            const double mNx = b.F[(dn * 6 + 3) * V + vn], mNy = b.F[(dn * 6 + 4) * V + vn], mNz = b.F[(dn * 6 + 5) * V + vn];
+            const double mPy = b.F[(dp * 6 + 4) * V + vp], mPz = b.F[(dp * 6 + 5) * V + vp];
+            se += (float)(fNx * fNx / (2.0f * mat.a1) + mNx * mNx / (2.0 * mat.a2) +
+                          (mNz * mNz - mNz * mPz + mPz * mPz) / (3.0 * mat.b3) + (mNy * mNy - mNy * mPy + mPy * mPy) / (3.0 * mat.b3));
+        }
+    }
+    red[0][threadIdx.x] = ke; red[1][threadIdx.x] = v2; red[2][threadIdx.x] = se;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + w]);
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int head = (sc.histHead + VX3_HIST - 1) % VX3_HIST;
+        const double dtD = sc.dtFixed > 0 ? sc.dtFixed : sc.dtFrac * sc.optDt;
+        H[head] = kin ? red[0][0] : -1.0;
+        H[VX3_HIST + head] = kin && strainE ? red[0][0] + red[2][0] : -1.0;
+        H[2 * VX3_HIST + head] = vel ? sqrt(red[1][0]) * dtD : -1.0;
+        sc.histHead = head;
+    }
+    __syncthreads();
+}
+
+// CVX_Sim::StopConditionMet (VoxCAD/src/VX1/VX_Sim.cpp:1015-1055) after `steps` steps at time sc.t
+__device__ bool voxcadStopMet(const DevBatch<real>& b, const SimCtl& sc, int s, int steps) {
+    if (steps < 2 * VX3_HIST) return false;
+    const double* H = b.hist + (size_t)s * 3 * VX3_HIST;
+    auto at = [&](int q, int age) { return H[q * VX3_HIST + (sc.histHead + age) % VX3_HIST]; };
+    const double inv = 1.0 / (float)sc.nv;
+    const double thr = 0.001 * sc.scValue;
+    const int jump = VX3_HIST / 10;
+    switch (sc.scType) {
+    case 1: return steps > (int)(sc.scValue + 0.5);
+    case 2: return sc.t > sc.scValue;
+    case 3: return sc.t > sc.tempPeriod * sc.scValue;
+    case 4: {
+        const double ini = at(1, 0);
+        for (int i = jump; i < VX3_HIST; i += jump) {
+            if (at(1, i) == -1) return false;
+            if (fabs(at(1, i) - ini) * inv > thr) return false;
+        }
+        return true;
+    }
+    case 5:
+        for (int i = 0; i < VX3_HIST; i += jump) {
+            if (at(0, i) == -1) return false;
+            if (at(0, i) * inv > thr) return false;
+        }
+        return true;
+    case 6:
+        for (int i = 0; i < VX3_HIST; i += jump) {
+            if (at(2, i) == -1) return false;
+            if (at(2, i) > thr) return false;
+        }
+        return true;
+    default: return false;
+    }
+}
+
 // Per-simulation bookkeeping after the voxel pass (doTimeStep tail, VX3_VoxelyzeKernel.cu:241,279,314-334,352,
 // and the stop test the driver loop makes before the next step, VX3_SimulationManager.cu:62-64).
 __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
@@ -546,6 +659,7 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     const bool sample = !diverged && sc.cycleStep > 0 && steps % sc.cycleStep == 0;
     double com[3];
     if (sample) blockCoM(b, sc, com);
+    if (!diverged && !(sc.flags & SC_COL_OVERFLOW) && (sc.scType >= 4 || sc.eqMode)) voxcadStats(b, sc, s);
     if (threadIdx.x != 0) return;
     sc.steps = steps;
     sc.flags &= ~SC_REBUILD;
@@ -576,6 +690,8 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     if (sc.maxSteps > 0 && steps >= sc.maxSteps) { sc.state = 1; return; }
     if (sc.stopLen > 0 &&
         mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], 0, sc.t, sc.recentAngle, 0, 0, sc.nv, sQ) > 0)
+        sc.state = 1;
+    else if (sc.scType && voxcadStopMet(b, sc, s, steps))
         sc.state = 1;
 }
 

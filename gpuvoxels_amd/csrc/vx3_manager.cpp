@@ -64,7 +64,22 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
     cfg.path = opt.path;
     vx3h_batch* b = nullptr;
     check(vx3h_batch_create(device, &cfg, &b), "batch creation");
-    struct Job { std::string name; int slot; vx3h_vxa* v; vx3h_vxa_info info; };
+    struct Job {
+        std::string name;
+        int slot;
+        vx3h_vxa* v;
+        vx3h_vxa_info info;
+        vx3h_sim_desc desc;   // the loader's description, with the VoxCAD stop keys when asked for
+    };
+    auto describe = [&](vx3h_vxa* v, const vx3h_vxa_info& info) {
+        vx3h_sim_desc d = *vx3h_vxa_desc(v);
+        if (opt.voxcad_stop) {
+            d.stop_condition_type = info.stop_condition_type;
+            d.stop_condition_value = info.stop_condition_value;
+            d.equilibrium_mode = info.equilibrium_mode;
+        }
+        return d;
+    };
     std::vector<Job> jobs, history;
     try {
         for (const std::string& f : files) {
@@ -86,13 +101,14 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
                 fprintf(stderr, "warning: %s: ignoring%s\n", f.c_str(), missing.c_str());
             }
             if (info.record_step_size > 0) {   // recorded simulations run alone, after the batch
-                history.push_back({baseName(f), -1, v, info});
+                history.push_back({baseName(f), -1, v, info, describe(v, info)});
                 continue;
             }
             int slot = -1;
-            vx3h_status st = vx3h_sim_admit(b, vx3h_vxa_desc(v), &slot);
+            const vx3h_sim_desc d = describe(v, info);
+            vx3h_status st = vx3h_sim_admit(b, &d, &slot);
             if (st != VX3H_OK) { vx3h_vxa_free(v); check(st, f); }
-            jobs.push_back({baseName(f), slot, v, info});
+            jobs.push_back({baseName(f), slot, v, info, d});
         }
         // run until every simulation is done or diverged
         for (;;) {
@@ -150,8 +166,8 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
         b = nullptr;
         for (Job& j : history) {
             check(vx3h_batch_create(device, &cfg, &b), "batch creation");
-            check(vx3h_sim_admit(b, vx3h_vxa_desc(j.v), &j.slot), j.name);
-            runHistory(b, j.slot, j.info, vx3h_vxa_desc(j.v)->dt_frac);
+            check(vx3h_sim_admit(b, &j.desc, &j.slot), j.name);
+            runHistory(b, j.slot, j.info, j.desc.dt_frac);
             collect(b, j);
             vx3h_batch_destroy(b);
             b = nullptr;

@@ -44,6 +44,7 @@ struct ManagerOptions {
     bool allow_unsupported = false;     // run VXAs that enable features this engine ignores
     bool verbose = true;
     FILE* history_out = nullptr;        // playback stream of RecordHistory simulations [stdout]
+    bool voxcad_stop = false;           // apply the VXA's VoxCAD stop condition and equilibrium mode
 };
 
 class SimulationManager {

@@ -1,6 +1,6 @@
 // vx3_node_worker -- runs the simulations of a task file and writes their report.
 //
-//   vx3_node_worker -i task.vxt -o report.xml [-f] [--f32] [--allow-unsupported]
+//   vx3_node_worker -i task.vxt -o report.xml [-f] [--f32] [--allow-unsupported] [--voxcad-stop]
 //
 // Command-line surface of Voxelyze3/src/Executables/vx3_node_worker.cu:32-143: the
 // task file holds <vxa>base.vxa</vxa>, <input_dir>dir</input_dir> and
@@ -9,7 +9,9 @@
 // to the output path.  Exit status 1 on bad arguments, an existing output without -f,
 // a missing task file or no GPU; 0 otherwise.  Additions: --f32 runs the FP32 state
 // engine (default FP64), --allow-unsupported runs VXAs that enable features this
-// engine does not simulate (attach/detach, cilia, signals) without them.
+// engine does not simulate (attach/detach, cilia, signals) without them, and
+// --voxcad-stop honours the VXA's StopConditionType/Value and EquilibriumMode the way
+// VoxCAD does (the Voxelyze3 worker reads and ignores them).
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -33,7 +35,8 @@ const char* USAGE =
     "  -o [ --output ] arg    output file path for the report (e.g. report_1.xml)\n"
     "  -f [ --force ]         overwrite output file if exists\n"
     "  --f32                  FP32 state engine (default FP64)\n"
-    "  --allow-unsupported    ignore features this engine does not simulate\n";
+    "  --allow-unsupported    ignore features this engine does not simulate\n"
+    "  --voxcad-stop          honour StopConditionType/Value and EquilibriumMode as VoxCAD does\n";
 
 bool isFile(const std::string& p) {
     struct stat s;
@@ -96,6 +99,7 @@ int main(int argc, char** argv) {
         else if (a == "-f" || a == "--force") force = true;
         else if (a == "--f32") opt.precision = VX3H_PREC_F32;
         else if (a == "--allow-unsupported") opt.allow_unsupported = true;
+        else if (a == "--voxcad-stop") opt.voxcad_stop = true;
         else help = true;
     }
     if (help || input.empty() || output.empty()) { printf("%s\n", USAGE); return 1; }

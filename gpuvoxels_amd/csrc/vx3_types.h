@@ -98,7 +98,15 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int nb0, nbCap;                  // neighbour-list range in nbList
     int h0, hBits;                   // hash-table range: (1 << hBits) buckets from bucket h0
     int rebuilds;                    // neighbour-list rebuilds so far
+    // VoxCAD stop-condition family and equilibrium mode (VoxCAD/src/VX1/VX_Sim.cpp:862-891,1015-1160)
+    int scType;                      // VX3H_SC_*
+    int eqMode;                      // zero all motion whenever the kinetic energy falls
+    double scValue;                  // StopConditionValue
+    double dtFixed;                  // > 0: the fixed step (else DtFrac*OptimalDt) for the max-move history
+    int histHead;                    // ring index of the newest entry of the 500-step histories
 };
+
+#define VX3_HIST 500                 // HISTORY_SIZE (VoxCAD/src/VX1/VX_Sim.h)
 
 struct TokenDev { int op; int pad; double value; };
 

@@ -477,7 +477,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
     if (b->graph) hipGraphDestroy(b->graph);
     for (auto* arrs : {&b->vArrs, &b->lArrs})
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
-    for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims}) if (p) hipFree(p);
+    for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims, (void*)b->db.hist}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
     for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool})
         for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
@@ -499,6 +499,8 @@ extern "C" vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* d, int precision,
 
 extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, int* slot) {
     if (!b || !desc || !slot) FAILV(VX3H_EINVAL, "null argument");
+    if (desc->stop_condition_type < VX3H_SC_NONE || desc->stop_condition_type > VX3H_SC_MIN_MAXMOVE)
+        FAILV(VX3H_EINVAL, "unknown stop condition type");
     HIPCHK(hipSetDevice(b->device));
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
@@ -516,11 +518,17 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         SimCtl* np = nullptr;
         HIPCHK(hipMalloc(&np, nc * sizeof(SimCtl)));
         HIPCHK(hipMemset(np, 0, nc * sizeof(SimCtl)));
+        double* nh = nullptr;
+        const size_t hs = 3 * VX3_HIST * sizeof(double);
+        HIPCHK(hipMalloc(&nh, nc * hs));
         if (b->db.sims) {
             HIPCHK(hipMemcpy(np, b->db.sims, b->simCap * sizeof(SimCtl), hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(nh, b->db.hist, b->simCap * hs, hipMemcpyDeviceToDevice));
             HIPCHK(hipFree(b->db.sims));
+            HIPCHK(hipFree(b->db.hist));
         }
         b->db.sims = np;
+        b->db.hist = nh;
         b->simCap = nc;
         b->layout++;
     }
@@ -613,8 +621,10 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     for (int i = 0; i < NV; i++) memcpy(&phases[i], &H.phase[i], 8);
     std::sort(phases.begin(), phases.end());
     const int nPhases = (int)(std::unique(phases.begin(), phases.end()) - phases.begin());
+    // VoxCAD stop conditions and equilibrium mode are evaluated by the stream path's control pass
+    const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
-                       nPhases <= RES_MAX_PHASES;
+                       nPhases <= RES_MAX_PHASES && !voxcad;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -623,6 +633,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         }
         if (shape < 0 && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "simulation too large for the resident kernel");
     }
+    if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
+        FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -644,6 +656,15 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     c.nvm = (int)vmIdx.size();
     c.nlm = (int)lmIdx.size();
     c.stopLen = (int)H.stopTok.size();
+    c.scType = desc->stop_condition_type;
+    c.scValue = desc->stop_condition_value;
+    c.eqMode = desc->equilibrium_mode != 0;
+    c.dtFixed = H.dtFixed;
+    c.histHead = 0;
+    if (voxcad) {
+        const std::vector<double> unset(3 * VX3_HIST, -1.0);   // ClearHistories
+        HIPCHK(hipMemcpy(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     if (NV == 0) c.state = VX3H_SIM_DONE;   // "No links and no voxels" (VX3_SimulationManager.cu:15-19)
     else if (!H.stopTok.empty() &&
              evalMathTree(H.stopTok.data(), (int)H.stopTok.size(), c.com[0], c.com[1], c.com[2], 0, 0.0, 0, 0, 0, NV) > 0)

@@ -828,6 +828,10 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     in.has_force_field = ptFind(doc, S + "ForceField") != nullptr;
     in.record_voxel = ptGet<int>(doc, S + "RecordHistory.RecordVoxel", 1);
     in.record_link = ptGet<int>(doc, S + "RecordHistory.RecordLink", 0);
+    // CVX_Sim::ReadVXA (VXA/VX_Sim.cpp:260-277): read, then ignored by the Voxelyze3 driver
+    in.stop_condition_type = ptGet<int>(doc, S + "StopCondition.StopConditionType", 0);
+    in.stop_condition_value = ptGet<double>(doc, S + "StopCondition.StopConditionValue", 0.0);
+    in.equilibrium_mode = ptGet<int>(doc, S + "EquilibriumMode.EquilibriumModeEnabled", 0) != 0;
     const XNode* rp = ptFind(doc, doc.name + ".RawPrint");
     if (rp) o.rawPrint = rp->data;
 

@@ -139,6 +139,11 @@ class Model:
     fitness_tokens: list = field(default_factory=list)
     max_steps: int = 0
     save_position_of_all_voxels: int = 0
+    # VoxCAD stop-condition family (SC_* of _abi) and equilibrium mode; the Voxelyze3
+    # driver ignores them, so they are off unless set here
+    stop_condition_type: int = 0
+    stop_condition_value: float = 0.0
+    equilibrium_mode: int = 0
     name: str = "sim"
 
     @property
@@ -199,4 +204,7 @@ class Model:
                 setattr(d, "%s_tokens" % name, arr)
         d.max_steps = self.max_steps
         d.save_position_of_all_voxels = self.save_position_of_all_voxels
+        d.stop_condition_type = self.stop_condition_type
+        d.stop_condition_value = self.stop_condition_value
+        d.equilibrium_mode = self.equilibrium_mode
         return d, keep

@@ -86,9 +86,14 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                    "</Damping>" % (_num(zeta.zeta_internal), _num(zeta.zeta_collision), _num(zeta.zeta_global)))
     out.append("<Collisions><SelfColEnabled>%d</SelfColEnabled></Collisions>" % int(bool(m.collisions_enabled)))
     out.append("<Features><VolumeEffectsEnabled>%d</VolumeEffectsEnabled></Features>" % int(bool(m.volume_effects)))
-    if m.stop_tokens:
-        out.append("<StopCondition><StopConditionFormula>%s</StopConditionFormula></StopCondition>"
-                   % mathtree_xml(m.stop_tokens))
+    if m.stop_tokens or m.stop_condition_type:
+        out.append("<StopCondition>%s%s</StopCondition>"
+                   % ("<StopConditionFormula>%s</StopConditionFormula>" % mathtree_xml(m.stop_tokens) if m.stop_tokens
+                      else "",
+                      "<StopConditionType>%d</StopConditionType><StopConditionValue>%s</StopConditionValue>"
+                      % (m.stop_condition_type, _num(m.stop_condition_value)) if m.stop_condition_type else ""))
+    if m.equilibrium_mode:
+        out.append("<EquilibriumMode><EquilibriumModeEnabled>1</EquilibriumModeEnabled></EquilibriumMode>")
     if m.fitness_tokens:
         out.append("<FitnessFunction>%s</FitnessFunction>" % mathtree_xml(m.fitness_tokens))
     out.append("<AttachDetach><EnableCollision>%d</EnableCollision><boundingRadius>%s</boundingRadius>"

@@ -131,7 +131,27 @@ typedef struct vx3h_sim_desc {
     int    n_fitness_tokens; const vx3h_token* fitness_tokens;  /* FitnessFunction */
     int    max_steps;        /* step budget; 0 = unlimited (the reference caps at 1e6) */
     int    save_position_of_all_voxels;
+    /* VoxCAD's stop-condition family and equilibrium mode (VoxCAD/src/VX1/VX_Sim.cpp:862-891,
+     * 1015-1160; StopConditionType of VXA/VX_Sim.cpp:260-277).  The Voxelyze3 driver reads
+     * these keys but ignores them (VX3_VoxelyzeKernel.cu:162-182), so the loader leaves
+     * them 0 here and reports the VXA's values in vx3h_vxa_info; a caller that wants
+     * VoxCAD's behaviour copies them in.  Checked after the MathTree stop condition;
+     * none of them stops a simulation before 1000 steps.  Simulations using 4-6 or
+     * equilibrium mode run on the stream path. */
+    int    stop_condition_type;   /* VX3H_SC_* */
+    double stop_condition_value;
+    int    equilibrium_mode;      /* zero all motion whenever the kinetic energy falls */
 } vx3h_sim_desc;
+
+enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */
+    VX3H_SC_NONE = 0,
+    VX3H_SC_MAX_TIME_STEPS = 1,     /* steps > (int)(value + 0.5) */
+    VX3H_SC_MAX_SIM_TIME = 2,       /* time > value */
+    VX3H_SC_TEMP_CYCLES = 3,        /* time > temperature period * value */
+    VX3H_SC_CONST_MAXENERGY = 4,    /* total energy per voxel within 0.001*value of its value 50..450 steps ago */
+    VX3H_SC_MIN_KE = 5,             /* kinetic energy per voxel below 0.001*value at 10 samples 50 steps apart */
+    VX3H_SC_MIN_MAXMOVE = 6         /* largest voxel move per step below 0.001*value at those samples */
+};
 
 /* ---------------------------------------------------------------- VXA / VXD loader */
 typedef struct vx3h_vxa vx3h_vxa;   /* owns a parsed description */
@@ -152,6 +172,9 @@ typedef struct vx3h_vxa_info {
     double max_dist_close_pairs;           /* MaxDistInVoxelLengthsToCountAsPair */
     int    has_force_field;                /* a ForceField element is present */
     int    record_voxel, record_link;      /* RecordHistory.RecordVoxel [1] / RecordLink [0] */
+    int    stop_condition_type;            /* StopCondition.StopConditionType [0] */
+    double stop_condition_value;           /* StopCondition.StopConditionValue [0] */
+    int    equilibrium_mode;               /* EquilibriumMode.EquilibriumModeEnabled [0] */
 } vx3h_vxa_info;
 vx3h_status vx3h_vxa_get_info(const vx3h_vxa* v, vx3h_vxa_info* info, const char** raw_print);
 

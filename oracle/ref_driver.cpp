@@ -13,6 +13,7 @@
  *    Voxelyze3/src/VX3/VX3_VoxelyzeKernel.cu:184-359,477-534,625-650.
  */
 #include "vx_oracle.h"
+#include "vx_voxcad_stop.h"
 
 #include "Voxelyze.h"
 #include "VX_Collision.h"
@@ -57,6 +58,7 @@ struct vxo_sim {
     int maxSteps = 0;
     std::vector<vx3h_token> stopTok, fitTok;
     float lastDt = 0;
+    VoxcadStop voxcad;               // VoxCAD stop-condition family and equilibrium mode
 };
 
 static int matIndex(vxo_sim* h, CVX_MaterialVoxel* m) {
@@ -158,6 +160,9 @@ vxo_sim* vxo_create(const vx3h_sim_desc* d) {
     h->TempAmplitude = d->temp_amplitude;
     h->TempPeriod = d->temp_period;
     h->maxSteps = d->max_steps;
+    h->voxcad.type = d->stop_condition_type;
+    h->voxcad.value = d->stop_condition_value;
+    h->voxcad.eq = d->equilibrium_mode != 0;
     if (d->n_stop_tokens > 0) h->stopTok.assign(d->stop_tokens, d->stop_tokens + d->n_stop_tokens);
     if (d->n_fitness_tokens > 0) h->fitTok.assign(d->fitness_tokens, d->fitness_tokens + d->n_fitness_tokens);
     indexVoxels(h);
@@ -212,6 +217,13 @@ void ref_vxa_voxels(vxo_sim* h, int* ijk, int* mat, double* phase) {
         mat[i] = matIndex(h, v->material());
         phase[i] = v->phaseOffset;
     }
+}
+/* Stop-condition type and value and the equilibrium-mode switch CVX_Sim::ReadVXA read
+ * (VX_Sim.cpp:260-277). */
+void ref_vxa_stop(vxo_sim* h, int* type, double* value, int* eq) {
+    *type = h->sim ? (int)h->sim->GetStopConditionType() : 0;
+    *value = h->sim ? h->sim->GetStopConditionValue() : 0;
+    *eq = h->sim && h->sim->IsFeatureEnabled(VXSFEAT_EQUILIBRIUM_MODE) ? 1 : 0;
 }
 /* Colour channels [0,255] and matid of each material the importer made, in palette
  * order (CVX_Sim::CopyMat, VX_Sim.cpp:368-389); returns the material count. */
@@ -328,7 +340,7 @@ static bool stopMet(vxo_sim* h) {
         vxo_mathtree_eval(h->stopTok.data(), (int)h->stopTok.size(), h->currentCoM.x, h->currentCoM.y, h->currentCoM.z, 0, h->currentTime,
                           h->recentAngle, 0, 0, h->vx->voxelCount()) > 0)
         return true;
-    return false;
+    return h->voxcad.type != 0 && h->voxcad.met(h->CurStepCount, h->currentTime, h->TempPeriod, h->vx->voxelCount());
 }
 
 static bool stepVX3(vxo_sim* h) {
@@ -364,6 +376,18 @@ static bool stepVX3(vxo_sim* h) {
         else h->recentAngle = acos((B - A).Dot(C - B) / (B.Dist(A) * C.Dist(B)));
     }
     h->currentTime += dt;
+    CVoxelyze* vx = h->vx;
+    h->voxcad.afterStep(
+        vx->isCollisionsEnabled(), h->dtFixed > 0 ? h->dtFixed : h->DtFrac * h->OptimalDt,
+        [&] { for (int i = 0; i < vx->voxelCount(); i++) vx->voxel(i)->haltMotion(); },
+        [&](bool vel, bool kin, bool strainE, double& maxVel2, double& totKE, double& totSE) {
+            for (int i = 0; i < vx->voxelCount(); i++) {
+                CVX_Voxel* v = vx->voxel(i);
+                if (vel) { const double v2 = v->velocity().Length2(); if (v2 > maxVel2) maxVel2 = v2; }
+                if (kin) totKE += v->kineticEnergy();
+            }
+            if (strainE) for (int i = 0; i < vx->linkCount(); i++) totSE += vx->link(i)->strainEnergy();
+        });
     return true;
 }
 
@@ -431,6 +455,10 @@ void vxo_link_state(vxo_sim* h, float* strain, float* maxStrain, float* off, int
             for (int k = 0; k < 4; k++) { f12[12 * i + 3 * k] = vs[k]->x; f12[12 * i + 3 * k + 1] = vs[k]->y; f12[12 * i + 3 * k + 2] = vs[k]->z; }
         }
     }
+}
+void vxo_voxcad_history(vxo_sim* h, double* k, double* t, double* m) {
+    const VoxcadStop& c = h->voxcad;
+    for (int i = 0; i < VoxcadStop::HIST_LEN; i++) { k[i] = c.KinEHistory[i]; t[i] = c.TotEHistory[i]; m[i] = c.MaxMoveHistory[i]; }
 }
 void vxo_corner_offset(vxo_sim* h, int v, int corner, float* out) {
     Vec3D<float> r = h->vx->voxel(v)->cornerOffset((CVX_Voxel::voxelCorner)corner);

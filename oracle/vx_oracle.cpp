@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "vx_mathtree.h"
+#include "vx_voxcad_stop.h"
 
 namespace vxo {
 
@@ -528,6 +529,12 @@ struct Link {
         return positiveEnd ? 2.0f * strain * strainRatio / (1.0f + strainRatio) : 2.0f * strain / (1.0f + strainRatio);
     }
     bool isFailed() const { return mat->isFailed(maxStrain); }
+    // VX_Link.cpp:243-249 (tensile, torsion, bending about z and about y)
+    float strainEnergy() const {
+        return forceNeg.x * forceNeg.x / (2.0f * mat->_a1) + momentNeg.x * momentNeg.x / (2.0 * mat->_a2) +
+               (momentNeg.z * momentNeg.z - momentNeg.z * momentPos.z + momentPos.z * momentPos.z) / (3.0 * mat->_b3) +
+               (momentNeg.y * momentNeg.y - momentNeg.y * momentPos.y + momentPos.y * momentPos.y) / (3.0 * mat->_b3);
+    }
     bool isLocalVelocityValid() const { return boolStates & LOCAL_VELOCITY_VALID ? true : false; }
     void setBoolState(int flag, bool set = true) { set ? boolStates |= flag : boolStates &= ~flag; }
     void updateRestLength();
@@ -573,6 +580,10 @@ struct Voxel {
     double baseSizeAverage() const { Vd b = baseSize(); return (b.x + b.y + b.z) / 3.0f; }
     Vd velocity() const { return linMom * mat->_massInverse; }
     Vd angularVelocity() const { return angMom * mat->_momentInertiaInverse; }
+    // VX_Voxel.h:104
+    float kineticEnergy() const {
+        return (float)(0.5 * (mat->_massInverse * linMom.Length2() + mat->_momentInertiaInverse * angMom.Length2()));
+    }
     float floorPenetration() const { return (float)(baseSizeAverage() / 2 - mat->nomSize / 2 - pos.z); }
     float dampingMultiplier() const { return 2 * mat->_sqrtMass * mat->zetaInternal / previousDt; }
     static bool isNegative(int d) { return d % 2 == 1; }
@@ -1180,6 +1191,7 @@ struct Sim {
     int maxSteps = 0;
     std::vector<vx3h_token> stopTok, fitTok;
     float lastDt = 0;
+    VoxcadStop voxcad;   // VoxCAD stop-condition family and equilibrium mode
 
     // VX3_VoxelyzeKernel.cu:477-493
     void updateCurrentCenterOfMass() {
@@ -1228,7 +1240,7 @@ struct Sim {
             mathtree_eval(stopTok.data(), (int)stopTok.size(), currentCoM.x, currentCoM.y, currentCoM.z, 0, currentTime, recentAngle, 0, 0,
                           (int)vx.voxelsList.size()) > 0)
             return true;
-        return false;
+        return voxcad.type != 0 && voxcad.met(CurStepCount, currentTime, TempPeriod, (int)vx.voxelsList.size());
     }
     // VX3_VoxelyzeKernel.cu:237-359
     bool doTimeStepVX3() {
@@ -1254,6 +1266,15 @@ struct Sim {
             else recentAngle = acos((B - A).Dot(C - B) / (B.Dist(A) * C.Dist(B)));
         }
         currentTime += dt;
+        voxcad.afterStep(
+            vx.collisions, dtFixed > 0 ? dtFixed : DtFrac * OptimalDt, [&] { for (Voxel* v : vx.voxelsList) v->haltMotion(); },
+            [&](bool vel, bool kin, bool strainE, double& maxVel2, double& totKE, double& totSE) {
+                for (Voxel* v : vx.voxelsList) {
+                    if (vel) { const double v2 = v->velocity().Length2(); if (v2 > maxVel2) maxVel2 = v2; }
+                    if (kin) totKE += v->kineticEnergy();
+                }
+                if (strainE) for (Link* l : vx.linksList) totSE += l->strainEnergy();
+            });
         return true;
     }
 };
@@ -1346,6 +1367,9 @@ static Sim* build(const vx3h_sim_desc* d) {
     s->TempAmplitude = d->temp_amplitude;
     s->TempPeriod = d->temp_period;
     s->maxSteps = d->max_steps;
+    s->voxcad.type = d->stop_condition_type;
+    s->voxcad.value = d->stop_condition_value;
+    s->voxcad.eq = d->equilibrium_mode != 0;
     if (d->n_stop_tokens > 0) s->stopTok.assign(d->stop_tokens, d->stop_tokens + d->n_stop_tokens);
     if (d->n_fitness_tokens > 0) s->fitTok.assign(d->fitness_tokens, d->fitness_tokens + d->n_fitness_tokens);
     // VX3_SimulationManager.cu:54-55
@@ -1507,6 +1531,10 @@ void vxo_link_state(vxo_sim* h, float* strain, float* maxStrain, float* off, int
             for (int k = 0; k < 4; k++) { f12[12 * i + 3 * k] = vs[k]->x; f12[12 * i + 3 * k + 1] = vs[k]->y; f12[12 * i + 3 * k + 2] = vs[k]->z; }
         }
     }
+}
+void vxo_voxcad_history(vxo_sim* h, double* k, double* t, double* m) {
+    const VoxcadStop& c = h->s->voxcad;
+    for (int i = 0; i < VoxcadStop::HIST_LEN; i++) { k[i] = c.KinEHistory[i]; t[i] = c.TotEHistory[i]; m[i] = c.MaxMoveHistory[i]; }
 }
 void vxo_corner_offset(vxo_sim* h, int v, int corner, float* out) {
     Vf r = h->s->vx.voxelsList[v]->cornerOffset(corner);

@@ -66,6 +66,8 @@ void vxo_voxel_state(vxo_sim* s, double* pos, double* orient, double* lin_mom, d
 void vxo_link_state(vxo_sim* s, float* strain, float* max_strain, float* strain_offset,
                     int* small_angle, int* bool_states, double* rest_length, double* forces12);
 void vxo_corner_offset(vxo_sim* s, int v, int corner, float* out3);
+/* VoxCAD stop-condition histories, 500 entries each, newest first (-1 = not computed) */
+void vxo_voxcad_history(vxo_sim* s, double* kinetic, double* total, double* max_move);
 int  vxo_collision_pairs(vxo_sim* s, int* pairs, int cap);        /* returns count */
 /* derived material constants, for the host builder's parity checks.
  * voxel mats: 14 floats each; link mats (list order): 20 floats each */

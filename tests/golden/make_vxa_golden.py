@@ -46,6 +46,8 @@ def reference_vectors(text):
     rgba, mid = np.zeros(4 * cap, np.int32), np.zeros(cap, np.int32)
     nm = lib.ref_vxa_colors(C.c_void_p(r.h), rgba.ctypes.data_as(C.POINTER(C.c_int)), mid.ctypes.data_as(C.POINTER(C.c_int)),
                             cap)
+    st_type, st_val, eq = C.c_int(), C.c_double(), C.c_int()
+    lib.ref_vxa_stop(C.c_void_p(r.h), C.byref(st_type), C.byref(st_val), C.byref(eq))
     vmc = r.voxel_material_consts()
     lidx, lmc = r.link_material_consts()
     r.step(STEPS)
@@ -54,7 +56,8 @@ def reference_vectors(text):
     return dict(nv=np.int64(nv), nl=np.int64(r.nl), ijk=ijk, mat=mat, phase=ph, ext_dof=dof, ext_trans=tr, ext_rot=ro,
                 ext_force=fo, ext_moment=mo, env_grav=np.float32(g.value), env_floor=np.int32(fl.value),
                 env_dtfrac=np.float64(dtf.value), env_temp_amp=np.float64(ta.value), env_temp_period=np.float64(tp.value),
-                mat_color=rgba[: 4 * nm].reshape(-1, 4), mat_id=mid[:nm], vmat_consts=vmc, lmat_index=lidx, lmat_consts=lmc, sample=s, pos=st["pos"][s], orient=st["orient"][s],
+                mat_color=rgba[: 4 * nm].reshape(-1, 4), mat_id=mid[:nm],
+                stop=np.array([st_type.value, st_val.value, eq.value], np.float64), vmat_consts=vmc, lmat_index=lidx, lmat_consts=lmc, sample=s, pos=st["pos"][s], orient=st["orient"][s],
                 pos_sum=np.float64(st["pos"].sum()), orient_sum=np.float64(st["orient"].sum()))
 
 

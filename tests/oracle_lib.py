@@ -46,6 +46,7 @@ def _bind(lib):
         "vxo_voxel_state": (None, [P, pD, pD, pD, pD, pI, pF]),
         "vxo_link_state": (None, [P, pF, pF, pF, pI, pI, pD, pD]),
         "vxo_corner_offset": (None, [P, I, I, pF]),
+        "vxo_voxcad_history": (None, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "vxo_collision_pairs": (I, [P, pI, I]),
         "vxo_voxel_material_consts": (I, [P, pF, I]),
         "vxo_link_material_consts": (I, [P, pI, pF, I]),
@@ -216,6 +217,12 @@ class OSim:
         out = np.zeros(3, np.float32)
         self.lib.vxo_corner_offset(self.h, v, corner, _p(out, C.c_float))
         return out
+
+    def voxcad_history(self):
+        """VoxCAD stop-condition histories (kinetic, total energy, max move), newest first."""
+        k, t, m = np.zeros(500), np.zeros(500), np.zeros(500)
+        self.lib.vxo_voxcad_history(self.h, _p(k, C.c_double), _p(t, C.c_double), _p(m, C.c_double))
+        return k, t, m
 
     def reset_time(self):
         self.lib.vxo_reset_time(self.h)

@@ -60,7 +60,8 @@ def test_ctypes_layout_matches_header(tmp_path):
 def test_constants_match_header():
     text = open(HEADER).read()
     for name, val in [("VX3H_SIM_FAILED", _abi.SIM_FAILED), ("VX3H_MDL_BILINEAR", _abi.MDL_BILINEAR),
-                      ("VX3H_MDL_DATA", _abi.MDL_DATA), ("VX3H_MAX_CURVE", 16), ("VX3H_MAX_TOKENS", 256)]:
+                      ("VX3H_MDL_DATA", _abi.MDL_DATA), ("VX3H_MAX_CURVE", 16), ("VX3H_MAX_TOKENS", 256),
+                      ("VX3H_SC_MIN_KE", _abi.SC_MIN_KE), ("VX3H_SC_MIN_MAXMOVE", _abi.SC_MIN_MAXMOVE)]:
         m = re.search(r"\b%s\s*=?\s*(\d+)" % name, text)
         assert m and int(m.group(1)) == val, name
 

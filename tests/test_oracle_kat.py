@@ -314,3 +314,112 @@ def test_corner_offsets_follow_thermal_size():
     h = 0.5 * 0.01 * float(np.float32(1) + np.float32(12.5) * np.float32(0.01))
     assert np.array_equal(o.corner_offset(0, 7), np.float32([h, h, h]))
     assert np.array_equal(o.corner_offset(0, 0), np.float32([-h, -h, -h]))
+
+
+# ---- VoxCAD stop-condition family and equilibrium mode (VoxCAD/src/VX1/VX_Sim.cpp:862-891,
+# 1015-1160, 1833-1837), on top of the VX3 step
+
+def _drop_cube(sc=0, value=0.0, eq=0, lift=2, n=3):
+    m = Model(voxel_size=0.01)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, zeta_internal=1.0, zeta_global=0.01, cte=0.0)]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    m.ijk = np.stack([x.ravel(), y.ravel(), z.ravel() + lift], axis=1).astype(np.int32)
+    m.mat = np.zeros(n ** 3, np.int32)
+    m.gravity_mult = np.float32(1.0)
+    m.floor_enabled = 1
+    m.stop_condition_type, m.stop_condition_value, m.equilibrium_mode = sc, value, eq
+    return m
+
+
+@pytest.mark.parametrize("value,stop", [(10.0, 1000), (1500.4, 1501), (1500.6, 1502)])
+def test_stop_after_steps(value, stop):
+    # steps > (int)(value + 0.5), never before 2 * HISTORY_SIZE = 1000 steps
+    o = OSim(_drop_cube(1, value))
+    assert o.step(100000) == stop and o.status().state == 2 - 1
+
+
+@pytest.mark.parametrize("sc,value", [(2, 0.05), (3, 0.2)])
+def test_stop_after_time_and_temperature_cycles(sc, value):
+    m = _drop_cube(sc, value)
+    m.temp_period = 0.25
+    limit = value if sc == 2 else m.temp_period * value
+    o = OSim(m)
+    n = o.step(100000)
+    t = o.status().current_time
+    assert t > limit and n >= 1000
+    before = OSim(m)
+    before.step(n - 1)
+    assert before.status().current_time <= limit or n == 1000
+
+
+def test_free_fall_histories_closed_form():
+    # one voxel falling without floor or drag: p_n = n*m*g*dt, kinetic energy p^2/(2m),
+    # largest move per step |p|/m*dt
+    m = _drop_cube(5, 1e-9, n=1)
+    m.materials[0].zeta_global = 0.0
+    m.floor_enabled = 0
+    o = OSim(m)
+    o.step(300)
+    dt = float(np.float32(o.status().dt))
+    mass = 1e3 * 0.01 ** 3
+    k, t, mm = o.voxcad_history()
+    p = np.abs(o.state()["lin_mom"][0, 2])
+    assert p == pytest.approx(300 * mass * 9.80665 * dt, rel=1e-5)
+    assert k[0] == pytest.approx(p * p / (2 * mass), rel=1e-5)
+    assert k[1] == pytest.approx((299 * mass * 9.80665 * dt) ** 2 / (2 * mass), rel=1e-5)
+    assert (t == -1).all() and (mm == -1).all()          # only what the condition needs
+    m.stop_condition_type = 6
+    o = OSim(m)
+    o.step(300)
+    k, t, mm = o.voxcad_history()
+    rec_dt = o.status().recommended_dt * m.dt_frac      # the max-move history uses DtFrac * OptimalDt in double
+    assert mm[0] == pytest.approx(300 * 9.80665 * dt * rec_dt, rel=1e-5) and (k == -1).all()
+
+
+def test_strain_energy_of_a_stretched_bond():
+    # voxel 0 fixed, voxel 1 pulled along +x by F: at rest the tensile strain energy is
+    # F^2/(2*a1) with a1 = E*s, the bond's axial stiffness
+    m, second, _ = two_voxels("X_POS", ALL & ~TX, force=(0.5, 0, 0))
+    m.stop_condition_type, m.stop_condition_value = 4, 1e-12
+    m.gravity_mult = np.float32(0.0)
+    m.max_steps = 4000
+    o = OSim(m)
+    o.step(4000)
+    k, t, _ = o.voxcad_history()
+    assert k[0] < 1e-12
+    assert t[0] - k[0] == pytest.approx(0.5 ** 2 / (2 * 1e6 * 0.001), rel=1e-3)
+
+
+def test_equilibrium_zeroes_motion_when_kinetic_energy_falls():
+    # after each step the motion is zeroed (kinetic energy exactly 0) iff the kinetic
+    # energy of the three previous steps summed below that of the three before them
+    o = OSim(_drop_cube(0, 0.0, eq=1, lift=1))
+    ke = []
+    for _ in range(3000):
+        o.step(1)
+        ke.append(o.voxcad_history()[0][0])
+    hist = [-1.0] * 6
+    zeroed = 0
+    for n, e in enumerate(ke):
+        k = hist[-1:-7:-1]      # newest first
+        dec = sum(k[:3]) < sum(k[3:6]) and 0 not in k
+        assert (e == 0.0) == dec, n
+        zeroed += dec
+        hist.append(e)
+    assert zeroed > 10
+
+
+def test_min_kinetic_energy_stop_brute_force():
+    # stop at the first step n >= 1000 at which the kinetic energy per voxel at steps n,
+    # n-50, ..., n-450 is at most 0.001*value
+    value = 1e-3
+    o = OSim(_drop_cube(5, value))
+    ke = []
+    while o.status().state == 0:
+        if o.step(1) == 0:
+            break
+        ke.append(o.voxcad_history()[0][0])
+    nv = 27
+    first = next(n for n in range(1000, len(ke) + 1)
+                 if all(ke[n - 1 - a] / nv <= 0.001 * value for a in range(0, 500, 50)))
+    assert len(ke) == first

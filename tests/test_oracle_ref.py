@@ -102,3 +102,52 @@ def test_failed_links_corner_offsets():
     a, _ = run_both(m, 400, 100)
     fail = a.link_state()["max_strain"] > 20.0 / 1e6 + (22.0 - 20.0) / 2e5   # failure strain 3e-5
     assert fail.any() and not fail.all()
+
+
+def voxcad_model(sc, value, eq=0, lift=2, n=3):
+    """A cube dropped from `lift` voxels above the floor, with a VoxCAD stop condition."""
+    m = workloads.cube(n, steps=0)
+    m.ijk = m.ijk + np.array([0, 0, lift], np.int32)
+    m.vary_temp_enabled = 0
+    m.stop_condition_type, m.stop_condition_value, m.equilibrium_mode = sc, value, eq
+    return m
+
+
+@pytest.mark.parametrize("sc,value,eq", [(_abi.SC_MAX_TIME_STEPS, 1500.4, 0), (_abi.SC_MAX_SIM_TIME, 0.04, 0),
+                                         (_abi.SC_TEMP_CYCLES, 0.1, 0), (_abi.SC_CONST_MAXENERGY, 1e-3, 0),
+                                         (_abi.SC_MIN_KE, 1e-3, 0), (_abi.SC_MIN_KE, 1e-3, 1),
+                                         (_abi.SC_MIN_MAXMOVE, 1e-3, 0), (_abi.SC_NONE, 0.0, 1)])
+def test_voxcad_stop_conditions(sc, value, eq):
+    # CVX_Sim's stop-condition family and equilibrium mode on the reference's own
+    # engine (kineticEnergy, strainEnergy, velocity, haltMotion) against the oracle:
+    # same stop step, same histories, bit-exact states
+    m = voxcad_model(sc, value, eq)
+    m.max_steps = 3000 if sc == _abi.SC_NONE else 0
+    a, b = pair(m)
+    while a.status().state == _abi.SIM_RUNNING:
+        na, nb = a.step(500), b.step(500)
+        assert na == nb
+        same_state(a, b)
+        for ha, hb in zip(a.voxcad_history(), b.voxcad_history()):
+            assert np.array_equal(ha, hb)
+        sa, sb = a.status(), b.status()
+        assert (sa.steps, sa.state, sa.current_time) == (sb.steps, sb.state, sb.current_time)
+        assert sa.steps < 20000
+    assert a.status().state == _abi.SIM_DONE
+    assert a.status().steps >= 1000          # no VoxCAD condition stops before 2 * HISTORY_SIZE steps
+
+
+def test_voxcad_stop_with_collisions_and_strain_energy():
+    # contacts add the velocity statistic to every step; a constant-energy stop on the
+    # drop scene sums strain energy over links
+    m = workloads.drop(3, gap=1, shift=1, steps=0)
+    m.stop_condition_type, m.stop_condition_value, m.equilibrium_mode = _abi.SC_CONST_MAXENERGY, 1e-3, 1
+    m.max_steps = 6000
+    a, b = pair(m)
+    while a.status().state == _abi.SIM_RUNNING:
+        assert a.step(500) == b.step(500)
+        same_state(a, b)
+        for ha, hb in zip(a.voxcad_history(), b.voxcad_history()):
+            assert np.array_equal(ha, hb)
+    k, t, mm = a.voxcad_history()
+    assert (mm >= 0).all() and (t >= k).all() and (t > k).any()

@@ -48,13 +48,15 @@ def loader_vectors(text, steps=200):
     return v, dict(nv=nv, nl=o.nl, ijk=ijk, mat=mat, phase=ph, ext_dof=dof, ext_trans=tr, ext_rot=ro, ext_force=fo,
                    ext_moment=mo, env_grav=np.float32(d.gravity_mult), env_floor=int(d.floor_enabled),
                    env_dtfrac=d.dt_frac, env_temp_amp=d.temp_amplitude, env_temp_period=d.temp_period, vmat_consts=vmc,
-                   lmat_index=lidx, lmat_consts=lmc, mat_color=color, mat_id=mid, state=st)
+                   lmat_index=lidx, lmat_consts=lmc, mat_color=color, mat_id=mid,
+                   stop=np.array([v.info.stop_condition_type, v.info.stop_condition_value, v.info.equilibrium_mode],
+                                 np.float64), state=st)
 
 
 def check_against(ref, got):
     assert int(ref["nv"]) == got["nv"] and int(ref["nl"]) == got["nl"]
     for k in ("ijk", "mat", "phase", "ext_dof", "ext_trans", "ext_rot", "ext_force", "ext_moment", "vmat_consts",
-              "lmat_index", "lmat_consts", "mat_color", "mat_id"):
+              "lmat_index", "lmat_consts", "mat_color", "mat_id", "stop"):
         assert np.array_equal(np.asarray(ref[k]), np.asarray(got[k])), k
     assert np.float32(ref["env_grav"]) == got["env_grav"]
     assert int(ref["env_floor"]) == got["env_floor"]

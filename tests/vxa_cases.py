@@ -83,7 +83,11 @@ def _display_colors():
     text = text.replace(mats[2], re.sub(r"<Display>.*?</Display>", "", mats[2]))
     text = text.replace(mats[3], re.sub(r"<Green>[^<]*</Green><Blue>[^<]*</Blue>", "",
                                         re.sub(r"<Mechanical>.*?</Mechanical>", "", mats[3], flags=re.S)))
-    return text
+    # VoxCAD stop-condition keys, which the Voxelyze3 driver reads and ignores
+    return text.replace("</Simulator>", "<StopCondition><StopConditionType>5</StopConditionType>"
+                                        "<StopConditionValue>0.25</StopConditionValue></StopCondition>"
+                                        "<EquilibriumMode><EquilibriumModeEnabled>1</EquilibriumModeEnabled>"
+                                        "</EquilibriumMode></Simulator>")
 
 
 CASES = {
