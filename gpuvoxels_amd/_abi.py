@@ -42,6 +42,8 @@ class MaterialDesc(C.Structure):
         ("thermal_on_after", C.c_double),
         ("matid", C.c_int),
         ("color", C.c_int * 4),
+        ("cilia", C.c_double),
+        ("cilia_on_after", C.c_double),
     ]
 
 
@@ -94,6 +96,11 @@ class SimDesc(C.Structure):
         ("stop_condition_type", C.c_int),
         ("stop_condition_value", C.c_double),
         ("equilibrium_mode", C.c_int),
+        ("enable_cilia", C.c_int),
+        ("base_cilia_force", C.POINTER(C.c_double)),
+        ("shift_cilia_force", C.POINTER(C.c_double)),
+        ("n_force_field_tokens", C.c_int * 3),
+        ("force_field_tokens", C.POINTER(Token) * 3),
     ]
 
 

@@ -20,6 +20,9 @@ template <typename T> struct DevBatch {
     int* vijk;                // 3*Vcap lattice index (externals / reporting)
     float *psx, *psy, *psz;   // Poisson strain of the voxel (CVX_Voxel::poissonsStrain)
     T* F;                     // 36*Vcap link end loads: F[(d*6+c)*Vcap + v]
+    double* cil;              // 8*Vcap cilia: base force xyz, shift force xyz (voxel frame), Cilia scale,
+                              // switch-on time; cil[k*Vcap + v]
+    T* xf;                    // 6*Vcap this step's global cilia force xyz and force-field force xyz
     float* es;                // 6*Vcap axial strain of the link end in slot d: es[d*Vcap + v]
     // link state [Lcap]
     int *lneg, *lpos;

@@ -394,14 +394,33 @@ struct Contacts {
     }
 };
 
-// at most 128 VGPRs (four waves per SIMD): the voxel pass is a gather over the force
-// slots, latency-bound at the three waves its unconstrained allocation allows
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_voxels(DevBatch<real> b) {
+// The extra forces of a simulation with cilia or a force field, added where the
+// reference adds them: the cilia force at the end of force() (VX3_Voxel.cu:376), the
+// force field right after force() returns (VX3_Voxel.cu:178-184).
+template <typename Inner> struct WithExtras {
+    Inner inner;
+    const real* xf;
+    size_t V;
+    int v;
+    template <typename T> __device__ __forceinline__ void subtract(V3<T>& f) const {
+        inner.subtract(f);
+        f.add(V3<T>(xf[0 * V + v], xf[1 * V + v], xf[2 * V + v]));
+        f.x += xf[3 * V + v];
+        f.y += xf[4 * V + v];
+        f.z += xf[5 * V + v];
+    }
+};
+
+// The voxel pass of the simulations without (EXTRAS false) or with (true) cilia or a
+// force field; the two are separate kernels so that the common one keeps its
+// registers.
+template <bool EXTRAS> __device__ __forceinline__ void voxelPass(const DevBatch<real>& b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
     if (!simRunning(b.sims, s)) return;
     const SimCtl& sc = b.sims[s];
+    if (((sc.flags & SC_EXTRAS) != 0) != EXTRAS) return;
     if (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW)) return;   // doTimeStep returns before the voxel loop
     const VoxMatDev& m = b.vmatTab[b.vmat[v]];
     const float tf = b.tfac[v];
@@ -423,11 +442,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
         V3<real> pos(b.px[v], b.py[v], b.pz[v]), lm(b.mx[v], b.my[v], b.mz[v]), am(b.ax[v], b.ay[v], b.az[v]);
         Q4<real> o(b.ow[v], b.ox[v], b.oy[v], b.oz[v]);
-        if ((sc.flags & SC_COLLISIONS) && !(fl & VF_INTERIOR)) {
-            Contacts con{&b, v, b.nbStart[v], b.nbCount[v], (real)sc.voxSize};
-            voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, con, pos, lm, o, am, fl);
+        if constexpr (EXTRAS) {
+            if ((sc.flags & SC_COLLISIONS) && !(fl & VF_INTERIOR)) {
+                WithExtras<Contacts> con{Contacts{&b, v, b.nbStart[v], b.nbCount[v], (real)sc.voxSize}, b.xf, V, v};
+                voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, con, pos, lm, o, am, fl);
+            } else {
+                WithExtras<NoContacts> con{NoContacts(), b.xf, V, v};
+                voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, con, pos, lm, o, am, fl);
+            }
         } else {
-            voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, NoContacts(), pos, lm, o, am, fl);
+            if ((sc.flags & SC_COLLISIONS) && !(fl & VF_INTERIOR)) {
+                Contacts con{&b, v, b.nbStart[v], b.nbCount[v], (real)sc.voxSize};
+                voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, con, pos, lm, o, am, fl);
+            } else {
+                voxelStep<real>(m, ext, sc.dt, (real)sc.voxSize, ix, iy, iz, tf, f, mo, NoContacts(), pos, lm, o, am, fl);
+            }
         }
         b.px[v] = pos.x; b.py[v] = pos.y; b.pz[v] = pos.z;
         b.mx[v] = lm.x; b.my[v] = lm.y; b.mz[v] = lm.z;
@@ -445,6 +474,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     if (sc.flags & SC_POISSON) poissonStrain(b, v);
 }
+
+// at most 128 VGPRs (four waves per SIMD): the voxel pass is a gather over the force
+// slots, latency-bound at the three waves its unconstrained allocation allows
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_voxels(DevBatch<real> b) { voxelPass<false>(b); }
+__global__ void __launch_bounds__(256) k_voxels_extras(DevBatch<real> b) { voxelPass<true>(b); }
 
 // ---- MathTree (queue evaluator of Utils/VX3_MathTree.h:50-192), device side, bounded programs.
 // The value queue q lives in LDS (dynamically indexed), supplied by the caller.
@@ -508,6 +542,42 @@ __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, doubl
         q[vc++] = r;
     }
     return q[pc < VX3_DEV_MT_MAX ? pc : 0];
+}
+
+// Extra global forces of the simulations with VX3 cilia or a force field, for the
+// voxel pass that follows, from the state before this step's voxel update:
+// cilia as gpu_update_cilia_force (VX3_VoxelyzeKernel.cu:846-859): a surface voxel
+// whose material has Cilia != 0 and whose switch-on time has passed is pushed by
+// orient.RotateVec3D(base + localSignal*shift) * Cilia, localSignal being 0 (signals
+// are not simulated); the force field's three programs of the voxel position at the
+// step's start time (VX3_Voxel.cu:178-184; hit, closeness and close pairs are 0).
+__global__ void __launch_bounds__(256) k_extras(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!simRunning(b.sims, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_EXTRAS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+    const size_t V = (size_t)b.Vcap;
+    V3<real> c(0, 0, 0);
+    if (sc.flags & SC_CILIA) {
+        const double coef = b.cil[6 * V + v], on = b.cil[7 * V + v];
+        if (!(b.vflags[v] & VF_INTERIOR) && coef != 0 && !(on > sc.t)) {
+            const double sig = 0.0;
+            const V3<real> f((real)(b.cil[0 * V + v] + sig * b.cil[3 * V + v]), (real)(b.cil[1 * V + v] + sig * b.cil[4 * V + v]),
+                             (real)(b.cil[2 * V + v] + sig * b.cil[5 * V + v]));
+            const Q4<real> o(b.ow[v], b.ox[v], b.oy[v], b.oz[v]);
+            c = o.rot(f).mul((real)coef);
+        }
+    }
+    double ff[3] = {0, 0, 0};
+    double q[VX3_DEV_MT_MAX + 1];
+    for (int k = 0; k < 3; k++)
+        if (sc.ffLen[k] > 0)
+            ff[k] = mathTreeEval(b.tokens + sc.ffOff[k], sc.ffLen[k], (double)b.px[v], (double)b.py[v], (double)b.pz[v], 0, sc.t,
+                                 sc.recentAngle, 0, 0, sc.nv, q);
+    b.xf[0 * V + v] = c.x; b.xf[1 * V + v] = c.y; b.xf[2 * V + v] = c.z;
+    b.xf[3 * V + v] = (real)ff[0]; b.xf[4 * V + v] = (real)ff[1]; b.xf[5 * V + v] = (real)ff[2];
 }
 
 // Block-wide deterministic FP64 center of mass of measured voxels (VX3_VoxelyzeKernel.cu:477-493).

@@ -7,6 +7,7 @@
 namespace vx3 {
 struct StepOpts {
     bool contacts;     // some simulation has collisions: run the collision passes
+    bool extras;       // some simulation has cilia or a force field: run the extra-force pass
     int block;         // workgroup size of the element passes
 };
 // Kernel passes of one step, in launch order; ev (optional) receives a start/stop

@@ -92,7 +92,6 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             if (raw && *raw) printf("%s\n", raw);   // RawPrint (VX3_SimulationManager.cu:319-321)
             std::string missing;
             if (info.enable_attach || info.enable_detach) missing += " attach/detach";
-            if (info.enable_cilia) missing += " cilia";
             if (info.enable_signals) missing += " signals";
             if (info.secondary_experiment) missing += " secondary-experiment";
             if (info.max_dist_close_pairs > 0) missing += " close-pair counting";

@@ -9,7 +9,7 @@
 // to the output path.  Exit status 1 on bad arguments, an existing output without -f,
 // a missing task file or no GPU; 0 otherwise.  Additions: --f32 runs the FP32 state
 // engine (default FP64), --allow-unsupported runs VXAs that enable features this
-// engine does not simulate (attach/detach, cilia, signals) without them, and
+// engine does not simulate (attach/detach, signals) without them, and
 // --voxcad-stop honours the VXA's StopConditionType/Value and EquilibriumMode the way
 // VoxCAD does (the Voxelyze3 worker reads and ignores them).
 #include <sys/stat.h>

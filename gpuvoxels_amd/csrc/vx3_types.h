@@ -33,7 +33,9 @@ enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2 };
 // per-sim control bits
 enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32, SC_RESIDENT = 64,
        SC_REBUILD = 128,        // collision neighbour lists are rebuilt this step
-       SC_COL_OVERFLOW = 256 }; // a collision table overflowed: the simulation fails
+       SC_COL_OVERFLOW = 256,   // a collision table overflowed: the simulation fails
+       SC_EXTRAS = 512,         // cilia or a force field add to the voxel force (xf arrays)
+       SC_CILIA = 1024 };
 #define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
@@ -104,6 +106,8 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     double scValue;                  // StopConditionValue
     double dtFixed;                  // > 0: the fixed step (else DtFrac*OptimalDt) for the max-move history
     int histHead;                    // ring index of the newest entry of the 500-step histories
+    // VX3 force field (VX3_Voxel.cu:178-184): MathTree programs per axis in the token pool
+    int ffOff[3], ffLen[3];
 };
 
 #define VX3_HIST 500                 // HISTORY_SIZE (VoxCAD/src/VX1/VX_Sim.h)

@@ -475,6 +475,32 @@ bool buildModel(const vx3h_sim_desc* d, HostModel& H, std::string& err) {
     if (d->n_stop_tokens > 0) H.stopTok.assign(d->stop_tokens, d->stop_tokens + d->n_stop_tokens);
     if (d->n_fitness_tokens > 0) H.fitTok.assign(d->fitness_tokens, d->fitness_tokens + d->n_fitness_tokens);
     H.saveAll = d->save_position_of_all_voxels;
+    if (d->enable_cilia) {
+        H.vmCilia.resize(d->n_materials);
+        H.vmCiliaOn.resize(d->n_materials);
+        bool any = false;
+        for (int i = 0; i < d->n_materials; i++) {
+            H.vmCilia[i] = d->materials[i].cilia;
+            H.vmCiliaOn[i] = d->materials[i].cilia_on_after;
+            any |= d->materials[i].cilia != 0;
+        }
+        if (any) {
+            H.cilia.assign(6 * (size_t)H.nv, 0.0);
+            for (int v = 0; v < H.nv; v++)
+                for (int k = 0; k < 3; k++) {
+                    if (d->base_cilia_force) H.cilia[6 * (size_t)v + k] = d->base_cilia_force[3 * (size_t)v + k];
+                    if (d->shift_cilia_force) H.cilia[6 * (size_t)v + 3 + k] = d->shift_cilia_force[3 * (size_t)v + k];
+                }
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        const int n = d->n_force_field_tokens[k];
+        if (n < 0 || n > VX3H_MAX_TOKENS || (n > 0 && !d->force_field_tokens[k])) {
+            err = "bad force-field program";
+            return false;
+        }
+        if (n > 0) H.ffTok[k].assign(d->force_field_tokens[k], d->force_field_tokens[k] + n);
+    }
     return true;
 }
 

@@ -46,6 +46,13 @@ struct HostModel {
     double boundingRadius = 0.75, watchDistance = 1.0;
     std::vector<vx3h_token> stopTok, fitTok;
     int saveAll = 0;
+    // VX3 extras: cilia (6 per voxel: base then shift force, voxel frame; empty when
+    // cilia are off) with each voxel-material's Cilia scale and switch-on time, and
+    // the force-field programs per axis
+    std::vector<double> cilia;
+    std::vector<double> vmCilia, vmCiliaOn;
+    std::vector<vx3h_token> ffTok[3];
+    bool extras() const { return !cilia.empty() || !ffTok[0].empty() || !ffTok[1].empty() || !ffTok[2].empty(); }
 };
 
 // Returns false and fills err on an invalid description.

@@ -167,6 +167,7 @@ struct vx3h_batch {
     void* dSimMats = nullptr; size_t capSimMats = 0;
     DevPool nbPool, hashPool, nearPool;
     bool anyColl = false;
+    bool anyExtras = false;             // some simulation has cilia or a force field
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
     // resident path: kernel shapes and, per shape, the sims it runs
@@ -198,7 +199,7 @@ struct vx3h_batch {
 
 namespace {
 
-size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8; }
+size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8; }
 size_t linkBytes(size_t rs) { return rs * 9 + 4 * 5 + 4 * 3; }
 
 void initSpecs(vx3h_batch* b) {
@@ -218,6 +219,7 @@ void initSpecs(vx3h_batch* b) {
     V(&d.cpx, R, 1, false); V(&d.cpy, R, 1, false); V(&d.cpz, R, 1, false);
     V(&d.cmx, R, 1, false); V(&d.cmy, R, 1, false); V(&d.cmz, R, 1, false); V(&d.ctf, 4, 1, false);
     V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
+    V(&d.cil, 8, 8, true); V(&d.xf, R, 6, true);
     b->nbPool.arrs = {{(void**)&d.nbList, 4}};
     b->hashPool.arrs = {{(void**)&d.hashCnt, 4}, {(void**)&d.hashEnt, 4 * VX3_COL_BUCKET}, {(void**)&d.hashSim, 4}};
     b->nearPool.arrs = {{(void**)&d.nearList, 4}};
@@ -323,6 +325,7 @@ char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx;
 vx3h_status launchSteps(vx3h_batch* b, int n) {
     StepOpts o;
     o.contacts = b->anyColl;
+    o.extras = b->anyExtras;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
@@ -505,6 +508,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
     if (H.stopTok.size() > 64) FAILV(VX3H_EUNSUPPORTED, "stop condition longer than 64 tokens");
+    for (int k = 0; k < 3; k++)
+        if (H.ffTok[k].size() > 64) FAILV(VX3H_EUNSUPPORTED, "force-field program longer than 64 tokens");
     const size_t rs = b->realSize;
     const size_t need = (size_t)H.nv * voxelBytes(rs) + (size_t)H.nl * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
     if (b->bytesUsed + need > b->budget) FAILV(VX3H_ENOMEM, "simulation does not fit the batch's device memory budget");
@@ -558,7 +563,12 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     b->db.simMats = (const int*)b->dSimMats;
     int tok0 = (int)b->tokens.size();
     for (const vx3h_token& t : H.stopTok) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
-    if (!H.stopTok.empty()) { vx3h_status st = uploadTable(b, b->dTok, b->capTok, b->tokens); if (st) return st; }
+    int ffOff[3];
+    for (int k = 0; k < 3; k++) {
+        ffOff[k] = (int)b->tokens.size();
+        for (const vx3h_token& t : H.ffTok[k]) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
+    }
+    if ((int)b->tokens.size() > tok0) { vx3h_status st = uploadTable(b, b->dTok, b->capTok, b->tokens); if (st) return st; }
     b->db.vmatTab = (const VoxMatDev*)b->dVmat;
     b->db.lmatTab = (const LinkMatDev*)b->dLmat;
     b->db.extTab = (const ExtDev*)b->dExt;
@@ -592,6 +602,14 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         HIPCHK(hipMemset(d.psz + v0, 0, NV * 4));
         for (int k = 0; k < 6; k++) HIPCHK(hipMemset(d.es + (size_t)k * d.Vcap + v0, 0, NV * 4));
         HIPCHK(hipMemcpy(d.vsim + v0, vs.data(), NV * 4, hipMemcpyHostToDevice));
+        if (!H.cilia.empty()) {   // per voxel: base, shift, the material's Cilia scale and switch-on time
+            std::vector<double> col(NV);
+            for (int k = 0; k < 8; k++) {
+                for (int i = 0; i < NV; i++)
+                    col[i] = k < 6 ? H.cilia[6 * (size_t)i + k] : k == 6 ? H.vmCilia[H.vmat[i]] : H.vmCiliaOn[H.vmat[i]];
+                HIPCHK(hipMemcpy(d.cil + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+            }
+        }
     }
     if (NL) {
         std::vector<int> ln(NL), lp(NL), lm(NL), ls(NL, s), lf(NL, LF_SMALL_ANGLE);
@@ -624,7 +642,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     // VoxCAD stop conditions and equilibrium mode are evaluated by the stream path's control pass
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
-                       nPhases <= RES_MAX_PHASES && !voxcad;
+                       nPhases <= RES_MAX_PHASES && !voxcad && !H.extras();
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -635,6 +653,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     }
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
+    if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -661,6 +680,9 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     c.eqMode = desc->equilibrium_mode != 0;
     c.dtFixed = H.dtFixed;
     c.histHead = 0;
+    for (int k = 0; k < 3; k++) { c.ffOff[k] = ffOff[k]; c.ffLen[k] = (int)H.ffTok[k].size(); }
+    if (H.extras()) { c.flags |= SC_EXTRAS; b->anyExtras = true; }
+    if (!H.cilia.empty()) c.flags |= SC_CILIA;
     if (voxcad) {
         const std::vector<double> unset(3 * VX3_HIST, -1.0);   // ClearHistories
         HIPCHK(hipMemcpy(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -824,6 +846,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     StepOpts o;
     o.contacts = b->anyColl;
+    o.extras = b->anyExtras;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];

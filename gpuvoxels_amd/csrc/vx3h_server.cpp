@@ -31,7 +31,8 @@ struct DescCopy {
     std::vector<int> ijk, mat;
     std::vector<double> phase;
     std::vector<vx3h_external> exts;
-    std::vector<vx3h_token> stop, fit;
+    std::vector<vx3h_token> stop, fit, ff[3];
+    std::vector<double> baseCilia, shiftCilia;
     void take(const vx3h_sim_desc& s) {
         d = s;
         mats.assign(s.materials, s.materials + s.n_materials);
@@ -48,6 +49,15 @@ struct DescCopy {
         d.externals = exts.empty() ? nullptr : exts.data();
         d.stop_tokens = stop.empty() ? nullptr : stop.data();
         d.fitness_tokens = fit.empty() ? nullptr : fit.data();
+        if (s.base_cilia_force) baseCilia.assign(s.base_cilia_force, s.base_cilia_force + 3 * (size_t)s.n_voxels);
+        if (s.shift_cilia_force) shiftCilia.assign(s.shift_cilia_force, s.shift_cilia_force + 3 * (size_t)s.n_voxels);
+        d.base_cilia_force = s.base_cilia_force ? baseCilia.data() : nullptr;
+        d.shift_cilia_force = s.shift_cilia_force ? shiftCilia.data() : nullptr;
+        for (int k = 0; k < 3; k++) {
+            if (s.n_force_field_tokens[k] > 0)
+                ff[k].assign(s.force_field_tokens[k], s.force_field_tokens[k] + s.n_force_field_tokens[k]);
+            d.force_field_tokens[k] = ff[k].empty() ? nullptr : ff[k].data();
+        }
     }
 };
 

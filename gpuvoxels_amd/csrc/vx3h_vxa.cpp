@@ -334,6 +334,7 @@ struct PalMat {                 // CVXC_Material (VX_Object.cpp:1373-1460), SING
     int fixed = 0;
     double E = 0, plasticMod = 0, yieldStress = 0, failStress = 0, density = 0, nu = 0, cte = 0, uS = 0, uD = 0;
     float rgba[4] = {0.5f, 0.5f, 0.5f, 1.0f};   // Display colour (CVXC_Material::ClearMaterial defaults)
+    double cilia = 0, ciliaOn = 0;              // Cilia, TurnOnCiliaAfterThisManySeconds
 };
 
 bool readRegion(Rip& r, Region& g) {
@@ -478,7 +479,8 @@ struct vx3h_vxa {
     std::vector<int> ijk, mat;
     std::vector<double> phase;
     std::vector<vx3h_external> exts;
-    std::vector<vx3h_token> stop, fit;
+    std::vector<vx3h_token> stop, fit, ff[3];
+    std::vector<double> baseCilia, shiftCilia;
     std::string rawPrint;
     vx3h_vxa_info info;
 };
@@ -559,6 +561,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     int XV = 1, YV = 1, ZV = 1;
     std::vector<unsigned char> cells;
     std::vector<double> cellPhase;
+    std::vector<double> cellCilia[2];   // BaseCiliaForce, ShiftCiliaForce: 3 per occupied cell
     bool havePhase = false;
     if (r.find("VXC") || r.find("DMF")) {
         const std::string version = r.attr("Version");
@@ -621,6 +624,8 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                     if (!r.load("isMeasured", m.isMeasured)) m.isMeasured = 1;
                     if (!r.load("TurnOnThermalExpansionAfterThisManySeconds", m.thermalOnAfter)) m.thermalOnAfter = 0;
                     if (!r.load("Fixed", m.fixed)) m.fixed = 0;
+                    if (!r.load("TurnOnCiliaAfterThisManySeconds", m.ciliaOn)) m.ciliaOn = 0.0;
+                    if (!r.load("Cilia", m.cilia)) m.cilia = 0;
                     if (!r.load("Elastic_Mod", m.E)) m.E = 0;
                     if (!r.load("Plastic_Mod", m.plasticMod)) m.plasticMod = 0;
                     if (!r.load("Yield_Stress", m.yieldStress)) m.yieldStress = 0;
@@ -681,6 +686,24 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                 r.up();
                 r.up();
             }
+            // CVXC_Structure::ReadXML (VX_Object.cpp:1886-1958): three values per cell k of a
+            // layer at 3k..3k+2, for the occupied cells; a short layer pads with zeros
+            const char* ciliaTags[2] = {"BaseCiliaForce", "ShiftCiliaForce"};
+            for (int c = 0; c < 2; c++) {
+                if (!r.find(ciliaTags[c])) continue;
+                for (int z = 0; z < ZV; z++) {
+                    std::string raw;
+                    r.load("Layer", raw, true);
+                    std::vector<std::string> vals = splitComma(raw);
+                    for (size_t k = 0; k < layer; k++) {
+                        if ((signed char)cells[z * layer + k] > 0)
+                            for (int j = 0; j < 3; j++)
+                                cellCilia[c].push_back(3 * k + j < vals.size() ? atof(vals[3 * k + j].c_str()) : 0.0);
+                    }
+                }
+                r.up();
+                r.up();
+            }
             r.up();
         }
         r.up();
@@ -718,6 +741,8 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
         d.is_measured = p.isMeasured;
         d.fixed = p.fixed != 0;
         d.thermal_on_after = p.thermalOnAfter;
+        d.cilia = p.cilia;
+        d.cilia_on_after = p.ciliaOn;
         d.matid = (int)i + 1;   // palette index; 0 is the erase material
         for (int k = 0; k < 4; k++) {   // GetRedi() etc. (VX_Object.h:419-425), then setRed's clamp
             const int c = (int)(p.rgba[k] * 255);
@@ -744,6 +769,13 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     const int NV = (int)o.mat.size();
     o.phase.assign(NV, 0.0);
     if (havePhase) for (int i = 0; i < NV && i < (int)cellPhase.size(); i++) o.phase[i] = cellPhase[i];
+    // per voxel cilia vectors (VX_Sim.cpp:102-105), zero where the structure has none
+    std::vector<double>* ciliaOut[2] = {&o.baseCilia, &o.shiftCilia};
+    for (int c = 0; c < 2; c++) {
+        if (cellCilia[c].empty()) continue;
+        ciliaOut[c]->assign(3 * (size_t)NV, 0.0);
+        for (size_t i = 0; i < ciliaOut[c]->size() && i < cellCilia[c].size(); i++) (*ciliaOut[c])[i] = cellCilia[c][i];
+    }
 
     // boundary conditions -> externals (Import, VX_Sim.cpp:108-141; GetNumTouching, VX_Environment.cpp:215-232)
     if (!bcs.empty()) {
@@ -815,6 +847,9 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     // ---- VX3 simulator keys (readVXD, VX3_SimulationManager.cu:323-367)
     if (!parseMathTree(doc, doc.name + ".Simulator.StopCondition.StopConditionFormula", o.stop, err)) return VX3H_EPARSE;
     if (!parseMathTree(doc, doc.name + ".Simulator.FitnessFunction", o.fit, err)) return VX3H_EPARSE;
+    const char* ffAxes[3] = {"x_forcefield", "y_forcefield", "z_forcefield"};   // VX3_SimulationManager.cu:349-354
+    for (int k = 0; k < 3; k++)
+        if (!parseMathTree(doc, doc.name + ".Simulator.ForceField." + ffAxes[k], o.ff[k], err)) return VX3H_EPARSE;
     const std::string S = doc.name + ".Simulator.";
     vx3h_vxa_info& in = o.info;
     memset(&in, 0, sizeof(in));
@@ -866,6 +901,13 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     d.fitness_tokens = o.fit.data();
     d.max_steps = 1000000;   // the driver loop's cap (VX3_SimulationManager.cu:62)
     d.save_position_of_all_voxels = ptGet<int>(doc, S + "SavePositionOfAllVoxels", 0);
+    d.enable_cilia = in.enable_cilia;
+    d.base_cilia_force = o.baseCilia.empty() ? nullptr : o.baseCilia.data();
+    d.shift_cilia_force = o.shiftCilia.empty() ? nullptr : o.shiftCilia.data();
+    for (int k = 0; k < 3; k++) {
+        d.n_force_field_tokens[k] = (int)o.ff[k].size();
+        d.force_field_tokens[k] = o.ff[k].empty() ? nullptr : o.ff[k].data();
+    }
     return VX3H_OK;
 }
 

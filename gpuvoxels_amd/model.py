@@ -38,6 +38,8 @@ class Material:
     thermal_on_after: float = 0.0
     matid: int = 0
     color: tuple = (0.5, 0.5, 0.5, 1.0)   # Display Red, Green, Blue, Alpha in [0, 1]
+    cilia: float = 0.0                    # scale of the cilia force of voxels of this material
+    cilia_on_after: float = 0.0           # TurnOnCiliaAfterThisManySeconds
 
     def to_c(self) -> _abi.MaterialDesc:
         m = _abi.MaterialDesc()
@@ -62,6 +64,8 @@ class Material:
         m.fixed = self.fixed
         m.thermal_on_after = self.thermal_on_after
         m.matid = self.matid
+        m.cilia = self.cilia
+        m.cilia_on_after = self.cilia_on_after
         for k in range(4):   # CVXC_Material::GetRedi etc. then CVX_Material::setRed's clamp
             m.color[k] = min(255, max(0, int(np.float32(self.color[k]) * np.float32(255))))
         return m
@@ -144,6 +148,12 @@ class Model:
     stop_condition_type: int = 0
     stop_condition_value: float = 0.0
     equilibrium_mode: int = 0
+    # VX3 cilia (surface voxels pushed along base_cilia rotated with the voxel) and
+    # force field (a MathTree token list per axis, empty = none)
+    enable_cilia: int = 0
+    base_cilia: Optional[np.ndarray] = None   # (V, 3) N, voxel frame
+    shift_cilia: Optional[np.ndarray] = None  # (V, 3)
+    force_field: list = field(default_factory=lambda: [[], [], []])
     name: str = "sim"
 
     @property
@@ -207,4 +217,18 @@ class Model:
         d.stop_condition_type = self.stop_condition_type
         d.stop_condition_value = self.stop_condition_value
         d.equilibrium_mode = self.equilibrium_mode
+        d.enable_cilia = self.enable_cilia
+        for name, arr in (("base_cilia_force", self.base_cilia), ("shift_cilia_force", self.shift_cilia)):
+            if arr is not None:
+                a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+                if a.size != 3 * self.n_voxels:
+                    raise ValueError("%s needs 3 values per voxel" % name)
+                keep.append(a)
+                setattr(d, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+        for k, toks in enumerate(self.force_field):
+            if toks:
+                arr = (_abi.Token * len(toks))(*[_abi.Token(op, val) for op, val in toks])
+                keep.append(arr)
+                d.n_force_field_tokens[k] = len(toks)
+                d.force_field_tokens[k] = C.cast(arr, C.POINTER(_abi.Token))
         return d, keep

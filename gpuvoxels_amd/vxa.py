@@ -101,6 +101,12 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                % (int(bool(m.collisions_enabled)), _num(m.bounding_radius), _num(m.watch_distance)))
     out.append("<EnableExpansion>%d</EnableExpansion>" % int(bool(m.enable_expansion)))
     out.append("<SavePositionOfAllVoxels>%d</SavePositionOfAllVoxels>" % int(bool(m.save_position_of_all_voxels)))
+    if m.enable_cilia:
+        out.append("<EnableCilia>1</EnableCilia>")
+    if any(m.force_field):
+        out.append("<ForceField>%s</ForceField>" % "".join(
+            "<%s>%s</%s>" % (axis, mathtree_xml(toks), axis)
+            for axis, toks in zip(("x_forcefield", "y_forcefield", "z_forcefield"), m.force_field) if toks))
     out.append(extra_simulator)
     out.append("</Simulator>")
     out.append("<Environment>")
@@ -136,7 +142,9 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                 "<isMeasured>%d</isMeasured>" % int(mt.is_measured),
                 "<Fixed>%d</Fixed>" % int(mt.fixed),
                 "<TurnOnThermalExpansionAfterThisManySeconds>%s</TurnOnThermalExpansionAfterThisManySeconds>"
-                % _num(mt.thermal_on_after)]
+                % _num(mt.thermal_on_after),
+                "<Cilia>%s</Cilia>" % _num(mt.cilia),
+                "<TurnOnCiliaAfterThisManySeconds>%s</TurnOnCiliaAfterThisManySeconds>" % _num(mt.cilia_on_after)]
         if mt.strain_data:
             mech.append("<SSData><NumDataPts>%d</NumDataPts><StrainData>%s</StrainData><StressData>%s</StressData></SSData>"
                         % (len(mt.strain_data), "".join("<Strain>%s</Strain>" % _num(e) for e in mt.strain_data),
@@ -163,10 +171,13 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
     for z in range(ZV):
         out.append("<Layer><![CDATA[%s]]></Layer>" % ",".join(repr(float(v)) for v in ph[z].ravel()))
     out.append("</PhaseOffset>")
-    for tag in ("BaseCiliaForce", "ShiftCiliaForce"):
+    for tag, vec in (("BaseCiliaForce", m.base_cilia), ("ShiftCiliaForce", m.shift_cilia)):
+        cv = np.zeros((ZV, YV, XV, 3))
+        if vec is not None:
+            cv[ijk[:, 2], ijk[:, 1], ijk[:, 0]] = np.asarray(vec, dtype=np.float64).reshape(-1, 3)
         out.append("<%s>" % tag)
         for z in range(ZV):
-            out.append("<Layer><![CDATA[%s]]></Layer>" % ",".join(["0"] * (3 * XV * YV)))
+            out.append("<Layer><![CDATA[%s]]></Layer>" % ",".join(repr(float(v)) for v in cv[z].ravel()))
         out.append("</%s>" % tag)
     out.append("</Structure></VXC></VXA>")
     return "\n".join(out)

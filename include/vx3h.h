@@ -87,6 +87,8 @@ typedef struct vx3h_material_desc {
     int    matid;               /* palette id reported in the report's <mats> */
     int    color[4];            /* Display Red/Green/Blue/Alpha as 0-255 (int)(x*255), clamped
                                    (VXA/VX_Object.h:419-425, old/VX_Material.cpp:245-283) */
+    double cilia;               /* Cilia: scale of the voxel's cilia force (VX3_Voxel.cu:376) */
+    double cilia_on_after;      /* TurnOnCiliaAfterThisManySeconds */
 } vx3h_material_desc;
 
 /* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
@@ -141,6 +143,19 @@ typedef struct vx3h_sim_desc {
     int    stop_condition_type;   /* VX3H_SC_* */
     double stop_condition_value;
     int    equilibrium_mode;      /* zero all motion whenever the kinetic energy falls */
+    /* VX3 cilia (Simulator.EnableCilia; VX3_VoxelyzeKernel.cu:296-303,846-859): each
+     * surface voxel whose material has Cilia != 0, from TurnOnCiliaAfterThisManySeconds
+     * on, is pushed by orient.RotateVec3D(base + localSignal*shift) * Cilia, with
+     * localSignal 0 (signals are not simulated).  The arrays hold 3 per voxel
+     * (Structure.BaseCiliaForce / ShiftCiliaForce) or are NULL for zeros. */
+    int    enable_cilia;
+    const double* base_cilia_force;
+    const double* shift_cilia_force;
+    /* VX3 force field (Simulator.ForceField.{x,y,z}_forcefield; VX3_Voxel.cu:178-184):
+     * a MathTree per axis of the voxel position, added to every moving voxel's force;
+     * 0 tokens = no force on that axis. */
+    int    n_force_field_tokens[3];
+    const vx3h_token* force_field_tokens[3];
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */

@@ -218,6 +218,18 @@ void ref_vxa_voxels(vxo_sim* h, int* ijk, int* mat, double* phase) {
         phase[i] = v->phaseOffset;
     }
 }
+/* Cilia data the reference's importer attached (VX_Sim.cpp:102-105, 381-386): per voxel
+ * base and shift force (3 each), per material Cilia and TurnOnCiliaAfterThisManySeconds. */
+void ref_vxa_cilia(vxo_sim* h, double* base, double* shift, double* matCilia, double* matOn) {
+    for (int i = 0; i < h->vx->voxelCount(); i++) {
+        CVX_Voxel* v = h->vx->voxel(i);
+        for (int k = 0; k < 3; k++) { base[3 * i + k] = v->baseCiliaForce[k]; shift[3 * i + k] = v->shiftCiliaForce[k]; }
+    }
+    for (int i = 0; i < h->vx->materialCount(); i++) {
+        matCilia[i] = h->vx->material(i)->Cilia;
+        matOn[i] = h->vx->material(i)->TurnOnCiliaAfterThisManySeconds;
+    }
+}
 /* Stop-condition type and value and the equilibrium-mode switch CVX_Sim::ReadVXA read
  * (VX_Sim.cpp:260-277). */
 void ref_vxa_stop(vxo_sim* h, int* type, double* value, int* eq) {

@@ -552,6 +552,12 @@ struct Voxel {
     Link* links[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     Vd pos, linMom, angMom;
     Qd orient;
+    // VX3 extras (GPU-only in the reference): this step's cilia force, added at the end
+    // of force() (VX3_Voxel.cu:376), and force-field force, added after force()
+    // (VX3_Voxel.cu:178-184); used when the simulation has either
+    bool extras = false;
+    Vd ciliaForce, fieldForce;
+    Vd baseCilia, shiftCilia;
     int boolStates = 0;
     float temp = 0;
     Vf pStrain;
@@ -850,6 +856,7 @@ Vd Voxel::force() {
     if (isCollisionsEnabled()) {
         for (Collision* c : *colWatch) totalForce -= c->contactForce(this);
     }
+    if (extras) totalForce += ciliaForce;
     return totalForce;
 }
 Vd Voxel::moment() {
@@ -889,6 +896,11 @@ void Voxel::timeStep(float dt) {
         return;
     }
     Vd curForce = force();
+    if (extras) {
+        curForce.x += fieldForce.x;
+        curForce.y += fieldForce.y;
+        curForce.z += fieldForce.z;
+    }
     Vd fricForce = curForce;
     if (isFloorEnabled()) floorForce(dt, &curForce);
     fricForce = curForce - fricForce;
@@ -1192,6 +1204,36 @@ struct Sim {
     std::vector<vx3h_token> stopTok, fitTok;
     float lastDt = 0;
     VoxcadStop voxcad;   // VoxCAD stop-condition family and equilibrium mode
+    // VX3 cilia and force field
+    bool extras = false, ciliaOn = false;
+    std::vector<double> matCilia, matCiliaOn;   // per material (voxelMats order)
+    std::vector<vx3h_token> ffTok[3];
+
+    // gpu_update_cilia_force (VX3_VoxelyzeKernel.cu:846-859) with localSignal 0, and the
+    // force field of VX3_Voxel::timeStep (VX3_Voxel.cu:178-184), from the state at the
+    // start of the voxel update
+    void updateExtras() {
+        if (!extras) return;
+        const int nv = (int)vx.voxelsList.size();
+        for (Voxel* v : vx.voxelsList) {
+            v->ciliaForce = Vd(0, 0, 0);
+            const int mi = matOf(v);   // voxel materials are in description order
+            if (ciliaOn && v->isSurface() && matCilia[mi] != 0 && !(matCiliaOn[mi] > currentTime)) {
+                const double localSignal = 0.0;
+                v->ciliaForce = v->orient.RotateVec3D(v->baseCilia + v->shiftCilia * localSignal) * matCilia[mi];
+            }
+            double f[3];
+            for (int k = 0; k < 3; k++)
+                f[k] = ffTok[k].empty() ? 0.0
+                                        : mathtree_eval(ffTok[k].data(), (int)ffTok[k].size(), v->pos.x, v->pos.y, v->pos.z, 0,
+                                                        currentTime, recentAngle, 0, 0, nv);
+            v->fieldForce = Vd(f[0], f[1], f[2]);
+        }
+    }
+    int matOf(const Voxel* v) const {
+        for (size_t i = 0; i < vx.voxelMats.size(); i++) if (vx.voxelMats[i] == v->mat) return (int)i;
+        return 0;
+    }
 
     // VX3_VoxelyzeKernel.cu:477-493
     void updateCurrentCenterOfMass() {
@@ -1254,6 +1296,7 @@ struct Sim {
             dt = DtFrac * OptimalDt;
         }
         lastDt = dt;
+        updateExtras();
         if (!vx.doTimeStep(dt)) return false;
         int CycleStep = int(TempPeriod / dt);
         if (CycleStep > 0 && CurStepCount % CycleStep == 0) {
@@ -1367,6 +1410,19 @@ static Sim* build(const vx3h_sim_desc* d) {
     s->TempAmplitude = d->temp_amplitude;
     s->TempPeriod = d->temp_period;
     s->maxSteps = d->max_steps;
+    // VX3 cilia and force field
+    for (int k = 0; k < 3; k++)
+        if (d->n_force_field_tokens[k] > 0) s->ffTok[k].assign(d->force_field_tokens[k], d->force_field_tokens[k] + d->n_force_field_tokens[k]);
+    if (d->enable_cilia)
+        for (int i = 0; i < d->n_materials; i++) s->ciliaOn |= d->materials[i].cilia != 0;
+    s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty();
+    for (int i = 0; i < d->n_materials; i++) { s->matCilia.push_back(d->materials[i].cilia); s->matCiliaOn.push_back(d->materials[i].cilia_on_after); }
+    for (size_t i = 0; i < vx.voxelsList.size(); i++) {
+        Voxel* v = vx.voxelsList[i];
+        v->extras = s->extras;
+        if (d->base_cilia_force) v->baseCilia = Vd(d->base_cilia_force[3 * i], d->base_cilia_force[3 * i + 1], d->base_cilia_force[3 * i + 2]);
+        if (d->shift_cilia_force) v->shiftCilia = Vd(d->shift_cilia_force[3 * i], d->shift_cilia_force[3 * i + 1], d->shift_cilia_force[3 * i + 2]);
+    }
     s->voxcad.type = d->stop_condition_type;
     s->voxcad.value = d->stop_condition_value;
     s->voxcad.eq = d->equilibrium_mode != 0;

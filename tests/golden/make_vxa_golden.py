@@ -46,6 +46,11 @@ def reference_vectors(text):
     rgba, mid = np.zeros(4 * cap, np.int32), np.zeros(cap, np.int32)
     nm = lib.ref_vxa_colors(C.c_void_p(r.h), rgba.ctypes.data_as(C.POINTER(C.c_int)), mid.ctypes.data_as(C.POINTER(C.c_int)),
                             cap)
+    cb, cs = np.zeros(3 * nv), np.zeros(3 * nv)
+    mcil, mon = np.zeros(cap), np.zeros(cap)
+    dp = C.POINTER(C.c_double)
+    lib.ref_vxa_cilia(C.c_void_p(r.h), cb.ctypes.data_as(dp), cs.ctypes.data_as(dp), mcil.ctypes.data_as(dp),
+                      mon.ctypes.data_as(dp))
     st_type, st_val, eq = C.c_int(), C.c_double(), C.c_int()
     lib.ref_vxa_stop(C.c_void_p(r.h), C.byref(st_type), C.byref(st_val), C.byref(eq))
     vmc = r.voxel_material_consts()
@@ -57,7 +62,8 @@ def reference_vectors(text):
                 ext_force=fo, ext_moment=mo, env_grav=np.float32(g.value), env_floor=np.int32(fl.value),
                 env_dtfrac=np.float64(dtf.value), env_temp_amp=np.float64(ta.value), env_temp_period=np.float64(tp.value),
                 mat_color=rgba[: 4 * nm].reshape(-1, 4), mat_id=mid[:nm],
-                stop=np.array([st_type.value, st_val.value, eq.value], np.float64), vmat_consts=vmc, lmat_index=lidx, lmat_consts=lmc, sample=s, pos=st["pos"][s], orient=st["orient"][s],
+                stop=np.array([st_type.value, st_val.value, eq.value], np.float64),
+                cilia_base=cb, cilia_shift=cs, mat_cilia=np.stack([mcil[:nm], mon[:nm]], axis=1), vmat_consts=vmc, lmat_index=lidx, lmat_consts=lmc, sample=s, pos=st["pos"][s], orient=st["orient"][s],
                 pos_sum=np.float64(st["pos"].sum()), orient_sum=np.float64(st["orient"].sum()))
 
 

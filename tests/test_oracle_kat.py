@@ -11,7 +11,7 @@ reference's own unit tests and to closed forms:
 import numpy as np
 import pytest
 
-from gpuvoxels_amd.model import Ext, Material, Model
+from gpuvoxels_amd.model import MT, Ext, Material, Model
 from oracle_lib import OSim
 
 TX, TY, TZ = 1, 2, 4
@@ -423,3 +423,85 @@ def test_min_kinetic_energy_stop_brute_force():
     first = next(n for n in range(1000, len(ke) + 1)
                  if all(ke[n - 1 - a] / nv <= 0.001 * value for a in range(0, 500, 50)))
     assert len(ke) == first
+
+
+# ---- VX3 cilia and force field (VX3_VoxelyzeKernel.cu:846-859, VX3_Voxel.cu:178-184,376)
+
+def _free_voxel(n=1, **kw):
+    m = Model(voxel_size=0.01)
+    m.materials = [Material(elastic_mod=1e6, density=1e3, zeta_internal=1.0, zeta_global=0.0, **kw)]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    m.ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+    m.mat = np.zeros(n ** 3, np.int32)
+    return m
+
+
+def test_cilia_push_a_free_voxel_after_switch_on():
+    # p = n * F * Cilia * dt once t >= TurnOnCiliaAfterThisManySeconds, nothing before
+    m = _free_voxel(cilia=2.5, cilia_on_after=0.0)
+    m.enable_cilia = 1
+    m.base_cilia = np.array([[1e-3, 0.0, 0.0]])
+    m.shift_cilia = np.array([[5e-3, 5e-3, 5e-3]])      # scaled by the signal, which stays 0
+    o = OSim(m)
+    o.step(100)
+    dt = float(np.float32(o.status().dt))
+    p = o.state()["lin_mom"][0]
+    assert p[0] == pytest.approx(100 * 1e-3 * 2.5 * dt, rel=1e-9) and p[1] == 0 and p[2] == 0
+    m.materials[0].cilia_on_after = 50.5 * dt
+    o = OSim(m)
+    o.step(100)
+    assert o.state()["lin_mom"][0][0] == pytest.approx(49 * 1e-3 * 2.5 * dt, rel=1e-9)
+    m.enable_cilia = 0                                  # EnableCilia off: no force
+    o = OSim(m)
+    o.step(10)
+    assert (o.state()["lin_mom"][0] == 0).all()
+
+
+def test_cilia_rotate_with_the_voxel():
+    # a voxel held at 90 degrees about z pushes along +y for a base force along +x
+    m = _free_voxel(cilia=1.0)
+    m.enable_cilia = 1
+    m.base_cilia = np.array([[1e-3, 0.0, 0.0]])
+    m.externals = [Ext(0, dof_fixed=RX | RY | RZ, rotate=(0.0, 0.0, np.pi / 2))]
+    o = OSim(m)
+    o.step(102)    # the first step still starts from the identity orientation
+    dt = float(np.float32(o.status().dt))
+    p = o.state()["lin_mom"][0]
+    assert p[1] == pytest.approx(101 * 1e-3 * dt, rel=1e-9)
+    assert p[0] == pytest.approx(1e-3 * dt, rel=1e-9)
+
+
+def test_cilia_act_on_surface_voxels_only():
+    # the centre voxel of a 3x3x3 cube is interior: its cilia do nothing
+    m = _free_voxel(n=3, cilia=1.0)
+    m.enable_cilia = 1
+    base = np.zeros((27, 3))
+    base[13] = (1e-3, 0, 0)
+    m.base_cilia = base
+    o = OSim(m)
+    o.step(200)
+    assert np.abs(o.state()["lin_mom"]).max() == 0
+    base[13] = 0
+    base[0] = (1e-3, 0, 0)
+    m.base_cilia = base
+    o = OSim(m)
+    o.step(200)
+    dt = float(np.float32(o.status().dt))
+    assert o.state()["lin_mom"][:, 0].sum() == pytest.approx(200 * 1e-3 * dt, rel=1e-3)
+
+
+def test_force_field_constant_and_time_dependent():
+    # x: constant c -> p = n*c*dt; y: the variable t -> p = dt * sum of the step start times
+    m = _free_voxel()
+    m.force_field = [[(MT["mtCONST"], 2e-3), (MT["mtEND"], 0.0)], [(MT["mtVAR"], 4.0), (MT["mtEND"], 0.0)], []]
+    o = OSim(m)
+    dt = None
+    ts = []
+    for _ in range(50):
+        ts.append(o.status().current_time)
+        o.step(1)
+        dt = float(np.float32(o.status().dt))
+    p = o.state()["lin_mom"][0]
+    assert p[0] == pytest.approx(50 * 2e-3 * dt, rel=1e-9)
+    assert p[1] == pytest.approx(dt * sum(ts), rel=1e-9) and p[1] > 0
+    assert p[2] == 0

@@ -39,8 +39,20 @@ def loader_vectors(text, steps=200):
             tr[3 * e.voxel + k], ro[3 * e.voxel + k] = e.translate[k], e.rotate[k]
             fo[3 * e.voxel + k], mo[3 * e.voxel + k] = e.force[k], e.moment[k]
     color = np.array([list(d.materials[i].color) for i in range(d.n_materials)], np.int32).reshape(-1, 4)
+    mcil = np.array([[d.materials[i].cilia, d.materials[i].cilia_on_after] for i in range(d.n_materials)]).reshape(-1, 2)
+    cb = np.ctypeslib.as_array(d.base_cilia_force, (3 * nv,)).copy() if d.base_cilia_force else np.zeros(3 * nv)
+    cs = np.ctypeslib.as_array(d.shift_cilia_force, (3 * nv,)).copy() if d.shift_cilia_force else np.zeros(3 * nv)
     mid = np.array([d.materials[i].matid for i in range(d.n_materials)], np.int32)
+    # the trajectory pins the loader's physics fields against the reference's CPU engine,
+    # which has no VX3 cilia or force field: those run separately (test_gpu_extras.py)
+    saved = d.enable_cilia, list(d.n_force_field_tokens)
+    d.enable_cilia = 0
+    for k in range(3):
+        d.n_force_field_tokens[k] = 0
     o = OSim.from_desc(d)
+    d.enable_cilia = saved[0]
+    for k in range(3):
+        d.n_force_field_tokens[k] = saved[1][k]
     vmc = o.voxel_material_consts()
     lidx, lmc = o.link_material_consts()
     o.step(steps)
@@ -48,7 +60,8 @@ def loader_vectors(text, steps=200):
     return v, dict(nv=nv, nl=o.nl, ijk=ijk, mat=mat, phase=ph, ext_dof=dof, ext_trans=tr, ext_rot=ro, ext_force=fo,
                    ext_moment=mo, env_grav=np.float32(d.gravity_mult), env_floor=int(d.floor_enabled),
                    env_dtfrac=d.dt_frac, env_temp_amp=d.temp_amplitude, env_temp_period=d.temp_period, vmat_consts=vmc,
-                   lmat_index=lidx, lmat_consts=lmc, mat_color=color, mat_id=mid,
+                   lmat_index=lidx, lmat_consts=lmc, mat_color=color, mat_id=mid, cilia_base=cb, cilia_shift=cs,
+                   mat_cilia=mcil,
                    stop=np.array([v.info.stop_condition_type, v.info.stop_condition_value, v.info.equilibrium_mode],
                                  np.float64), state=st)
 
@@ -56,7 +69,7 @@ def loader_vectors(text, steps=200):
 def check_against(ref, got):
     assert int(ref["nv"]) == got["nv"] and int(ref["nl"]) == got["nl"]
     for k in ("ijk", "mat", "phase", "ext_dof", "ext_trans", "ext_rot", "ext_force", "ext_moment", "vmat_consts",
-              "lmat_index", "lmat_consts", "mat_color", "mat_id", "stop"):
+              "lmat_index", "lmat_consts", "mat_color", "mat_id", "stop", "cilia_base", "cilia_shift", "mat_cilia"):
         assert np.array_equal(np.asarray(ref[k]), np.asarray(got[k])), k
     assert np.float32(ref["env_grav"]) == got["env_grav"]
     assert int(ref["env_floor"]) == got["env_floor"]

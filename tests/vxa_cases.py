@@ -90,6 +90,28 @@ def _display_colors():
                                         "</EquilibriumMode></Simulator>")
 
 
+def _cilia_and_force_field():
+    # VX3 cilia: per-voxel base and shift forces on the structure, Cilia scale and
+    # switch-on time per material; a short ShiftCiliaForce layer pads with zeros; and a
+    # force field per axis
+    m = workloads.robots(1, seed=17, grid=6, steps=0)[0]
+    m.name = "cilia"
+    rng = np.random.default_rng(5)
+    m.enable_cilia = 1
+    m.base_cilia = rng.normal(0, 1e-4, (m.n_voxels, 3))
+    m.shift_cilia = rng.normal(0, 1e-4, (m.n_voxels, 3))
+    for k, mt in enumerate(m.materials):
+        mt.cilia = [0.0, 2.5, 8.0, 1.0][k]
+        mt.cilia_on_after = [0.0, 0.0, 0.01, 0.0][k]
+    m.force_field = [[(MT["mtCONST"], 1e-4), (MT["mtEND"], 0.0)], [],
+                     [(MT["mtVAR"], 4.0), (MT["mtCONST"], -1e-3), (MT["mtMUL"], 0.0), (MT["mtEND"], 0.0)]]
+    text = vxa.model_to_vxa(m)
+    shift = re.search(r"<ShiftCiliaForce>.*?</ShiftCiliaForce>", text, re.S).group(0)
+    layers = re.findall(r"<Layer><!\[CDATA\[(.*?)\]\]></Layer>", shift)
+    short = shift.replace(layers[-1], ",".join(layers[-1].split(",")[:6]))   # whole cells: a partial one is out of range there
+    return text.replace(shift, short)
+
+
 CASES = {
     "robot": lambda: vxa.model_to_vxa(workloads.robots(2, seed=3, steps=0)[1]),
     "cube6": lambda: vxa.model_to_vxa(workloads.cube(6, steps=0)),
@@ -100,6 +122,7 @@ CASES = {
     "legacy_thermal": _legacy_thermal_and_defaults,
     "robot_stop": _robot_with_stop,
     "colors": _display_colors,
+    "cilia": _cilia_and_force_field,
 }
 
 
