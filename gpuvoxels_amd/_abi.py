@@ -101,6 +101,7 @@ class SimDesc(C.Structure):
         ("shift_cilia_force", C.POINTER(C.c_double)),
         ("n_force_field_tokens", C.c_int * 3),
         ("force_field_tokens", C.POINTER(Token) * 3),
+        ("enable_detach", C.c_int),
     ]
 
 

@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     const VoxMatDev& mn = b.vmatTab[b.vmat[vn]];
     const VoxMatDev& mp = b.vmatTab[b.vmat[vp]];
     if (mn.fixed && mp.fixed) return;   // VX3_VoxelyzeKernel.cu:572
+    if (b.lflags[l] & LF_GONE) return;  // detached (VX3_VoxelyzeKernel.cu:574)
     const real nom = (real)sc.voxSize;
     // CVX_Link::updateRestLength: 0.5*(baseSize_neg + baseSize_pos)
     const real restLength = (real)0.5 * (nom * (real)b.tfac[vn] + nom * (real)b.tfac[vp]);
@@ -158,6 +159,13 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
         b.es[dp * V + vp] = 2.0f * st.strain * r / (1.0f + r);
     }
     if (st.strain > 100) atomicOr(&b.sims[s].flags, SC_DIVERGED_PENDING);   // Voxelyze.cpp:265
+    // gpu_update_detach (VX3_VoxelyzeKernel.cu:946-967): a failed link leaves both voxels
+    // before their update; their surface state follows at the next step's start
+    if ((sc.flags & SC_DETACH) && mat.epsilonFail != -1.0f && st.maxStrain > mat.epsilonFail) {
+        b.lflags[l] = st.flags | LF_GONE;
+        atomicAnd(&b.vflags[vn], ~(1 << (VF_LINKMASK_SHIFT + dn)));
+        atomicAnd(&b.vflags[vp], ~(1 << (VF_LINKMASK_SHIFT + dp)));
+    }
 }
 
 // ---- collisions (CVoxelyze::updateCollisions / regenerateCollisions, Voxelyze.cpp:671-751)
@@ -544,20 +552,28 @@ __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, doubl
     return q[pc < VX3_DEV_MT_MAX ? pc : 0];
 }
 
-// Extra global forces of the simulations with VX3 cilia or a force field, for the
-// voxel pass that follows, from the state before this step's voxel update:
+// Start of a step for the simulations with VX3 detach, cilia or a force field.
+// Detach: a voxel's surface state follows its link mask as of the previous step's
+// end (regenerateSurfaceVoxels runs at the next step, VX3_VoxelyzeKernel.cu:281-285).
+// Extra global forces for the voxel pass, from the state before this step's update:
 // cilia as gpu_update_cilia_force (VX3_VoxelyzeKernel.cu:846-859): a surface voxel
 // whose material has Cilia != 0 and whose switch-on time has passed is pushed by
 // orient.RotateVec3D(base + localSignal*shift) * Cilia, localSignal being 0 (signals
 // are not simulated); the force field's three programs of the voxel position at the
 // step's start time (VX3_Voxel.cu:178-184; hit, closeness and close pairs are 0).
-__global__ void __launch_bounds__(256) k_extras(DevBatch<real> b) {
+__global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
     if (!simRunning(b.sims, s)) return;
     const SimCtl& sc = b.sims[s];
-    if (!(sc.flags & SC_EXTRAS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+    if (sc.flags & SC_DETACH) {   // CVX_Voxel::updateSurface: interior iff all six links remain
+        const int fl = b.vflags[v];
+        const int all = 0x3F << VF_LINKMASK_SHIFT;
+        const int nf = (fl & all) == all ? (fl | VF_INTERIOR) : (fl & ~VF_INTERIOR);
+        if (nf != fl) b.vflags[v] = nf;
+    }
+    if (!(sc.flags & SC_EXTRAS)) return;
     const size_t V = (size_t)b.Vcap;
     V3<real> c(0, 0, 0);
     if (sc.flags & SC_CILIA) {

@@ -73,6 +73,7 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
     };
     auto describe = [&](vx3h_vxa* v, const vx3h_vxa_info& info) {
         vx3h_sim_desc d = *vx3h_vxa_desc(v);
+        if (d.enable_detach && d.collisions_enabled) d.enable_detach = 0;   // reached only with allow_unsupported
         if (opt.voxcad_stop) {
             d.stop_condition_type = info.stop_condition_type;
             d.stop_condition_value = info.stop_condition_value;
@@ -91,7 +92,8 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             check(vx3h_vxa_get_info(v, &info, &raw), f);
             if (raw && *raw) printf("%s\n", raw);   // RawPrint (VX3_SimulationManager.cu:319-321)
             std::string missing;
-            if (info.enable_attach || info.enable_detach) missing += " attach/detach";
+            if (info.enable_attach) missing += " attach";
+            if (info.enable_detach && vx3h_vxa_desc(v)->collisions_enabled) missing += " detach-with-collisions";
             if (info.enable_signals) missing += " signals";
             if (info.secondary_experiment) missing += " secondary-experiment";
             if (info.max_dist_close_pairs > 0) missing += " close-pair counting";

@@ -29,13 +29,15 @@ enum {
     VF_LINKMASK_SHIFT = 8,           // bits 8..13: link present in direction d
 };
 // link state bits
-enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2 };
+enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2,
+       LF_GONE = 4 };   // detached: no longer updated, no longer in its voxels' link masks
 // per-sim control bits
 enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32, SC_RESIDENT = 64,
        SC_REBUILD = 128,        // collision neighbour lists are rebuilt this step
        SC_COL_OVERFLOW = 256,   // a collision table overflowed: the simulation fails
        SC_EXTRAS = 512,         // cilia or a force field add to the voxel force (xf arrays)
-       SC_CILIA = 1024 };
+       SC_CILIA = 1024,
+       SC_DETACH = 2048 };      // failed links detach
 #define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)

@@ -168,6 +168,7 @@ struct vx3h_batch {
     DevPool nbPool, hashPool, nearPool;
     bool anyColl = false;
     bool anyExtras = false;             // some simulation has cilia or a force field
+    bool anyDetach = false;             // some simulation detaches failed links
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
     // resident path: kernel shapes and, per shape, the sims it runs
@@ -326,6 +327,7 @@ vx3h_status launchSteps(vx3h_batch* b, int n) {
     StepOpts o;
     o.contacts = b->anyColl;
     o.extras = b->anyExtras;
+    o.prestep = b->anyExtras || b->anyDetach;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
@@ -504,6 +506,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (!b || !desc || !slot) FAILV(VX3H_EINVAL, "null argument");
     if (desc->stop_condition_type < VX3H_SC_NONE || desc->stop_condition_type > VX3H_SC_MIN_MAXMOVE)
         FAILV(VX3H_EINVAL, "unknown stop condition type");
+    if (desc->enable_detach && desc->collisions_enabled)
+        FAILV(VX3H_EUNSUPPORTED, "detach with collisions is not supported (the contact exclusion lists are built once)");
     HIPCHK(hipSetDevice(b->device));
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
@@ -642,7 +646,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     // VoxCAD stop conditions and equilibrium mode are evaluated by the stream path's control pass
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
-                       nPhases <= RES_MAX_PHASES && !voxcad && !H.extras();
+                       nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -654,6 +658,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
+    if (desc->enable_detach && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "detach runs on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -683,6 +688,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     for (int k = 0; k < 3; k++) { c.ffOff[k] = ffOff[k]; c.ffLen[k] = (int)H.ffTok[k].size(); }
     if (H.extras()) { c.flags |= SC_EXTRAS; b->anyExtras = true; }
     if (!H.cilia.empty()) c.flags |= SC_CILIA;
+    if (desc->enable_detach) { c.flags |= SC_DETACH; b->anyDetach = true; }
     if (voxcad) {
         const std::vector<double> unset(3 * VX3_HIST, -1.0);   // ClearHistories
         HIPCHK(hipMemcpy(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -847,6 +853,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     StepOpts o;
     o.contacts = b->anyColl;
     o.extras = b->anyExtras;
+    o.prestep = b->anyExtras || b->anyDetach;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];
@@ -1188,14 +1195,17 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
     const SlotHost& sh = b->slots[slot];
     const int NV = sh.nv, NL = sh.nl;
     std::vector<double> pos(3 * (size_t)NV), ori(4 * (size_t)NV), tf(NV), strain(NL), maxStrain(NL);
+    std::vector<int> vflags(NV), lflags(NL);
     vx3h_state_view view{};
     view.pos = pos.data();
     view.orient = ori.data();
     view.temp = tf.data();
+    view.vox_flags = vflags.data();
     view.link_strain = strain.data();
     view.link_max_strain = maxStrain.data();
     vx3h_status st = vx3h_sim_read_state(b, slot, &view);
     if (st) return st;
+    if (NL) HIPCHK(hipMemcpy(lflags.data(), b->db.lflags + sh.l0, NL * 4, hipMemcpyDeviceToHost));
     int n = 0;
     if ((st = vx3h_batch_poll(b, nullptr, 0, &n))) return st;
     SimCtl c;
@@ -1205,7 +1215,7 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
     if (record_voxel) {
         appendf(t, "<<<Step%d Time:%f>>>", step_label, c.t);
         for (int v = 0; v < NV; v++) {
-            if (!sh.surface[v]) continue;
+            if (vflags[v] & VF_INTERIOR) continue;   // surface voxels as of this step's start
             const double* p = &pos[3 * (size_t)v];
             const double* q = &ori[4 * (size_t)v];
             appendf(t, "%.1f,%.1f,%.1f,", p[0] * vs, p[1] * vs, p[2] * vs);
@@ -1219,7 +1229,7 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
                 for (int i = 0; i < 3; i++) {
                     const int l = sh.vlinks[6 * (size_t)v + 2 * i + (posLink ? 0 : 1)];
                     double sv;
-                    if (l >= 0 && !(sh.lfail[l] != -1.0f && (float)maxStrain[l] > sh.lfail[l])) {
+                    if (l >= 0 && !(lflags[l] & LF_GONE) && !(sh.lfail[l] != -1.0f && (float)maxStrain[l] > sh.lfail[l])) {
                         const float s = (float)strain[l], r = sh.lratio[l];
                         const float ax = posLink ? 2.0f * s * r / (1.0f + r) : 2.0f * s / (1.0f + r);   // axialStrain
                         sv = (1 + ax) * (posLink ? 1 : -1);
@@ -1238,6 +1248,7 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
     if (record_link) {
         appendf(t, "|[[[%d]]]", step_label);
         for (int l = 0; l < NL; l++) {
+            if (lflags[l] & LF_GONE) continue;   // detached links are not drawn
             const double* a = &pos[3 * (size_t)sh.lpos[l]];
             const double* z = &pos[3 * (size_t)sh.lneg[l]];
             appendf(t, "%.4f,%.4f,%.4f,", a[0], a[1], a[2]);

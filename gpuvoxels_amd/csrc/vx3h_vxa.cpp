@@ -902,6 +902,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     d.max_steps = 1000000;   // the driver loop's cap (VX3_SimulationManager.cu:62)
     d.save_position_of_all_voxels = ptGet<int>(doc, S + "SavePositionOfAllVoxels", 0);
     d.enable_cilia = in.enable_cilia;
+    d.enable_detach = in.enable_detach;
     d.base_cilia_force = o.baseCilia.empty() ? nullptr : o.baseCilia.data();
     d.shift_cilia_force = o.shiftCilia.empty() ? nullptr : o.shiftCilia.data();
     for (int k = 0; k < 3; k++) {

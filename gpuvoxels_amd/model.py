@@ -154,6 +154,7 @@ class Model:
     base_cilia: Optional[np.ndarray] = None   # (V, 3) N, voxel frame
     shift_cilia: Optional[np.ndarray] = None  # (V, 3)
     force_field: list = field(default_factory=lambda: [[], [], []])
+    enable_detach: int = 0                    # failed links leave their voxels (VX3 detach)
     name: str = "sim"
 
     @property
@@ -231,4 +232,5 @@ class Model:
                 keep.append(arr)
                 d.n_force_field_tokens[k] = len(toks)
                 d.force_field_tokens[k] = C.cast(arr, C.POINTER(_abi.Token))
+        d.enable_detach = self.enable_detach
         return d, keep

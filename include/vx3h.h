@@ -156,6 +156,11 @@ typedef struct vx3h_sim_desc {
      * 0 tokens = no force on that axis. */
     int    n_force_field_tokens[3];
     const vx3h_token* force_field_tokens[3];
+    /* VX3 detach (Simulator.AttachDetach.EnableDetach; gpu_update_detach,
+     * VX3_VoxelyzeKernel.cu:946-967): a link whose maximum strain passes its material's
+     * failure strain leaves both voxels after the link update of that step, before the
+     * voxel update.  Not with collisions. */
+    int    enable_detach;
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */

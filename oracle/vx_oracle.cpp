@@ -493,6 +493,7 @@ struct Collision;
 
 struct Link {
     Voxel *pVNeg = nullptr, *pVPos = nullptr;
+    bool gone = false;   // VX3 detach: no longer updated, no longer in its voxels' links
     Vd forceNeg, forcePos, momentNeg, momentPos;
     float strain = 0, maxStrain = 0, strainOffset = 0;
     int boolStates = 0;
@@ -958,6 +959,7 @@ struct Engine {
     double voxSize = 0.001;
     float currentTime = 0, ambientTemp = 0, grav = 0;
     bool floor = false, collisions = false;
+    bool detach = false, surfaceChanged = false;   // VX3 detach of failed links
     float boundingRadius = 0.75f, watchDistance = 1.0f;
     std::vector<MaterialVoxel*> voxelMats;
     std::list<MaterialLink*> linkMats;
@@ -1173,10 +1175,23 @@ struct Engine {
         bool Diverged = false;
         for (Link* l : linksList) {
             if (l->pVPos->mat->fixed && l->pVNeg->mat->fixed) continue;
+            if (l->gone) continue;
             l->updateForces();
             if (l->axialStrain() > 100) Diverged = true;
         }
         if (Diverged) return false;
+        // gpu_update_detach (VX3_VoxelyzeKernel.cu:946-967); the voxels' surface state is
+        // refreshed at the next step (regenerateSurfaceVoxels, :281-285)
+        if (detach)
+            for (Link* l : linksList) {
+                if (l->gone || !l->isFailed()) continue;
+                l->gone = true;
+                for (int i = 0; i < 6; i++) {
+                    if (l->pVNeg->links[i] == l) l->pVNeg->links[i] = nullptr;
+                    if (l->pVPos->links[i] == l) l->pVPos->links[i] = nullptr;
+                }
+                surfaceChanged = true;
+            }
         if (collisions) updateCollisions();
         for (Voxel* v : voxelsList) {
             if (v->mat->fixed) continue;
@@ -1296,6 +1311,10 @@ struct Sim {
             dt = DtFrac * OptimalDt;
         }
         lastDt = dt;
+        if (vx.surfaceChanged) {
+            vx.surfaceChanged = false;
+            for (Voxel* v : vx.voxelsList) v->updateSurface();
+        }
         updateExtras();
         if (!vx.doTimeStep(dt)) return false;
         int CycleStep = int(TempPeriod / dt);
@@ -1416,6 +1435,7 @@ static Sim* build(const vx3h_sim_desc* d) {
     if (d->enable_cilia)
         for (int i = 0; i < d->n_materials; i++) s->ciliaOn |= d->materials[i].cilia != 0;
     s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty();
+    vx.detach = d->enable_detach != 0;
     for (int i = 0; i < d->n_materials; i++) { s->matCilia.push_back(d->materials[i].cilia); s->matCiliaOn.push_back(d->materials[i].cilia_on_after); }
     for (size_t i = 0; i < vx.voxelsList.size(); i++) {
         Voxel* v = vx.voxelsList[i];

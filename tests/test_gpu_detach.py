@@ -1,0 +1,70 @@
+"""VX3 detach on the GPU engine against the oracle (gpu_update_detach,
+VX3_VoxelyzeKernel.cu:946-967): failed links leave their voxels before the voxel
+update of the step they fail in, the voxels' surface state follows at the next step.
+FP64 positions within 1e-9 m, voxel flags and link strains equal."""
+import numpy as np
+import pytest
+
+from gpuvoxels_amd import _abi, engine, workloads
+from oracle_lib import OSim
+from test_gpu_extras import ciliated
+from test_oracle_kat import _shrinking_core
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(m, steps):
+    b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
+    s = b.admit(m)
+    b.step(steps)
+    o = OSim(m)
+    o.step(steps)
+    g = b.read_state(s, nl=o.nl)
+    return b, s, g, o
+
+
+def test_shrinking_core_detaches_like_the_oracle():
+    b, s, g, o = run_pair(_shrinking_core(1), 2000)
+    r, lr = o.state(), o.link_state()
+    assert np.max(np.abs(g["pos"] - r["pos"])) < 1e-9
+    assert np.max(np.abs(g["strain"] - lr["strain"])) < 1e-6
+    assert np.array_equal(g["flags"] & 2, r["flags"] & 2)        # interior bit, centre voxel now surface
+    assert not (g["flags"][13] & 2)
+    # the history record leaves the six detached links out and draws the centre voxel
+    rec = b.history_record(s, 1999, 1, 1)
+    assert rec.split("|[[[1999]]]")[1].count(";") == o.nl - 6
+    assert rec.split("<<<>>>")[0].count(";") == 27
+    b.close()
+
+
+def test_ciliated_robots_with_fragile_links():
+    # cilia follow the surface, which grows as links fail
+    models = []
+    for i in range(2):   # dense bodies (p = 0.85), so that failures turn interior voxels into surface
+        m = workloads.robot(np.random.default_rng(21 + i), grid=7, p=0.85, steps=0)
+        for mt in m.materials:
+            mt.mat_model, mt.fail_stress = _abi.MDL_LINEAR_FAIL, 500.0
+        m.enable_detach = 1
+        models.append(ciliated(m, seed=i, field=False))
+    b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
+    slots = [b.admit(m) for m in models]
+    before = {s: int((b.read_state(s)["flags"] & 2 > 0).sum()) for s in slots}
+    b.step(1500)
+    for m, s in zip(models, slots):
+        o = OSim(m)
+        o.step(1500)
+        g, r = b.read_state(s, nl=o.nl), o.state()
+        assert np.max(np.abs(g["pos"] - r["pos"])) < 1e-9
+        assert np.array_equal(g["flags"] & 2, r["flags"] & 2)
+        assert int((g["flags"] & 2 > 0).sum()) < before[s]      # failures opened the bodies up
+    b.close()
+
+
+def test_detach_with_collisions_refused():
+    m = workloads.drop(3, steps=0)
+    m.enable_detach = 1
+    b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
+    with pytest.raises(engine.VX3Error) as e:
+        b.admit(m)
+    assert e.value.status == _abi.EUNSUPPORTED
+    b.close()

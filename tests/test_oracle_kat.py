@@ -505,3 +505,50 @@ def test_force_field_constant_and_time_dependent():
     assert p[0] == pytest.approx(50 * 2e-3 * dt, rel=1e-9)
     assert p[1] == pytest.approx(dt * sum(ts), rel=1e-9) and p[1] > 0
     assert p[2] == 0
+
+
+# ---- VX3 detach (gpu_update_detach, VX3_VoxelyzeKernel.cu:946-967)
+
+def _shrinking_core(detach):
+    """3x3x3 cube whose centre voxel contracts with the actuation until its six links
+    pass their failure strain (1e4 Pa / 1e6 Pa)."""
+    m = Model(voxel_size=0.01)
+    kw = dict(elastic_mod=1e6, density=1e3, u_static=1.0, u_dynamic=0.5, zeta_internal=1.0, zeta_global=0.01,
+              zeta_collision=0.8, mat_model=1, fail_stress=1e4)
+    m.materials = [Material(cte=0.0, **kw), Material(cte=0.02, **kw)]
+    z, y, x = np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij")
+    m.ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+    m.mat = np.zeros(27, np.int32)
+    m.mat[13] = 1
+    m.gravity_mult = np.float32(1.0)
+    m.floor_enabled = 1
+    m.temp_enabled = m.vary_temp_enabled = 1
+    m.temp_amplitude, m.temp_period = 10.0, 0.1
+    m.enable_detach = detach
+    return m
+
+
+def test_detach_frees_failed_links_and_surface_follows_a_step_later():
+    INTERIOR = 2       # the engine's "SURFACE" bit: set while all six links are present
+    o = OSim(_shrinking_core(1))
+    fail = 1e4 / 1e6
+    k = None
+    for n in range(1, 3000):
+        o.step(1)
+        ms = o.link_state()["max_strain"]
+        if k is None and (ms > fail).any():
+            k = n
+            frozen = o.link_state()["strain"].copy()
+            assert o.state()["flags"][13] & INTERIOR      # not yet: the surface follows at the next step
+        elif k is not None:
+            assert not (o.state()["flags"][13] & INTERIOR)
+            failed = frozen.copy()
+            gone = o.link_state()["max_strain"] > fail
+            assert np.array_equal(o.link_state()["strain"][gone], failed[gone])   # detached links no longer update
+            if n > k + 20:
+                break
+    assert k is not None and gone.sum() == 6                # the centre's six links
+    # without detach the failed links stay in place
+    o2 = OSim(_shrinking_core(0))
+    o2.step(k + 20)
+    assert o2.state()["flags"][13] & INTERIOR
