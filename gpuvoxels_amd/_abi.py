@@ -44,6 +44,7 @@ class MaterialDesc(C.Structure):
         ("color", C.c_int * 4),
         ("cilia", C.c_double),
         ("cilia_on_after", C.c_double),
+        ("remove_after", C.c_double),
     ]
 
 
@@ -102,6 +103,8 @@ class SimDesc(C.Structure):
         ("n_force_field_tokens", C.c_int * 3),
         ("force_field_tokens", C.POINTER(Token) * 3),
         ("enable_detach", C.c_int),
+        ("secondary_experiment", C.c_int),
+        ("reinit_after", C.c_double),
     ]
 
 
@@ -146,6 +149,7 @@ class StateView(C.Structure):
         ("link_strain", C.POINTER(C.c_double)),
         ("link_max_strain", C.POINTER(C.c_double)),
         ("link_small_angle", C.POINTER(C.c_int)),
+        ("init_pos", C.POINTER(C.c_double)),
     ]
 
 

@@ -23,6 +23,8 @@ template <typename T> struct DevBatch {
     double* cil;              // 8*Vcap cilia: base force xyz, shift force xyz (voxel frame), Cilia scale,
                               // switch-on time; cil[k*Vcap + v]
     T* xf;                    // 6*Vcap this step's global cilia force xyz and force-field force xyz
+    double* ip;               // 3*Vcap initial positions saved by a secondary experiment's reset
+    double* rt;               // Vcap removal time of the voxel's material (secondary experiment), 0 = never
     float* es;                // 6*Vcap axial strain of the link end in slot d: es[d*Vcap + v]
     // link state [Lcap]
     int *lneg, *lpos;

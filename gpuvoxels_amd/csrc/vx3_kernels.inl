@@ -430,6 +430,7 @@ template <bool EXTRAS> __device__ __forceinline__ void voxelPass(const DevBatch<
     const SimCtl& sc = b.sims[s];
     if (((sc.flags & SC_EXTRAS) != 0) != EXTRAS) return;
     if (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW)) return;   // doTimeStep returns before the voxel loop
+    if ((sc.flags & SC_SECONDARY) && (b.vflags[v] & VF_REMOVED)) return;   // VX3_VoxelyzeKernel.cu:586,632
     const VoxMatDev& m = b.vmatTab[b.vmat[v]];
     const float tf = b.tfac[v];
     if (!m.fixed) {
@@ -567,7 +568,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
     const int s = b.vsim[v];
     if (!simRunning(b.sims, s)) return;
     const SimCtl& sc = b.sims[s];
-    if (sc.flags & SC_DETACH) {   // CVX_Voxel::updateSurface: interior iff all six links remain
+    if (sc.flags & (SC_DETACH | SC_SECONDARY)) {   // CVX_Voxel::updateSurface: interior iff all six links remain
         const int fl = b.vflags[v];
         const int all = 0x3F << VF_LINKMASK_SHIFT;
         const int nf = (fl & all) == all ? (fl | VF_INTERIOR) : (fl & ~VF_INTERIOR);
@@ -578,7 +579,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
     V3<real> c(0, 0, 0);
     if (sc.flags & SC_CILIA) {
         const double coef = b.cil[6 * V + v], on = b.cil[7 * V + v];
-        if (!(b.vflags[v] & VF_INTERIOR) && coef != 0 && !(on > sc.t)) {
+        if (!(b.vflags[v] & (VF_INTERIOR | VF_REMOVED)) && coef != 0 && !(on > sc.t)) {
             const double sig = 0.0;
             const V3<real> f((real)(b.cil[0 * V + v] + sig * b.cil[3 * V + v]), (real)(b.cil[1 * V + v] + sig * b.cil[4 * V + v]),
                              (real)(b.cil[2 * V + v] + sig * b.cil[5 * V + v]));
@@ -732,6 +733,55 @@ __device__ bool voxcadStopMet(const DevBatch<real>& b, const SimCtl& sc, int s, 
     }
 }
 
+// The secondary experiment at the end of a step (VX3_VoxelyzeKernel.cu:336-350):
+// removeVoxels (:365-397) takes out the voxels of each material whose removal time is
+// positive and earlier than the step's start time, with their links (gone from both
+// voxels' masks; surfaces follow at the next step's start); then, once the start time
+// passes the reset time, saveInitialPosition (:155-160) stores the current positions.
+// Returns true when the initial center of mass is to be reset too.
+__device__ bool secondaryExperiment(const DevBatch<real>& b, SimCtl& sc) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    for (int v = sc.v0 + threadIdx.x; v < sc.v0 + sc.nv; v += blockDim.x) {
+        const double t = b.rt[v];   // the voxel material's removal time, 0 = never
+        if (t > 0 && t < sc.t && !(b.vflags[v] & VF_REMOVED)) {
+            atomicOr(&b.vflags[v], VF_REMOVED);
+            any = 1;
+        }
+    }
+    __syncthreads();
+    if (any) {
+        for (int l = sc.l0 + threadIdx.x; l < sc.l0 + sc.nl; l += blockDim.x) {
+            const int lf = b.lflags[l];
+            if (lf & LF_GONE) continue;
+            const int vn = b.lneg[l], vp = b.lpos[l];
+            if (!((b.vflags[vn] | b.vflags[vp]) & VF_REMOVED)) continue;
+            const int axis = b.lmeta[l] & 3;
+            b.lflags[l] = lf | LF_GONE;
+            atomicAnd(&b.vflags[vn], ~(1 << (VF_LINKMASK_SHIFT + 2 * axis)));
+            atomicAnd(&b.vflags[vp], ~(1 << (VF_LINKMASK_SHIFT + 2 * axis + 1)));
+        }
+        __syncthreads();
+        // the Poisson strain the next link pass uses follows the voxels' remaining links
+        // (CVX_Voxel::strain is evaluated lazily there), so the value the voxel pass
+        // stored for it is recomputed
+        if (sc.flags & SC_POISSON)
+            for (int v = sc.v0 + threadIdx.x; v < sc.v0 + sc.nv; v += blockDim.x)
+                if (!(b.vflags[v] & VF_REMOVED)) poissonStrain(b, v);
+        __syncthreads();
+    }
+    const bool reinit = !sc.reinitDone && sc.reinitAfter < sc.t;
+    if (reinit) {
+        const size_t V = (size_t)b.Vcap;
+        for (int v = sc.v0 + threadIdx.x; v < sc.v0 + sc.nv; v += blockDim.x) {
+            b.ip[0 * V + v] = b.px[v]; b.ip[1 * V + v] = b.py[v]; b.ip[2 * V + v] = b.pz[v];
+        }
+    }
+    __syncthreads();
+    return reinit;
+}
+
 // Per-simulation bookkeeping after the voxel pass (doTimeStep tail, VX3_VoxelyzeKernel.cu:241,279,314-334,352,
 // and the stop test the driver loop makes before the next step, VX3_SimulationManager.cu:62-64).
 __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
@@ -746,6 +796,8 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     double com[3];
     if (sample) blockCoM(b, sc, com);
     if (!diverged && !(sc.flags & SC_COL_OVERFLOW) && (sc.scType >= 4 || sc.eqMode)) voxcadStats(b, sc, s);
+    bool reinit = false;
+    if ((sc.flags & SC_SECONDARY) && !diverged && !(sc.flags & SC_COL_OVERFLOW)) reinit = secondaryExperiment(b, sc);
     if (threadIdx.x != 0) return;
     sc.steps = steps;
     sc.flags &= ~SC_REBUILD;
@@ -771,6 +823,10 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
             double l2 = sqrt((B[0] - C[0]) * (B[0] - C[0]) + (B[1] - C[1]) * (B[1] - C[1]) + (B[2] - C[2]) * (B[2] - C[2]));
             sc.recentAngle = acos(dot / (l1 * l2));
         }
+    }
+    if (reinit) {   // InitializeCenterOfMass: the last sampled center of mass
+        for (int k = 0; k < 3; k++) sc.com0[k] = sc.com[k];
+        sc.reinitDone = 1;
     }
     sc.t += (double)sc.dt;
     if (sc.maxSteps > 0 && steps >= sc.maxSteps) { sc.state = 1; return; }

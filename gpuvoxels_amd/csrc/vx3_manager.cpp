@@ -74,6 +74,7 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
     auto describe = [&](vx3h_vxa* v, const vx3h_vxa_info& info) {
         vx3h_sim_desc d = *vx3h_vxa_desc(v);
         if (d.enable_detach && d.collisions_enabled) d.enable_detach = 0;   // reached only with allow_unsupported
+        if (d.secondary_experiment && d.collisions_enabled) d.secondary_experiment = 0;
         if (opt.voxcad_stop) {
             d.stop_condition_type = info.stop_condition_type;
             d.stop_condition_value = info.stop_condition_value;
@@ -95,7 +96,7 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             if (info.enable_attach) missing += " attach";
             if (info.enable_detach && vx3h_vxa_desc(v)->collisions_enabled) missing += " detach-with-collisions";
             if (info.enable_signals) missing += " signals";
-            if (info.secondary_experiment) missing += " secondary-experiment";
+            if (info.secondary_experiment && vx3h_vxa_desc(v)->collisions_enabled) missing += " secondary-experiment-with-collisions";
             if (info.max_dist_close_pairs > 0) missing += " close-pair counting";
             if (!missing.empty()) {
                 if (!opt.allow_unsupported) { vx3h_vxa_free(v); throw std::runtime_error(f + ": not supported by this engine:" + missing); }
@@ -149,12 +150,10 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
                 r.voxel_init_pos.resize(3 * V);
                 r.voxel_position.resize(3 * V);
                 r.voxel_mats.resize(V);
-                for (int i = 0; i < V; i++) {
-                    for (int k = 0; k < 3; k++) r.voxel_init_pos[3 * i + k] = d->ijk[3 * i + k] * d->voxel_size;
-                    r.voxel_mats[i] = d->materials[d->material[i]].matid;
-                }
+                for (int i = 0; i < V; i++) r.voxel_mats[i] = d->materials[d->material[i]].matid;
                 vx3h_state_view view{};
                 view.pos = r.voxel_position.data();
+                view.init_pos = r.voxel_init_pos.data();   // d_initialPosition (VX3_SimulationManager.cu:439-446)
                 check(vx3h_sim_read_state(bt, j.slot, &view), j.name);
             }
             if (opt.verbose)

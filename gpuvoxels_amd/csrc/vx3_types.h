@@ -25,6 +25,7 @@ enum {
     VF_INTERIOR = 1 << 1,            // all six links present (the reference's "SURFACE" bit)
     VF_FLOOR = 1 << 2,
     VF_STATIC = 1 << 3,              // floor static friction
+    VF_REMOVED = 1 << 4,             // removed by a secondary experiment: frozen, no links
     VF_COLL = 1 << 5,
     VF_LINKMASK_SHIFT = 8,           // bits 8..13: link present in direction d
 };
@@ -37,7 +38,8 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_COL_OVERFLOW = 256,   // a collision table overflowed: the simulation fails
        SC_EXTRAS = 512,         // cilia or a force field add to the voxel force (xf arrays)
        SC_CILIA = 1024,
-       SC_DETACH = 2048 };      // failed links detach
+       SC_DETACH = 2048,        // failed links detach
+       SC_SECONDARY = 4096 };   // secondary experiment: timed voxel removal, initial-position reset
 #define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
@@ -110,6 +112,9 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int histHead;                    // ring index of the newest entry of the 500-step histories
     // VX3 force field (VX3_Voxel.cu:178-184): MathTree programs per axis in the token pool
     int ffOff[3], ffLen[3];
+    // VX3 secondary experiment: the initial-position reset (removal times are per voxel)
+    double reinitAfter;
+    int reinitDone;
 };
 
 #define VX3_HIST 500                 // HISTORY_SIZE (VoxCAD/src/VX1/VX_Sim.h)

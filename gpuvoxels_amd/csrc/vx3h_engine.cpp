@@ -200,7 +200,7 @@ struct vx3h_batch {
 
 namespace {
 
-size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8; }
+size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8 + 8 * 3 + 8; }
 size_t linkBytes(size_t rs) { return rs * 9 + 4 * 5 + 4 * 3; }
 
 void initSpecs(vx3h_batch* b) {
@@ -220,7 +220,7 @@ void initSpecs(vx3h_batch* b) {
     V(&d.cpx, R, 1, false); V(&d.cpy, R, 1, false); V(&d.cpz, R, 1, false);
     V(&d.cmx, R, 1, false); V(&d.cmy, R, 1, false); V(&d.cmz, R, 1, false); V(&d.ctf, 4, 1, false);
     V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
-    V(&d.cil, 8, 8, true); V(&d.xf, R, 6, true);
+    V(&d.cil, 8, 8, true); V(&d.xf, R, 6, true); V(&d.ip, 8, 3, true); V(&d.rt, 8, 1, false);
     b->nbPool.arrs = {{(void**)&d.nbList, 4}};
     b->hashPool.arrs = {{(void**)&d.hashCnt, 4}, {(void**)&d.hashEnt, 4 * VX3_COL_BUCKET}, {(void**)&d.hashSim, 4}};
     b->nearPool.arrs = {{(void**)&d.nearList, 4}};
@@ -508,6 +508,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         FAILV(VX3H_EINVAL, "unknown stop condition type");
     if (desc->enable_detach && desc->collisions_enabled)
         FAILV(VX3H_EUNSUPPORTED, "detach with collisions is not supported (the contact exclusion lists are built once)");
+    if (desc->secondary_experiment && desc->collisions_enabled)
+        FAILV(VX3H_EUNSUPPORTED, "voxel removal with collisions is not supported (the contact exclusion lists are built once)");
     HIPCHK(hipSetDevice(b->device));
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
@@ -572,6 +574,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         ffOff[k] = (int)b->tokens.size();
         for (const vx3h_token& t : H.ffTok[k]) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
     }
+
     if ((int)b->tokens.size() > tok0) { vx3h_status st = uploadTable(b, b->dTok, b->capTok, b->tokens); if (st) return st; }
     b->db.vmatTab = (const VoxMatDev*)b->dVmat;
     b->db.lmatTab = (const LinkMatDev*)b->dLmat;
@@ -646,7 +649,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     // VoxCAD stop conditions and equilibrium mode are evaluated by the stream path's control pass
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
-                       nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach;
+                       nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach &&
+                       !desc->secondary_experiment;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -658,7 +662,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
-    if (desc->enable_detach && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "detach runs on the stream path");
+    if ((desc->enable_detach || desc->secondary_experiment) && b->cfg.path == VX3H_PATH_RESIDENT)
+        FAILV(VX3H_EUNSUPPORTED, "detach and secondary experiments run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -689,6 +694,19 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (H.extras()) { c.flags |= SC_EXTRAS; b->anyExtras = true; }
     if (!H.cilia.empty()) c.flags |= SC_CILIA;
     if (desc->enable_detach) { c.flags |= SC_DETACH; b->anyDetach = true; }
+    if (desc->secondary_experiment) {
+        c.flags |= SC_SECONDARY;
+        b->anyDetach = true;   // the step-start pass refreshes surfaces after removals too
+        c.reinitAfter = desc->reinit_after;
+        c.reinitDone = 0;
+        std::vector<double> col(NV);
+        for (int k = 0; k < 3; k++) {   // saveInitialPosition at creation: the lattice positions
+            for (int i = 0; i < NV; i++) col[i] = H.ijk[3 * i + k] * H.voxSize;
+            if (NV) HIPCHK(hipMemcpy(d.ip + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+        }
+        for (int i = 0; i < NV; i++) col[i] = desc->materials[desc->material[i]].remove_after;
+        if (NV) HIPCHK(hipMemcpy(d.rt + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+    }
     if (voxcad) {
         const std::vector<double> unset(3 * VX3_HIST, -1.0);   // ClearHistories
         HIPCHK(hipMemcpy(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -971,13 +989,20 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
     o.dt = c.dt;
     o.recommended_dt = c.optDt;
     o.recent_angle = c.recentAngle;
-    for (int k = 0; k < 3; k++) o.initial_com[k] = sh.com0[k];
+    // initial center of mass and positions: the admission's, or a secondary experiment's reset
+    for (int k = 0; k < 3; k++) o.initial_com[k] = c.com0[k];
     std::vector<double> x, y, z;
     vx3h_status st;
     const size_t rs = b->realSize;
     if ((st = readReal(b, at(b->db.px, rs, sh.v0), sh.nv, x))) return st;
     if ((st = readReal(b, at(b->db.py, rs, sh.v0), sh.nv, y))) return st;
     if ((st = readReal(b, at(b->db.pz, rs, sh.v0), sh.nv, z))) return st;
+    std::vector<double> ip;
+    if (c.flags & SC_SECONDARY) {
+        ip.resize(3 * (size_t)sh.nv);
+        for (int k = 0; k < 3; k++)
+            HIPCHK(hipMemcpy(ip.data() + (size_t)k * sh.nv, b->db.ip + (size_t)k * b->db.Vcap + sh.v0, sh.nv * 8, hipMemcpyDeviceToHost));
+    }
     // updateCurrentCenterOfMass (VX3_VoxelyzeKernel.cu:477-493) + collectResults totals (VX3_SimulationManager.cu:442-452)
     double tot = 0, sx = 0, sy = 0, sz = 0, dist = 0;
     int nm = 0;
@@ -987,13 +1012,16 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
         sx += x[i] * m; sy += y[i] * m; sz += z[i] * m;
         tot += m;
         nm++;
-        double dx = x[i] - sh.ijk[3 * i] * sh.voxSize, dy = y[i] - sh.ijk[3 * i + 1] * sh.voxSize, dz = z[i] - sh.ijk[3 * i + 2] * sh.voxSize;
+        const double ix = ip.empty() ? sh.ijk[3 * i] * sh.voxSize : ip[i];
+        const double iy = ip.empty() ? sh.ijk[3 * i + 1] * sh.voxSize : ip[sh.nv + i];
+        const double iz = ip.empty() ? sh.ijk[3 * i + 2] * sh.voxSize : ip[2 * (size_t)sh.nv + i];
+        double dx = x[i] - ix, dy = y[i] - iy, dz = z[i] - iz;
         dist += sqrt(dx * dx + dy * dy + dz * dz);
     }
     if (tot != 0) { double inv = 1.0 / tot; o.current_com[0] = inv * sx; o.current_com[1] = inv * sy; o.current_com[2] = inv * sz; }
     o.num_measured_voxel = nm;
     o.total_distance_of_all_voxels = dist;
-    double off[3] = {o.current_com[0] - sh.com0[0], o.current_com[1] - sh.com0[1], o.current_com[2] - sh.com0[2]};
+    double off[3] = {o.current_com[0] - c.com0[0], o.current_com[1] - c.com0[1], o.current_com[2] - c.com0[2]};
     // computeFitness (VX3_VoxelyzeKernel.cu:530-534)
     o.fitness = evalMathTree(sh.fit.data(), (int)sh.fit.size(), off[0], off[1], off[2], 0, c.t, c.recentAngle, 0, 0, sh.nv);
     if (posOut) {
@@ -1118,6 +1146,18 @@ extern "C" vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_v
         }
     }
     if (view->vox_flags) HIPCHK(hipMemcpy(view->vox_flags, d.vflags + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
+    if (view->init_pos) {
+        SimCtl c;
+        HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(SimCtl), hipMemcpyDeviceToHost));
+        std::vector<double> col(sh.nv);
+        for (int k = 0; k < 3; k++) {
+            if (c.flags & SC_SECONDARY)
+                HIPCHK(hipMemcpy(col.data(), d.ip + (size_t)k * d.Vcap + sh.v0, sh.nv * 8, hipMemcpyDeviceToHost));
+            else
+                for (int i = 0; i < sh.nv; i++) col[i] = sh.ijk[3 * i + k] * sh.voxSize;
+            for (int i = 0; i < sh.nv; i++) view->init_pos[3 * i + k] = col[i];
+        }
+    }
     if (view->temp) {
         std::vector<float> tf(sh.nv);
         HIPCHK(hipMemcpy(tf.data(), d.tfac + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
@@ -1215,7 +1255,7 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
     if (record_voxel) {
         appendf(t, "<<<Step%d Time:%f>>>", step_label, c.t);
         for (int v = 0; v < NV; v++) {
-            if (vflags[v] & VF_INTERIOR) continue;   // surface voxels as of this step's start
+            if (vflags[v] & (VF_INTERIOR | VF_REMOVED)) continue;   // surface voxels as of this step's start
             const double* p = &pos[3 * (size_t)v];
             const double* q = &ori[4 * (size_t)v];
             appendf(t, "%.1f,%.1f,%.1f,", p[0] * vs, p[1] * vs, p[2] * vs);

@@ -335,6 +335,7 @@ struct PalMat {                 // CVXC_Material (VX_Object.cpp:1373-1460), SING
     double E = 0, plasticMod = 0, yieldStress = 0, failStress = 0, density = 0, nu = 0, cte = 0, uS = 0, uD = 0;
     float rgba[4] = {0.5f, 0.5f, 0.5f, 1.0f};   // Display colour (CVXC_Material::ClearMaterial defaults)
     double cilia = 0, ciliaOn = 0;              // Cilia, TurnOnCiliaAfterThisManySeconds
+    double removeAfter = 0;                     // RemoveFromSimulationAfterThisManySeconds
 };
 
 bool readRegion(Rip& r, Region& g) {
@@ -625,6 +626,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                     if (!r.load("TurnOnThermalExpansionAfterThisManySeconds", m.thermalOnAfter)) m.thermalOnAfter = 0;
                     if (!r.load("Fixed", m.fixed)) m.fixed = 0;
                     if (!r.load("TurnOnCiliaAfterThisManySeconds", m.ciliaOn)) m.ciliaOn = 0.0;
+                    if (!r.load("RemoveFromSimulationAfterThisManySeconds", m.removeAfter)) m.removeAfter = 0.0;
                     if (!r.load("Cilia", m.cilia)) m.cilia = 0;
                     if (!r.load("Elastic_Mod", m.E)) m.E = 0;
                     if (!r.load("Plastic_Mod", m.plasticMod)) m.plasticMod = 0;
@@ -743,6 +745,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
         d.thermal_on_after = p.thermalOnAfter;
         d.cilia = p.cilia;
         d.cilia_on_after = p.ciliaOn;
+        d.remove_after = p.removeAfter;
         d.matid = (int)i + 1;   // palette index; 0 is the erase material
         for (int k = 0; k < 4; k++) {   // GetRedi() etc. (VX_Object.h:419-425), then setRed's clamp
             const int c = (int)(p.rgba[k] * 255);
@@ -903,6 +906,8 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     d.save_position_of_all_voxels = ptGet<int>(doc, S + "SavePositionOfAllVoxels", 0);
     d.enable_cilia = in.enable_cilia;
     d.enable_detach = in.enable_detach;
+    d.secondary_experiment = in.secondary_experiment;
+    d.reinit_after = ptGet<double>(doc, S + "ReinitializeInitialPositionAfterThisManySeconds", 0.0);   // VX3_SimulationManager.cu:365
     d.base_cilia_force = o.baseCilia.empty() ? nullptr : o.baseCilia.data();
     d.shift_cilia_force = o.shiftCilia.empty() ? nullptr : o.shiftCilia.data();
     for (int k = 0; k < 3; k++) {

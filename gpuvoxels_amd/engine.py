@@ -308,8 +308,9 @@ class Batch:
     def read_state(self, slot, nl=None):
         nv = self.nv[slot]
         pos, ori, lm, am = np.zeros(3 * nv), np.zeros(4 * nv), np.zeros(3 * nv), np.zeros(3 * nv)
-        fl, tf = np.zeros(nv, np.int32), np.zeros(nv)
+        fl, tf, ip = np.zeros(nv, np.int32), np.zeros(nv), np.zeros(3 * nv)
         v = _abi.StateView()
+        v.init_pos = _p(ip, C.c_double)
         v.pos, v.orient, v.lin_mom, v.ang_mom = (_p(pos, C.c_double), _p(ori, C.c_double), _p(lm, C.c_double),
                                                  _p(am, C.c_double))
         v.vox_flags, v.temp = _p(fl, C.c_int), _p(tf, C.c_double)
@@ -321,7 +322,7 @@ class Batch:
             out.update(strain=st, max_strain=ms, small_angle=sa)
         check(lib().vx3h_sim_read_state(self.h, slot, C.byref(v)))
         out.update(pos=pos.reshape(-1, 3), orient=ori.reshape(-1, 4), lin_mom=lm.reshape(-1, 3),
-                   ang_mom=am.reshape(-1, 3), flags=fl, tfac=tf)
+                   ang_mom=am.reshape(-1, 3), flags=fl, tfac=tf, init_pos=ip.reshape(-1, 3))
         return out
 
 

@@ -40,6 +40,7 @@ class Material:
     color: tuple = (0.5, 0.5, 0.5, 1.0)   # Display Red, Green, Blue, Alpha in [0, 1]
     cilia: float = 0.0                    # scale of the cilia force of voxels of this material
     cilia_on_after: float = 0.0           # TurnOnCiliaAfterThisManySeconds
+    remove_after: float = 0.0             # RemoveFromSimulationAfterThisManySeconds (secondary experiment)
 
     def to_c(self) -> _abi.MaterialDesc:
         m = _abi.MaterialDesc()
@@ -66,6 +67,7 @@ class Material:
         m.matid = self.matid
         m.cilia = self.cilia
         m.cilia_on_after = self.cilia_on_after
+        m.remove_after = self.remove_after
         for k in range(4):   # CVXC_Material::GetRedi etc. then CVX_Material::setRed's clamp
             m.color[k] = min(255, max(0, int(np.float32(self.color[k]) * np.float32(255))))
         return m
@@ -155,6 +157,8 @@ class Model:
     shift_cilia: Optional[np.ndarray] = None  # (V, 3)
     force_field: list = field(default_factory=lambda: [[], [], []])
     enable_detach: int = 0                    # failed links leave their voxels (VX3 detach)
+    secondary_experiment: int = 0             # timed voxel removal and initial-position reset
+    reinit_after: float = 0.0                 # ReinitializeInitialPositionAfterThisManySeconds
     name: str = "sim"
 
     @property
@@ -233,4 +237,6 @@ class Model:
                 d.n_force_field_tokens[k] = len(toks)
                 d.force_field_tokens[k] = C.cast(arr, C.POINTER(_abi.Token))
         d.enable_detach = self.enable_detach
+        d.secondary_experiment = self.secondary_experiment
+        d.reinit_after = self.reinit_after
         return d, keep

@@ -97,12 +97,17 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
     if m.fitness_tokens:
         out.append("<FitnessFunction>%s</FitnessFunction>" % mathtree_xml(m.fitness_tokens))
     out.append("<AttachDetach><EnableCollision>%d</EnableCollision><boundingRadius>%s</boundingRadius>"
-               "<watchDistance>%s</watchDistance></AttachDetach>"
-               % (int(bool(m.collisions_enabled)), _num(m.bounding_radius), _num(m.watch_distance)))
+               "<watchDistance>%s</watchDistance>%s</AttachDetach>"
+               % (int(bool(m.collisions_enabled)), _num(m.bounding_radius), _num(m.watch_distance),
+                  "<EnableDetach>1</EnableDetach>" if m.enable_detach else ""))
     out.append("<EnableExpansion>%d</EnableExpansion>" % int(bool(m.enable_expansion)))
     out.append("<SavePositionOfAllVoxels>%d</SavePositionOfAllVoxels>" % int(bool(m.save_position_of_all_voxels)))
     if m.enable_cilia:
         out.append("<EnableCilia>1</EnableCilia>")
+    if m.secondary_experiment:
+        out.append("<SecondaryExperiment>1</SecondaryExperiment><ReinitializeInitialPositionAfterThisManySeconds>%s"
+                   "</ReinitializeInitialPositionAfterThisManySeconds>" % _num(m.reinit_after))
+
     if any(m.force_field):
         out.append("<ForceField>%s</ForceField>" % "".join(
             "<%s>%s</%s>" % (axis, mathtree_xml(toks), axis)
@@ -144,7 +149,9 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                 "<TurnOnThermalExpansionAfterThisManySeconds>%s</TurnOnThermalExpansionAfterThisManySeconds>"
                 % _num(mt.thermal_on_after),
                 "<Cilia>%s</Cilia>" % _num(mt.cilia),
-                "<TurnOnCiliaAfterThisManySeconds>%s</TurnOnCiliaAfterThisManySeconds>" % _num(mt.cilia_on_after)]
+                "<TurnOnCiliaAfterThisManySeconds>%s</TurnOnCiliaAfterThisManySeconds>" % _num(mt.cilia_on_after),
+                "<RemoveFromSimulationAfterThisManySeconds>%s</RemoveFromSimulationAfterThisManySeconds>"
+                % _num(mt.remove_after)]
         if mt.strain_data:
             mech.append("<SSData><NumDataPts>%d</NumDataPts><StrainData>%s</StrainData><StressData>%s</StressData></SSData>"
                         % (len(mt.strain_data), "".join("<Strain>%s</Strain>" % _num(e) for e in mt.strain_data),

@@ -89,6 +89,7 @@ typedef struct vx3h_material_desc {
                                    (VXA/VX_Object.h:419-425, old/VX_Material.cpp:245-283) */
     double cilia;               /* Cilia: scale of the voxel's cilia force (VX3_Voxel.cu:376) */
     double cilia_on_after;      /* TurnOnCiliaAfterThisManySeconds */
+    double remove_after;        /* RemoveFromSimulationAfterThisManySeconds (secondary experiment) */
 } vx3h_material_desc;
 
 /* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
@@ -161,6 +162,15 @@ typedef struct vx3h_sim_desc {
      * failure strain leaves both voxels after the link update of that step, before the
      * voxel update.  Not with collisions. */
     int    enable_detach;
+    /* VX3 secondary experiment (Simulator.SecondaryExperiment; VX3_VoxelyzeKernel.cu:
+     * 336-350, 365-397): at the end of each step, every material whose remove_after is
+     * positive and earlier than the step's start time loses its voxels (frozen in
+     * place, still counted in the center of mass) and their links; once the start time
+     * passes reinit_after (ReinitializeInitialPositionAfterThisManySeconds), the
+     * initial center of mass becomes the last sampled one and the initial positions
+     * the current ones.  Not with collisions. */
+    int    secondary_experiment;
+    double reinit_after;
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */
@@ -260,11 +270,14 @@ typedef struct vx3h_state_view {
     double* orient;    /* 4*V (w,x,y,z) */
     double* lin_mom;   /* 3*V */
     double* ang_mom;   /* 3*V */
-    int*    vox_flags; /* V: bit 3 = floor static friction */
+    int*    vox_flags; /* V: bit 1 = interior (all six links), bit 3 = floor static friction,
+                          bit 4 = removed (secondary experiment) */
     double* temp;      /* V */
     double* link_strain;  /* L */
     double* link_max_strain; /* L */
     int*    link_small_angle; /* L */
+    double* init_pos;  /* 3*V initial positions (the lattice positions, or those saved by a
+                          secondary experiment's re-initialization) */
 } vx3h_state_view;
 vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_view* view);
 

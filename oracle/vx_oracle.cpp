@@ -557,6 +557,7 @@ struct Voxel {
     // of force() (VX3_Voxel.cu:376), and force-field force, added after force()
     // (VX3_Voxel.cu:178-184); used when the simulation has either
     bool extras = false;
+    bool removed = false;   // VX3 secondary experiment: frozen, without links
     Vd ciliaForce, fieldForce;
     Vd baseCilia, shiftCilia;
     int boolStates = 0;
@@ -1195,6 +1196,7 @@ struct Engine {
         if (collisions) updateCollisions();
         for (Voxel* v : voxelsList) {
             if (v->mat->fixed) continue;
+            if (v->removed) continue;   // VX3_VoxelyzeKernel.cu:586
             v->timeStep(dt);
         }
         currentTime += dt;
@@ -1219,6 +1221,34 @@ struct Sim {
     std::vector<vx3h_token> stopTok, fitTok;
     float lastDt = 0;
     VoxcadStop voxcad;   // VoxCAD stop-condition family and equilibrium mode
+    // VX3 secondary experiment (VX3_VoxelyzeKernel.cu:336-350)
+    bool secondary = false, reinitDone = false;
+    double reinitAfter = 0;
+    std::vector<double> matRemoveAfter;
+    std::vector<bool> matRemoved;
+    std::vector<Vd> initPos;   // d_initialPosition: lattice positions, or those saved at the reset
+
+    // removeVoxels (VX3_VoxelyzeKernel.cu:365-397)
+    void removeVoxels() {
+        for (size_t i = 0; i < matRemoveAfter.size(); i++) {
+            if (matRemoved[i] || !(matRemoveAfter[i] > 0) || !(matRemoveAfter[i] < currentTime)) continue;
+            for (Voxel* v : vx.voxelsList) {
+                if (v->mat != vx.voxelMats[i] || v->removed) continue;
+                v->removed = true;
+                for (int k = 0; k < 6; k++) {
+                    Link* l = v->links[k];
+                    if (!l) continue;
+                    l->gone = true;
+                    Voxel* nb = l->pVNeg == v ? l->pVPos : l->pVNeg;
+                    for (int m = 0; m < 6; m++)
+                        if (nb->links[m] == l) { nb->links[m] = nullptr; break; }
+                    v->links[k] = nullptr;
+                }
+            }
+            matRemoved[i] = true;
+            vx.surfaceChanged = true;
+        }
+    }
     // VX3 cilia and force field
     bool extras = false, ciliaOn = false;
     std::vector<double> matCilia, matCiliaOn;   // per material (voxelMats order)
@@ -1233,7 +1263,7 @@ struct Sim {
         for (Voxel* v : vx.voxelsList) {
             v->ciliaForce = Vd(0, 0, 0);
             const int mi = matOf(v);   // voxel materials are in description order
-            if (ciliaOn && v->isSurface() && matCilia[mi] != 0 && !(matCiliaOn[mi] > currentTime)) {
+            if (ciliaOn && v->isSurface() && !v->removed && matCilia[mi] != 0 && !(matCiliaOn[mi] > currentTime)) {
                 const double localSignal = 0.0;
                 v->ciliaForce = v->orient.RotateVec3D(v->baseCilia + v->shiftCilia * localSignal) * matCilia[mi];
             }
@@ -1284,6 +1314,7 @@ struct Sim {
     void updateTemperature() {
         if (!vary || !(TempPeriod > 0)) return;
         for (Voxel* v : vx.voxelsList) {
+            if (v->removed) continue;   // VX3_VoxelyzeKernel.cu:632
             if (v->mat->thermalOnAfter > currentTime) continue;
             if (v->mat->fixed) continue;
             double T = TempAmplitude * sin(2 * 3.1415926f * (currentTime / TempPeriod + v->phaseOffset));
@@ -1326,6 +1357,14 @@ struct Sim {
             Vd A = comHist[0], B = comHist[1], C = currentCoM;
             if (B == C || A == B || angleSampleTimes < 3) recentAngle = 0;
             else recentAngle = acos((B - A).Dot(C - B) / (B.Dist(A) * C.Dist(B)));
+        }
+        if (secondary) {
+            removeVoxels();
+            if (!reinitDone && reinitAfter < currentTime) {
+                reinitDone = true;
+                initialCoM = currentCoM;   // InitializeCenterOfMass
+                for (size_t i = 0; i < vx.voxelsList.size(); i++) initPos[i] = vx.voxelsList[i]->pos;   // saveInitialPosition
+            }
         }
         currentTime += dt;
         voxcad.afterStep(
@@ -1436,6 +1475,11 @@ static Sim* build(const vx3h_sim_desc* d) {
         for (int i = 0; i < d->n_materials; i++) s->ciliaOn |= d->materials[i].cilia != 0;
     s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty();
     vx.detach = d->enable_detach != 0;
+    s->secondary = d->secondary_experiment != 0;
+    s->reinitAfter = d->reinit_after;
+    for (int i = 0; i < d->n_materials; i++) s->matRemoveAfter.push_back(d->materials[i].remove_after);
+    s->matRemoved.assign(d->n_materials, false);
+    for (Voxel* v : vx.voxelsList) s->initPos.push_back(v->originalPosition());
     for (int i = 0; i < d->n_materials; i++) { s->matCilia.push_back(d->materials[i].cilia); s->matCiliaOn.push_back(d->materials[i].cilia_on_after); }
     for (size_t i = 0; i < vx.voxelsList.size(); i++) {
         Voxel* v = vx.voxelsList[i];
@@ -1570,11 +1614,11 @@ void vxo_vx3_status(vxo_sim* h, vx3h_sim_status* st) {
     st->recent_angle = s->recentAngle;
     int nm = 0;
     double tot = 0;
-    for (Voxel* v : s->vx.voxelsList) {
+    for (size_t i = 0; i < s->vx.voxelsList.size(); i++) {
+        Voxel* v = s->vx.voxelsList[i];
         if (!v->mat->isMeasured) continue;
         nm++;
-        Vd p0 = v->originalPosition();
-        tot += Vd(v->pos.x, v->pos.y, v->pos.z).Dist(p0);
+        tot += Vd(v->pos.x, v->pos.y, v->pos.z).Dist(s->initPos[i]);   // d_initialPosition (VX3_SimulationManager.cu:439-452)
     }
     st->num_measured_voxel = nm;
     st->total_distance_of_all_voxels = tot;
@@ -1588,7 +1632,7 @@ void vxo_voxel_state(vxo_sim* h, double* pos, double* orient, double* lm, double
         if (orient) { orient[4 * i] = v->orient.w; orient[4 * i + 1] = v->orient.x; orient[4 * i + 2] = v->orient.y; orient[4 * i + 3] = v->orient.z; }
         if (lm) { lm[3 * i] = v->linMom.x; lm[3 * i + 1] = v->linMom.y; lm[3 * i + 2] = v->linMom.z; }
         if (am) { am[3 * i] = v->angMom.x; am[3 * i + 1] = v->angMom.y; am[3 * i + 2] = v->angMom.z; }
-        if (flags) flags[i] = v->boolStates;
+        if (flags) flags[i] = v->boolStates | (v->removed ? 16 : 0);
         if (temp) temp[i] = v->temp;
     }
 }

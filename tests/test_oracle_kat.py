@@ -552,3 +552,81 @@ def test_detach_frees_failed_links_and_surface_follows_a_step_later():
     o2 = OSim(_shrinking_core(0))
     o2.step(k + 20)
     assert o2.state()["flags"][13] & INTERIOR
+
+
+# ---- VX3 secondary experiment (VX3_VoxelyzeKernel.cu:336-350, 365-397)
+
+def _layered_cube(remove_after, reinit_after=1e9):
+    """3x3x3 cube on the floor whose top layer is a second material, removed once the
+    step start time passes remove_after."""
+    m = Model(voxel_size=0.01)
+    kw = dict(elastic_mod=1e6, density=1e3, u_static=1.0, u_dynamic=0.5, zeta_internal=1.0, zeta_global=0.01,
+              zeta_collision=0.8)
+    m.materials = [Material(**kw), Material(remove_after=remove_after, **kw)]
+    z, y, x = np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij")
+    m.ijk = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.int32)
+    m.mat = (z.ravel() == 2).astype(np.int32)
+    m.gravity_mult = np.float32(1.0)
+    m.floor_enabled = 1
+    m.temp_period = 0.002
+    m.secondary_experiment = 1
+    m.reinit_after = reinit_after
+    return m
+
+
+def test_removed_voxels_freeze_and_lose_their_links():
+    REMOVED, INTERIOR = 16, 2
+    o = OSim(_layered_cube(1e-9))
+    dt = None
+    n_removed_at = None
+    for n in range(1, 200):
+        t_start = o.status().current_time
+        o.step(1)
+        dt = o.status().dt
+        fl = o.state()["flags"]
+        top = np.arange(18, 27)
+        if n_removed_at is None and (fl[top] & REMOVED).any():
+            n_removed_at = n
+            assert t_start > 0 and (fl[top] & REMOVED).all() and not (fl[:18] & REMOVED).any()
+            frozen = o.state()["pos"][top].copy()
+            assert fl[13] & INTERIOR          # the middle voxel loses its top link; the surface follows next step
+        elif n_removed_at is not None:
+            assert np.array_equal(o.state()["pos"][top], frozen)
+            assert not (fl[13] & INTERIOR)
+    assert n_removed_at == 2                 # the first step starts at t = 0, not after 1e-9
+    o = OSim(_layered_cube(0.0))             # a removal time of 0 means never
+    o.step(50)
+    assert not (o.state()["flags"] & REMOVED).any()
+    # the removal time counts: a later one removes later
+    o = OSim(_layered_cube(100.5 * float(np.float32(dt))))
+    o.step(100)
+    assert not (o.state()["flags"] & REMOVED).any()
+    o.step(2)
+    assert (o.state()["flags"][18:] & REMOVED).all()
+
+
+def test_initial_position_reset():
+    # once the step start time passes reinit_after, the initial center of mass becomes the
+    # last sampled one (sampled every CycleStep = int(TempPeriod/dt) steps) and the total
+    # distance counts from the positions at that moment
+    probe = OSim(_layered_cube(1e9))
+    probe.step(1)
+    dt = float(np.float32(probe.status().dt))
+    cycle = int(0.002 / dt)
+    m = _layered_cube(1e9, reinit_after=(cycle + 10.5) * dt)
+    m.fitness_tokens = [(MT["mtVAR"], 2.0), (MT["mtEND"], 0.0)]      # z displacement of the center of mass
+    twin = OSim(m)
+    twin.step(cycle)
+    sampled = np.array(twin.status().current_com[:])
+    o = OSim(m)
+    o.step(cycle + 11)
+    assert np.array_equal(np.array(o.status().initial_com[:]), np.array(OSim(m).status().initial_com[:]))
+    o.step(1)                                                         # the reset, at the end of this step
+    s2 = o.status()
+    assert np.array_equal(np.array(s2.initial_com[:]), sampled)
+    pos2 = o.state()["pos"].copy()
+    o.step(300)
+    s3 = o.status()
+    d = np.linalg.norm(o.state()["pos"] - pos2, axis=1).sum()
+    assert s3.total_distance_of_all_voxels == pytest.approx(d, rel=1e-12)
+    assert s3.fitness == pytest.approx(s3.current_com[2] - sampled[2], rel=1e-12)
