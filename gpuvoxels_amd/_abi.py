@@ -45,6 +45,11 @@ class MaterialDesc(C.Structure):
         ("cilia", C.c_double),
         ("cilia_on_after", C.c_double),
         ("remove_after", C.c_double),
+        ("is_pacemaker", C.c_int),
+        ("pacemaker_period", C.c_double),
+        ("signal_value_decay", C.c_double),
+        ("signal_time_delay", C.c_double),
+        ("inactive_period", C.c_double),
     ]
 
 
@@ -105,6 +110,7 @@ class SimDesc(C.Structure):
         ("enable_detach", C.c_int),
         ("secondary_experiment", C.c_int),
         ("reinit_after", C.c_double),
+        ("enable_signals", C.c_int),
     ]
 
 

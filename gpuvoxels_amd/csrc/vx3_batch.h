@@ -25,6 +25,10 @@ template <typename T> struct DevBatch {
     T* xf;                    // 6*Vcap this step's global cilia force xyz and force-field force xyz
     double* ip;               // 3*Vcap initial positions saved by a secondary experiment's reset
     double* rt;               // Vcap removal time of the voxel's material (secondary experiment), 0 = never
+    double* sg;               // 8*Vcap signal state: local signal, its decay time, next pacemaker pulse,
+                              // deaf until, pending value and time, posted value and time
+    double* sp;               // 5*Vcap signal material: pacemaker, period, value decay, time delay, inactive period
+    int* nbr;                 // 6*Vcap neighbour voxel per link direction (signal simulations), -1 = none
     float* es;                // 6*Vcap axial strain of the link end in slot d: es[d*Vcap + v]
     // link state [Lcap]
     int *lneg, *lpos;

@@ -402,6 +402,46 @@ struct Contacts {
     }
 };
 
+// ---- signals (VX3_Voxel.cu:279-348).  sg[k*V + v]: 0 local signal, 1 its next decay
+// time, 2 next pacemaker pulse, 3 deaf until, 4/5 pending signal value/time, 6/7 posted
+// value/time; sp[k*V + v]: 0 pacemaker, 1 period, 2 value decay, 3 time delay, 4 inactive period.
+// VX3_Voxel::receiveSignal
+__device__ __forceinline__ void receiveSignal(const DevBatch<real>& b, size_t V, int v, double value, double activeTime, bool force) {
+    if (!force && b.sg[3 * V + v] > activeTime) return;
+    if (value < 0.1) return;
+    b.sg[3 * V + v] = activeTime + b.sp[4 * V + v];
+    b.sg[0 * V + v] = value;
+    double pv = value * b.sp[2 * V + v];
+    if (pv < 0.1) pv = 0;
+    b.sg[4 * V + v] = pv;
+    b.sg[5 * V + v] = activeTime;
+}
+// End of a moving voxel's update: propagateSignal (its pending signal is posted for the
+// neighbours), packMaker, localSignalDecay -- all on the voxel's own state
+__device__ __forceinline__ void signalsOwn(const DevBatch<real>& b, size_t V, int v, double t) {
+    double outV = 0, outT = 0;
+    if (!(b.sg[5 * V + v] > t) && !(b.sg[4 * V + v] < 0.1)) {
+        outV = b.sg[4 * V + v];
+        outT = t + b.sp[3 * V + v];
+        b.sg[4 * V + v] = 0;
+        b.sg[5 * V + v] = 0;
+        b.sg[3 * V + v] = t + 2 * b.sp[3 * V + v] + b.sp[4 * V + v];
+    }
+    if (b.sp[0 * V + v] != 0 && !(b.sg[2 * V + v] > t)) {
+        receiveSignal(b, V, v, 100, t, true);
+        b.sg[2 * V + v] = t + b.sp[1 * V + v];
+    }
+    if (!(b.sg[1 * V + v] > t)) {
+        if (b.sg[0 * V + v] < 0.1) b.sg[0 * V + v] = 0;
+        else {
+            b.sg[0 * V + v] = b.sg[0 * V + v] * 0.9;
+            b.sg[1 * V + v] = t + 0.01;
+        }
+    }
+    b.sg[6 * V + v] = outV;
+    b.sg[7 * V + v] = outT;
+}
+
 // The extra forces of a simulation with cilia or a force field, added where the
 // reference adds them: the cilia force at the end of force() (VX3_Voxel.cu:376), the
 // force field right after force() returns (VX3_Voxel.cu:178-184).
@@ -472,6 +512,9 @@ template <bool EXTRAS> __device__ __forceinline__ void voxelPass(const DevBatch<
         b.ow[v] = o.w; b.ox[v] = o.x; b.oy[v] = o.y; b.oz[v] = o.z;
         b.ax[v] = am.x; b.ay[v] = am.y; b.az[v] = am.z;
         b.vflags[v] = fl;
+        if constexpr (EXTRAS) {
+            if (sc.flags & SC_SIGNALS) signalsOwn(b, V, v, sc.t);
+        }
     }
     // thermal factor for the next step: VX3 updateTemperature at t + dt (VX3_VoxelyzeKernel.cu:625-650)
     if ((sc.flags & SC_VARY) && !m.fixed) {
@@ -580,7 +623,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
     if (sc.flags & SC_CILIA) {
         const double coef = b.cil[6 * V + v], on = b.cil[7 * V + v];
         if (!(b.vflags[v] & (VF_INTERIOR | VF_REMOVED)) && coef != 0 && !(on > sc.t)) {
-            const double sig = 0.0;
+            const double sig = (sc.flags & SC_SIGNALS) ? b.sg[v] : 0.0;   // localSignal
             const V3<real> f((real)(b.cil[0 * V + v] + sig * b.cil[3 * V + v]), (real)(b.cil[1 * V + v] + sig * b.cil[4 * V + v]),
                              (real)(b.cil[2 * V + v] + sig * b.cil[5 * V + v]));
             const Q4<real> o(b.ow[v], b.ox[v], b.oy[v], b.oz[v]);
@@ -595,6 +638,26 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
                                  sc.recentAngle, 0, 0, sc.nv, q);
     b.xf[0 * V + v] = c.x; b.xf[1 * V + v] = c.y; b.xf[2 * V + v] = c.z;
     b.xf[3 * V + v] = (real)ff[0]; b.xf[4 * V + v] = (real)ff[1]; b.xf[5 * V + v] = (real)ff[2];
+}
+
+// Signals, second half: every voxel takes the signals its linked neighbours posted this
+// step (the receiveSignal calls of their propagateSignal), in link-direction order.
+__global__ void __launch_bounds__(256) k_signals(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!simRunning(b.sims, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_SIGNALS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+    const int fl = b.vflags[v];
+    if (fl & VF_REMOVED) return;
+    const size_t V = (size_t)b.Vcap;
+    for (int d = 0; d < 6; d++) {
+        if (!(fl & (1 << (VF_LINKMASK_SHIFT + d)))) continue;
+        const int nb = b.nbr[d * V + v];
+        const double val = b.sg[6 * V + nb];
+        if (val > 0) receiveSignal(b, V, v, val, b.sg[7 * V + nb], false);
+    }
 }
 
 // Block-wide deterministic FP64 center of mass of measured voxels (VX3_VoxelyzeKernel.cu:477-493).

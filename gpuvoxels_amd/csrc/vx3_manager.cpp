@@ -95,7 +95,6 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             std::string missing;
             if (info.enable_attach) missing += " attach";
             if (info.enable_detach && vx3h_vxa_desc(v)->collisions_enabled) missing += " detach-with-collisions";
-            if (info.enable_signals) missing += " signals";
             if (info.secondary_experiment && vx3h_vxa_desc(v)->collisions_enabled) missing += " secondary-experiment-with-collisions";
             if (info.max_dist_close_pairs > 0) missing += " close-pair counting";
             if (!missing.empty()) {

@@ -31,6 +31,7 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs));
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_voxels_extras, dim3(gv), dim3(bs), 0, s, b);
     VX3_EV(PASS_VOXELS, hipLaunchKernelGGL(VX_NS::k_voxels, dim3(gv), dim3(bs), 0, s, b));
+    if (o.signals) hipLaunchKernelGGL(VX_NS::k_signals, dim3(gv), dim3(bs), 0, s, b);
     VX3_EV(PASS_CONTROL, hipLaunchKernelGGL(VX_NS::k_control, dim3(b.nSims), dim3(256), 0, s, b));
 #undef VX3_EV
 }

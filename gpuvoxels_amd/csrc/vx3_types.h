@@ -39,7 +39,8 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_EXTRAS = 512,         // cilia or a force field add to the voxel force (xf arrays)
        SC_CILIA = 1024,
        SC_DETACH = 2048,        // failed links detach
-       SC_SECONDARY = 4096 };   // secondary experiment: timed voxel removal, initial-position reset
+       SC_SECONDARY = 4096,     // secondary experiment: timed voxel removal, initial-position reset
+       SC_SIGNALS = 8192 };     // signals between linked voxels (runs with SC_EXTRAS)
 #define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)

@@ -169,6 +169,7 @@ struct vx3h_batch {
     bool anyColl = false;
     bool anyExtras = false;             // some simulation has cilia or a force field
     bool anyDetach = false;             // some simulation detaches failed links
+    bool anySignals = false;            // some simulation has signals
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
     // resident path: kernel shapes and, per shape, the sims it runs
@@ -200,7 +201,7 @@ struct vx3h_batch {
 
 namespace {
 
-size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8 + 8 * 3 + 8; }
+size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8 + 8 * 3 + 8 + 8 * 8 + 8 * 5 + 4 * 6; }
 size_t linkBytes(size_t rs) { return rs * 9 + 4 * 5 + 4 * 3; }
 
 void initSpecs(vx3h_batch* b) {
@@ -221,6 +222,7 @@ void initSpecs(vx3h_batch* b) {
     V(&d.cmx, R, 1, false); V(&d.cmy, R, 1, false); V(&d.cmz, R, 1, false); V(&d.ctf, 4, 1, false);
     V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
     V(&d.cil, 8, 8, true); V(&d.xf, R, 6, true); V(&d.ip, 8, 3, true); V(&d.rt, 8, 1, false);
+    V(&d.sg, 8, 8, true); V(&d.sp, 8, 5, true); V(&d.nbr, 4, 6, true);
     b->nbPool.arrs = {{(void**)&d.nbList, 4}};
     b->hashPool.arrs = {{(void**)&d.hashCnt, 4}, {(void**)&d.hashEnt, 4 * VX3_COL_BUCKET}, {(void**)&d.hashSim, 4}};
     b->nearPool.arrs = {{(void**)&d.nearList, 4}};
@@ -328,6 +330,7 @@ vx3h_status launchSteps(vx3h_batch* b, int n) {
     o.contacts = b->anyColl;
     o.extras = b->anyExtras;
     o.prestep = b->anyExtras || b->anyDetach;
+    o.signals = b->anySignals;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
@@ -650,7 +653,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
                        nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach &&
-                       !desc->secondary_experiment;
+                       !desc->secondary_experiment && !desc->enable_signals;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -662,8 +665,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
-    if ((desc->enable_detach || desc->secondary_experiment) && b->cfg.path == VX3H_PATH_RESIDENT)
-        FAILV(VX3H_EUNSUPPORTED, "detach and secondary experiments run on the stream path");
+    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals) && b->cfg.path == VX3H_PATH_RESIDENT)
+        FAILV(VX3H_EUNSUPPORTED, "detach, secondary experiments and signals run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -694,6 +697,29 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (H.extras()) { c.flags |= SC_EXTRAS; b->anyExtras = true; }
     if (!H.cilia.empty()) c.flags |= SC_CILIA;
     if (desc->enable_detach) { c.flags |= SC_DETACH; b->anyDetach = true; }
+    if (desc->enable_signals && NV) {
+        c.flags |= SC_SIGNALS | SC_EXTRAS;
+        b->anyExtras = b->anySignals = true;
+        const std::vector<double> zero(NV, 0.0);
+        for (int k = 0; k < 8; k++) HIPCHK(hipMemcpy(d.sg + (size_t)k * d.Vcap + v0, zero.data(), NV * 8, hipMemcpyHostToDevice));
+        std::vector<double> col(NV);
+        for (int k = 0; k < 5; k++) {
+            for (int i = 0; i < NV; i++) {
+                const vx3h_material_desc& md = desc->materials[desc->material[i]];
+                col[i] = k == 0 ? (double)(md.is_pacemaker != 0) : k == 1 ? md.pacemaker_period : k == 2 ? md.signal_value_decay
+                         : k == 3 ? md.signal_time_delay : md.inactive_period;
+            }
+            HIPCHK(hipMemcpy(d.sp + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+        }
+        std::vector<int> nb(NV);
+        for (int k = 0; k < 6; k++) {
+            for (int i = 0; i < NV; i++) {
+                const int l = H.voxelLinks[6 * (size_t)i + k];
+                nb[i] = l < 0 ? -1 : v0 + (H.lneg[l] == i ? H.lpos[l] : H.lneg[l]);
+            }
+            HIPCHK(hipMemcpy(d.nbr + (size_t)k * d.Vcap + v0, nb.data(), NV * 4, hipMemcpyHostToDevice));
+        }
+    }
     if (desc->secondary_experiment) {
         c.flags |= SC_SECONDARY;
         b->anyDetach = true;   // the step-start pass refreshes surfaces after removals too
@@ -872,6 +898,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     o.contacts = b->anyColl;
     o.extras = b->anyExtras;
     o.prestep = b->anyExtras || b->anyDetach;
+    o.signals = b->anySignals;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];
@@ -1246,10 +1273,15 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
     vx3h_status st = vx3h_sim_read_state(b, slot, &view);
     if (st) return st;
     if (NL) HIPCHK(hipMemcpy(lflags.data(), b->db.lflags + sh.l0, NL * 4, hipMemcpyDeviceToHost));
+    std::vector<double> sig;
     int n = 0;
     if ((st = vx3h_batch_poll(b, nullptr, 0, &n))) return st;
     SimCtl c;
     HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(SimCtl), hipMemcpyDeviceToHost));
+    if (c.flags & SC_SIGNALS) {
+        sig.resize(NV);
+        HIPCHK(hipMemcpy(sig.data(), b->db.sg + sh.v0, NV * 8, hipMemcpyDeviceToHost));
+    }
     const double vs = 1 / 0.001;
     std::string t;
     if (record_voxel) {
@@ -1280,7 +1312,7 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
             appendf(t, "%.1f,%.1f,%.1f,%.1f,%.1f,%.1f,", corner[0][0] * vs, corner[0][1] * vs, corner[0][2] * vs,
                     corner[1][0] * vs, corner[1][1] * vs, corner[1][2] * vs);
             appendf(t, "%d,", sh.matid[v]);
-            appendf(t, "%.1f,", 0.0);   // localSignal: signals are not simulated
+            appendf(t, "%.1f,", sig.empty() ? 0.0 : sig[v]);   // localSignal
             appendf(t, ";");
         }
         appendf(t, "<<<>>>");

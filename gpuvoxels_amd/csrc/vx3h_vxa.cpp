@@ -336,6 +336,8 @@ struct PalMat {                 // CVXC_Material (VX_Object.cpp:1373-1460), SING
     float rgba[4] = {0.5f, 0.5f, 0.5f, 1.0f};   // Display colour (CVXC_Material::ClearMaterial defaults)
     double cilia = 0, ciliaOn = 0;              // Cilia, TurnOnCiliaAfterThisManySeconds
     double removeAfter = 0;                     // RemoveFromSimulationAfterThisManySeconds
+    int isPaceMaker = 0;                        // signals (VX_Object.cpp:1420-1425)
+    double paceMakerPeriod = 0, signalValueDecay = 0.9, signalTimeDelay = 0.03, inactivePeriod = 0.03;
 };
 
 bool readRegion(Rip& r, Region& g) {
@@ -627,6 +629,11 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                     if (!r.load("Fixed", m.fixed)) m.fixed = 0;
                     if (!r.load("TurnOnCiliaAfterThisManySeconds", m.ciliaOn)) m.ciliaOn = 0.0;
                     if (!r.load("RemoveFromSimulationAfterThisManySeconds", m.removeAfter)) m.removeAfter = 0.0;
+                    if (!r.load("isPaceMaker", m.isPaceMaker)) m.isPaceMaker = 0;
+                    if (!r.load("PaceMakerPeriod", m.paceMakerPeriod)) m.paceMakerPeriod = 0;
+                    if (!r.load("signalValueDecay", m.signalValueDecay)) m.signalValueDecay = 0.9;
+                    if (!r.load("signalTimeDelay", m.signalTimeDelay)) m.signalTimeDelay = 0.03;
+                    if (!r.load("inactivePeriod", m.inactivePeriod)) m.inactivePeriod = 0.03;
                     if (!r.load("Cilia", m.cilia)) m.cilia = 0;
                     if (!r.load("Elastic_Mod", m.E)) m.E = 0;
                     if (!r.load("Plastic_Mod", m.plasticMod)) m.plasticMod = 0;
@@ -746,6 +753,11 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
         d.cilia = p.cilia;
         d.cilia_on_after = p.ciliaOn;
         d.remove_after = p.removeAfter;
+        d.is_pacemaker = p.isPaceMaker != 0;
+        d.pacemaker_period = p.paceMakerPeriod;
+        d.signal_value_decay = p.signalValueDecay;
+        d.signal_time_delay = p.signalTimeDelay;
+        d.inactive_period = p.inactivePeriod;
         d.matid = (int)i + 1;   // palette index; 0 is the erase material
         for (int k = 0; k < 4; k++) {   // GetRedi() etc. (VX_Object.h:419-425), then setRed's clamp
             const int c = (int)(p.rgba[k] * 255);
@@ -907,6 +919,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     d.enable_cilia = in.enable_cilia;
     d.enable_detach = in.enable_detach;
     d.secondary_experiment = in.secondary_experiment;
+    d.enable_signals = in.enable_signals;
     d.reinit_after = ptGet<double>(doc, S + "ReinitializeInitialPositionAfterThisManySeconds", 0.0);   // VX3_SimulationManager.cu:365
     d.base_cilia_force = o.baseCilia.empty() ? nullptr : o.baseCilia.data();
     d.shift_cilia_force = o.shiftCilia.empty() ? nullptr : o.shiftCilia.data();

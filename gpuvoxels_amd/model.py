@@ -41,6 +41,11 @@ class Material:
     cilia: float = 0.0                    # scale of the cilia force of voxels of this material
     cilia_on_after: float = 0.0           # TurnOnCiliaAfterThisManySeconds
     remove_after: float = 0.0             # RemoveFromSimulationAfterThisManySeconds (secondary experiment)
+    is_pacemaker: int = 0                 # signals: fires 100 every pacemaker_period
+    pacemaker_period: float = 0.0
+    signal_value_decay: float = 0.9
+    signal_time_delay: float = 0.03
+    inactive_period: float = 0.03
 
     def to_c(self) -> _abi.MaterialDesc:
         m = _abi.MaterialDesc()
@@ -68,6 +73,11 @@ class Material:
         m.cilia = self.cilia
         m.cilia_on_after = self.cilia_on_after
         m.remove_after = self.remove_after
+        m.is_pacemaker = self.is_pacemaker
+        m.pacemaker_period = self.pacemaker_period
+        m.signal_value_decay = self.signal_value_decay
+        m.signal_time_delay = self.signal_time_delay
+        m.inactive_period = self.inactive_period
         for k in range(4):   # CVXC_Material::GetRedi etc. then CVX_Material::setRed's clamp
             m.color[k] = min(255, max(0, int(np.float32(self.color[k]) * np.float32(255))))
         return m
@@ -159,6 +169,7 @@ class Model:
     enable_detach: int = 0                    # failed links leave their voxels (VX3 detach)
     secondary_experiment: int = 0             # timed voxel removal and initial-position reset
     reinit_after: float = 0.0                 # ReinitializeInitialPositionAfterThisManySeconds
+    enable_signals: int = 0                   # VX3 signals between linked voxels
     name: str = "sim"
 
     @property
@@ -239,4 +250,5 @@ class Model:
         d.enable_detach = self.enable_detach
         d.secondary_experiment = self.secondary_experiment
         d.reinit_after = self.reinit_after
+        d.enable_signals = self.enable_signals
         return d, keep

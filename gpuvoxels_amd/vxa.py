@@ -104,6 +104,8 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
     out.append("<SavePositionOfAllVoxels>%d</SavePositionOfAllVoxels>" % int(bool(m.save_position_of_all_voxels)))
     if m.enable_cilia:
         out.append("<EnableCilia>1</EnableCilia>")
+    if m.enable_signals:
+        out.append("<EnableSignals>1</EnableSignals>")
     if m.secondary_experiment:
         out.append("<SecondaryExperiment>1</SecondaryExperiment><ReinitializeInitialPositionAfterThisManySeconds>%s"
                    "</ReinitializeInitialPositionAfterThisManySeconds>" % _num(m.reinit_after))
@@ -151,7 +153,11 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                 "<Cilia>%s</Cilia>" % _num(mt.cilia),
                 "<TurnOnCiliaAfterThisManySeconds>%s</TurnOnCiliaAfterThisManySeconds>" % _num(mt.cilia_on_after),
                 "<RemoveFromSimulationAfterThisManySeconds>%s</RemoveFromSimulationAfterThisManySeconds>"
-                % _num(mt.remove_after)]
+                % _num(mt.remove_after),
+                "<isPaceMaker>%d</isPaceMaker><PaceMakerPeriod>%s</PaceMakerPeriod><signalValueDecay>%s</signalValueDecay>"
+                "<signalTimeDelay>%s</signalTimeDelay><inactivePeriod>%s</inactivePeriod>"
+                % (int(mt.is_pacemaker), _num(mt.pacemaker_period), _num(mt.signal_value_decay),
+                   _num(mt.signal_time_delay), _num(mt.inactive_period))]
         if mt.strain_data:
             mech.append("<SSData><NumDataPts>%d</NumDataPts><StrainData>%s</StrainData><StressData>%s</StressData></SSData>"
                         % (len(mt.strain_data), "".join("<Strain>%s</Strain>" % _num(e) for e in mt.strain_data),

@@ -90,6 +90,12 @@ typedef struct vx3h_material_desc {
     double cilia;               /* Cilia: scale of the voxel's cilia force (VX3_Voxel.cu:376) */
     double cilia_on_after;      /* TurnOnCiliaAfterThisManySeconds */
     double remove_after;        /* RemoveFromSimulationAfterThisManySeconds (secondary experiment) */
+    /* signals (VX3_Voxel.cu:279-348; VXA defaults VX_Object.cpp:1420-1425) */
+    int    is_pacemaker;        /* isPaceMaker [0]: fires a signal of 100 every pacemaker_period */
+    double pacemaker_period;    /* PaceMakerPeriod [0] */
+    double signal_value_decay;  /* signalValueDecay [0.9]: value passed on = value received * decay */
+    double signal_time_delay;   /* signalTimeDelay [0.03] s before a neighbour receives it */
+    double inactive_period;     /* inactivePeriod [0.03] s deaf after receiving */
 } vx3h_material_desc;
 
 /* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
@@ -171,6 +177,14 @@ typedef struct vx3h_sim_desc {
      * the current ones.  Not with collisions. */
     int    secondary_experiment;
     double reinit_after;
+    /* VX3 signals (Simulator.EnableSignals; VX3_Voxel.cu:271-348): after its update each
+     * moving voxel passes a pending signal to its linked neighbours, fires if it is a
+     * pacemaker, and lets its local signal decay; the local signal scales the cilia
+     * shift force.  The reference's voxels write their neighbours' state concurrently;
+     * here every voxel first updates its own state and posts what it sends, then
+     * every voxel takes what its neighbours posted, in link-direction order
+     * (X+, X-, Y+, Y-, Z+, Z-). */
+    int    enable_signals;
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */

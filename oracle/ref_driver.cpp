@@ -468,6 +468,9 @@ void vxo_link_state(vxo_sim* h, float* strain, float* maxStrain, float* off, int
         }
     }
 }
+void vxo_local_signal(vxo_sim* h, double* out) {   // the CPU engine has no signals
+    for (int i = 0; i < h->vx->voxelCount(); i++) out[i] = 0;
+}
 void vxo_voxcad_history(vxo_sim* h, double* k, double* t, double* m) {
     const VoxcadStop& c = h->voxcad;
     for (int i = 0; i < VoxcadStop::HIST_LEN; i++) { k[i] = c.KinEHistory[i]; t[i] = c.TotEHistory[i]; m[i] = c.MaxMoveHistory[i]; }

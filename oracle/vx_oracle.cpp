@@ -558,6 +558,44 @@ struct Voxel {
     // (VX3_Voxel.cu:178-184); used when the simulation has either
     bool extras = false;
     bool removed = false;   // VX3 secondary experiment: frozen, without links
+    // VX3 signals (VX3_Voxel.h:305-309): local signal and its decay time, next pacemaker
+    // pulse, deaf-until time, pending signal, and the signal posted this step
+    double localSignal = 0, localSignaldt = 0, nextPulse = 0, inactiveUntil = 0, pendingValue = 0, pendingTime = 0;
+    double postedValue = 0, postedTime = 0;
+    int isPaceMaker = 0;
+    double paceMakerPeriod = 0, signalValueDecay = 0.9, signalTimeDelay = 0.03, inactivePeriod = 0.03;
+    // VX3_Voxel::receiveSignal (VX3_Voxel.cu:301-322)
+    void receiveSignal(double value, double activeTime, bool force) {
+        if (!force && inactiveUntil > activeTime) return;
+        if (value < 0.1) return;
+        inactiveUntil = activeTime + inactivePeriod;
+        localSignal = value;
+        pendingValue = value * signalValueDecay;
+        if (pendingValue < 0.1) pendingValue = 0;
+        pendingTime = activeTime;
+    }
+    // the end of VX3_Voxel::timeStep (VX3_Voxel.cu:271-276) on this voxel's own state:
+    // propagateSignal posts the pending signal, packMaker, localSignalDecay
+    void signalsOwn(double t) {
+        postedValue = postedTime = 0;
+        if (!(pendingTime > t) && !(pendingValue < 0.1)) {
+            postedValue = pendingValue;
+            postedTime = t + signalTimeDelay;
+            pendingValue = pendingTime = 0;
+            inactiveUntil = t + 2 * signalTimeDelay + inactivePeriod;
+        }
+        if (isPaceMaker && !(nextPulse > t)) {
+            receiveSignal(100, t, true);
+            nextPulse = t + paceMakerPeriod;
+        }
+        if (!(localSignaldt > t)) {
+            if (localSignal < 0.1) localSignal = 0;
+            else {
+                localSignal = localSignal * 0.9;
+                localSignaldt = t + 0.01;
+            }
+        }
+    }
     Vd ciliaForce, fieldForce;
     Vd baseCilia, shiftCilia;
     int boolStates = 0;
@@ -1250,7 +1288,7 @@ struct Sim {
         }
     }
     // VX3 cilia and force field
-    bool extras = false, ciliaOn = false;
+    bool extras = false, ciliaOn = false, signals = false;
     std::vector<double> matCilia, matCiliaOn;   // per material (voxelMats order)
     std::vector<vx3h_token> ffTok[3];
 
@@ -1264,7 +1302,7 @@ struct Sim {
             v->ciliaForce = Vd(0, 0, 0);
             const int mi = matOf(v);   // voxel materials are in description order
             if (ciliaOn && v->isSurface() && !v->removed && matCilia[mi] != 0 && !(matCiliaOn[mi] > currentTime)) {
-                const double localSignal = 0.0;
+                const double localSignal = signals ? v->localSignal : 0.0;
                 v->ciliaForce = v->orient.RotateVec3D(v->baseCilia + v->shiftCilia * localSignal) * matCilia[mi];
             }
             double f[3];
@@ -1348,6 +1386,19 @@ struct Sim {
         }
         updateExtras();
         if (!vx.doTimeStep(dt)) return false;
+        if (signals) {   // every moving voxel on its own state, then every voxel takes its neighbours' posts
+            for (Voxel* v : vx.voxelsList)
+                if (!v->mat->fixed && !v->removed) v->signalsOwn(currentTime);
+            for (Voxel* v : vx.voxelsList) {
+                if (v->removed) continue;
+                for (int d = 0; d < 6; d++) {
+                    Link* l = v->links[d];
+                    if (!l) continue;
+                    Voxel* nb = l->pVNeg == v ? l->pVPos : l->pVNeg;
+                    if (nb->postedValue > 0) v->receiveSignal(nb->postedValue, nb->postedTime, false);
+                }
+            }
+        }
         int CycleStep = int(TempPeriod / dt);
         if (CycleStep > 0 && CurStepCount % CycleStep == 0) {
             angleSampleTimes++;
@@ -1473,7 +1524,8 @@ static Sim* build(const vx3h_sim_desc* d) {
         if (d->n_force_field_tokens[k] > 0) s->ffTok[k].assign(d->force_field_tokens[k], d->force_field_tokens[k] + d->n_force_field_tokens[k]);
     if (d->enable_cilia)
         for (int i = 0; i < d->n_materials; i++) s->ciliaOn |= d->materials[i].cilia != 0;
-    s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty();
+    s->signals = d->enable_signals != 0;
+    s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty() || s->signals;
     vx.detach = d->enable_detach != 0;
     s->secondary = d->secondary_experiment != 0;
     s->reinitAfter = d->reinit_after;
@@ -1486,6 +1538,12 @@ static Sim* build(const vx3h_sim_desc* d) {
         v->extras = s->extras;
         if (d->base_cilia_force) v->baseCilia = Vd(d->base_cilia_force[3 * i], d->base_cilia_force[3 * i + 1], d->base_cilia_force[3 * i + 2]);
         if (d->shift_cilia_force) v->shiftCilia = Vd(d->shift_cilia_force[3 * i], d->shift_cilia_force[3 * i + 1], d->shift_cilia_force[3 * i + 2]);
+        const vx3h_material_desc& md = d->materials[d->material[i]];
+        v->isPaceMaker = md.is_pacemaker != 0;
+        v->paceMakerPeriod = md.pacemaker_period;
+        v->signalValueDecay = md.signal_value_decay;
+        v->signalTimeDelay = md.signal_time_delay;
+        v->inactivePeriod = md.inactive_period;
     }
     s->voxcad.type = d->stop_condition_type;
     s->voxcad.value = d->stop_condition_value;
@@ -1651,6 +1709,9 @@ void vxo_link_state(vxo_sim* h, float* strain, float* maxStrain, float* off, int
             for (int k = 0; k < 4; k++) { f12[12 * i + 3 * k] = vs[k]->x; f12[12 * i + 3 * k + 1] = vs[k]->y; f12[12 * i + 3 * k + 2] = vs[k]->z; }
         }
     }
+}
+void vxo_local_signal(vxo_sim* h, double* out) {
+    for (size_t i = 0; i < h->s->vx.voxelsList.size(); i++) out[i] = h->s->vx.voxelsList[i]->localSignal;
 }
 void vxo_voxcad_history(vxo_sim* h, double* k, double* t, double* m) {
     const VoxcadStop& c = h->s->voxcad;

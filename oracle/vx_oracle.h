@@ -68,6 +68,8 @@ void vxo_link_state(vxo_sim* s, float* strain, float* max_strain, float* strain_
 void vxo_corner_offset(vxo_sim* s, int v, int corner, float* out3);
 /* VoxCAD stop-condition histories, 500 entries each, newest first (-1 = not computed) */
 void vxo_voxcad_history(vxo_sim* s, double* kinetic, double* total, double* max_move);
+/* VX3 local signal of each voxel (0 where the engine has no signals) */
+void vxo_local_signal(vxo_sim* s, double* out);
 int  vxo_collision_pairs(vxo_sim* s, int* pairs, int cap);        /* returns count */
 /* derived material constants, for the host builder's parity checks.
  * voxel mats: 14 floats each; link mats (list order): 20 floats each */

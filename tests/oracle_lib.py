@@ -47,6 +47,7 @@ def _bind(lib):
         "vxo_link_state": (None, [P, pF, pF, pF, pI, pI, pD, pD]),
         "vxo_corner_offset": (None, [P, I, I, pF]),
         "vxo_voxcad_history": (None, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "vxo_local_signal": (None, [P, C.POINTER(C.c_double)]),
         "vxo_collision_pairs": (I, [P, pI, I]),
         "vxo_voxel_material_consts": (I, [P, pF, I]),
         "vxo_link_material_consts": (I, [P, pI, pF, I]),
@@ -216,6 +217,11 @@ class OSim:
     def corner_offset(self, v, corner):
         out = np.zeros(3, np.float32)
         self.lib.vxo_corner_offset(self.h, v, corner, _p(out, C.c_float))
+        return out
+
+    def local_signal(self):
+        out = np.zeros(self.nv)
+        self.lib.vxo_local_signal(self.h, _p(out, C.c_double))
         return out
 
     def voxcad_history(self):

@@ -630,3 +630,73 @@ def test_initial_position_reset():
     d = np.linalg.norm(o.state()["pos"] - pos2, axis=1).sum()
     assert s3.total_distance_of_all_voxels == pytest.approx(d, rel=1e-12)
     assert s3.fitness == pytest.approx(s3.current_com[2] - sampled[2], rel=1e-12)
+
+
+# ---- VX3 signals (VX3_Voxel.cu:279-348)
+
+def _chain(n=6, period=10.0, **kw):
+    """A weightless row of n voxels along x; voxel 0 is a pacemaker."""
+    m = Model(voxel_size=0.01)
+    base = dict(elastic_mod=1e6, density=1e3, zeta_internal=1.0, zeta_global=0.0)
+    m.materials = [Material(is_pacemaker=1, pacemaker_period=period, **base, **kw), Material(**base, **kw)]
+    m.ijk = np.stack([np.arange(n), np.zeros(n), np.zeros(n)], axis=1).astype(np.int32)
+    m.mat = np.array([0] + [1] * (n - 1), np.int32)
+    m.enable_signals = 1
+    return m
+
+
+def test_signal_travels_down_a_chain():
+    # a pulse of 100 starts at the pacemaker; a receiver shows the value at once but
+    # passes it on only signalTimeDelay (0.03 s, to within a step) later, 0.9 of it; the
+    # refractory period keeps it from echoing back
+    o = OSim(_chain())
+    first, value = {}, {}
+    times = []
+    for n in range(1, 4500):
+        times.append(o.status().current_time)
+        o.step(1)
+        sig = o.local_signal()
+        for k in range(6):
+            if k not in first and sig[k] > 0:
+                first[k], value[k] = n, sig[k]
+    dt = float(np.float32(o.status().dt))
+    assert value[0] == pytest.approx(90.0)                   # 100, decayed once in the same step
+    for k in range(1, 6):
+        assert value[k] == pytest.approx(100 * 0.9 ** k, rel=1e-12)
+    assert first[1] == 2                                       # the pacemaker posts at step 2
+    for k in range(2, 6):
+        hop = (first[k] - first[k - 1]) * dt
+        assert 0.03 <= hop + 1e-12 and hop <= 0.03 + 2 * dt + 1e-12
+
+
+def test_pacemaker_period_and_decay():
+    # a lone pacemaker fires every period; in between its local signal decays by 0.9 every 0.01 s
+    m = _chain(n=1, period=0.05)
+    o = OSim(m)
+    sig, t0 = [], []
+    for _ in range(2500):
+        t0.append(o.status().current_time)
+        o.step(1)
+        sig.append(o.local_signal()[0])
+    sig = np.array(sig)
+    fires = [i for i in range(1, len(sig)) if sig[i] > sig[i - 1]]
+    gaps = np.diff([t0[i] for i in [0] + fires])
+    assert len(fires) >= 2 and np.all(gaps >= 0.05 - 1e-12) and np.all(gaps <= 0.05 + 2e-4)
+    assert sig.min() > 0 and sig.max() == 100.0
+    assert np.all(np.isin(np.round(sig[1:fires[0]], 9), np.round(90 * 0.9 ** np.arange(8), 9)))   # 0.9 per 0.01 s
+
+
+def test_signal_scales_the_cilia_shift_force():
+    # the cilia force is RotateVec3D(base + localSignal*shift)*Cilia with the signal as
+    # of the step's start
+    m = _chain(n=1, period=0.05, cilia=2.0)
+    m.enable_cilia = 1
+    m.base_cilia = np.array([[0.0, 0.0, 0.0]])
+    m.shift_cilia = np.array([[1e-5, 0.0, 0.0]])
+    o = OSim(m)
+    used = 0.0
+    for _ in range(400):
+        used += o.local_signal()[0]
+        o.step(1)
+    dt = float(np.float32(o.status().dt))
+    assert o.state()["lin_mom"][0][0] == pytest.approx(used * 1e-5 * 2.0 * dt, rel=1e-9) and used > 0
