@@ -662,6 +662,12 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         }
         if (shape < 0 && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "simulation too large for the resident kernel");
     }
+    if (shape < 0 && NV > 0 && b->cfg.path == VX3H_PATH_RESIDENT) {   // a forced resident path takes only what fits
+        if (H.collisions) FAILV(VX3H_EUNSUPPORTED, "collisions run on the stream path");
+        if (!resOk && (int)H.vmats.size() > RES_MAX_VMATS) FAILV(VX3H_EUNSUPPORTED, "too many voxel materials for the resident kernel");
+        if (!resOk && (int)H.lmats.size() > RES_MAX_LMATS) FAILV(VX3H_EUNSUPPORTED, "too many link materials for the resident kernel");
+        if (!resOk && nPhases > RES_MAX_PHASES) FAILV(VX3H_EUNSUPPORTED, "too many distinct phase offsets for the resident kernel");
+    }
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");

@@ -225,7 +225,8 @@ vx3h_status vx3h_vxa_get_info(const vx3h_vxa* v, vx3h_vxa_info* info, const char
 /* ---------------------------------------------------------------- batch engine */
 enum {
     VX3H_PATH_STREAM = 0,    /* state in HBM; link pass + voxel pass per step over every resident sim */
-    VX3H_PATH_RESIDENT = 1,  /* one workgroup per sim, state held in LDS/registers across many steps */
+    VX3H_PATH_RESIDENT = 1,  /* one workgroup per sim, state held in LDS/registers across many steps;
+                                admitting a sim it cannot take fails with VX3H_EUNSUPPORTED */
     VX3H_PATH_AUTO = 2       /* resident where the sim fits, stream otherwise */
 };
 enum { VX3H_PREC_F32 = 0, VX3H_PREC_F64 = 1 };
