@@ -22,6 +22,20 @@
  *   VX3 driver           Voxelyze3/src/VX3/VX3_VoxelyzeKernel.cu:184-359,477-534,625-650,
  *                        VX3_SimulationManager.cu:52-117
  *   MathTree             Voxelyze3/src/Utils/VX3_MathTree.h:50-192
+ *   VX3 extras           cilia VX3_VoxelyzeKernel.cu:846-859, VX3_Voxel.cu:376; force field
+ *                        VX3_Voxel.cu:178-184; detach VX3_VoxelyzeKernel.cu:946-967; voxel
+ *                        removal and reset :336-350,365-397; signals VX3_Voxel.cu:271-348
+ *   VoxCAD stop family   vx_voxcad_stop.h (VoxCAD/src/VX1/VX_Sim.cpp:862-891,1015-1160,1833-1837)
+ *
+ * How each part is pinned: the CPU engine and importer bit-exactly against the
+ * reference's own sources compiled in place (oracle/_ref); the VX3 driver and the
+ * VX3 extras, which exist only as GPU code in the reference, by closed forms and
+ * brute-force restatements in tests/test_oracle_kat.py (phase temperature, dt, CoM,
+ * fitness, stop counts; cilia momentum, rotation and surface rule; force fields;
+ * detach; removal and reset; signal hops, pacemaker period and decay, cilia
+ * coupling).  Signals follow a deterministic two-phase order where the reference's
+ * threads race (VX3_Voxel.cu:323-348): that order is this oracle's definition, so
+ * signal parity with the reference is unpinned beyond the single-hop behaviour.
  */
 #include "vx_oracle.h"
 
