@@ -50,6 +50,7 @@ class MaterialDesc(C.Structure):
         ("signal_value_decay", C.c_double),
         ("signal_time_delay", C.c_double),
         ("inactive_period", C.c_double),
+        ("is_target", C.c_int),
     ]
 
 
@@ -141,6 +142,7 @@ class SimStatus(C.Structure):
         ("total_distance_of_all_voxels", C.c_double),
         ("recent_angle", C.c_double),
         ("recommended_dt", C.c_double),
+        ("collision_count", C.c_int),
     ]
 
 

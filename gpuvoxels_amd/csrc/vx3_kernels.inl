@@ -596,28 +596,32 @@ __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, doubl
     return q[pc < VX3_DEV_MT_MAX ? pc : 0];
 }
 
-// Start of a step for the simulations with VX3 detach, cilia or a force field.
-// Detach: a voxel's surface state follows its link mask as of the previous step's
-// end (regenerateSurfaceVoxels runs at the next step, VX3_VoxelyzeKernel.cu:281-285).
-// Extra global forces for the voxel pass, from the state before this step's update:
+// Step passes of the simulations with VX3 detach, removal, cilia or a force field.
+// forces == 0, at the step's start: a voxel's surface state follows its link mask as
+// of the previous step's end (regenerateSurfaceVoxels, VX3_VoxelyzeKernel.cu:281-285).
+// forces == 1, before the voxel pass: extra global forces, from the state before this
+// step's update:
 // cilia as gpu_update_cilia_force (VX3_VoxelyzeKernel.cu:846-859): a surface voxel
 // whose material has Cilia != 0 and whose switch-on time has passed is pushed by
 // orient.RotateVec3D(base + localSignal*shift) * Cilia, localSignal being 0 (signals
 // are not simulated); the force field's three programs of the voxel position at the
 // step's start time (VX3_Voxel.cu:178-184; hit, closeness and close pairs are 0).
-__global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
+__global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
     if (!simRunning(b.sims, s)) return;
     const SimCtl& sc = b.sims[s];
-    if (sc.flags & (SC_DETACH | SC_SECONDARY)) {   // CVX_Voxel::updateSurface: interior iff all six links remain
+    if (!forces) {   // before the link pass: CVX_Voxel::updateSurface, interior iff all six links remain
+        if (!(sc.flags & (SC_DETACH | SC_SECONDARY))) return;
         const int fl = b.vflags[v];
         const int all = 0x3F << VF_LINKMASK_SHIFT;
         const int nf = (fl & all) == all ? (fl | VF_INTERIOR) : (fl & ~VF_INTERIOR);
         if (nf != fl) b.vflags[v] = nf;
+        return;
     }
-    if (!(sc.flags & SC_EXTRAS)) return;
+    // after the link and contact passes (gpu_update_cilia_force runs after updateAttach)
+    if (!(sc.flags & SC_EXTRAS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
     const size_t V = (size_t)b.Vcap;
     V3<real> c(0, 0, 0);
     if (sc.flags & SC_CILIA) {
@@ -634,10 +638,44 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b) {
     double q[VX3_DEV_MT_MAX + 1];
     for (int k = 0; k < 3; k++)
         if (sc.ffLen[k] > 0)
-            ff[k] = mathTreeEval(b.tokens + sc.ffOff[k], sc.ffLen[k], (double)b.px[v], (double)b.py[v], (double)b.pz[v], 0, sc.t,
-                                 sc.recentAngle, 0, 0, sc.nv, q);
+            ff[k] = mathTreeEval(b.tokens + sc.ffOff[k], sc.ffLen[k], (double)b.px[v], (double)b.py[v], (double)b.pz[v],
+                                 (double)sc.hits, sc.t, sc.recentAngle, 0, 0, sc.nv, q);
     b.xf[0 * V + v] = c.x; b.xf[1 * V + v] = c.y; b.xf[2 * V + v] = c.z;
     b.xf[3 * V + v] = (real)ff[0]; b.xf[4 * V + v] = (real)ff[1]; b.xf[5 * V + v] = (real)ff[2];
+}
+
+// Target contacts (handle_collision_attachment, VX3_VoxelyzeKernel.cu:682-722): each
+// watched pair of a target and another voxel, not both material-fixed, whose centres
+// are within 0.625 x the sum of their average base sizes counts one hit per step, seen
+// from the non-target voxel, which with signals receives 100 at once.  Positions and
+// thermal factors from the contact snapshot (this step's start).
+__global__ void __launch_bounds__(256) k_hits(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!simRunning(b.sims, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_TARGETS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+    const int fl = b.vflags[v];
+    if (fl & (VF_INTERIOR | VF_TARGET)) return;
+    const VoxMatDev& mv = b.vmatTab[b.vmat[v]];
+    const double nom = sc.voxSize;
+    const double bv = nom * (double)b.ctf[v];
+    int n = 0;
+    for (int k = 0; k < b.nbCount[v]; k++) {
+        const int u = b.nbList[b.nbStart[v] + k];
+        if (!(b.vflags[u] & VF_TARGET)) continue;
+        if (mv.fixed && b.vmatTab[b.vmat[u]].fixed) continue;
+        const double bu = nom * (double)b.ctf[u];
+        const double wd = ((bv + bv + bv) / 3.0f + (bu + bu + bu) / 3.0f) * 0.625;   // baseSizeAverage, COLLISION_ENVELOPE_RADIUS
+        const double dx = (double)b.cpx[v] - (double)b.cpx[u], dy = (double)b.cpy[v] - (double)b.cpy[u],
+                     dz = (double)b.cpz[v] - (double)b.cpz[u];
+        if (dx > wd || dx < -wd || dy > wd || dy < -wd || dz > wd || dz < -wd) continue;
+        if (sqrt(dx * dx + dy * dy + dz * dz) > wd) continue;
+        n++;
+        if (sc.flags & SC_SIGNALS) receiveSignal(b, (size_t)b.Vcap, v, 100, sc.t, true);
+    }
+    if (n) atomicAdd(&b.sims[s].hits, n);
 }
 
 // Signals, second half: every voxel takes the signals its linked neighbours posted this
@@ -894,7 +932,8 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     sc.t += (double)sc.dt;
     if (sc.maxSteps > 0 && steps >= sc.maxSteps) { sc.state = 1; return; }
     if (sc.stopLen > 0 &&
-        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], 0, sc.t, sc.recentAngle, 0, 0, sc.nv, sQ) > 0)
+        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], (double)sc.hits, sc.t, sc.recentAngle, 0, 0,
+                     sc.nv, sQ) > 0)
         sc.state = 1;
     else if (sc.scType && voxcadStopMet(b, sc, s, steps))
         sc.state = 1;

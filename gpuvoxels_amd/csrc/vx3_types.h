@@ -27,6 +27,7 @@ enum {
     VF_STATIC = 1 << 3,              // floor static friction
     VF_REMOVED = 1 << 4,             // removed by a secondary experiment: frozen, no links
     VF_COLL = 1 << 5,
+    VF_TARGET = 1 << 6,              // material isTarget: contacts with other materials count as hits
     VF_LINKMASK_SHIFT = 8,           // bits 8..13: link present in direction d
 };
 // link state bits
@@ -40,7 +41,8 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_CILIA = 1024,
        SC_DETACH = 2048,        // failed links detach
        SC_SECONDARY = 4096,     // secondary experiment: timed voxel removal, initial-position reset
-       SC_SIGNALS = 8192 };     // signals between linked voxels (runs with SC_EXTRAS)
+       SC_SIGNALS = 8192,       // signals between linked voxels (runs with SC_EXTRAS)
+       SC_TARGETS = 16384 };    // target contacts are counted (runs with SC_EXTRAS)
 #define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
@@ -116,6 +118,7 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     // VX3 secondary experiment: the initial-position reset (removal times are per voxel)
     double reinitAfter;
     int reinitDone;
+    int hits;                        // collisionCount: target contacts so far (MathTree variable hit)
 };
 
 #define VX3_HIST 500                 // HISTORY_SIZE (VoxCAD/src/VX1/VX_Sim.h)

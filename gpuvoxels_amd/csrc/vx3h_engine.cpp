@@ -329,7 +329,7 @@ vx3h_status launchSteps(vx3h_batch* b, int n) {
     StepOpts o;
     o.contacts = b->anyColl;
     o.extras = b->anyExtras;
-    o.prestep = b->anyExtras || b->anyDetach;
+    o.prestep = b->anyDetach;
     o.signals = b->anySignals;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
@@ -601,7 +601,11 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         if ((st = copyReal(b, at(d.ow, rs, v0), one))) return st;
         std::vector<int> vm(NV), vs(NV, s), ve(NV);
         for (int i = 0; i < NV; i++) { vm[i] = vmIdx[H.vmat[i]]; ve[i] = H.vext[i] >= 0 ? ext0 + H.vext[i] : -1; }
-        HIPCHK(hipMemcpy(d.vflags + v0, H.vflags.data(), NV * 4, hipMemcpyHostToDevice));
+        std::vector<int> vf(H.vflags);
+        if (H.collisions)   // target materials (handle_collision_attachment counts their contacts)
+            for (int i = 0; i < NV; i++)
+                if (desc->materials[desc->material[i]].is_target) vf[i] |= VF_TARGET;
+        HIPCHK(hipMemcpy(d.vflags + v0, vf.data(), NV * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d.tfac + v0, H.tfac0.data(), NV * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d.phase + v0, H.phase.data(), NV * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d.vmat + v0, vm.data(), NV * 4, hipMemcpyHostToDevice));
@@ -703,6 +707,13 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (H.extras()) { c.flags |= SC_EXTRAS; b->anyExtras = true; }
     if (!H.cilia.empty()) c.flags |= SC_CILIA;
     if (desc->enable_detach) { c.flags |= SC_DETACH; b->anyDetach = true; }
+    bool targets = false;
+    if (H.collisions)
+        for (int i = 0; i < desc->n_materials; i++) targets |= desc->materials[i].is_target != 0;
+    if (targets) {
+        c.flags |= SC_TARGETS | SC_EXTRAS;
+        b->anyExtras = true;
+    }
     if (desc->enable_signals && NV) {
         c.flags |= SC_SIGNALS | SC_EXTRAS;
         b->anyExtras = b->anySignals = true;
@@ -903,7 +914,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     StepOpts o;
     o.contacts = b->anyColl;
     o.extras = b->anyExtras;
-    o.prestep = b->anyExtras || b->anyDetach;
+    o.prestep = b->anyDetach;
     o.signals = b->anySignals;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
@@ -1021,6 +1032,7 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
     o.current_time = c.t;
     o.dt = c.dt;
     o.recommended_dt = c.optDt;
+    o.collision_count = c.hits;
     o.recent_angle = c.recentAngle;
     // initial center of mass and positions: the admission's, or a secondary experiment's reset
     for (int k = 0; k < 3; k++) o.initial_com[k] = c.com0[k];
@@ -1056,7 +1068,7 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
     o.total_distance_of_all_voxels = dist;
     double off[3] = {o.current_com[0] - c.com0[0], o.current_com[1] - c.com0[1], o.current_com[2] - c.com0[2]};
     // computeFitness (VX3_VoxelyzeKernel.cu:530-534)
-    o.fitness = evalMathTree(sh.fit.data(), (int)sh.fit.size(), off[0], off[1], off[2], 0, c.t, c.recentAngle, 0, 0, sh.nv);
+    o.fitness = evalMathTree(sh.fit.data(), (int)sh.fit.size(), off[0], off[1], off[2], c.hits, c.t, c.recentAngle, 0, 0, sh.nv);
     if (posOut) {
         posOut->resize(3 * sh.nv);
         for (int i = 0; i < sh.nv; i++) { (*posOut)[3 * i] = x[i]; (*posOut)[3 * i + 1] = y[i]; (*posOut)[3 * i + 2] = z[i]; }

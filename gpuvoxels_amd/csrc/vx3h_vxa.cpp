@@ -338,6 +338,7 @@ struct PalMat {                 // CVXC_Material (VX_Object.cpp:1373-1460), SING
     double removeAfter = 0;                     // RemoveFromSimulationAfterThisManySeconds
     int isPaceMaker = 0;                        // signals (VX_Object.cpp:1420-1425)
     double paceMakerPeriod = 0, signalValueDecay = 0.9, signalTimeDelay = 0.03, inactivePeriod = 0.03;
+    int isTarget = 0;
 };
 
 bool readRegion(Rip& r, Region& g) {
@@ -634,6 +635,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                     if (!r.load("signalValueDecay", m.signalValueDecay)) m.signalValueDecay = 0.9;
                     if (!r.load("signalTimeDelay", m.signalTimeDelay)) m.signalTimeDelay = 0.03;
                     if (!r.load("inactivePeriod", m.inactivePeriod)) m.inactivePeriod = 0.03;
+                    if (!r.load("isTarget", m.isTarget)) m.isTarget = 0;
                     if (!r.load("Cilia", m.cilia)) m.cilia = 0;
                     if (!r.load("Elastic_Mod", m.E)) m.E = 0;
                     if (!r.load("Plastic_Mod", m.plasticMod)) m.plasticMod = 0;
@@ -758,6 +760,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
         d.signal_value_decay = p.signalValueDecay;
         d.signal_time_delay = p.signalTimeDelay;
         d.inactive_period = p.inactivePeriod;
+        d.is_target = p.isTarget != 0;
         d.matid = (int)i + 1;   // palette index; 0 is the erase material
         for (int k = 0; k < 4; k++) {   // GetRedi() etc. (VX_Object.h:419-425), then setRed's clamp
             const int c = (int)(p.rgba[k] * 255);

@@ -46,6 +46,7 @@ class Material:
     signal_value_decay: float = 0.9
     signal_time_delay: float = 0.03
     inactive_period: float = 0.03
+    is_target: int = 0                    # contacts with other materials count as hits
 
     def to_c(self) -> _abi.MaterialDesc:
         m = _abi.MaterialDesc()
@@ -78,6 +79,7 @@ class Material:
         m.signal_value_decay = self.signal_value_decay
         m.signal_time_delay = self.signal_time_delay
         m.inactive_period = self.inactive_period
+        m.is_target = self.is_target
         for k in range(4):   # CVXC_Material::GetRedi etc. then CVX_Material::setRed's clamp
             m.color[k] = min(255, max(0, int(np.float32(self.color[k]) * np.float32(255))))
         return m

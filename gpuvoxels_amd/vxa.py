@@ -155,9 +155,9 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                 "<RemoveFromSimulationAfterThisManySeconds>%s</RemoveFromSimulationAfterThisManySeconds>"
                 % _num(mt.remove_after),
                 "<isPaceMaker>%d</isPaceMaker><PaceMakerPeriod>%s</PaceMakerPeriod><signalValueDecay>%s</signalValueDecay>"
-                "<signalTimeDelay>%s</signalTimeDelay><inactivePeriod>%s</inactivePeriod>"
+                "<signalTimeDelay>%s</signalTimeDelay><inactivePeriod>%s</inactivePeriod><isTarget>%d</isTarget>"
                 % (int(mt.is_pacemaker), _num(mt.pacemaker_period), _num(mt.signal_value_decay),
-                   _num(mt.signal_time_delay), _num(mt.inactive_period))]
+                   _num(mt.signal_time_delay), _num(mt.inactive_period), int(mt.is_target))]
         if mt.strain_data:
             mech.append("<SSData><NumDataPts>%d</NumDataPts><StrainData>%s</StrainData><StressData>%s</StressData></SSData>"
                         % (len(mt.strain_data), "".join("<Strain>%s</Strain>" % _num(e) for e in mt.strain_data),

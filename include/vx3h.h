@@ -96,6 +96,8 @@ typedef struct vx3h_material_desc {
     double signal_value_decay;  /* signalValueDecay [0.9]: value passed on = value received * decay */
     double signal_time_delay;   /* signalTimeDelay [0.03] s before a neighbour receives it */
     double inactive_period;     /* inactivePeriod [0.03] s deaf after receiving */
+    int    is_target;           /* isTarget [0]: contacts with other materials count as hits
+                                   (VX3_VoxelyzeKernel.cu:711-722) */
 } vx3h_material_desc;
 
 /* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
@@ -185,6 +187,13 @@ typedef struct vx3h_sim_desc {
      * every voxel takes what its neighbours posted, in link-direction order
      * (X+, X-, Y+, Y-, Z+, Z-). */
     int    enable_signals;
+    /* With collisions, a surface voxel pair of one target and one other material whose
+     * centres are within 0.625 x the sum of their sizes counts one hit per step
+     * (collisionCount, the MathTree variable hit) and, with signals, sends 100 to the
+     * non-target voxel (handle_collision_attachment, VX3_VoxelyzeKernel.cu:682-722).
+     * Pairs come from this engine's contact lists (the watch radius, 2.5 voxels by
+     * default, covers the 1.25-voxel envelope; voxels within the nearby link-hop
+     * radius of each other are not paired). */
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */
@@ -262,6 +271,7 @@ typedef struct vx3h_sim_status {
     double recent_angle;
     double recommended_dt;        /* optimal step before DtFrac (recommendedTimeStep, VX3_VoxelyzeKernel.cu:184-217);
                                      0 until the first step */
+    int    collision_count;       /* target contacts so far (collisionCount, the MathTree variable hit) */
 } vx3h_sim_status;
 
 vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out);

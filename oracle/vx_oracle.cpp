@@ -49,6 +49,7 @@
 #include <math.h>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <list>
 #include <map>
 #include <string>
@@ -572,6 +573,7 @@ struct Voxel {
     // (VX3_Voxel.cu:178-184); used when the simulation has either
     bool extras = false;
     bool removed = false;   // VX3 secondary experiment: frozen, without links
+    bool target = false;    // material isTarget
     // VX3 signals (VX3_Voxel.h:305-309): local signal and its decay time, next pacemaker
     // pulse, deaf-until time, pending signal, and the signal posted this step
     double localSignal = 0, localSignaldt = 0, nextPulse = 0, inactiveUntil = 0, pendingValue = 0, pendingTime = 0;
@@ -1013,6 +1015,7 @@ struct Engine {
     float currentTime = 0, ambientTemp = 0, grav = 0;
     bool floor = false, collisions = false;
     bool detach = false, surfaceChanged = false;   // VX3 detach of failed links
+    std::function<void()> afterContacts;           // VX3 target contacts, between the contact and voxel updates
     float boundingRadius = 0.75f, watchDistance = 1.0f;
     std::vector<MaterialVoxel*> voxelMats;
     std::list<MaterialLink*> linkMats;
@@ -1246,6 +1249,7 @@ struct Engine {
                 surfaceChanged = true;
             }
         if (collisions) updateCollisions();
+        if (afterContacts) afterContacts();
         for (Voxel* v : voxelsList) {
             if (v->mat->fixed) continue;
             if (v->removed) continue;   // VX3_VoxelyzeKernel.cu:586
@@ -1273,6 +1277,24 @@ struct Sim {
     std::vector<vx3h_token> stopTok, fitTok;
     float lastDt = 0;
     VoxcadStop voxcad;   // VoxCAD stop-condition family and equilibrium mode
+    int hits = 0;        // collisionCount (the MathTree variable hit)
+
+    // handle_collision_attachment (VX3_VoxelyzeKernel.cu:682-722) over the watched pairs:
+    // a target and another voxel, not both fixed, with centres within 0.625 x the sum
+    // of their average base sizes, count a hit; with signals the other voxel receives 100
+    void targetContacts() {
+        for (Collision* c : vx.collisionsList) {
+            Voxel *a = c->pV1, *b = c->pV2;
+            if (a->target == b->target) continue;
+            if (a->mat->fixed && b->mat->fixed) continue;
+            const double wd = (a->baseSizeAverage() + b->baseSizeAverage()) * 0.625;
+            const Vd d = a->pos - b->pos;
+            if (d.x > wd || d.x < -wd || d.y > wd || d.y < -wd || d.z > wd || d.z < -wd) continue;
+            if (d.Length() > wd) continue;
+            hits++;
+            if (signals) (a->target ? b : a)->receiveSignal(100, currentTime, true);
+        }
+    }
     // VX3 secondary experiment (VX3_VoxelyzeKernel.cu:336-350)
     bool secondary = false, reinitDone = false;
     double reinitAfter = 0;
@@ -1322,7 +1344,7 @@ struct Sim {
             double f[3];
             for (int k = 0; k < 3; k++)
                 f[k] = ffTok[k].empty() ? 0.0
-                                        : mathtree_eval(ffTok[k].data(), (int)ffTok[k].size(), v->pos.x, v->pos.y, v->pos.z, 0,
+                                        : mathtree_eval(ffTok[k].data(), (int)ffTok[k].size(), v->pos.x, v->pos.y, v->pos.z, hits,
                                                         currentTime, recentAngle, 0, 0, nv);
             v->fieldForce = Vd(f[0], f[1], f[2]);
         }
@@ -1377,7 +1399,7 @@ struct Sim {
     bool stopConditionMet() {
         if (maxSteps > 0 && CurStepCount >= maxSteps) return true;
         if (!stopTok.empty() &&
-            mathtree_eval(stopTok.data(), (int)stopTok.size(), currentCoM.x, currentCoM.y, currentCoM.z, 0, currentTime, recentAngle, 0, 0,
+            mathtree_eval(stopTok.data(), (int)stopTok.size(), currentCoM.x, currentCoM.y, currentCoM.z, hits, currentTime, recentAngle, 0, 0,
                           (int)vx.voxelsList.size()) > 0)
             return true;
         return voxcad.type != 0 && voxcad.met(CurStepCount, currentTime, TempPeriod, (int)vx.voxelsList.size());
@@ -1398,8 +1420,7 @@ struct Sim {
             vx.surfaceChanged = false;
             for (Voxel* v : vx.voxelsList) v->updateSurface();
         }
-        updateExtras();
-        if (!vx.doTimeStep(dt)) return false;
+        if (!vx.doTimeStep(dt)) return false;   // target contacts and extra forces run inside, after the contact update
         if (signals) {   // every moving voxel on its own state, then every voxel takes its neighbours' posts
             for (Voxel* v : vx.voxelsList)
                 if (!v->mat->fixed && !v->removed) v->signalsOwn(currentTime);
@@ -1539,7 +1560,17 @@ static Sim* build(const vx3h_sim_desc* d) {
     if (d->enable_cilia)
         for (int i = 0; i < d->n_materials; i++) s->ciliaOn |= d->materials[i].cilia != 0;
     s->signals = d->enable_signals != 0;
-    s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty() || s->signals;
+    bool anyTarget = false;
+    if (d->collisions_enabled)
+        for (size_t i = 0; i < vx.voxelsList.size(); i++)
+            if (d->materials[d->material[i]].is_target) { vx.voxelsList[i]->target = true; anyTarget = true; }
+    // VX3 order (VX3_VoxelyzeKernel.cu:281-303): links, then updateAttach (target contacts), then
+    // the cilia force, then the voxels (with the force field)
+    vx.afterContacts = [s, anyTarget] {
+        if (anyTarget) s->targetContacts();
+        s->updateExtras();
+    };
+    s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty() || s->signals || anyTarget;
     vx.detach = d->enable_detach != 0;
     s->secondary = d->secondary_experiment != 0;
     s->reinitAfter = d->reinit_after;
@@ -1679,9 +1710,10 @@ void vxo_vx3_status(vxo_sim* h, vx3h_sim_status* st) {
     st->current_time = s->currentTime;
     st->dt = s->lastDt;
     st->recommended_dt = s->OptimalDt;
+    st->collision_count = s->hits;
     st->initial_com[0] = s->initialCoM.x; st->initial_com[1] = s->initialCoM.y; st->initial_com[2] = s->initialCoM.z;
     st->current_com[0] = s->currentCoM.x; st->current_com[1] = s->currentCoM.y; st->current_com[2] = s->currentCoM.z;
-    st->fitness = mathtree_eval(s->fitTok.data(), (int)s->fitTok.size(), off.x, off.y, off.z, 0, s->currentTime, s->recentAngle, 0, 0,
+    st->fitness = mathtree_eval(s->fitTok.data(), (int)s->fitTok.size(), off.x, off.y, off.z, s->hits, s->currentTime, s->recentAngle, 0, 0,
                                 (int)s->vx.voxelsList.size());
     st->recent_angle = s->recentAngle;
     int nm = 0;

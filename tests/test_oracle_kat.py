@@ -700,3 +700,45 @@ def test_signal_scales_the_cilia_shift_force():
         o.step(1)
     dt = float(np.float32(o.status().dt))
     assert o.state()["lin_mom"][0][0] == pytest.approx(used * 1e-5 * 2.0 * dt, rel=1e-9) and used > 0
+
+
+# ---- VX3 target contacts (handle_collision_attachment, VX3_VoxelyzeKernel.cu:682-722)
+
+def _target_drop():
+    from gpuvoxels_amd import workloads
+    m = workloads.drop(3, gap=1, shift=1, steps=0)
+    m.materials = [workloads.base_material(cte=0.0), workloads.base_material(cte=0.0, is_target=1)]
+    m.mat = (m.ijk[:, 2] >= 3).astype(np.int32)      # the upper cube is the target
+    return m
+
+
+def test_target_hits_brute_force():
+    # each step counts the watched pairs of one target and one other voxel whose
+    # centres are within 0.625 x (size_a + size_b) at the step's start
+    m = _target_drop()
+    target = m.mat == 1
+    o = OSim(m)
+    o.step(2000)
+    checked = 0
+    for _ in range(500):
+        pos, pairs, h0 = o.state()["pos"].copy(), o.collision_pairs().copy(), o.status().collision_count
+        o.step(1)
+        if not np.array_equal(pairs, o.collision_pairs()):
+            continue          # the watch list was rebuilt during the step
+        want = sum(1 for a, b in pairs if target[a] != target[b]
+                   and np.linalg.norm(pos[a] - pos[b]) <= (0.01 + 0.01) * 0.625)
+        assert o.status().collision_count - h0 == want
+        checked += want > 0
+    assert checked > 50
+
+
+def test_target_hit_sends_a_signal_and_feeds_the_fitness():
+    m = _target_drop()
+    m.enable_signals = 1
+    m.fitness_tokens = [(MT["mtVAR"], 3.0), (MT["mtEND"], 0.0)]     # hit
+    o = OSim(m)
+    o.step(2600)
+    st = o.status()
+    assert st.collision_count > 0 and st.fitness == st.collision_count
+    sig = o.local_signal()
+    assert sig[m.mat == 0].max() > 0 and sig[m.mat == 1].max() == 0   # only non-target voxels receive
