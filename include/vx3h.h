@@ -155,7 +155,7 @@ typedef struct vx3h_sim_desc {
     /* VX3 cilia (Simulator.EnableCilia; VX3_VoxelyzeKernel.cu:296-303,846-859): each
      * surface voxel whose material has Cilia != 0, from TurnOnCiliaAfterThisManySeconds
      * on, is pushed by orient.RotateVec3D(base + localSignal*shift) * Cilia, with
-     * localSignal 0 (signals are not simulated).  The arrays hold 3 per voxel
+     * localSignal the voxel's signal (0 unless enable_signals).  The arrays hold 3 per voxel
      * (Structure.BaseCiliaForce / ShiftCiliaForce) or are NULL for zeros. */
     int    enable_cilia;
     const double* base_cilia_force;
