@@ -51,6 +51,7 @@ class MaterialDesc(C.Structure):
         ("signal_time_delay", C.c_double),
         ("inactive_period", C.c_double),
         ("is_target", C.c_int),
+        ("sticky", C.c_int),
     ]
 
 
@@ -112,6 +113,10 @@ class SimDesc(C.Structure):
         ("secondary_experiment", C.c_int),
         ("reinit_after", C.c_double),
         ("enable_signals", C.c_int),
+        ("enable_attach", C.c_int),
+        ("n_attach_tokens", C.c_int * 5),
+        ("attach_tokens", C.POINTER(Token) * 5),
+        ("safety_guard", C.c_int),
     ]
 
 
@@ -143,6 +148,7 @@ class SimStatus(C.Structure):
         ("recent_angle", C.c_double),
         ("recommended_dt", C.c_double),
         ("collision_count", C.c_int),
+        ("n_links", C.c_int),
     ]
 
 
@@ -158,6 +164,7 @@ class StateView(C.Structure):
         ("link_max_strain", C.POINTER(C.c_double)),
         ("link_small_angle", C.POINTER(C.c_int)),
         ("init_pos", C.POINTER(C.c_double)),
+        ("link_capacity", C.c_int),
     ]
 
 

@@ -39,6 +39,7 @@ template <typename T> struct DevBatch {
     T *a1x, *a1y, *a1z;       // previous angle1v
     T *a2x, *a2y, *a2z;       // previous angle2v
     float *strain, *maxStrain, *strainOffset;
+    int* lnew;                // steps left in which a link made by attach carries 1/100 of its force
     // collisions: per voxel the watched partners in ascending index order (both
     // directions), the order CVX_Voxel::force walks its colWatch list
     float* lastWatch;         // 3*Vcap position at the last rebuild (CVX_Voxel::lastColWatchPosition)

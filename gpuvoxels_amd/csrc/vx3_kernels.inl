@@ -107,6 +107,10 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     const VoxMatDev& mp = b.vmatTab[b.vmat[vp]];
     if (mn.fixed && mp.fixed) return;   // VX3_VoxelyzeKernel.cu:572
     if (b.lflags[l] & LF_GONE) return;  // detached (VX3_VoxelyzeKernel.cu:574)
+    if (b.lflags[l] & LF_FRESH) {       // made by attach after this step's link pass in the reference
+        b.lflags[l] &= ~LF_FRESH;
+        return;
+    }
     const real nom = (real)sc.voxSize;
     // CVX_Link::updateRestLength: 0.5*(baseSize_neg + baseSize_pos)
     const real restLength = (real)0.5 * (nom * (real)b.tfac[vn] + nom * (real)b.tfac[vp]);
@@ -137,6 +141,11 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     const Q4<real> oN(b.ow[vn], b.ox[vn], b.oy[vn], b.oz[vn]), oP(b.ow[vp], b.ox[vp], b.oy[vp], b.oz[vp]);
     V3<real> fN, mN, fP, mP;
     linkForces<real>(mat, axis, pN, oN, pP, oP, restLength, area, tss, mat.nu != 0.0f, dmN, dmP, st, fN, mN, fP, mP);
+    if ((sc.flags & SC_ATTACH) && b.lnew[l] > 0) {   // isNewLink (VX3_Link.cu:206-213)
+        const real g = (real)0.01;
+        fN = fN.mul(g); mN = mN.mul(g); fP = fP.mul(g); mP = mP.mul(g);
+        b.lnew[l] -= 1;
+    }
     b.q2x[l] = st.pos2.x; b.q2y[l] = st.pos2.y; b.q2z[l] = st.pos2.z;
     b.a1x[l] = st.a1v.x; b.a1y[l] = st.a1v.y; b.a1z[l] = st.a1v.z;
     b.a2x[l] = st.a2v.x; b.a2y[l] = st.a2v.y; b.a2z[l] = st.a2v.z;
@@ -613,10 +622,24 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
     if (!simRunning(b.sims, s)) return;
     const SimCtl& sc = b.sims[s];
     if (!forces) {   // before the link pass: CVX_Voxel::updateSurface, interior iff all six links remain
-        if (!(sc.flags & (SC_DETACH | SC_SECONDARY))) return;
+        if (!(sc.flags & (SC_DETACH | SC_SECONDARY | SC_ATTACH))) return;
         const int fl = b.vflags[v];
         const int all = 0x3F << VF_LINKMASK_SHIFT;
         const int nf = (fl & all) == all ? (fl | VF_INTERIOR) : (fl & ~VF_INTERIOR);
+        if (nf != fl) b.vflags[v] = nf;
+        return;
+    }
+    if (forces == 2) {   // after the voxel pass: VX3_Voxel::enableAttach (VX3_VoxelyzeKernel.cu:608-621)
+        if (!(sc.flags & SC_ATTACH) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+        const int fl = b.vflags[v];
+        if ((fl & VF_REMOVED) || b.vmatTab[b.vmat[v]].fixed) return;   // not updated, keeps its value
+        bool on = true;
+        double q[VX3_DEV_MT_MAX + 1];
+        for (int k = 0; k < 5 && on; k++)   // all five conditions > 0; a missing one does not restrict
+            if (sc.atLen[k] > 0)
+                on = mathTreeEval(b.tokens + sc.atOff[k], sc.atLen[k], (double)b.px[v], (double)b.py[v], (double)b.pz[v],
+                                  (double)sc.hits, sc.t, sc.recentAngle, 0, 0, sc.nv, q) > 0;
+        const int nf = on ? (fl | VF_ATTACH) : (fl & ~VF_ATTACH);
         if (nf != fl) b.vflags[v] = nf;
         return;
     }
@@ -642,6 +665,157 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
                                  (double)sc.hits, sc.t, sc.recentAngle, 0, 0, sc.nv, q);
     b.xf[0 * V + v] = c.x; b.xf[1 * V + v] = c.y; b.xf[2 * V + v] = c.z;
     b.xf[3 * V + v] = (real)ff[0]; b.xf[4 * V + v] = (real)ff[1]; b.xf[5 * V + v] = (real)ff[2];
+}
+
+// ---- VX3 attach (handle_collision_attachment, VX3_VoxelyzeKernel.cu:724-826, over the
+// surface pairs of gpu_update_attach, :833-843)
+//
+// One workgroup per simulation.  Only voxels that can attach take part: on the surface,
+// not removed, their attach conditions held at their last update, of a sticky material
+// that is not fixed.  Those are compacted in voxel order; the pairs (higher, lower)
+// within the envelope and not directly linked are collected in parallel and sorted;
+// one lane then takes them in ascending order, as the reference's threads would if
+// they ran one after another: a pair within five links (counting the links made
+// before it) is skipped, the dominant axis of the offset in the first voxel's frame
+// picks the faces, and the link is made when both faces are free.
+#define VX3_ATT_MAXV 4096   // attachable voxels per simulation and step
+#define VX3_ATT_MAXP 1024   // candidate pairs per simulation and step
+
+// is_neighbor (VX3_VoxelyzeKernel.cu:651-680): t is reachable from a within depth links,
+// never stepping straight back over the link just taken
+__device__ bool linkedWithin(const DevBatch<real>& b, size_t V, int a, int t, int depth) {
+    if (a == t) return true;
+    int node[6], skip[6], next[6];
+    int top = 0;
+    node[0] = a; skip[0] = -1; next[0] = 0;
+    while (top >= 0) {
+        if (next[top] >= 6 || top >= depth) { top--; continue; }
+        const int u = node[top], d = next[top]++;
+        if (d == skip[top] || !(b.vflags[u] & (1 << (VF_LINKMASK_SHIFT + d)))) continue;
+        const int w = b.nbr[d * V + u];
+        if (w == t) return true;
+        top++;
+        node[top] = w; skip[top] = d ^ 1; next[top] = 0;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
+    const int s = blockIdx.x;
+    if (s >= b.nSims || !simRunning(b.sims, s)) return;
+    SimCtl& sc = b.sims[s];
+    if (!(sc.flags & SC_ATTACH) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+    __shared__ int cand[VX3_ATT_MAXV];
+    __shared__ unsigned long long pairs[VX3_ATT_MAXP], sorted[VX3_ATT_MAXP];
+    __shared__ int wcnt[4], nc, np;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const size_t V = (size_t)b.Vcap;
+    if (tid == 0) { nc = 0; np = 0; }
+    __syncthreads();
+    for (int base = 0; base < sc.nv; base += 256) {
+        const int i = base + tid;
+        bool ok = false;
+        if (i < sc.nv) {
+            const int v = sc.v0 + i, fl = b.vflags[v];
+            const VoxMatDev& m = b.vmatTab[b.vmat[v]];
+            ok = !(fl & (VF_INTERIOR | VF_REMOVED)) && (fl & VF_ATTACH) && m.sticky && !m.fixed;
+        }
+        const unsigned long long bal = __ballot(ok);
+        if (lane == 0) wcnt[wv] = __popcll(bal);
+        __syncthreads();
+        int off = nc + __popcll(bal & ((1ull << lane) - 1));
+        for (int w = 0; w < wv; w++) off += wcnt[w];
+        if (ok && off < VX3_ATT_MAXV) cand[off] = i;
+        __syncthreads();
+        if (tid == 0) nc += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        __syncthreads();
+    }
+    if (nc > VX3_ATT_MAXV) {   // more attachable voxels than the pass holds: the simulation fails
+        if (tid == 0) atomicOr(&sc.flags, SC_COL_OVERFLOW);
+        return;
+    }
+    const int n = nc;
+    const double nom = sc.voxSize;
+    for (int x = tid; x < n; x += 256) {
+        const int v1 = sc.v0 + cand[x];
+        const double b1 = nom * (double)b.tfac[v1];
+        const int m1 = b.vmat[v1];
+        for (int y = 0; y < x; y++) {
+            const int v2 = sc.v0 + cand[y];
+            if (b.vmat[v2] != m1) continue;
+            const double b2 = nom * (double)b.tfac[v2];
+            const double wd = ((b1 + b1 + b1) / 3.0f + (b2 + b2 + b2) / 3.0f) * 0.625;   // COLLISION_ENVELOPE_RADIUS
+            const double dx = (double)b.px[v1] - (double)b.px[v2], dy = (double)b.py[v1] - (double)b.py[v2],
+                         dz = (double)b.pz[v1] - (double)b.pz[v2];
+            if (dx > wd || dx < -wd || dy > wd || dy < -wd || dz > wd || dz < -wd) continue;
+            if (sqrt(dx * dx + dy * dy + dz * dz) > wd) continue;
+            bool linked = false;   // is_neighbor depth 1
+            for (int d = 0; d < 6; d++)
+                if ((b.vflags[v1] & (1 << (VF_LINKMASK_SHIFT + d))) && b.nbr[d * V + v1] == v2) linked = true;
+            if (linked) continue;
+            const int k = atomicAdd(&np, 1);
+            if (k < VX3_ATT_MAXP) pairs[k] = ((unsigned long long)cand[x] << 32) | (unsigned)cand[y];
+        }
+    }
+    __syncthreads();
+    if (np > VX3_ATT_MAXP) {
+        if (tid == 0) atomicOr(&sc.flags, SC_COL_OVERFLOW);
+        return;
+    }
+    const int p = np;
+    for (int k = tid; k < p; k += 256) {   // rank sort: the keys are distinct
+        const unsigned long long key = pairs[k];
+        int r = 0;
+        for (int j = 0; j < p; j++) r += pairs[j] < key;
+        sorted[r] = key;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    int nl = sc.nl;
+    bool made = false;
+    for (int k = 0; k < p; k++) {
+        const int v1 = sc.v0 + (int)(sorted[k] >> 32), v2 = sc.v0 + (int)(sorted[k] & 0xffffffffu);
+        if (linkedWithin(b, V, v1, v2, 5)) continue;
+        // the offset to the second voxel in the first voxel's frame
+        const Q4<real> o1(b.ow[v1], b.ox[v1], b.oy[v1], b.oz[v1]);
+        const V3<real> e(b.px[v2] - b.px[v1], b.py[v2] - b.py[v1], b.pz[v2] - b.pz[v1]);
+        const V3<real> ea = o1.rotInv(e);
+        const real fx = fabs(ea.x), fy = fabs(ea.y), fz = fabs(ea.z);
+        int axis;
+        real c;
+        if (fx >= fy && fx >= fz) { axis = 0; c = ea.x; }
+        else if (fy >= fx && fy >= fz) { axis = 1; c = ea.y; }
+        else { axis = 2; c = ea.z; }
+        const bool rev = c < 0;   // the second voxel lies on the first one's negative side
+        const int d1 = 2 * axis + (rev ? 1 : 0), d2 = d1 ^ 1;
+        if ((b.vflags[v1] & (1 << (VF_LINKMASK_SHIFT + d1))) || (b.vflags[v2] & (1 << (VF_LINKMASK_SHIFT + d2)))) continue;
+        if (nl >= sc.nlMax) { sc.flags |= SC_COL_OVERFLOW; break; }
+        const int l = sc.l0 + nl++;
+        const int vneg = rev ? v2 : v1, vpos = rev ? v1 : v2;   // VX3_Link device ctor (VX3_Link.cu:30-55)
+        b.lneg[l] = vneg;
+        b.lpos[l] = vpos;
+        b.lmeta[l] = axis | (b.vmatTab[b.vmat[v1]].selfLink << 2);
+        b.lflags[l] = LF_SMALL_ANGLE | LF_FRESH;   // VX3_Link::reset
+        b.q2x[l] = 0; b.q2y[l] = 0; b.q2z[l] = 0;
+        b.a1x[l] = 0; b.a1y[l] = 0; b.a1z[l] = 0;
+        b.a2x[l] = 0; b.a2y[l] = 0; b.a2z[l] = 0;
+        b.strain[l] = 0; b.maxStrain[l] = 0; b.strainOffset[l] = 0;
+        b.lnew[l] = sc.safety;
+        b.lsim[l] = s;
+        const int dn = 2 * axis, dp = 2 * axis + 1;
+        for (int q = 0; q < 6; q++) {   // no force from the new link until its first update
+            b.F[(dn * 6 + q) * V + vneg] = 0;
+            b.F[(dp * 6 + q) * V + vpos] = 0;
+        }
+        b.es[dn * V + vneg] = 0;
+        b.es[dp * V + vpos] = 0;
+        b.nbr[dn * V + vneg] = vpos;
+        b.nbr[dp * V + vpos] = vneg;
+        b.vflags[vneg] |= 1 << (VF_LINKMASK_SHIFT + dn);
+        b.vflags[vpos] |= 1 << (VF_LINKMASK_SHIFT + dp);
+        made = true;
+    }
+    if (made) sc.nl = nl;
 }
 
 // Target contacts (handle_collision_attachment, VX3_VoxelyzeKernel.cu:682-722): each

@@ -10,6 +10,7 @@ struct StepOpts {
     bool extras;       // some simulation has cilia or a force field: run its voxel pass
     bool prestep;      // some simulation has cilia, a force field or detach: run the step-start pass
     bool signals;      // some simulation has signals: run the signal-delivery pass
+    bool attach;       // some simulation attaches sticky voxels: run the attach pass and conditions
     int block;         // workgroup size of the element passes
 };
 // Kernel passes of one step, in launch order; ev (optional) receives a start/stop

@@ -27,6 +27,7 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
         if (ev) hipEventRecord(ev[2 * (k) + 1], s);         \
     } while (0)
     if (o.prestep) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 0);
+    if (o.attach) hipLaunchKernelGGL(VX_NS::k_attach, dim3(b.nSims), dim3(256), 0, s, b);
     if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
     if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs));
     if (o.contacts && o.extras) hipLaunchKernelGGL(VX_NS::k_hits, dim3(gv), dim3(bs), 0, s, b);
@@ -34,6 +35,7 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_voxels_extras, dim3(gv), dim3(bs), 0, s, b);
     VX3_EV(PASS_VOXELS, hipLaunchKernelGGL(VX_NS::k_voxels, dim3(gv), dim3(bs), 0, s, b));
     if (o.signals) hipLaunchKernelGGL(VX_NS::k_signals, dim3(gv), dim3(bs), 0, s, b);
+    if (o.attach) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 2);
     VX3_EV(PASS_CONTROL, hipLaunchKernelGGL(VX_NS::k_control, dim3(b.nSims), dim3(256), 0, s, b));
 #undef VX3_EV
 }

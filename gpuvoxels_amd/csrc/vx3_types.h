@@ -28,11 +28,13 @@ enum {
     VF_REMOVED = 1 << 4,             // removed by a secondary experiment: frozen, no links
     VF_COLL = 1 << 5,
     VF_TARGET = 1 << 6,              // material isTarget: contacts with other materials count as hits
+    VF_ATTACH = 1 << 7,              // VX3 enableAttach: the attach conditions held at the last update
     VF_LINKMASK_SHIFT = 8,           // bits 8..13: link present in direction d
 };
 // link state bits
 enum { LF_LOCAL_VELOCITY_VALID = 1, LF_SMALL_ANGLE = 2,
-       LF_GONE = 4 };   // detached: no longer updated, no longer in its voxels' link masks
+       LF_GONE = 4,     // detached: no longer updated, no longer in its voxels' link masks
+       LF_FRESH = 8 };  // made by attach this step: no force until the next step's link pass
 // per-sim control bits
 enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, SC_FLOOR = 16, SC_COLLISIONS = 32, SC_RESIDENT = 64,
        SC_REBUILD = 128,        // collision neighbour lists are rebuilt this step
@@ -42,7 +44,8 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_DETACH = 2048,        // failed links detach
        SC_SECONDARY = 4096,     // secondary experiment: timed voxel removal, initial-position reset
        SC_SIGNALS = 8192,       // signals between linked voxels (runs with SC_EXTRAS)
-       SC_TARGETS = 16384 };    // target contacts are counted (runs with SC_EXTRAS)
+       SC_TARGETS = 16384,      // target contacts are counted (runs with SC_EXTRAS)
+       SC_ATTACH = 32768 };     // touching sticky surface voxels get new links
 #define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
@@ -59,6 +62,8 @@ struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_M
     double thermalOnAfter;
     int fixed;                       // VX3 material-fixed
     int isMeasured;
+    int sticky;                      // VX3 attach: 1 + the material's index in its simulation, 0 = not sticky
+    int selfLink;                    // link-material table index of a link between two voxels of this material
 };
 
 struct LinkMatDev {                 // CVX_MaterialLink derived constants (VX_MaterialLink.cpp:121-141)
@@ -119,6 +124,10 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     double reinitAfter;
     int reinitDone;
     int hits;                        // collisionCount: target contacts so far (MathTree variable hit)
+    // VX3 attach: link range capacity (links are made at l0 + nl), new-link guard steps,
+    // attach-condition programs in the token pool
+    int nlMax, safety;
+    int atOff[5], atLen[5];
 };
 
 #define VX3_HIST 500                 // HISTORY_SIZE (VoxCAD/src/VX1/VX_Sim.h)

@@ -362,6 +362,33 @@ bool buildModel(const vx3h_sim_desc* d, HostModel& H, std::string& err) {
         }
         H.lmat[l] = it->second;
     }
+    // VX3 attach makes links between two voxels of one sticky material at run time
+    // (VX3_Link device ctor, VX3_Link.cu:30-55: combinedMaterial(m, m), strain ratio 1)
+    H.vmSelf.assign(NM, -1);
+    if (d->enable_attach)
+        for (int m = 0; m < NM; m++) {
+            if (!d->materials[m].sticky || d->materials[m].fixed) continue;
+            int lm = -1;
+            for (int j = 0; j < (int)pairMats.size(); j++)
+                if (pairMats[j].first == m && pairMats[j].second == m) { lm = j; break; }
+            if (lm < 0) {
+                lm = (int)pairMats.size();
+                pairMats.push_back({m, m});
+                linkMatsHost.push_back(combine(mats[m], mats[m]));
+            }
+            const int key = lm * NM + m;
+            auto it = vidx.find(key);
+            if (it == vidx.end()) {
+                LinkMatDev o;
+                linkDerived(linkMatsHost[lm], o);
+                o.strainRatio = mats[m].E / mats[m].E;
+                vidx[key] = (int)H.lmats.size();
+                H.lmats.push_back(o);
+                it = vidx.find(key);
+            }
+            H.vmSelf[m] = it->second;
+            H.vmats[m].sticky = m + 1;
+        }
 
     // flags: floor, static friction set by CVX_Voxel::reset, interior, link mask
     H.vflags.assign(NV, 0);
@@ -501,6 +528,15 @@ bool buildModel(const vx3h_sim_desc* d, HostModel& H, std::string& err) {
         }
         if (n > 0) H.ffTok[k].assign(d->force_field_tokens[k], d->force_field_tokens[k] + n);
     }
+    if (d->enable_attach)
+        for (int k = 0; k < 5; k++) {
+            const int n = d->n_attach_tokens[k];
+            if (n < 0 || n > VX3H_MAX_TOKENS || (n > 0 && !d->attach_tokens[k])) {
+                err = "bad attach-condition program";
+                return false;
+            }
+            if (n > 0) H.atTok[k].assign(d->attach_tokens[k], d->attach_tokens[k] + n);
+        }
     return true;
 }
 

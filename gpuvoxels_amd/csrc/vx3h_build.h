@@ -52,6 +52,10 @@ struct HostModel {
     std::vector<double> cilia;
     std::vector<double> vmCilia, vmCiliaOn;
     std::vector<vx3h_token> ffTok[3];
+    // VX3 attach: per voxel-material the device link-material index of a link between two
+    // of its voxels (-1 unless sticky), and the attach-condition programs
+    std::vector<int> vmSelf;
+    std::vector<vx3h_token> atTok[5];
     bool extras() const { return !cilia.empty() || !ffTok[0].empty() || !ffTok[1].empty() || !ffTok[2].empty(); }
 };
 

@@ -103,6 +103,8 @@ struct RangeAlloc {
 struct SlotHost {
     bool used = false;
     int v0 = 0, nv = 0, l0 = 0, nl = 0;
+    int nlCap = 0;               // link range size: nl, or room for the links attach makes
+    int nlSeen = 0;              // links known to the host shadows below (nl + attached ones read back)
     int shape = -1;              // resident kernel shape, -1 = stream path
     std::vector<int> ijk;
     std::vector<int> matid;
@@ -169,6 +171,7 @@ struct vx3h_batch {
     bool anyColl = false;
     bool anyExtras = false;             // some simulation has cilia or a force field
     bool anyDetach = false;             // some simulation detaches failed links
+    bool anyAttach = false;             // some simulation attaches sticky voxels
     bool anySignals = false;            // some simulation has signals
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
@@ -202,7 +205,7 @@ struct vx3h_batch {
 namespace {
 
 size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8 + 8 * 3 + 8 + 8 * 8 + 8 * 5 + 4 * 6; }
-size_t linkBytes(size_t rs) { return rs * 9 + 4 * 5 + 4 * 3; }
+size_t linkBytes(size_t rs) { return rs * 9 + 4 * 6 + 4 * 3; }
 
 void initSpecs(vx3h_batch* b) {
     DevBatch<float>& d = b->db;
@@ -230,7 +233,7 @@ void initSpecs(vx3h_batch* b) {
     L(&d.q2x, R, 1, false); L(&d.q2y, R, 1, false); L(&d.q2z, R, 1, false);
     L(&d.a1x, R, 1, false); L(&d.a1y, R, 1, false); L(&d.a1z, R, 1, false);
     L(&d.a2x, R, 1, false); L(&d.a2y, R, 1, false); L(&d.a2z, R, 1, false);
-    L(&d.strain, 4, 1, false); L(&d.maxStrain, 4, 1, false); L(&d.strainOffset, 4, 1, false);
+    L(&d.strain, 4, 1, false); L(&d.maxStrain, 4, 1, false); L(&d.strainOffset, 4, 1, false); L(&d.lnew, 4, 1, false);
 }
 
 vx3h_status growArrays(vx3h_batch* b, std::vector<ArrSpec>& arrs, int& cap, int need, int used, bool isVoxel) {
@@ -331,6 +334,7 @@ vx3h_status launchSteps(vx3h_batch* b, int n) {
     o.extras = b->anyExtras;
     o.prestep = b->anyDetach;
     o.signals = b->anySignals;
+    o.attach = b->anyAttach;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
@@ -501,7 +505,8 @@ extern "C" vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* d, int precision,
     HostModel H;
     if (!buildModel(d, H, g_err)) return VX3H_EINVAL;
     size_t rs = precision == VX3H_PREC_F64 ? 8 : 4;
-    *bytes = (size_t)H.nv * voxelBytes(rs) + (size_t)H.nl * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
+    const int NLcap = d->enable_attach && H.nv > 0 ? std::max(H.nl, 3 * H.nv) : H.nl;
+    *bytes = (size_t)H.nv * voxelBytes(rs) + (size_t)NLcap * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
     return VX3H_OK;
 }
 
@@ -513,14 +518,21 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         FAILV(VX3H_EUNSUPPORTED, "detach with collisions is not supported (the contact exclusion lists are built once)");
     if (desc->secondary_experiment && desc->collisions_enabled)
         FAILV(VX3H_EUNSUPPORTED, "voxel removal with collisions is not supported (the contact exclusion lists are built once)");
+    if (desc->enable_attach && desc->collisions_enabled)
+        FAILV(VX3H_EUNSUPPORTED, "attach with collisions is not supported (the contact exclusion lists are built once)");
     HIPCHK(hipSetDevice(b->device));
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
     if (H.stopTok.size() > 64) FAILV(VX3H_EUNSUPPORTED, "stop condition longer than 64 tokens");
     for (int k = 0; k < 3; k++)
         if (H.ffTok[k].size() > 64) FAILV(VX3H_EUNSUPPORTED, "force-field program longer than 64 tokens");
+    for (int k = 0; k < 5; k++)
+        if (H.atTok[k].size() > 64) FAILV(VX3H_EUNSUPPORTED, "attach-condition program longer than 64 tokens");
+    const bool attach = desc->enable_attach != 0 && H.nv > 0;
+    // attach needs room for new links: every voxel has six faces, so at most 3 links per voxel
+    const int NLcap = attach ? std::max(H.nl, 3 * H.nv) : H.nl;
     const size_t rs = b->realSize;
-    const size_t need = (size_t)H.nv * voxelBytes(rs) + (size_t)H.nl * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
+    const size_t need = (size_t)H.nv * voxelBytes(rs) + (size_t)NLcap * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
     if (b->bytesUsed + need > b->budget) FAILV(VX3H_ENOMEM, "simulation does not fit the batch's device memory budget");
     HIPCHK(hipStreamSynchronize(b->stream));
     // slot
@@ -549,14 +561,17 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     // ranges
     bool gv = false, gl = false;
     int v0 = H.nv ? b->va.alloc(H.nv, b->db.Vcap, gv) : 0;
-    int l0 = H.nl ? b->la.alloc(H.nl, b->db.Lcap, gl) : 0;
+    int l0 = NLcap ? b->la.alloc(NLcap, b->db.Lcap, gl) : 0;
     if (gv) { vx3h_status st = growArrays(b, b->vArrs, b->db.Vcap, b->va.hi, v0, true); if (st) return st; }
     if (gl) { vx3h_status st = growArrays(b, b->lArrs, b->db.Lcap, b->la.hi, l0, false); if (st) return st; }
     // tables
     std::vector<int> vmIdx(H.vmats.size()), lmIdx(H.lmats.size());
     size_t nvm = b->vmats.items.size(), nlm = b->lmats.items.size();
-    for (size_t i = 0; i < H.vmats.size(); i++) vmIdx[i] = b->vmats.add(H.vmats[i]);
     for (size_t i = 0; i < H.lmats.size(); i++) lmIdx[i] = b->lmats.add(H.lmats[i]);
+    for (size_t i = 0; i < H.vmats.size(); i++) {
+        if (H.vmSelf[i] >= 0) H.vmats[i].selfLink = lmIdx[H.vmSelf[i]];
+        vmIdx[i] = b->vmats.add(H.vmats[i]);
+    }
     if (b->vmats.items.size() != nvm || !b->dVmat) { vx3h_status st = uploadTable(b, b->dVmat, b->capVmat, b->vmats.items); if (st) return st; }
     if (b->lmats.items.size() != nlm || !b->dLmat) { vx3h_status st = uploadTable(b, b->dLmat, b->capLmat, b->lmats.items); if (st) return st; }
     int ext0 = (int)b->exts.size();
@@ -572,10 +587,14 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     b->db.simMats = (const int*)b->dSimMats;
     int tok0 = (int)b->tokens.size();
     for (const vx3h_token& t : H.stopTok) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
-    int ffOff[3];
+    int ffOff[3], atOff[5];
     for (int k = 0; k < 3; k++) {
         ffOff[k] = (int)b->tokens.size();
         for (const vx3h_token& t : H.ffTok[k]) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
+    }
+    for (int k = 0; k < 5; k++) {
+        atOff[k] = (int)b->tokens.size();
+        for (const vx3h_token& t : H.atTok[k]) { TokenDev td; td.op = t.op; td.pad = 0; td.value = t.value; b->tokens.push_back(td); }
     }
 
     if ((int)b->tokens.size() > tok0) { vx3h_status st = uploadTable(b, b->dTok, b->capTok, b->tokens); if (st) return st; }
@@ -605,6 +624,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         if (H.collisions)   // target materials (handle_collision_attachment counts their contacts)
             for (int i = 0; i < NV; i++)
                 if (desc->materials[desc->material[i]].is_target) vf[i] |= VF_TARGET;
+        if (attach)   // VX3_Voxel::enableAttach starts true (VX3_Voxel.h:302)
+            for (int i = 0; i < NV; i++) vf[i] |= VF_ATTACH;
         HIPCHK(hipMemcpy(d.vflags + v0, vf.data(), NV * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d.tfac + v0, H.tfac0.data(), NV * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d.phase + v0, H.phase.data(), NV * 8, hipMemcpyHostToDevice));
@@ -644,6 +665,11 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         HIPCHK(hipMemset(d.strainOffset + l0, 0, NL * 4));
         HIPCHK(hipMemcpy(d.lsim + l0, ls.data(), NL * 4, hipMemcpyHostToDevice));
     }
+    if (NLcap) HIPCHK(hipMemset(d.lnew + l0, 0, (size_t)NLcap * 4));
+    if (NLcap > NL) {   // room for attached links: not owned until made
+        HIPCHK(hipMemset(d.lsim + l0 + NL, 0xFF, (size_t)(NLcap - NL) * 4));
+        HIPCHK(hipMemset(d.lflags + l0 + NL, 0, (size_t)(NLcap - NL) * 4));
+    }
     // execution path: resident when the sim fits one of the on-chip kernel shapes
     int shape = -1;
     int perAxis[3] = {0, 0, 0};
@@ -657,7 +683,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
                        nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach &&
-                       !desc->secondary_experiment && !desc->enable_signals;
+                       !desc->secondary_experiment && !desc->enable_signals && !attach;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -675,8 +701,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
-    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals) && b->cfg.path == VX3H_PATH_RESIDENT)
-        FAILV(VX3H_EUNSUPPORTED, "detach, secondary experiments and signals run on the stream path");
+    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals || attach) && b->cfg.path == VX3H_PATH_RESIDENT)
+        FAILV(VX3H_EUNSUPPORTED, "detach, attach, secondary experiments and signals run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -714,6 +740,24 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         c.flags |= SC_TARGETS | SC_EXTRAS;
         b->anyExtras = true;
     }
+    if (attach) {
+        c.flags |= SC_ATTACH | SC_EXTRAS;
+        b->anyAttach = b->anyExtras = true;
+        b->anyDetach = true;   // the step-start pass refreshes surfaces after attachments
+        c.safety = desc->safety_guard;
+        for (int k = 0; k < 5; k++) { c.atOff[k] = atOff[k]; c.atLen[k] = (int)H.atTok[k].size(); }
+    }
+    c.nlMax = NLcap;
+    if ((desc->enable_signals || attach) && NV) {   // neighbour per link direction
+        std::vector<int> nb(NV);
+        for (int k = 0; k < 6; k++) {
+            for (int i = 0; i < NV; i++) {
+                const int l = H.voxelLinks[6 * (size_t)i + k];
+                nb[i] = l < 0 ? -1 : v0 + (H.lneg[l] == i ? H.lpos[l] : H.lneg[l]);
+            }
+            HIPCHK(hipMemcpy(d.nbr + (size_t)k * d.Vcap + v0, nb.data(), NV * 4, hipMemcpyHostToDevice));
+        }
+    }
     if (desc->enable_signals && NV) {
         c.flags |= SC_SIGNALS | SC_EXTRAS;
         b->anyExtras = b->anySignals = true;
@@ -727,14 +771,6 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
                          : k == 3 ? md.signal_time_delay : md.inactive_period;
             }
             HIPCHK(hipMemcpy(d.sp + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
-        }
-        std::vector<int> nb(NV);
-        for (int k = 0; k < 6; k++) {
-            for (int i = 0; i < NV; i++) {
-                const int l = H.voxelLinks[6 * (size_t)i + k];
-                nb[i] = l < 0 ? -1 : v0 + (H.lneg[l] == i ? H.lpos[l] : H.lneg[l]);
-            }
-            HIPCHK(hipMemcpy(d.nbr + (size_t)k * d.Vcap + v0, nb.data(), NV * 4, hipMemcpyHostToDevice));
         }
     }
     if (desc->secondary_experiment) {
@@ -831,6 +867,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh = SlotHost();
     sh.used = true;
     sh.v0 = v0; sh.nv = NV; sh.l0 = l0; sh.nl = NL;
+    sh.nlCap = NLcap; sh.nlSeen = NL;
     sh.ijk = H.ijk;
     sh.voxSize = H.voxSize;
     for (int k = 0; k < 3; k++) sh.com0[k] = H.com0[k];
@@ -916,6 +953,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     o.extras = b->anyExtras;
     o.prestep = b->anyDetach;
     o.signals = b->anySignals;
+    o.attach = b->anyAttach;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];
@@ -1033,6 +1071,7 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
     o.dt = c.dt;
     o.recommended_dt = c.optDt;
     o.collision_count = c.hits;
+    o.n_links = c.nl;
     o.recent_angle = c.recentAngle;
     // initial center of mass and positions: the admission's, or a secondary experiment's reset
     for (int k = 0; k < 3; k++) o.initial_com[k] = c.com0[k];
@@ -1103,13 +1142,13 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
     HIPCHK(hipStreamSynchronize(b->stream));
     SlotHost& sh = b->slots[slot];
     if (sh.nv) HIPCHK(hipMemset(b->db.vsim + sh.v0, 0xFF, (size_t)sh.nv * 4));
-    if (sh.nl) HIPCHK(hipMemset(b->db.lsim + sh.l0, 0xFF, (size_t)sh.nl * 4));
+    if (sh.nlCap) HIPCHK(hipMemset(b->db.lsim + sh.l0, 0xFF, (size_t)sh.nlCap * 4));
     SimCtl c;
     memset(&c, 0, sizeof(c));
     c.state = VX3H_SIM_EMPTY;
     HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
     b->va.release(sh.v0, sh.nv);
-    b->la.release(sh.l0, sh.nl);
+    b->la.release(sh.l0, sh.nlCap);
     if (sh.hN) {
         HIPCHK(hipMemset(b->db.hashSim + sh.h0, 0xFF, (size_t)sh.hN * 4));
         b->hashPool.ra.release(sh.h0, sh.hN);
@@ -1160,12 +1199,43 @@ extern "C" vx3h_status vx3h_sim_collision_pairs(vx3h_batch* b, int slot, int* pa
     return VX3H_OK;
 }
 
+// Brings the host shadows of a simulation's topology up to the links attach has made on
+// the device (they follow the admitted links in the order they were made).
+static vx3h_status syncAttached(vx3h_batch* b, int slot) {
+    SlotHost& sh = b->slots[slot];
+    if (sh.nlCap <= sh.nl) return VX3H_OK;
+    SimCtl c;
+    HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(SimCtl), hipMemcpyDeviceToHost));
+    const int n = std::min(c.nl, sh.nlCap) - sh.nlSeen;
+    if (n <= 0) return VX3H_OK;
+    std::vector<int> ln(n), lp(n), lm(n);
+    const size_t o = (size_t)sh.l0 + sh.nlSeen;
+    HIPCHK(hipMemcpy(ln.data(), b->db.lneg + o, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lp.data(), b->db.lpos + o, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(lm.data(), b->db.lmeta + o, n * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        const int li = sh.nlSeen + i, vn = ln[i] - sh.v0, vp = lp[i] - sh.v0, axis = lm[i] & 3;
+        const LinkMatDev& m = b->lmats.items[lm[i] >> 2];
+        sh.lneg.push_back(vn);
+        sh.lpos.push_back(vp);
+        sh.lratio.push_back(m.strainRatio);
+        sh.lfail.push_back(m.epsilonFail);
+        sh.vlinks[6 * (size_t)vn + 2 * axis] = li;
+        sh.vlinks[6 * (size_t)vp + 2 * axis + 1] = li;
+    }
+    sh.nlSeen += n;
+    return VX3H_OK;
+}
+
 extern "C" vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_view* view) {
     if (!b || !view) FAILV(VX3H_EINVAL, "null argument");
     if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
+    if (view->link_capacity > 0) { vx3h_status s0 = syncAttached(b, slot); if (s0) return s0; }
     const SlotHost& sh = b->slots[slot];
+    // links written: the admitted ones, or with a capacity also those attach made
+    const int nlOut = view->link_capacity > 0 ? std::min(view->link_capacity, sh.nlSeen) : sh.nl;
     const DevBatch<float>& d = b->db;
     const size_t rs = b->realSize;
     std::vector<double> t0, t1, t2, t3;
@@ -1208,21 +1278,21 @@ extern "C" vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_v
         HIPCHK(hipMemcpy(tf.data(), d.tfac + sh.v0, sh.nv * 4, hipMemcpyDeviceToHost));
         for (int i = 0; i < sh.nv; i++) view->temp[i] = tf[i];   // thermal size factor 1 + T*CTE
     }
-    if (sh.nl && (view->link_strain || view->link_max_strain)) {
-        std::vector<float> a(sh.nl);
+    if (nlOut && (view->link_strain || view->link_max_strain)) {
+        std::vector<float> a(nlOut);
         if (view->link_strain) {
-            HIPCHK(hipMemcpy(a.data(), d.strain + sh.l0, sh.nl * 4, hipMemcpyDeviceToHost));
-            for (int i = 0; i < sh.nl; i++) view->link_strain[i] = a[i];
+            HIPCHK(hipMemcpy(a.data(), d.strain + sh.l0, nlOut * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < nlOut; i++) view->link_strain[i] = a[i];
         }
         if (view->link_max_strain) {
-            HIPCHK(hipMemcpy(a.data(), d.maxStrain + sh.l0, sh.nl * 4, hipMemcpyDeviceToHost));
-            for (int i = 0; i < sh.nl; i++) view->link_max_strain[i] = a[i];
+            HIPCHK(hipMemcpy(a.data(), d.maxStrain + sh.l0, nlOut * 4, hipMemcpyDeviceToHost));
+            for (int i = 0; i < nlOut; i++) view->link_max_strain[i] = a[i];
         }
     }
-    if (sh.nl && view->link_small_angle) {
-        std::vector<int> f(sh.nl);
-        HIPCHK(hipMemcpy(f.data(), d.lflags + sh.l0, sh.nl * 4, hipMemcpyDeviceToHost));
-        for (int i = 0; i < sh.nl; i++) view->link_small_angle[i] = (f[i] & LF_SMALL_ANGLE) ? 1 : 0;
+    if (nlOut && view->link_small_angle) {
+        std::vector<int> f(nlOut);
+        HIPCHK(hipMemcpy(f.data(), d.lflags + sh.l0, nlOut * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nlOut; i++) view->link_small_angle[i] = (f[i] & LF_SMALL_ANGLE) ? 1 : 0;
     }
     return VX3H_OK;
 }
@@ -1277,8 +1347,11 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
                                                char* buf, size_t cap, size_t* len) {
     if (!b) FAILV(VX3H_EINVAL, "null batch");
     if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    { vx3h_status s0 = syncAttached(b, slot); if (s0) return s0; }
     const SlotHost& sh = b->slots[slot];
-    const int NV = sh.nv, NL = sh.nl;
+    const int NV = sh.nv, NL = sh.nlSeen;   // the admitted links and those attach made
     std::vector<double> pos(3 * (size_t)NV), ori(4 * (size_t)NV), tf(NV), strain(NL), maxStrain(NL);
     std::vector<int> vflags(NV), lflags(NL);
     vx3h_state_view view{};
@@ -1288,6 +1361,7 @@ extern "C" vx3h_status vx3h_sim_history_record(vx3h_batch* b, int slot, int step
     view.vox_flags = vflags.data();
     view.link_strain = strain.data();
     view.link_max_strain = maxStrain.data();
+    view.link_capacity = NL;
     vx3h_status st = vx3h_sim_read_state(b, slot, &view);
     if (st) return st;
     if (NL) HIPCHK(hipMemcpy(lflags.data(), b->db.lflags + sh.l0, NL * 4, hipMemcpyDeviceToHost));

@@ -31,7 +31,7 @@ struct DescCopy {
     std::vector<int> ijk, mat;
     std::vector<double> phase;
     std::vector<vx3h_external> exts;
-    std::vector<vx3h_token> stop, fit, ff[3];
+    std::vector<vx3h_token> stop, fit, ff[3], att[5];
     std::vector<double> baseCilia, shiftCilia;
     void take(const vx3h_sim_desc& s) {
         d = s;
@@ -57,6 +57,10 @@ struct DescCopy {
             if (s.n_force_field_tokens[k] > 0)
                 ff[k].assign(s.force_field_tokens[k], s.force_field_tokens[k] + s.n_force_field_tokens[k]);
             d.force_field_tokens[k] = ff[k].empty() ? nullptr : ff[k].data();
+        }
+        for (int k = 0; k < 5; k++) {
+            if (s.n_attach_tokens[k] > 0) att[k].assign(s.attach_tokens[k], s.attach_tokens[k] + s.n_attach_tokens[k]);
+            d.attach_tokens[k] = att[k].empty() ? nullptr : att[k].data();
         }
     }
 };

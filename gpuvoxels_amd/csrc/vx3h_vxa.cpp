@@ -339,6 +339,7 @@ struct PalMat {                 // CVXC_Material (VX_Object.cpp:1373-1460), SING
     int isPaceMaker = 0;                        // signals (VX_Object.cpp:1420-1425)
     double paceMakerPeriod = 0, signalValueDecay = 0.9, signalTimeDelay = 0.03, inactivePeriod = 0.03;
     int isTarget = 0;
+    int sticky = 0;
 };
 
 bool readRegion(Rip& r, Region& g) {
@@ -483,7 +484,7 @@ struct vx3h_vxa {
     std::vector<int> ijk, mat;
     std::vector<double> phase;
     std::vector<vx3h_external> exts;
-    std::vector<vx3h_token> stop, fit, ff[3];
+    std::vector<vx3h_token> stop, fit, ff[3], att[5];
     std::vector<double> baseCilia, shiftCilia;
     std::string rawPrint;
     vx3h_vxa_info info;
@@ -636,6 +637,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
                     if (!r.load("signalTimeDelay", m.signalTimeDelay)) m.signalTimeDelay = 0.03;
                     if (!r.load("inactivePeriod", m.inactivePeriod)) m.inactivePeriod = 0.03;
                     if (!r.load("isTarget", m.isTarget)) m.isTarget = 0;
+                    if (!r.load("Sticky", m.sticky)) m.sticky = 0;   // VXA/VX_Object.cpp:1436
                     if (!r.load("Cilia", m.cilia)) m.cilia = 0;
                     if (!r.load("Elastic_Mod", m.E)) m.E = 0;
                     if (!r.load("Plastic_Mod", m.plasticMod)) m.plasticMod = 0;
@@ -761,6 +763,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
         d.signal_time_delay = p.signalTimeDelay;
         d.inactive_period = p.inactivePeriod;
         d.is_target = p.isTarget != 0;
+        d.sticky = p.sticky != 0;
         d.matid = (int)i + 1;   // palette index; 0 is the erase material
         for (int k = 0; k < 4; k++) {   // GetRedi() etc. (VX_Object.h:419-425), then setRed's clamp
             const int c = (int)(p.rgba[k] * 255);
@@ -868,6 +871,9 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     const char* ffAxes[3] = {"x_forcefield", "y_forcefield", "z_forcefield"};   // VX3_SimulationManager.cu:349-354
     for (int k = 0; k < 3; k++)
         if (!parseMathTree(doc, doc.name + ".Simulator.ForceField." + ffAxes[k], o.ff[k], err)) return VX3H_EPARSE;
+    for (int k = 0; k < 5; k++)   // VX3_SimulationManager.cu:334-343
+        if (!parseMathTree(doc, doc.name + ".Simulator.AttachDetach.AttachCondition.Condition_" + std::to_string(k), o.att[k], err))
+            return VX3H_EPARSE;
     const std::string S = doc.name + ".Simulator.";
     vx3h_vxa_info& in = o.info;
     memset(&in, 0, sizeof(in));
@@ -921,6 +927,12 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     d.save_position_of_all_voxels = ptGet<int>(doc, S + "SavePositionOfAllVoxels", 0);
     d.enable_cilia = in.enable_cilia;
     d.enable_detach = in.enable_detach;
+    d.enable_attach = in.enable_attach;
+    d.safety_guard = ptGet<int>(doc, S + "AttachDetach.SafetyGuard", 500);   // VX3_SimulationManager.cu:333
+    for (int k = 0; k < 5; k++) {
+        d.n_attach_tokens[k] = (int)o.att[k].size();
+        d.attach_tokens[k] = o.att[k].empty() ? nullptr : o.att[k].data();
+    }
     d.secondary_experiment = in.secondary_experiment;
     d.enable_signals = in.enable_signals;
     d.reinit_after = ptGet<double>(doc, S + "ReinitializeInitialPositionAfterThisManySeconds", 0.0);   // VX3_SimulationManager.cu:365

@@ -319,6 +319,7 @@ class Batch:
             st, ms, sa = np.zeros(nl), np.zeros(nl), np.zeros(nl, np.int32)
             v.link_strain, v.link_max_strain, v.link_small_angle = (_p(st, C.c_double), _p(ms, C.c_double),
                                                                     _p(sa, C.c_int))
+            v.link_capacity = nl   # links made by attach follow the admitted ones
             out.update(strain=st, max_strain=ms, small_angle=sa)
         check(lib().vx3h_sim_read_state(self.h, slot, C.byref(v)))
         out.update(pos=pos.reshape(-1, 3), orient=ori.reshape(-1, 4), lin_mom=lm.reshape(-1, 3),

@@ -47,6 +47,7 @@ class Material:
     signal_time_delay: float = 0.03
     inactive_period: float = 0.03
     is_target: int = 0                    # contacts with other materials count as hits
+    sticky: int = 0                       # surface voxels of this material attach to each other
 
     def to_c(self) -> _abi.MaterialDesc:
         m = _abi.MaterialDesc()
@@ -80,6 +81,7 @@ class Material:
         m.signal_time_delay = self.signal_time_delay
         m.inactive_period = self.inactive_period
         m.is_target = self.is_target
+        m.sticky = self.sticky
         for k in range(4):   # CVXC_Material::GetRedi etc. then CVX_Material::setRed's clamp
             m.color[k] = min(255, max(0, int(np.float32(self.color[k]) * np.float32(255))))
         return m
@@ -172,6 +174,9 @@ class Model:
     secondary_experiment: int = 0             # timed voxel removal and initial-position reset
     reinit_after: float = 0.0                 # ReinitializeInitialPositionAfterThisManySeconds
     enable_signals: int = 0                   # VX3 signals between linked voxels
+    enable_attach: int = 0                    # touching sticky voxels get new links (VX3 attach)
+    attach_conditions: list = field(default_factory=lambda: [[], [], [], [], []])   # MathTrees, all > 0
+    safety_guard: int = 500                   # steps a new link carries 1/100 of its force
     name: str = "sim"
 
     @property
@@ -253,4 +258,12 @@ class Model:
         d.secondary_experiment = self.secondary_experiment
         d.reinit_after = self.reinit_after
         d.enable_signals = self.enable_signals
+        d.enable_attach = self.enable_attach
+        for k, toks in enumerate(self.attach_conditions):
+            if toks:
+                arr = (_abi.Token * len(toks))(*[_abi.Token(op, val) for op, val in toks])
+                keep.append(arr)
+                d.n_attach_tokens[k] = len(toks)
+                d.attach_tokens[k] = C.cast(arr, C.POINTER(_abi.Token))
+        d.safety_guard = self.safety_guard
         return d, keep

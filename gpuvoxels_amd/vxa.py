@@ -24,6 +24,16 @@ for _op in ("mtSQRT", "mtSIN", "mtCOS", "mtTAN", "mtATAN", "mtLOG", "mtINT", "mt
 _VAR_NAMES = {v: k for k, v in MT_VARS.items()}
 
 
+def _attach_xml(m):
+    """EnableAttach, SafetyGuard and the attach conditions (VX3_SimulationManager.cu:329-343)."""
+    if not m.enable_attach:
+        return ""
+    conds = "".join("<Condition_%d>%s</Condition_%d>" % (k, mathtree_xml(t), k)
+                    for k, t in enumerate(m.attach_conditions) if t)
+    return ("<EnableAttach>1</EnableAttach><SafetyGuard>%d</SafetyGuard>" % m.safety_guard +
+            ("<AttachCondition>%s</AttachCondition>" % conds if conds else ""))
+
+
 def mathtree_xml(tokens: Sequence) -> str:
     """XML subtree of a token program given as (op, value) pairs ending with mtEND."""
     toks = [(MT_OPS[int(op)], float(v)) for op, v in tokens]
@@ -99,7 +109,7 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
     out.append("<AttachDetach><EnableCollision>%d</EnableCollision><boundingRadius>%s</boundingRadius>"
                "<watchDistance>%s</watchDistance>%s</AttachDetach>"
                % (int(bool(m.collisions_enabled)), _num(m.bounding_radius), _num(m.watch_distance),
-                  "<EnableDetach>1</EnableDetach>" if m.enable_detach else ""))
+                  ("<EnableDetach>1</EnableDetach>" if m.enable_detach else "") + _attach_xml(m)))
     out.append("<EnableExpansion>%d</EnableExpansion>" % int(bool(m.enable_expansion)))
     out.append("<SavePositionOfAllVoxels>%d</SavePositionOfAllVoxels>" % int(bool(m.save_position_of_all_voxels)))
     if m.enable_cilia:
@@ -156,8 +166,9 @@ def model_to_vxa(m: Model, regions: Optional[List[dict]] = None, extra_simulator
                 % _num(mt.remove_after),
                 "<isPaceMaker>%d</isPaceMaker><PaceMakerPeriod>%s</PaceMakerPeriod><signalValueDecay>%s</signalValueDecay>"
                 "<signalTimeDelay>%s</signalTimeDelay><inactivePeriod>%s</inactivePeriod><isTarget>%d</isTarget>"
+                "<Sticky>%d</Sticky>"
                 % (int(mt.is_pacemaker), _num(mt.pacemaker_period), _num(mt.signal_value_decay),
-                   _num(mt.signal_time_delay), _num(mt.inactive_period), int(mt.is_target))]
+                   _num(mt.signal_time_delay), _num(mt.inactive_period), int(mt.is_target), int(mt.sticky))]
         if mt.strain_data:
             mech.append("<SSData><NumDataPts>%d</NumDataPts><StrainData>%s</StrainData><StressData>%s</StressData></SSData>"
                         % (len(mt.strain_data), "".join("<Strain>%s</Strain>" % _num(e) for e in mt.strain_data),

@@ -98,6 +98,8 @@ typedef struct vx3h_material_desc {
     double inactive_period;     /* inactivePeriod [0.03] s deaf after receiving */
     int    is_target;           /* isTarget [0]: contacts with other materials count as hits
                                    (VX3_VoxelyzeKernel.cu:711-722) */
+    int    sticky;              /* Sticky [0]: surface voxels of this material attach to each other
+                                   (VX3_VoxelyzeKernel.cu:739) */
 } vx3h_material_desc;
 
 /* degree-of-freedom bits (Voxelyze3/src/old/types.h:48-55) */
@@ -194,6 +196,21 @@ typedef struct vx3h_sim_desc {
      * Pairs come from this engine's contact lists (the watch radius, 2.5 voxels by
      * default, covers the 1.25-voxel envelope; voxels within the nearby link-hop
      * radius of each other are not paired). */
+    /* VX3 attach (Simulator.AttachDetach.EnableAttach; handle_collision_attachment,
+     * VX3_VoxelyzeKernel.cu:724-826): at each step's start, two surface voxels of one
+     * sticky, non-fixed material whose centres are within 0.625 x the sum of their
+     * sizes, which are not within five links of each other and whose attach conditions
+     * both hold, get a new link along the dominant axis of their offset in the first
+     * voxel's frame, if both faces are free.  The new link carries no force in the step
+     * it is made and 1/100 of its force for the next safety_guard steps.  Pairs are
+     * taken in ascending (higher, lower) voxel order, each seeing the links made before
+     * it.  Attach conditions: up to five MathTrees of the voxel position, hit, t and
+     * angle, evaluated after each voxel update; all must be > 0 (0 tokens = no
+     * condition).  Not with collisions. */
+    int    enable_attach;
+    int    n_attach_tokens[5];
+    const vx3h_token* attach_tokens[5];
+    int    safety_guard;          /* SafetyGuard [500] */
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */
@@ -272,6 +289,7 @@ typedef struct vx3h_sim_status {
     double recommended_dt;        /* optimal step before DtFrac (recommendedTimeStep, VX3_VoxelyzeKernel.cu:184-217);
                                      0 until the first step */
     int    collision_count;       /* target contacts so far (collisionCount, the MathTree variable hit) */
+    int    n_links;               /* links so far: the admitted ones plus those made by attach */
 } vx3h_sim_status;
 
 vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out);
@@ -303,6 +321,9 @@ typedef struct vx3h_state_view {
     int*    link_small_angle; /* L */
     double* init_pos;  /* 3*V initial positions (the lattice positions, or those saved by a
                           secondary experiment's re-initialization) */
+    int     link_capacity;  /* 0: the link arrays hold the admitted links; > 0: they hold this
+                               many entries, and links made by attach follow the admitted ones
+                               in the order they were made */
 } vx3h_state_view;
 vx3h_status vx3h_sim_read_state(vx3h_batch* b, int slot, vx3h_state_view* view);
 

@@ -352,6 +352,7 @@ struct Material {
 };
 
 struct MaterialVoxel : Material {
+    bool sticky = false;   // VX3 attach
     double nomSize = 0.001;
     float gravMult = 0;
     float _mass = 0, _massInverse = 0, _sqrtMass = 0, _firstMoment = 0, _momentInertia = 0, _momentInertiaInverse = 0;
@@ -509,6 +510,8 @@ struct Collision;
 struct Link {
     Voxel *pVNeg = nullptr, *pVPos = nullptr;
     bool gone = false;   // VX3 detach: no longer updated, no longer in its voxels' links
+    bool fresh = false;  // VX3 attach: made this step, first updated at the next step's link pass
+    int isNewLink = 0;   // VX3 attach: steps left carrying 1/100 of its force (VX3_Link.cu:206-213)
     Vd forceNeg, forcePos, momentNeg, momentPos;
     float strain = 0, maxStrain = 0, strainOffset = 0;
     int boolStates = 0;
@@ -573,6 +576,7 @@ struct Voxel {
     // (VX3_Voxel.cu:178-184); used when the simulation has either
     bool extras = false;
     bool removed = false;   // VX3 secondary experiment: frozen, without links
+    bool enableAttach = true;   // VX3 attach conditions as of the last update (VX3_Voxel.h:302)
     bool target = false;    // material isTarget
     // VX3 signals (VX3_Voxel.h:305-309): local signal and its decay time, next pacemaker
     // pulse, deaf-until time, pending signal, and the signal posted this step
@@ -1015,6 +1019,8 @@ struct Engine {
     float currentTime = 0, ambientTemp = 0, grav = 0;
     bool floor = false, collisions = false;
     bool detach = false, surfaceChanged = false;   // VX3 detach of failed links
+    bool attach = false;                           // VX3 attach of touching sticky voxels
+    int safetyGuard = 500;
     std::function<void()> afterContacts;           // VX3 target contacts, between the contact and voxel updates
     float boundingRadius = 0.75f, watchDistance = 1.0f;
     std::vector<MaterialVoxel*> voxelMats;
@@ -1224,15 +1230,83 @@ struct Engine {
         if (collisionsStale) regenerateCollisions(watchRadiusMm * watchRadiusMm);
         for (Collision* c : collisionsList) c->updateContactForce();
     }
+    // is_neighbor (VX3_VoxelyzeKernel.cu:651-680)
+    static bool linkedWithin(Voxel* a, Voxel* t, Link* in, int depth) {
+        if (a == t) return true;
+        if (depth <= 0) return false;
+        for (int i = 0; i < 6; i++) {
+            Link* l = a->links[i];
+            if (!l || l == in) continue;
+            if (linkedWithin(l->pVNeg == a ? l->pVPos : l->pVNeg, t, l, depth - 1)) return true;
+        }
+        return false;
+    }
+    // the attach half of handle_collision_attachment (VX3_VoxelyzeKernel.cu:724-826) over the
+    // pairs of gpu_update_attach (:833-843), surface voxels first > second, taken one after
+    // another in that order
+    void updateAttach() {
+        std::vector<Voxel*> surf;
+        for (Voxel* v : voxelsList)
+            if (v->isSurface() && !v->removed) surf.push_back(v);
+        for (size_t i = 1; i < surf.size(); i++)
+            for (size_t j = 0; j < i; j++) {
+                Voxel *v1 = surf[i], *v2 = surf[j];
+                if (v1->mat->fixed && v2->mat->fixed) continue;
+                const double wd = (v1->baseSizeAverage() + v2->baseSizeAverage()) * 0.625;
+                const Vd diff = v1->pos - v2->pos;
+                if (diff.x > wd || diff.x < -wd || diff.y > wd || diff.y < -wd || diff.z > wd || diff.z < -wd) continue;
+                if (diff.Length() > wd) continue;
+                if (linkedWithin(v1, v2, nullptr, 1)) continue;
+                if (!v1->enableAttach || !v2->enableAttach) continue;
+                if (v1->mat->fixed || v2->mat->fixed) continue;
+                if (v1->mat != v2->mat || !v1->mat->sticky) continue;
+                if (linkedWithin(v1, v2, nullptr, 5)) continue;
+                const Vd e = v1->pos - v2->pos;
+                const Vd ea = v1->orient.RotateVec3DInv(-e);
+                const Vd f(fabs(ea.x), fabs(ea.y), fabs(ea.z));
+                int ax;
+                double c;
+                if (f.x >= f.y && f.x >= f.z) { ax = X_AXIS; c = ea.x; }
+                else if (f.y >= f.x && f.y >= f.z) { ax = Y_AXIS; c = ea.y; }
+                else { ax = Z_AXIS; c = ea.z; }
+                const bool rev = c < 0;
+                const int d1 = 2 * ax + (rev ? 1 : 0), d2 = d1 ^ 1;
+                if (v1->links[d1] || v2->links[d2]) continue;
+                Link* L = new Link();
+                L->axis = ax;
+                L->pVPos = rev ? v1 : v2;   // VX3_Link device ctor (VX3_Link.cu:30-55)
+                L->pVNeg = rev ? v2 : v1;
+                L->mat = combinedMaterial(v1->mat, v2->mat);
+                L->boolStates = 0;
+                L->valid = true;
+                L->reset();
+                L->fresh = true;
+                L->isNewLink = safetyGuard;
+                v1->links[d1] = L;
+                v2->links[d2] = L;
+                linksList.push_back(L);
+                surfaceChanged = true;
+            }
+    }
     // Voxelyze.cpp:251-284 (fixed-material skips follow VX3_VoxelyzeKernel.cu:572,588)
     bool doTimeStep(float dt) {
         if (dt == 0) return true;
         else if (dt < 0) dt = recommendedTimeStep();
         bool Diverged = false;
+        // VX3 runs updateAttach after the link pass (VX3_VoxelyzeKernel.cu:289-291); a link it
+        // makes carries no force until the next step's link pass, so attaching before this
+        // step's link pass and skipping the new links there is the same
+        if (attach) updateAttach();
         for (Link* l : linksList) {
             if (l->pVPos->mat->fixed && l->pVNeg->mat->fixed) continue;
             if (l->gone) continue;
+            if (l->fresh) { l->fresh = false; continue; }
             l->updateForces();
+            if (l->isNewLink) {   // VX3_Link.cu:206-213
+                l->forceNeg = l->forceNeg * 0.01; l->forcePos = l->forcePos * 0.01;
+                l->momentNeg = l->momentNeg * 0.01; l->momentPos = l->momentPos * 0.01;
+                l->isNewLink -= 1;
+            }
             if (l->axialStrain() > 100) Diverged = true;
         }
         if (Diverged) return false;
@@ -1278,6 +1352,7 @@ struct Sim {
     float lastDt = 0;
     VoxcadStop voxcad;   // VoxCAD stop-condition family and equilibrium mode
     int hits = 0;        // collisionCount (the MathTree variable hit)
+    std::vector<vx3h_token> attachTok[5];   // VX3 attach conditions
 
     // handle_collision_attachment (VX3_VoxelyzeKernel.cu:682-722) over the watched pairs:
     // a target and another voxel, not both fixed, with centres within 0.625 x the sum
@@ -1421,6 +1496,16 @@ struct Sim {
             for (Voxel* v : vx.voxelsList) v->updateSurface();
         }
         if (!vx.doTimeStep(dt)) return false;   // target contacts and extra forces run inside, after the contact update
+        if (vx.attach)   // VX3_VoxelyzeKernel.cu:608-621, after each voxel's update; a missing condition does not restrict
+            for (Voxel* v : vx.voxelsList) {
+                if (v->mat->fixed || v->removed) continue;
+                bool on = true;
+                for (int k = 0; k < 5 && on; k++)
+                    if (!attachTok[k].empty())
+                        on = mathtree_eval(attachTok[k].data(), (int)attachTok[k].size(), v->pos.x, v->pos.y, v->pos.z, hits,
+                                           currentTime, recentAngle, 0, 0, (int)vx.voxelsList.size()) > 0;
+                v->enableAttach = on;
+            }
         if (signals) {   // every moving voxel on its own state, then every voxel takes its neighbours' posts
             for (Voxel* v : vx.voxelsList)
                 if (!v->mat->fixed && !v->removed) v->signalsOwn(currentTime);
@@ -1476,6 +1561,7 @@ static bool copyMat(const vx3h_material_desc& d, MaterialVoxel* p) {
     p->isMeasured = d.is_measured;
     p->thermalOnAfter = d.thermal_on_after;
     p->fixed = d.fixed != 0;
+    p->sticky = d.sticky != 0;
     switch (d.mat_model) {
     case VX3H_MDL_LINEAR: p->setModelLinear((float)d.elastic_mod); break;
     case VX3H_MDL_LINEAR_FAIL: p->setModelLinear((float)d.elastic_mod, (float)d.fail_stress); break;
@@ -1572,6 +1658,11 @@ static Sim* build(const vx3h_sim_desc* d) {
     };
     s->extras = s->ciliaOn || !s->ffTok[0].empty() || !s->ffTok[1].empty() || !s->ffTok[2].empty() || s->signals || anyTarget;
     vx.detach = d->enable_detach != 0;
+    vx.attach = d->enable_attach != 0;
+    vx.safetyGuard = d->safety_guard;
+    if (vx.attach)
+        for (int k = 0; k < 5; k++)
+            if (d->n_attach_tokens[k] > 0) s->attachTok[k].assign(d->attach_tokens[k], d->attach_tokens[k] + d->n_attach_tokens[k]);
     s->secondary = d->secondary_experiment != 0;
     s->reinitAfter = d->reinit_after;
     for (int i = 0; i < d->n_materials; i++) s->matRemoveAfter.push_back(d->materials[i].remove_after);

@@ -145,7 +145,11 @@ class OSim:
         return self.lib.vxo_recommended_dt(self.h)
 
     def step(self, n):
-        return self.lib.vxo_vx3_step(self.h, n)
+        r = self.lib.vxo_vx3_step(self.h, n)
+        nl = C.c_int()
+        self.lib.vxo_counts(self.h, None, C.byref(nl))   # attach adds links
+        self.nl = nl.value
+        return r
 
     def status(self):
         st = _abi.SimStatus()

@@ -742,3 +742,96 @@ def test_target_hit_sends_a_signal_and_feeds_the_fitness():
     assert st.collision_count > 0 and st.fitness == st.collision_count
     sig = o.local_signal()
     assert sig[m.mat == 0].max() > 0 and sig[m.mat == 1].max() == 0   # only non-target voxels receive
+
+
+# ---- VX3 attach (handle_collision_attachment, VX3_VoxelyzeKernel.cu:724-826)
+
+def _sticky_bars(n=7, sticky=(1, 1), gap=2, guard=500, conditions=None, dt=None):
+    """Two n-voxel bars along x, one on the floor and one `gap` voxels above it, falling
+    onto it with collisions off; material 0 for the lower bar, 1 (or 0) for the upper."""
+    m = Model(voxel_size=0.01)
+    kw = dict(elastic_mod=1e6, density=1e3, u_static=1.0, u_dynamic=0.5, zeta_internal=1.0, zeta_global=0.01,
+              zeta_collision=0.8)
+    m.materials = [Material(sticky=sticky[0], **kw), Material(sticky=sticky[1], **kw)]
+    x = np.arange(n)
+    m.ijk = np.concatenate([np.stack([x, 0 * x, 0 * x], 1), np.stack([x, 0 * x, 0 * x + gap], 1)]).astype(np.int32)
+    m.mat = np.array([0] * n + [1 if sticky[0] != sticky[1] else 0] * n, np.int32)
+    m.gravity_mult = np.float32(1.0)
+    m.floor_enabled = 1
+    m.collisions_enabled = 0
+    m.enable_attach = 1
+    m.safety_guard = guard
+    if conditions:
+        m.attach_conditions = conditions
+    if dt:
+        m.dt_fixed = dt
+    return m
+
+
+def _links_between(o, n):
+    vneg, vpos, axis = o.topology()[:3]
+    return sorted((int(a), int(b), int(ax)) for a, b, ax in zip(vneg, vpos, axis) if (a < n) != (b < n))
+
+
+def test_attach_bars_take_pairs_in_order_and_skip_those_within_five_links():
+    # the upper bar falls until the vertical pairs come within 0.625 x (1 + 1) voxels; taken in
+    # order, (B0, A0) links first; (B1, A1) and (B2, A2) are then within 3 and 5 links, (B3, A3)
+    # 7 links apart attaches, and so on: new links at 0, 3 and 6, each along z from the lower
+    # voxel (negative end) to the upper one
+    n = 7
+    o = OSim(_sticky_bars(n))
+    nl0 = o.nl
+    o.step(3000)
+    assert o.nl == nl0 + 3
+    assert _links_between(o, n) == [(0, n, 2), (3, n + 3, 2), (6, n + 6, 2)]
+
+
+def test_attach_needs_one_sticky_material_and_its_conditions():
+    for m in (_sticky_bars(sticky=(0, 0)),                      # not sticky
+              _sticky_bars(sticky=(1, 2)),                      # two sticky materials
+              _sticky_bars(conditions=[[(MT["mtCONST"], 1.0), (MT["mtEND"], 0.0)], [],
+                                       [(MT["mtCONST"], -1.0), (MT["mtEND"], 0.0)], [], []])):   # one condition fails
+        o = OSim(m)
+        nl0 = o.nl
+        o.step(3000)
+        assert o.nl == nl0
+    # a condition of the voxel height, 11.5 mm - z > 0: the upper bar's voxels qualify only
+    # once they have fallen below it, later than the 12.5 mm envelope alone; the links are the same
+    cond = [[(MT["mtVAR"], 2.0), (MT["mtCONST"], 0.0115), (MT["mtSUB"], 0.0), (MT["mtEND"], 0.0)], [], [], [], []]
+    made = []
+    for c in (None, cond):
+        o = OSim(_sticky_bars(conditions=c))
+        k, nl0 = 0, o.nl
+        while o.nl == nl0 and k < 3000:
+            o.step(1)
+            k += 1
+        made.append((k, min(o.state()["pos"][7:, 2])))
+        o.step(3000 - k)
+        assert _links_between(o, 7) == [(0, 7, 2), (3, 10, 2), (6, 13, 2)]
+    assert made[1][0] > made[0][0] and made[1][1] < 0.0115 < made[0][1]
+
+
+def test_new_link_carries_no_force_then_one_hundredth():
+    # three runs in lockstep: no guard, the guard, and no sticky material.  They are equal
+    # through the step that makes the link (it carries no force there); at the next step
+    # the upper voxel's momentum change is G + L without the guard and G + L/100 with it,
+    # G being the change without the link
+    sims = [OSim(_sticky_bars(n=1, guard=0, dt=1e-4)), OSim(_sticky_bars(n=1, guard=500, dt=1e-4)),
+            OSim(_sticky_bars(n=1, sticky=(0, 0), dt=1e-4))]
+    made = None
+    for k in range(4000):
+        prev = [o.state() for o in sims]
+        for o in sims:
+            o.step(1)
+        if sims[1].nl == 1:
+            made = k
+            break
+    assert made is not None and sims[0].nl == 1 and sims[2].nl == 0
+    now = [o.state() for o in sims]
+    assert np.array_equal(now[0]["pos"], now[2]["pos"]) and np.array_equal(now[1]["lin_mom"], now[2]["lin_mom"])
+    for o in sims:
+        o.step(1)
+    after = [o.state() for o in sims]
+    X, Y, G = [a["lin_mom"][1, 2] - b["lin_mom"][1, 2] for a, b in zip(after, now)]
+    assert abs(X - G) > 1e3 * abs(G) * 1e-12
+    assert (Y - G) == pytest.approx(0.01 * (X - G), rel=1e-9)

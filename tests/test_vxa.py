@@ -285,3 +285,25 @@ def test_report_xml_layout_and_order():
     x = engine.report_xml("d", [("r.vxd", _status(1.0, nv=2), 0.01, pos)])
     assert "<init_pos>0.000000,0.000000,0.000000;0.010000,0.000000,0.000000;</init_pos>" in x
     assert "<pos>0.001000,0.000000,0.000000;0.011111,0.000000,0.000000;</pos><mats>1;2;</mats>" in x
+
+
+def test_attach_keys_round_trip():
+    # Sticky, EnableAttach, SafetyGuard and AttachCondition.Condition_k (VX3_SimulationManager.cu:329-343,
+    # VXA/VX_Object.cpp:1436) reach the description, and the oracle run from it attaches as from the model
+    import sys
+    sys.path.insert(0, __import__("os").path.dirname(__file__))
+    from test_oracle_kat import _sticky_bars, _links_between
+    from gpuvoxels_amd.model import MT
+    from gpuvoxels_amd.vxa import model_to_vxa
+    cond = [[(MT["mtVAR"], 2.0), (MT["mtCONST"], 0.0115), (MT["mtSUB"], 0.0), (MT["mtEND"], 0.0)], [], [], [], []]
+    m = _sticky_bars(7, conditions=cond, guard=77)
+    v = engine.Vxa(model_to_vxa(m))
+    d = v.desc
+    assert d.enable_attach == 1 and d.safety_guard == 77 and d.n_attach_tokens[0] == 4
+    assert [d.n_attach_tokens[k] for k in range(1, 5)] == [0, 0, 0, 0]
+    assert d.materials[0].sticky == 1
+    a, b = OSim(m), OSim.from_desc(d)
+    a.step(3000)
+    b.step(3000)
+    assert _links_between(a, 7) == _links_between(b, 7) == [(0, 7, 2), (3, 10, 2), (6, 13, 2)]
+    assert np.array_equal(a.state()["pos"], b.state()["pos"])
