@@ -24,7 +24,8 @@
  *   MathTree             Voxelyze3/src/Utils/VX3_MathTree.h:50-192
  *   VX3 extras           cilia VX3_VoxelyzeKernel.cu:846-859, VX3_Voxel.cu:376; force field
  *                        VX3_Voxel.cu:178-184; detach VX3_VoxelyzeKernel.cu:946-967; voxel
- *                        removal and reset :336-350,365-397; signals VX3_Voxel.cu:271-348
+ *                        removal and reset :336-350,365-397; signals VX3_Voxel.cu:271-348;
+ *                        attach VX3_VoxelyzeKernel.cu:608-621,651-680,724-843, VX3_Link.cu:30-55,206-213
  *   VoxCAD stop family   vx_voxcad_stop.h (VoxCAD/src/VX1/VX_Sim.cpp:862-891,1015-1160,1833-1837)
  *
  * How each part is pinned: the CPU engine and importer bit-exactly against the
@@ -33,7 +34,11 @@
  * brute-force restatements in tests/test_oracle_kat.py (phase temperature, dt, CoM,
  * fitness, stop counts; cilia momentum, rotation and surface rule; force fields;
  * detach; removal and reset; signal hops, pacemaker period and decay, cilia
- * coupling).  Signals follow a deterministic two-phase order where the reference's
+ * coupling; attach: which pairs link, worked out by hand from the five-link rule,
+ * the sticky-material and condition gates, no force in the step a link is made and
+ * exactly 1/100 of it at the next).  Attach takes pairs one after another in the
+ * reference's pair order where its threads race (VX3_VoxelyzeKernel.cu:833-843): that
+ * order is this oracle's definition, as for signals.  Signals follow a deterministic two-phase order where the reference's
  * threads race (VX3_Voxel.cu:323-348): that order is this oracle's definition, so
  * signal parity with the reference is unpinned beyond the single-hop behaviour.
  */
