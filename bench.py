@@ -19,9 +19,14 @@ workgroup size, steps per resident launch) is the autotuner's choice
 (vx3h_autotune) on this same workload, reported under `autotune`.
 
 Printed: one JSON line with voxel-steps/s (value), the roofline of the dominant
-kernel (HIP events on the engine's stream), the plain one-simulation-per-launch
-HIP path on the same robots, and the CPU engine timed on this host's cores on a
-bounded sample of whole robots, whose final fitness values also check the GPU's.
+kernel (HIP events on the engine's stream; for the resident kernel also its FP64
+rate against FP64 peak from the committed counter profile), the plain
+one-simulation-per-launch HIP path on the same robots, the other configurations as
+single simulations (6^3 cube, 64x64x32 slab with self-collision, 2,048 arena balls
+in FP64 and FP32), the serving stream (capacity, then Poisson arrivals at 50% and
+80% of it with p50/p99 latency, and the plain path on the same requests), the CPU
+engine timed on this host's cores on a bounded sample of whole robots, and the
+final fitness of every robot of the batch against the CPU engine.
 """
 import argparse
 import json
@@ -34,6 +39,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix)
 
 
 def log(*a):
@@ -49,6 +55,32 @@ def bytes_per_step(V, L, f64):
     if f64:
         return 232 * V + 196 * L   # 13*8*2 + 8 + 8 + 4 + 4 ; (9*8 + 16)*2 + 4 + 8 + 8
     return 124 * V + 124 * L       # 13*4*2 + 8 + 4 + 4 + 4 ; 13*4*2 + 4 + 8 + 8
+
+
+def measured_per_launch(fname, dtype, spl):
+    """A per-launch figure measured with rocprofv3 and kept in profiles/<fname> as lines
+    "<dtype> <steps per launch> <value>": the least-squares line a + b*steps through the
+    measured launch lengths, evaluated at spl (exact when spl was measured)."""
+    import numpy as np
+    f = os.path.join(ROOT, "profiles", fname)
+    if not os.path.exists(f):
+        return None
+    pts = []
+    for line in open(f):
+        w = line.split()
+        if len(w) == 3 and w[0] == dtype and w[1].isdigit():
+            pts.append((int(w[1]), float(w[2])))
+    if not pts:
+        return None
+    for n, v in pts:
+        if n == spl:
+            return v
+    if len(set(n for n, _ in pts)) == 1:
+        return pts[0][1] if len(pts) == 1 else float(np.mean([v for _, v in pts]))
+    x = np.array([n for n, _ in pts], float)
+    y = np.array([v for _, v in pts], float)
+    b_, a_ = np.polyfit(x, y, 1)
+    return float(a_ + b_ * spl)
 
 
 def run_batch(models, precision, path, block, steps, warmup, kt_steps, spl):
@@ -154,29 +186,19 @@ def gpu_fitness(models, precision, steps):
     return out
 
 
-def serving_run(precision, seconds, rate, seed=4):
-    """Poisson arrivals of robots whose voxel counts are log-uniform in [100, 50000]
-    (the 12^3 robot generator on scaled grids) and horizons uniform in [2000, 20000]
-    steps, served by the continuous-batching loop; per-request latency is submit to
-    result.  Returns (summary dict, request models)."""
+def serve_stream(precision, reqs, quantum=500):
+    """Runs an arrival stream [(arrival time, model, horizon)] through the
+    continuous-batching server in real time; latency is submit to result."""
     import numpy as np
-    from gpuvoxels_amd import engine, workloads
-    rng = np.random.default_rng(seed)
-    t, reqs = 0.0, []
-    while t < seconds:
-        t += rng.exponential(1.0 / rate)
-        target = float(np.exp(rng.uniform(np.log(100), np.log(50000))))
-        grid = max(5, int(round((target / 0.475) ** (1.0 / 3.0))))
-        m = workloads.robot(rng, grid=grid, steps=0, name="req")
-        reqs.append((t, m, int(rng.integers(2000, 20001))))
-    sv = engine.Server(precision=precision, quantum_steps=500)
-    lat, vsteps, i, tickets = [], 0.0, 0, {}
+    from gpuvoxels_amd import engine
+    sv = engine.Server(precision=precision, quantum_steps=quantum)
+    lat, vsteps, i = [], 0.0, 0
     t0 = sv.clock()
     first = last = None
     while True:
         now = sv.clock() - t0
         while i < len(reqs) and reqs[i][0] <= now:
-            tickets[sv.submit(reqs[i][1], reqs[i][2])] = i
+            sv.submit(reqs[i][1], reqs[i][2])
             i += 1
         active = sv.round()
         for r in sv.results():
@@ -191,11 +213,146 @@ def serving_run(precision, seconds, rate, seed=4):
     sv.close()
     lat = np.array(lat)
     span = last - first
-    return {"requests": len(reqs), "rate_per_s": rate, "arrival_seconds": seconds,
+    return {"requests": len(reqs), "value": vsteps / span, "unit": "voxel-steps/s", "seconds": span,
             "voxels_min_max": [int(min(m.n_voxels for _, m, _ in reqs)), int(max(m.n_voxels for _, m, _ in reqs))],
-            "value": vsteps / span, "unit": "voxel-steps/s", "seconds": span,
-            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
-            "dtype": "f64" if precision == 1 else "f32"}, reqs
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99))}
+
+
+def serving_legs(precision, seconds, burst, seed=4):
+    """The serving stream (robots of log-uniform 100..50,000 voxels, horizons uniform in
+    2,000..20,000 steps).  Capacity: `burst` requests submitted at once (the server
+    saturated).  Then Poisson arrivals for `seconds` at 50% and 80% of that capacity
+    (rate = fraction x capacity / mean voxel-steps per request), each reporting
+    throughput and p50/p99 latency, and the plain one-simulation-per-launch path on the
+    burst's requests."""
+    import numpy as np
+    from gpuvoxels_amd import workloads
+    rng = np.random.default_rng(seed + 1000)
+    reqs = [(0.0,) + workloads.serving_request(rng, name="burst%04d" % i) for i in range(burst)]
+    work = float(np.mean([m.n_voxels * h for _, m, h in reqs]))
+    cap = serve_stream(precision, reqs)
+    out = {"dtype": "f64" if precision == 1 else "f32", "capacity": cap, "mean_voxel_steps_per_request": work}
+    log("serving capacity: %.3e voxel-steps/s (%d requests at once)" % (cap["value"], burst))
+    for frac in (0.5, 0.8):
+        rate = frac * cap["value"] / work
+        leg = serve_stream(precision, workloads.serving_stream(seconds, rate, seed=seed))
+        leg.update({"load_fraction": frac, "rate_per_s": rate, "arrival_seconds": seconds})
+        out["load_%d" % int(frac * 100)] = leg
+        log("serving at %d%% load (%.1f req/s): %s" % (frac * 100, rate, leg))
+    pv, _ = plain_path([m for _, m, _ in reqs], precision, 20)
+    out["plain_path"] = {"value": pv, "unit": "voxel-steps/s", "requests": len(reqs), "steps": 20}
+    out["capacity_vs_plain"] = cap["value"] / pv
+    return out
+
+
+def config_leg(model, precision, steps, interval=0, kernel_steps=50, warmup=10):
+    """One configuration as a single simulation: admitted, `warmup` steps, then `steps`
+    timed steps bracketed by stream synchronization; per-pass kernel durations from HIP
+    events over `kernel_steps` further steps, with the link and voxel passes priced at
+    their algorithmic bytes (FP64: 196 B/link, 232 B/voxel; FP32: 124 B each)."""
+    from gpuvoxels_amd import _abi, engine
+    f64 = precision == _abi.PREC_F64
+    b = engine.Batch(precision=precision, path=_abi.PATH_AUTO, collision_rebuild_interval=interval)
+    t0 = time.perf_counter()
+    s = b.admit(model)
+    admit_s = time.perf_counter() - t0
+    V = model.n_voxels
+    L = int(len(engine.topology(model)[0]))
+    b.step(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    dt = time.perf_counter() - t0
+    st = b.poll()[s]
+    kt = b.kernel_times(kernel_steps) if kernel_steps else {}
+    b.close()
+    out = {"value": V * steps / dt, "unit": "voxel-steps/s", "voxels": V, "links": L, "steps": steps,
+           "ms_per_step": dt / steps * 1e3, "dtype": "f64" if f64 else "f32", "admit_s": admit_s,
+           "state": int(st.state), "steps_done": int(st.steps), "kernel_ms": kt,
+           "alg_frac_per_step": bytes_per_step(V, L, f64) / (dt / steps) / 1e9 / HBM_PEAK_GBS}
+    if "links" in kt:
+        out["links_hbm_frac"] = (196 if f64 else 124) * L / (kt["links"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+        out["links_hbm_frac_at_124B"] = 124 * L / (kt["links"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    if "voxels" in kt:
+        out["voxels_hbm_frac"] = (232 if f64 else 124) * V / (kt["voxels"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+    return out
+
+
+def precision_horizon(models, horizons, cpu_robots, threads, kind):
+    """Fitness error against horizon on the robot batch, as evidence for the precision
+    choice.  At each horizon (steps), relative fitness differences of:
+      f32_vs_f64     FP32 state vs FP64 state, every robot;
+      f64_vs_cpu     FP64 state vs the CPU engine, the first cpu_robots robots;
+      f64_perturbed_1e-9 / 1e-14  FP64 runs whose voxels start with random velocities
+                     of that many m/s (N(0, eps) per axis) vs the unperturbed FP64 run:
+                     how far the workload itself carries a perturbation of about FP32
+                     (1e-9 m/s against velocities of ~1e-2 m/s) or FP64 rounding size."""
+    import copy
+    import numpy as np
+    from gpuvoxels_amd import _abi, engine
+
+    def gpu_curve(ms, precision):
+        b = engine.Batch(precision=precision, path=_abi.PATH_AUTO)
+        slots = [b.admit(m) for m in ms]
+        out, done = [], 0
+        for h in horizons:
+            b.step(h - done)
+            done = h
+            st = b.poll()
+            out.append(np.array([st[s].fitness for s in slots]))
+        b.close()
+        return out
+
+    def stats(a, ref):
+        rel = np.abs(a - ref) / np.abs(ref)
+        return {"max": float(rel.max()), "median": float(np.median(rel)), "within_0.5%": int((rel <= 0.005).sum()),
+                "robots": int(len(rel))}
+
+    f64 = gpu_curve(models, _abi.PREC_F64)
+    f32 = gpu_curve(models, _abi.PREC_F32)
+    res = {"horizons": horizons, "robots": len(models)}
+    res["f32_vs_f64"] = [stats(a, r) for a, r in zip(f32, f64)]
+    for eps in (1e-9, 1e-14):
+        rng = np.random.default_rng(123)
+        pert = []
+        for m in models:
+            q = copy.copy(m)
+            q.init_velocity = rng.normal(0.0, eps, size=(m.n_voxels, 3))
+            pert.append(q)
+        res["f64_perturbed_%g" % eps] = [stats(a, r) for a, r in zip(gpu_curve(pert, _abi.PREC_F64), f64)]
+    # the CPU engine at the same horizons
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle_lib import OSim, load
+    load(kind)
+
+    def cpu_one(m):
+        o = OSim(m, kind=kind)
+        fits, done = [], 0
+        for h in horizons:
+            o.step(h - done)
+            done = h
+            fits.append(o.status().fitness)
+        return fits
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        cf = np.array(list(ex.map(cpu_one, models[:cpu_robots])))
+    res["f64_vs_cpu"] = [stats(f64[i][:cpu_robots], cf[:, i]) for i in range(len(horizons))]
+    res["f32_vs_cpu"] = [stats(f32[i][:cpu_robots], cf[:, i]) for i in range(len(horizons))]
+    res["cpu_kind"] = "reference" if kind == "ref" else "port"
+    return res
+
+
+def host_threads():
+    """Worker threads for the CPU engine: OMP_NUM_THREADS when set (the GPU host's
+    share of cores), else the cores this process may run on."""
+    n = os.environ.get("OMP_NUM_THREADS")
+    if n and n.isdigit() and int(n) > 0:
+        return int(n)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def main():
@@ -223,8 +380,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-seconds", type=float, default=6.0, help="arrival window of the serving run (0 = skip)")
-    ap.add_argument("--serving-rate", type=float, default=40.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serving-seconds", type=float, default=8.0,
+                    help="arrival window of each serving load level (0 = skip serving)")
+    ap.add_argument("--serving-burst", type=int, default=100,
+                    help="requests submitted at once to measure the serving capacity (also the plain path's requests)")
+    ap.add_argument("--fitness-robots", type=int, default=512,
+                    help="robots whose final fitness is checked against the CPU engine (0 = skip)")
+    ap.add_argument("--precision-horizon", action="store_true",
+                    help="instead of the benchmark: fitness error against horizon (FP32 vs FP64, FP64 vs the CPU "
+                         "engine, FP64 under tiny initial-velocity perturbations) on the robot batch")
+    ap.add_argument("--no-legs", dest="legs", action="store_false",
+                    help="skip the single-cube, slab and arena-balls configuration legs")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (two launches: warm-up + timed), a kernel of "
                          "known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -248,6 +414,13 @@ def main():
         copy_gbs = 2 * nb / (ms * 1e-3) / 1e9
         log("plain copy of %d MiB: %.3f ms, %.0f GB/s read+write" % (args.calib_copy_mib, ms, copy_gbs))
     models = workloads.robots(args.robots, seed=args.seed, steps=0)
+    if args.precision_horizon:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import ref_available
+        kind = "ref" if ref_available() else "oracle"
+        print(json.dumps(precision_horizon(models, [1000, 2000, 5000, 10000, 20000], args.cpu_robots, host_threads(),
+                                           kind)))
+        return
     V = sum(m.n_voxels for m in models)
     L = int(sum(len(engine.topology(m)[0]) for m in models))
     log("robots %d voxels %d links %d" % (len(models), V, L))
@@ -288,18 +461,24 @@ def main():
         ab, achieved = None, None
     traffic = args.traffic
     if traffic is None and dom == "resident":
-        # measured HBM bytes per resident launch, kept with the rocprofv3 summaries:
-        # lines "<dtype> <steps per launch> <bytes>"
-        tf = os.path.join(ROOT, "profiles", "hbm_traffic_resident_512robots.txt")
-        if os.path.exists(tf):
-            for line in open(tf):
-                f = line.split()
-                if len(f) == 3 and f[0] == args.precision and f[1].isdigit() and int(f[1]) == spl:
-                    traffic = float(f[2])
+        traffic = measured_per_launch("hbm_traffic_resident_512robots.txt", args.precision, spl)
+    flops_step = measured_per_launch("fp64_flops_resident_512robots.txt", args.precision, 1) if dom == "resident" else None
+    per_step_survey = 124 * (V + L)
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
             "kernel": dom, "alg_bytes_per_launch": ab, "steps_per_launch": spl, "avg_launch_ms": kt[dom],
-            "alg_bytes_per_step": per_step, "kernel_ms": kt}
+            "alg_bytes_per_step": per_step, "kernel_ms": kt,
+            # the same launches priced at 124 B per voxel and per link (FP32 structure-of-arrays bytes)
+            "frac_at_124B": (per_step_survey * spl / (kt[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS) if dom == "resident" else None}
+    if dom == "resident":
+        # the resident kernel keeps the state on chip: its measured HBM traffic per launch
+        # is ~0.1% of the algorithmic bytes, so what bounds it is on chip (FP64 issue, LDS
+        # and barrier waits; counters in profiles/), priced here against FP64 peak
+        roof["limiter"] = "on-chip: FP64 VALU issue, LDS access and barrier waits (HBM traffic per launch is measured, not bounding)"
+        if flops_step:
+            tf = flops_step * spl / (kt[dom] * 1e-3) / 1e12
+            roof["compute"] = {"kind": "fp64 vector", "flops_per_step": flops_step, "achieved": tf,
+                               "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS}
 
     out = {
         "metric": "voxel-steps/s on 512-robot batch vs plain path; p50/p99 per-sim latency",
@@ -334,34 +513,45 @@ def main():
         out["plain_path"] = {"value": pv, "unit": "voxel-steps/s", "steps": args.plain_steps, "robots": len(models),
                              "dtype": args.precision, "batched_speedup": value / pv}
         log("plain path %.3e voxel-steps/s (%.2f s) -> batched speedup %.1fx" % (pv, pdt, value / pv))
+    if args.legs:
+        legs = {}
+        m0 = workloads.cube(6, steps=0)
+        legs["C0_single_cube_f64"] = config_leg(m0, _abi.PREC_F64, 10000, kernel_steps=0)
+        m2 = workloads.slab(64, 64, 32, steps=0)
+        legs["C2_slab_f64"] = config_leg(m2, _abi.PREC_F64, 5000)
+        m3 = workloads.arena_balls(steps=0)
+        legs["C3_arena_balls_f64"] = config_leg(m3, _abi.PREC_F64, 3000, interval=8)
+        legs["C3_arena_balls_f32"] = config_leg(m3, _abi.PREC_F32, 3000, interval=8)
+        for k, v in legs.items():
+            log("leg %s: %.3e voxel-steps/s, %.4f ms/step, kernels %s" % (k, v["value"], v["ms_per_step"], v["kernel_ms"]))
+        out["config_legs"] = legs
     if args.serving_seconds > 0:
-        srv, reqs = serving_run(prec, args.serving_seconds, args.serving_rate)
-        # the plain path on the same requests: each alone, one step per launch sequence
-        sub = [m for _, m, _ in reqs[:8]]
-        pv, _ = plain_path(sub, prec, 20)
-        srv["plain_path"] = {"value": pv, "unit": "voxel-steps/s", "requests": len(sub), "steps": 20}
-        srv["speedup_vs_plain"] = srv["value"] / pv
-        out["serving"] = srv
-        log("serving: %s" % srv)
+        out["serving"] = serving_legs(prec, args.serving_seconds, args.serving_burst)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_lib import ref_available
         kind = "ref" if ref_available() else "oracle"
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         sample = models[: args.cpu_robots]
-        cv, cdt, cfit = cpu_baseline(sample, args.cpu_steps, threads, kind)
+        cv, cdt, _ = cpu_baseline(sample, args.cpu_steps, threads, kind)
         out["cpu_baseline"] = {"value": cv, "unit": "voxel-steps/s", "cores": threads,
                                "kind": "reference" if kind == "ref" else "port",
                                "sample": "%d whole robots (the batch's first) x %d steps, FP64 CPU engine, one robot "
-                                         "per thread" % (len(sample), args.cpu_steps), "seconds": cdt}
+                                         "per thread" % (len(sample), args.cpu_steps), "seconds": cdt,
+                               "host_cpus": os.cpu_count()}
         log("cpu baseline %.3e voxel-steps/s on %d threads (%.1f s, %s)" % (cv, threads, cdt, kind))
-        par = {"robots": len(sample), "steps": args.cpu_steps, "tolerance": 0.005}
-        for name, p in (("f64", _abi.PREC_F64), ("f32", _abi.PREC_F32)):
-            g = gpu_fitness(sample, p, args.cpu_steps)
-            rel = [abs(a - c) / abs(c) if c else abs(a) for a, c in zip(g, cfit)]
-            par[name] = {"max_rel_fitness_err": max(rel), "within": int(sum(r <= 0.005 for r in rel))}
-        out["fitness_parity"] = par
-        log("fitness parity vs CPU engine: %s" % par)
+        if args.fitness_robots:
+            # every robot of the batch over the full horizon: CPU engine vs both GPU precisions
+            chk = models[: args.fitness_robots]
+            _, fdt, cfit = cpu_baseline(chk, args.cpu_steps, threads, kind)
+            par = {"robots": len(chk), "steps": args.cpu_steps, "tolerance": 0.005, "cpu_seconds": fdt}
+            for name, p in (("f64", _abi.PREC_F64), ("f32", _abi.PREC_F32)):
+                g = gpu_fitness(chk, p, args.cpu_steps)
+                rel = [abs(a - c) / abs(c) if c else abs(a) for a, c in zip(g, cfit)]
+                par[name] = {"max_rel_fitness_err": max(rel), "median_rel_fitness_err": sorted(rel)[len(rel) // 2],
+                             "within": int(sum(r <= 0.005 for r in rel))}
+            out["fitness_parity"] = par
+            log("fitness parity vs CPU engine: %s" % par)
     print(json.dumps(out))
 
 

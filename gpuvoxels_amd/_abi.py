@@ -117,6 +117,8 @@ class SimDesc(C.Structure):
         ("n_attach_tokens", C.c_int * 5),
         ("attach_tokens", C.POINTER(Token) * 5),
         ("safety_guard", C.c_int),
+        ("init_velocity", C.POINTER(C.c_double)),
+        ("max_dist_close_pairs", C.c_double),
     ]
 
 
@@ -129,6 +131,7 @@ class BatchCfg(C.Structure):
         ("steps_per_launch", C.c_int),
         ("collision_rebuild_interval", C.c_int),
         ("use_graph", C.c_int),
+        ("collision_table_entries", C.c_int),
     ]
 
 
@@ -149,6 +152,8 @@ class SimStatus(C.Structure):
         ("recommended_dt", C.c_double),
         ("collision_count", C.c_int),
         ("n_links", C.c_int),
+        ("num_close_pairs", C.c_int),
+        ("target_closeness", C.c_double),
     ]
 
 

@@ -19,7 +19,6 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(os.path.dirname(HERE), "_build")
 LIB = os.path.join(HERE, "libvx3h.so")
 WORKER = os.path.join(HERE, "vx3_node_worker")
-LAUNCHER = os.path.join(HERE, "Voxelyze3")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -76,9 +75,6 @@ def build(verbose=False):
     if os.path.exists(worker_src):
         _run(["g++"] + CXX_FLAGS + ["-o", WORKER, worker_src, os.path.join(CSRC, "vx3_manager.cpp"), "-L" + HERE, "-lvx3h",
                                     "-Wl,-rpath,$ORIGIN"])
-    launcher_src = os.path.join(CSRC, "vx3_launcher.cpp")
-    if os.path.exists(launcher_src):
-        _run(["g++"] + CXX_FLAGS + ["-o", LAUNCHER, launcher_src])
     if verbose:
         print("built", LIB)
     return LIB

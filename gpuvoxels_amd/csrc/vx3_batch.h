@@ -52,9 +52,14 @@ template <typename T> struct DevBatch {
     int *nearStart, *nearCount;  // per voxel range in nearList: surface voxels within the link-hop radius
     int* nearList;            // sim-local indices, ascending
     int* nbList;              // global voxel indices
-    int* hashCnt;             // per bucket fill count
-    int* hashEnt;             // VX3_COL_BUCKET voxel indices per bucket
+    int* hashCnt;             // per bucket: surface voxels in it
+    int* hashStart;           // per bucket: offset of its first voxel within its block of VX3_COL_HBLOCK buckets
     int* hashSim;             // owning sim of each bucket, -1 if free
+    int* hashBlk;             // per bucket block: voxels in the block
+    int* hashBlkOff;          // per bucket block: offset of its first voxel in cellVox
+    int* cellVox;             // surface voxels sorted by bucket, a range of nv entries per sim
+    int *vbk, *vrank;         // per voxel: its bucket and its rank within the bucket (last rebuild)
+    int* tgtList;             // voxels of isTarget materials per sim (close pairs), a range per sim
     // tables
     const VoxMatDev* vmatTab;
     const LinkMatDev* lmatTab;
@@ -64,8 +69,10 @@ template <typename T> struct DevBatch {
     const TokenDev* tokens;
     const int* simMats;       // per sim: global voxel-material then link-material table indices it uses
     int Vcap, Lcap, Ccap;     // Ccap: nbList capacity
-    int hashTot;              // buckets in use (high-water mark)
+    int hashTot;              // buckets in use (high-water mark, whole blocks)
     int vHi, lHi, nSims;      // high-water marks: kernels cover [0, vHi) / [0, lHi) / [0, nSims)
+    int solo;                 // >= 0: only this simulation takes the step (on the per-step kernels,
+                              // whichever path it was admitted to); -1: every stream-path simulation
 };
 
 }  // namespace vx3

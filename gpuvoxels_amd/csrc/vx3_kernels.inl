@@ -20,8 +20,13 @@ namespace VX_NS {
 
 typedef VX_REAL real;
 
-__device__ __forceinline__ bool simRunning(const SimCtl* sims, int s) {
-    return s >= 0 && sims[s].state == 0 && !(sims[s].flags & SC_RESIDENT);
+// a stream-path simulation that takes this step (not finished, not on the resident
+// path, not standing still for a larger collision table)
+__device__ __forceinline__ bool simRunning(const DevBatch<real>& b, int s) {
+    if (s < 0 || b.sims[s].state != 0) return false;
+    const int f = b.sims[s].flags;
+    if (b.solo >= 0) return s == b.solo && !(f & SC_COL_GROW);
+    return !(f & (SC_RESIDENT | SC_COL_GROW));
 }
 
 // Precision-dependent transcendentals.  The FP64 state path rounds like the host C
@@ -97,7 +102,7 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= b.lHi) return;
     const int s = b.lsim[l];
-    if (!simRunning(b.sims, s)) return;
+    if (!simRunning(b, s)) return;
     const SimCtl& sc = b.sims[s];
     const int vn = b.lneg[l], vp = b.lpos[l];
     const int meta = b.lmeta[l];
@@ -126,9 +131,10 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
         area = 0.5f * (size * size + size * size);
         tss = 0.0f;
     }
-    // damping multipliers 2*sqrt(m)*zeta/previousDt (VX_Voxel.h:131); material-fixed voxels never step
-    const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : sc.dt);
-    const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : sc.dt);
+    // damping multipliers 2*sqrt(m)*zeta/previousDt (VX_Voxel.h:131), previousDt being the dt of the
+    // voxel's last update; material-fixed voxels never step
+    const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : sc.prevDt);
+    const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : sc.prevDt);
     LinkState<real> st;
     st.pos2 = V3<real>(b.q2x[l], b.q2y[l], b.q2z[l]);
     st.a1v = V3<real>(b.a1x[l], b.a1y[l], b.a1z[l]);
@@ -179,20 +185,29 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
 
 // ---- collisions (CVoxelyze::updateCollisions / regenerateCollisions, Voxelyze.cpp:671-751)
 //
-// Instead of the CPU engine's all-pairs regeneration, every surface voxel of a
-// simulation being rebuilt enters a spatial hash (cell edge = watch radius) and
-// finds its partners in the 27 neighbouring cells.  The partner criterion is the
-// CPU one: both voxels on the surface, centre distance^2 <= watchRadius^2, and the
-// partner not "nearby" (within the link-hop radius, CVX_Voxel::generateNearby).
-// Each voxel keeps its partners in ascending index order -- the order in which
-// CVX_Voxel::force walks the colWatch list built by the pair loop -- and computes
-// the contact force of each pair itself, in the voxel pass.
+// Instead of the CPU engine's all-pairs regeneration, the surface voxels of every
+// simulation being rebuilt are sorted into cells (edge = watch radius) by a counting
+// sort over a per-simulation hash of the cell coordinates, and each voxel finds its
+// partners in the 27 neighbouring cells.  The partner criterion is the CPU one: both
+// voxels on the surface, centre distance^2 <= watchRadius^2, and the partner not
+// "nearby" (within the link-hop radius, CVX_Voxel::generateNearby).  Each voxel keeps
+// its partners in ascending index order -- the order in which CVX_Voxel::force walks
+// the colWatch list built by the pair loop -- and computes the contact force of each
+// pair itself, in the voxel pass.
+//
+// Sort: k_col_insert counts the voxels of each hash bucket (and each voxel's rank in
+// its bucket), k_col_bscan turns the counts into offsets within blocks of
+// VX3_COL_HBLOCK buckets, k_col_boff into per-simulation offsets of those blocks, and
+// k_col_place scatters the voxels into the simulation's cell-ordered range of cellVox.
+// No bucket has a capacity.  The partner lists are counted, scanned and emitted; if a
+// simulation's partners exceed its neighbour table, it sets SC_COL_GROW and stands
+// still (no pass changes its state) until the host enlarges the table.
 
 __device__ __forceinline__ bool colVoxel(const DevBatch<real>& b, int v, int& s) {
     s = b.vsim[v];
-    if (!simRunning(b.sims, s)) return false;
+    if (!simRunning(b, s)) return false;
     const int f = b.sims[s].flags;
-    if (!(f & SC_COLLISIONS) || (f & SC_DIVERGED_PENDING)) return false;
+    if (!(f & SC_COLLISIONS) || (f & (SC_DIVERGED_PENDING | SC_COL_GROW))) return false;
     return !(b.vflags[v] & VF_INTERIOR);
 }
 
@@ -240,11 +255,16 @@ __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
     if (dx * dx + dy * dy + dz * dz > (real)sc.colRecalcSq) atomicOr(&sc.flags, SC_REBUILD);
 }
 
+__device__ __forceinline__ bool colRebuilding(const DevBatch<real>& b, int s) {
+    if (s < 0 || !simRunning(b, s)) return false;
+    const int f = b.sims[s].flags;
+    return (f & SC_REBUILD) && (f & SC_COLLISIONS) && !(f & (SC_DIVERGED_PENDING | SC_COL_GROW));
+}
+
 __global__ void __launch_bounds__(256) k_col_clear(DevBatch<real> b) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= b.hashTot) return;
-    const int s = b.hashSim[h];
-    if (s >= 0 && simRunning(b.sims, s) && (b.sims[s].flags & SC_REBUILD)) b.hashCnt[h] = 0;
+    if (colRebuilding(b, b.hashSim[h])) b.hashCnt[h] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_col_insert(DevBatch<real> b) {
@@ -260,9 +280,77 @@ __global__ void __launch_bounds__(256) k_col_insert(DevBatch<real> b) {
     int c[3];
     colCell(b, v, sc.colCell, c);
     const int h = sc.h0 + (int)(colHash(c[0], c[1], c[2]) & ((1u << sc.hBits) - 1));
-    const int slot = atomicAdd(&b.hashCnt[h], 1);
-    if (slot < VX3_COL_BUCKET) b.hashEnt[(size_t)h * VX3_COL_BUCKET + slot] = v;
-    else atomicOr(&sc.flags, SC_COL_OVERFLOW);
+    b.vbk[v] = h;
+    b.vrank[v] = atomicAdd(&b.hashCnt[h], 1);
+}
+
+// exclusive scan of the bucket counts within each block of VX3_COL_HBLOCK buckets
+// (a simulation's buckets are whole blocks); the block's total goes to hashBlk
+__global__ void __launch_bounds__(256) k_col_bscan(DevBatch<real> b) {
+    const int B = blockIdx.x;
+    const int h0 = B * VX3_COL_HBLOCK;
+    if (h0 >= b.hashTot || !colRebuilding(b, b.hashSim[h0])) return;
+    constexpr int PER = VX3_COL_HBLOCK / 256;
+    __shared__ int part[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int x[PER], sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; k++) { x[k] = b.hashCnt[h0 + tid * PER + k]; sum += x[k]; }
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) part[w] = incl;
+    __syncthreads();
+    int before = incl - sum;
+    for (int i = 0; i < w; i++) before += part[i];
+#pragma unroll
+    for (int k = 0; k < PER; k++) { b.hashStart[h0 + tid * PER + k] = before; before += x[k]; }
+    if (tid == 255) b.hashBlk[B] = before;
+}
+
+// per rebuilt simulation: offsets of its bucket blocks in its cellVox range
+__global__ void __launch_bounds__(256) k_col_boff(DevBatch<real> b) {
+    const int s = blockIdx.x;
+    if (s >= b.nSims || !colRebuilding(b, s)) return;
+    const SimCtl& sc = b.sims[s];
+    __shared__ int part[4];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int B0 = sc.h0 / VX3_COL_HBLOCK, nB = (1 << sc.hBits) / VX3_COL_HBLOCK;
+    if (tid == 0) carry = sc.c0;
+    __syncthreads();
+    for (int base = 0; base < nB; base += 256) {
+        const int i = base + tid;
+        const int x = i < nB ? b.hashBlk[B0 + i] : 0;
+        int incl = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) part[w] = incl;
+        __syncthreads();
+        int before = carry + incl - x;
+        for (int k = 0; k < w; k++) before += part[k];
+        if (i < nB) b.hashBlkOff[B0 + i] = before;
+        __syncthreads();
+        if (tid == 255) carry = before + x;
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int colBucketStart(const DevBatch<real>& b, int h) {
+    return b.hashBlkOff[h / VX3_COL_HBLOCK] + b.hashStart[h];
+}
+
+__global__ void __launch_bounds__(256) k_col_place(DevBatch<real> b) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    int s;
+    if (!colVoxel(b, v, s) || !(b.sims[s].flags & SC_REBUILD)) return;
+    const int h = b.vbk[v];
+    b.cellVox[colBucketStart(b, h) + b.vrank[v]] = v;
 }
 
 // visits every watched partner u of surface voxel v once (any order)
@@ -271,26 +359,23 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
     int c[3];
     colCell(b, v, sc.colCell, c);
     const unsigned mask = (1u << sc.hBits) - 1;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
-                const int h = sc.h0 + (int)(colHash(cx, cy, cz) & mask);
-                // a bucket shared by several of the 27 cells is read once, for the first of them
-                bool seen = false;
-                for (int ez = -1; ez <= dz && !seen; ez++)
-                    for (int ey = -1; ey <= 1 && !seen; ey++)
-                        for (int ex = -1; ex <= 1 && !seen; ex++) {
-                            if (ez == dz && (ey > dy || (ey == dy && ex >= dx))) continue;
-                            if (sc.h0 + (int)(colHash(c[0] + ex, c[1] + ey, c[2] + ez) & mask) == h) seen = true;
-                        }
-                if (seen) continue;
-                const int n = min(b.hashCnt[h], VX3_COL_BUCKET);
-                for (int k = 0; k < n; k++) {
-                    const int u = b.hashEnt[(size_t)h * VX3_COL_BUCKET + k];
-                    if (u != v && colWatched(b, sc, v, u)) f(u);
-                }
-            }
+    int hs[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++)
+        hs[k] = sc.h0 + (int)(colHash(c[0] + k % 3 - 1, c[1] + (k / 3) % 3 - 1, c[2] + k / 9 - 1) & mask);
+    for (int k = 0; k < 27; k++) {
+        const int h = hs[k];
+        bool seen = false;   // a bucket shared by several of the 27 cells is read once
+        for (int j = 0; j < k; j++) seen |= hs[j] == h;
+        if (seen) continue;
+        const int n = b.hashCnt[h];
+        if (!n) continue;
+        const int* cv = b.cellVox + colBucketStart(b, h);
+        for (int i = 0; i < n; i++) {
+            const int u = cv[i];
+            if (u != v && colWatched(b, sc, v, u)) f(u);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_col_count(DevBatch<real> b) {
@@ -305,16 +390,14 @@ __global__ void __launch_bounds__(256) k_col_count(DevBatch<real> b) {
     b.nbCount[v] = n;
 }
 
-// per rebuilt simulation: exclusive scan of the partner counts over its voxels
+// per rebuilt simulation: exclusive scan of the partner counts over its voxels; a total
+// beyond the neighbour table stops the simulation in place until the host grows it
 __global__ void __launch_bounds__(1024) k_col_scan(DevBatch<real> b) {
     __shared__ int part[1024 / 64];
     __shared__ int carry;
     const int s = blockIdx.x;
-    if (s >= b.nSims) return;
+    if (s >= b.nSims || !colRebuilding(b, s)) return;
     SimCtl& sc = b.sims[s];
-    if (sc.state != 0 || (sc.flags & SC_RESIDENT) || !(sc.flags & SC_COLLISIONS) || !(sc.flags & SC_REBUILD) ||
-        (sc.flags & SC_DIVERGED_PENDING))
-        return;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -337,8 +420,10 @@ __global__ void __launch_bounds__(1024) k_col_scan(DevBatch<real> b) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (carry > sc.nbCap) atomicOr(&sc.flags, SC_COL_OVERFLOW);
-        sc.rebuilds++;
+        if (carry > sc.nbCap) {
+            sc.nbNeed = carry;
+            atomicOr(&sc.flags, SC_COL_GROW);
+        } else sc.rebuilds++;
     }
 }
 
@@ -349,7 +434,7 @@ __global__ void __launch_bounds__(256) k_col_snap(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
-    if (!simRunning(b.sims, s) || !(b.sims[s].flags & SC_COLLISIONS) || (b.vflags[v] & VF_INTERIOR)) return;
+    if (!simRunning(b, s) || !(b.sims[s].flags & SC_COLLISIONS) || (b.vflags[v] & VF_INTERIOR)) return;
     b.cpx[v] = b.px[v]; b.cpy[v] = b.py[v]; b.cpz[v] = b.pz[v];
     b.cmx[v] = b.mx[v]; b.cmy[v] = b.my[v]; b.cmz[v] = b.mz[v];
     b.ctf[v] = b.tfac[v];
@@ -361,7 +446,7 @@ __global__ void __launch_bounds__(256) k_col_emit(DevBatch<real> b) {
     int s;
     if (!colVoxel(b, v, s)) return;
     const SimCtl& sc = b.sims[s];
-    if (!(sc.flags & SC_REBUILD) || (sc.flags & SC_COL_OVERFLOW)) return;
+    if (!(sc.flags & SC_REBUILD)) return;
     int* out = b.nbList + b.nbStart[v];
     int n = 0;
     colForPartners(b, sc, v, [&](int u) { out[n++] = u; });
@@ -475,7 +560,7 @@ template <bool EXTRAS> __device__ __forceinline__ void voxelPass(const DevBatch<
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
-    if (!simRunning(b.sims, s)) return;
+    if (!simRunning(b, s)) return;
     const SimCtl& sc = b.sims[s];
     if (((sc.flags & SC_EXTRAS) != 0) != EXTRAS) return;
     if (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW)) return;   // doTimeStep returns before the voxel loop
@@ -614,12 +699,13 @@ __device__ __forceinline__ double mathTreeEval(const TokenDev* buf, int n, doubl
 // whose material has Cilia != 0 and whose switch-on time has passed is pushed by
 // orient.RotateVec3D(base + localSignal*shift) * Cilia, localSignal being 0 (signals
 // are not simulated); the force field's three programs of the voxel position at the
-// step's start time (VX3_Voxel.cu:178-184; hit, closeness and close pairs are 0).
+// step's start time (VX3_Voxel.cu:178-184; hit, closeness and close pairs as of the last
+// sample).
 __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
-    if (!simRunning(b.sims, s)) return;
+    if (!simRunning(b, s)) return;
     const SimCtl& sc = b.sims[s];
     if (!forces) {   // before the link pass: CVX_Voxel::updateSurface, interior iff all six links remain
         if (!(sc.flags & (SC_DETACH | SC_SECONDARY | SC_ATTACH))) return;
@@ -638,7 +724,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
         for (int k = 0; k < 5 && on; k++)   // all five conditions > 0; a missing one does not restrict
             if (sc.atLen[k] > 0)
                 on = mathTreeEval(b.tokens + sc.atOff[k], sc.atLen[k], (double)b.px[v], (double)b.py[v], (double)b.pz[v],
-                                  (double)sc.hits, sc.t, sc.recentAngle, 0, 0, sc.nv, q) > 0;
+                                  (double)sc.hits, sc.t, sc.recentAngle, sc.closeness, sc.closePairs, sc.nv, q) > 0;
         const int nf = on ? (fl | VF_ATTACH) : (fl & ~VF_ATTACH);
         if (nf != fl) b.vflags[v] = nf;
         return;
@@ -662,7 +748,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
     for (int k = 0; k < 3; k++)
         if (sc.ffLen[k] > 0)
             ff[k] = mathTreeEval(b.tokens + sc.ffOff[k], sc.ffLen[k], (double)b.px[v], (double)b.py[v], (double)b.pz[v],
-                                 (double)sc.hits, sc.t, sc.recentAngle, 0, 0, sc.nv, q);
+                                 (double)sc.hits, sc.t, sc.recentAngle, sc.closeness, sc.closePairs, sc.nv, q);
     b.xf[0 * V + v] = c.x; b.xf[1 * V + v] = c.y; b.xf[2 * V + v] = c.z;
     b.xf[3 * V + v] = (real)ff[0]; b.xf[4 * V + v] = (real)ff[1]; b.xf[5 * V + v] = (real)ff[2];
 }
@@ -702,7 +788,7 @@ __device__ bool linkedWithin(const DevBatch<real>& b, size_t V, int a, int t, in
 
 __global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
     const int s = blockIdx.x;
-    if (s >= b.nSims || !simRunning(b.sims, s)) return;
+    if (s >= b.nSims || !simRunning(b, s)) return;
     SimCtl& sc = b.sims[s];
     if (!(sc.flags & SC_ATTACH) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
     __shared__ int cand[VX3_ATT_MAXV];
@@ -827,7 +913,7 @@ __global__ void __launch_bounds__(256) k_hits(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
-    if (!simRunning(b.sims, s)) return;
+    if (!simRunning(b, s)) return;
     const SimCtl& sc = b.sims[s];
     if (!(sc.flags & SC_TARGETS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
     const int fl = b.vflags[v];
@@ -858,7 +944,7 @@ __global__ void __launch_bounds__(256) k_signals(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
-    if (!simRunning(b.sims, s)) return;
+    if (!simRunning(b, s)) return;
     const SimCtl& sc = b.sims[s];
     if (!(sc.flags & SC_SIGNALS) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
     const int fl = b.vflags[v];
@@ -1057,6 +1143,38 @@ __device__ bool secondaryExperiment(const DevBatch<real>& b, SimCtl& sc) {
     return reinit;
 }
 
+// computeTargetCloseness (VX3_VoxelyzeKernel.cu:545-562) over the simulation's target
+// voxels: pairs closer than closeR are counted, 1/distance summed over all pairs (the
+// block's partial sums, so the last bits of the sum can differ from a sequential one).
+__device__ void blockCloseness(const DevBatch<real>& b, const SimCtl& sc, double& closeness, int& pairs) {
+    __shared__ double rs[256];
+    __shared__ int rc[256];
+    double ret = 0;
+    int cnt = 0;
+    const int* tg = b.tgtList + sc.tg0;
+    for (int i = threadIdx.x; i < sc.nTg; i += blockDim.x) {
+        const int vi = tg[i];
+        const double xi = b.px[vi], yi = b.py[vi], zi = b.pz[vi];
+        for (int j = i + 1; j < sc.nTg; j++) {
+            const int vj = tg[j];
+            const double dx = (double)b.px[vj] - xi, dy = (double)b.py[vj] - yi, dz = (double)b.pz[vj] - zi;
+            const double distance = sqrt(dx * dx + dy * dy + dz * dz);
+            if (distance < sc.closeR) cnt++;
+            ret += 1 / distance;
+        }
+    }
+    rs[threadIdx.x] = ret;
+    rc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) { rs[threadIdx.x] += rs[threadIdx.x + w]; rc[threadIdx.x] += rc[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    closeness = rs[0];
+    pairs = rc[0];
+    __syncthreads();
+}
+
 // Per-simulation bookkeeping after the voxel pass (doTimeStep tail, VX3_VoxelyzeKernel.cu:241,279,314-334,352,
 // and the stop test the driver loop makes before the next step, VX3_SimulationManager.cu:62-64).
 __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
@@ -1064,12 +1182,15 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     const int s = blockIdx.x;
     if (s >= b.nSims) return;
     SimCtl& sc = b.sims[s];
-    if (sc.state != 0 || (sc.flags & SC_RESIDENT)) return;
+    if (!simRunning(b, s)) return;
     const bool diverged = (sc.flags & SC_DIVERGED_PENDING) != 0;
     const int steps = sc.steps + 1;
     const bool sample = !diverged && sc.cycleStep > 0 && steps % sc.cycleStep == 0;
     double com[3];
     if (sample) blockCoM(b, sc, com);
+    double closeness = 0;
+    int closePairs = 0;
+    if (sample && sc.closeR > 0) blockCloseness(b, sc, closeness, closePairs);
     if (!diverged && !(sc.flags & SC_COL_OVERFLOW) && (sc.scType >= 4 || sc.eqMode)) voxcadStats(b, sc, s);
     bool reinit = false;
     if ((sc.flags & SC_SECONDARY) && !diverged && !(sc.flags & SC_COL_OVERFLOW)) reinit = secondaryExperiment(b, sc);
@@ -1098,15 +1219,18 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
             double l2 = sqrt((B[0] - C[0]) * (B[0] - C[0]) + (B[1] - C[1]) * (B[1] - C[1]) + (B[2] - C[2]) * (B[2] - C[2]));
             sc.recentAngle = acos(dot / (l1 * l2));
         }
+        if (sc.closeR > 0) { sc.closeness = closeness; sc.closePairs = closePairs; }
     }
     if (reinit) {   // InitializeCenterOfMass: the last sampled center of mass
         for (int k = 0; k < 3; k++) sc.com0[k] = sc.com[k];
         sc.reinitDone = 1;
     }
     sc.t += (double)sc.dt;
+    sc.prevDt = sc.dt;
     if (sc.maxSteps > 0 && steps >= sc.maxSteps) { sc.state = 1; return; }
     if (sc.stopLen > 0 &&
-        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], (double)sc.hits, sc.t, sc.recentAngle, 0, 0,
+        mathTreeEval(b.tokens + sc.stopOff, sc.stopLen, sc.com[0], sc.com[1], sc.com[2], (double)sc.hits, sc.t, sc.recentAngle,
+                     sc.closeness, sc.closePairs,
                      sc.nv, sQ) > 0)
         sc.state = 1;
     else if (sc.scType && voxcadStopMet(b, sc, s, steps))
@@ -1166,6 +1290,7 @@ __global__ void __launch_bounds__(256) k_dt(DevBatch<real> b, int s0, int s1) {
         if (opt < 1e-10) opt = 1e-10;
         sc.optDt = opt;
         if (!(sc.dt > 0)) sc.dt = (float)(sc.dtFrac * opt);   // doTimeStep: dt = DtFrac * OptimalDt (float)
+        if (!(sc.prevDt > 0)) sc.prevDt = sc.dt;
         sc.cycleStep = sc.tempPeriod / (double)sc.dt >= 1.0 ? (int)(sc.tempPeriod / (double)sc.dt) : 0;
     }
 }

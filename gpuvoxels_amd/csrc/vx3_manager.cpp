@@ -97,7 +97,6 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             if (info.enable_attach && vx3h_vxa_desc(v)->collisions_enabled) missing += " attach-with-collisions";
             if (info.enable_detach && vx3h_vxa_desc(v)->collisions_enabled) missing += " detach-with-collisions";
             if (info.secondary_experiment && vx3h_vxa_desc(v)->collisions_enabled) missing += " secondary-experiment-with-collisions";
-            if (info.max_dist_close_pairs > 0) missing += " close-pair counting";
             if (!missing.empty()) {
                 if (!opt.allow_unsupported) { vx3h_vxa_free(v); throw std::runtime_error(f + ": not supported by this engine:" + missing); }
                 fprintf(stderr, "warning: %s: ignoring%s\n", f.c_str(), missing.c_str());
@@ -144,6 +143,7 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             r.num_voxel = s->num_voxel;
             r.num_measured_voxel = s->num_measured_voxel;
             r.total_distance_of_all_voxels = s->total_distance_of_all_voxels;
+            r.numClosePairs = s->num_close_pairs;
             r.SavePositionOfAllVoxels = d->save_position_of_all_voxels != 0;
             if (r.SavePositionOfAllVoxels) {
                 const int V = d->n_voxels;

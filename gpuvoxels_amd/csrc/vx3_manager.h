@@ -65,4 +65,10 @@ private:
 };
 
 }  // namespace vx3
+
+// The reference's class names (Voxelyze3/src/VX3/VX3_SimulationManager.cuh:15,
+// VX3_SimulationResult.h:4): the same constructor, start() and h_results.
+using VX3_SimulationManager = vx3::SimulationManager;
+using VX3_SimulationResult = vx3::SimulationResult;
+
 #endif

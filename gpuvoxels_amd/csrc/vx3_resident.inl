@@ -113,19 +113,55 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         if (expensive(g)) atomicAdd(&sExp[b.lmeta[g] & 3], 1);
     }
     __syncthreads();
-    for (int l = tid; l < nl; l += BS) {
-        const int g = l0 + l;
-        const int ax = b.lmeta[g] & 3;
-        // expensive links take [0, sExp), the others [sExp, count); admission guarantees count <= LAX
-        const int slot = expensive(g) ? atomicAdd(&sExpAt[ax], 1) : sExp[ax] + atomicAdd(&sCnt[ax], 1);
-        sSL[ax * LAX + slot] = (unsigned short)l;
-        sVL[(2 * ax) * VMAX + (b.lneg[g] - v0)] = (unsigned short)slot;
-        sVL[(2 * ax + 1) * VMAX + (b.lpos[g] - v0)] = (unsigned short)slot;
+    // Within each class (axis, expensive or not) links take slots in link-list order:
+    // expensive links [0, sExp), the others [sExp, count) (admission guarantees count
+    // <= LAX).  Link order follows voxel order, so neighbouring lanes gather neighbouring
+    // voxels' poses and load slots, which keeps LDS bank conflicts low.  Ranks come from
+    // wave ballots per class and per-wave counts in LDS; sCnt[a] / sExpAt[a] run as the
+    // class bases over the chunks of BS links.
+    {
+        constexpr int NW = BS / 64;
+        __shared__ int sWc[NW][6];
+        const int lane = tid & 63, w = tid >> 6;
+        const unsigned long long below = (1ull << lane) - 1;
+        for (int base = 0; base < nl; base += BS) {
+            const int l = base + tid;
+            int ax = 0, cls = -1;
+            bool ex = false;
+            if (l < nl) {
+                const int g = l0 + l;
+                ax = b.lmeta[g] & 3;
+                ex = expensive(g);
+                cls = 2 * ax + (ex ? 0 : 1);
+            }
+            int rank = 0;
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const unsigned long long m = __ballot(cls == c);
+                if (lane == 0) sWc[w][c] = __popcll(m);
+                if (cls == c) rank = __popcll(m & below);
+            }
+            __syncthreads();
+            if (cls >= 0) {
+                for (int i = 0; i < w; i++) rank += sWc[i][cls];
+                const int slot = ex ? sExpAt[ax] + rank : sExp[ax] + sCnt[ax] + rank;
+                const int g = l0 + l;
+                sSL[ax * LAX + slot] = (unsigned short)l;
+                sVL[(2 * ax) * VMAX + (b.lneg[g] - v0)] = (unsigned short)slot;
+                sVL[(2 * ax + 1) * VMAX + (b.lpos[g] - v0)] = (unsigned short)slot;
+            }
+            __syncthreads();
+            if (tid < 3) {
+                for (int i = 0; i < NW; i++) { sExpAt[tid] += sWc[i][2 * tid]; sCnt[tid] += sWc[i][2 * tid + 1]; }
+            }
+            __syncthreads();
+        }
     }
-    __syncthreads();
     if (tid < 3) sCnt[tid] += sExp[tid];   // links of axis tid
-    // 2*sqrt(m)*zeta/previousDt (VX_Voxel.h:131); every step of a launch uses the sim's one dt
-    if (tid < nvm) sDm[tid] = 2 * sVM[tid].sqrtMass * sVM[tid].zetaInternal / (sVM[tid].fixed ? 0.0f : dtf);
+    // 2*sqrt(m)*zeta/previousDt (VX_Voxel.h:131): the first step's previousDt is the dt of the
+    // simulation's last step (which a caller-given dt may have set), every later one the launch's dt
+    const float prevDt0 = scg.prevDt > 0 ? scg.prevDt : dtf;
+    if (tid < nvm) sDm[tid] = 2 * sVM[tid].sqrtMass * sVM[tid].zetaInternal / (sVM[tid].fixed ? 0.0f : prevDt0);
     __syncthreads();
     auto localVmat = [&](int gidx) {
         int r = 0;
@@ -443,6 +479,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         t = tn;
         bool stop = maxSteps > 0 && steps >= maxSteps;
         if (!stop && stopLen > 0 && tid == 0) sRun = mathTreeEval(b.tokens + stopOff, stopLen, sCom[0], sCom[1], sCom[2], 0, t, sAngle, 0, 0, nv, sQ) > 0 ? 0 : 1;
+        if (it == 0 && prevDt0 != dtf && tid < nvm) sDm[tid] = 2 * sVM[tid].sqrtMass * sVM[tid].zetaInternal / (sVM[tid].fixed ? 0.0f : dtf);
         __syncthreads();   // next step's link pass reads the new poses
         if (stop || !sRun) { state = 1; break; }
     }
@@ -482,6 +519,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         b.psx[g] = sPsx[v]; b.psy[g] = sPsy[v]; b.psz[g] = sPsz[v];
     }
     if (tid == 0) {
+        if (steps != scg.steps) scg.prevDt = dtf;
         scg.t = t;
         scg.steps = steps;
         scg.angleSamples = angleSamples;

@@ -9,6 +9,9 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs) {
     hipLaunchKernelGGL(VX_NS::k_col_stale, dim3(gv), dim3(bs), 0, s, b);
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_clear, dim3((b.hashTot + 255) / 256), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_insert, dim3(gv), dim3(bs), 0, s, b);
+    if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_bscan, dim3(b.hashTot / VX3_COL_HBLOCK), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_boff, dim3(b.nSims), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_place, dim3(gv), dim3(bs), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_count, dim3(gv), dim3(bs), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_scan, dim3(b.nSims), dim3(1024), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_emit, dim3(gv), dim3(bs), 0, s, b);
@@ -28,8 +31,12 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     } while (0)
     if (o.prestep) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 0);
     if (o.attach) hipLaunchKernelGGL(VX_NS::k_attach, dim3(b.nSims), dim3(256), 0, s, b);
-    if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
+    // The collision lists are rebuilt from the positions at the step's start, before the
+    // link pass (which moves no voxel): the reference rebuilds them between the link and
+    // voxel passes from the same positions.  A simulation whose neighbour table proves too
+    // small stands still for this step, link pass included.
     if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs));
+    if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
     if (o.contacts && o.extras) hipLaunchKernelGGL(VX_NS::k_hits, dim3(gv), dim3(bs), 0, s, b);
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 1);
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_voxels_extras, dim3(gv), dim3(bs), 0, s, b);

@@ -45,8 +45,10 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_SECONDARY = 4096,     // secondary experiment: timed voxel removal, initial-position reset
        SC_SIGNALS = 8192,       // signals between linked voxels (runs with SC_EXTRAS)
        SC_TARGETS = 16384,      // target contacts are counted (runs with SC_EXTRAS)
-       SC_ATTACH = 32768 };     // touching sticky surface voxels get new links
-#define VX3_COL_BUCKET 32       // voxels per spatial-hash bucket
+       SC_ATTACH = 32768,       // touching sticky surface voxels get new links
+       SC_COL_GROW = 65536 };   // the collision neighbour table is too small: the simulation stands
+                                // still (no pass changes it) until the host enlarges the table
+#define VX3_COL_HBLOCK 1024     // hash buckets per scan block; a simulation's buckets are whole blocks
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
     float mass, massInverse, sqrtMass, momentInertiaInverse;
@@ -96,6 +98,8 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     double recentAngle;
     double voxSize;
     float dt;                        // step size (float, as doTimeStep's parameter)
+    float prevDt;                    // the last step's dt: CVX_Voxel::previousDt of the link damping
+                                     // (VX_Voxel.h:131); 0 before the first step
     float tInit;                     // temperature of voxels not driven by the actuation
     int state;                       // VX3H_SIM_*
     int steps, maxSteps, cycleStep, angleSamples;
@@ -110,7 +114,9 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int colInterval;                 // > 0: also rebuild every colInterval steps
     int colNearHops;                 // link-hop radius of the excluded "nearby" voxels (int(2*watchRadiusVx))
     int nb0, nbCap;                  // neighbour-list range in nbList
-    int h0, hBits;                   // hash-table range: (1 << hBits) buckets from bucket h0
+    int nbNeed;                      // entries the last rebuild needed when it set SC_COL_GROW
+    int h0, hBits;                   // hash-table range: (1 << hBits) buckets from bucket h0 (whole blocks)
+    int c0;                          // cell-sorted surface voxels: range [c0, c0 + nv) of cellVox
     int rebuilds;                    // neighbour-list rebuilds so far
     // VoxCAD stop-condition family and equilibrium mode (VoxCAD/src/VX1/VX_Sim.cpp:862-891,1015-1160)
     int scType;                      // VX3H_SC_*
@@ -124,6 +130,10 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     double reinitAfter;
     int reinitDone;
     int hits;                        // collisionCount: target contacts so far (MathTree variable hit)
+    // VX3 close pairs (computeTargetCloseness, VX3_VoxelyzeKernel.cu:545-562): radius (m, 0 = off),
+    // the sim's target voxels tgtList[tg0 .. tg0 + nTg), and the values of the last sample
+    double closeR, closeness;
+    int closePairs, tg0, nTg;
     // VX3 attach: link range capacity (links are made at l0 + nl), new-link guard steps,
     // attach-condition programs in the token pool
     int nlMax, safety;

@@ -115,7 +115,8 @@ struct SlotHost {
     std::vector<vx3h_token> fit;
     size_t bytes = 0;
     // collision pools: neighbour lists, hash buckets, nearby lists
-    int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0;
+    int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0, c0 = 0;   // hN in buckets (whole blocks)
+    int tg0 = 0, nTg = 0;        // close pairs: range of the sim's target voxels in tgtList
     // history records: topology and link constants of the sim, palette in order
     std::vector<int> vlinks;     // 6 per voxel, sim-local link index or -1
     std::vector<char> surface;   // per voxel: not all six links present
@@ -128,6 +129,7 @@ struct SlotHost {
 struct DevPool {
     std::vector<std::pair<void**, size_t>> arrs;   // slot, bytes per element
     int cap = 0;
+    int minCap = 4096;                             // elements allocated at first use
     RangeAlloc ra;
 };
 
@@ -167,7 +169,7 @@ struct vx3h_batch {
     void* dExt = nullptr; size_t capExt = 0;
     void* dTok = nullptr; size_t capTok = 0;
     void* dSimMats = nullptr; size_t capSimMats = 0;
-    DevPool nbPool, hashPool, nearPool;
+    DevPool nbPool, hashPool, nearPool, cellPool, tgtPool;
     bool anyColl = false;
     bool anyExtras = false;             // some simulation has cilia or a force field
     bool anyDetach = false;             // some simulation detaches failed links
@@ -200,11 +202,12 @@ struct vx3h_batch {
     std::vector<hipEvent_t> evPool;              // free events
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evUsed;
     long long timedSteps = 0;
+    int tableGrowths = 0;                        // collision neighbour tables enlarged so far
 };
 
 namespace {
 
-size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8 + 8 * 3 + 8 + 8 * 8 + 8 * 5 + 4 * 6; }
+size_t voxelBytes(size_t rs) { return rs * (13 + 36 + 6 + 6) + 4 * 5 + 4 * 3 + 4 * 4 + 4 * 6 + 4 + 8 + 8 * 8 + 8 * 3 + 8 + 8 * 8 + 8 * 5 + 4 * 6 + 4 * 2; }
 size_t linkBytes(size_t rs) { return rs * 9 + 4 * 6 + 4 * 3; }
 
 void initSpecs(vx3h_batch* b) {
@@ -221,13 +224,19 @@ void initSpecs(vx3h_batch* b) {
     V(&d.psx, 4, 1, false); V(&d.psy, 4, 1, false); V(&d.psz, 4, 1, false);
     V(&d.F, R, 36, true); V(&d.es, 4, 6, true);
     V(&d.lastWatch, 4, 3, false); V(&d.nbStart, 4, 1, false); V(&d.nbCount, 4, 1, false);
+    V(&d.vbk, 4, 1, false); V(&d.vrank, 4, 1, false);
     V(&d.cpx, R, 1, false); V(&d.cpy, R, 1, false); V(&d.cpz, R, 1, false);
     V(&d.cmx, R, 1, false); V(&d.cmy, R, 1, false); V(&d.cmz, R, 1, false); V(&d.ctf, 4, 1, false);
     V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
     V(&d.cil, 8, 8, true); V(&d.xf, R, 6, true); V(&d.ip, 8, 3, true); V(&d.rt, 8, 1, false);
     V(&d.sg, 8, 8, true); V(&d.sp, 8, 5, true); V(&d.nbr, 4, 6, true);
     b->nbPool.arrs = {{(void**)&d.nbList, 4}};
-    b->hashPool.arrs = {{(void**)&d.hashCnt, 4}, {(void**)&d.hashEnt, 4 * VX3_COL_BUCKET}, {(void**)&d.hashSim, 4}};
+    // the hash pool is allocated in blocks of VX3_COL_HBLOCK buckets
+    b->hashPool.arrs = {{(void**)&d.hashCnt, 4 * VX3_COL_HBLOCK}, {(void**)&d.hashStart, 4 * VX3_COL_HBLOCK},
+                        {(void**)&d.hashSim, 4 * VX3_COL_HBLOCK}, {(void**)&d.hashBlk, 4}, {(void**)&d.hashBlkOff, 4}};
+    b->cellPool.arrs = {{(void**)&d.cellVox, 4}};
+    b->hashPool.minCap = 16;
+    b->tgtPool.arrs = {{(void**)&d.tgtList, 4}};
     b->nearPool.arrs = {{(void**)&d.nearList, 4}};
     L(&d.lneg, 4, 1, false); L(&d.lpos, 4, 1, false); L(&d.lmeta, 4, 1, false); L(&d.lsim, 4, 1, false); L(&d.lflags, 4, 1, false);
     L(&d.q2x, R, 1, false); L(&d.q2y, R, 1, false); L(&d.q2z, R, 1, false);
@@ -269,7 +278,7 @@ vx3h_status poolAlloc(vx3h_batch* b, DevPool& p, int n, int& start) {
     bool grow = false;
     start = p.ra.alloc(n, p.cap, grow);
     if (!grow) return VX3H_OK;
-    const int ncap = std::max(p.ra.hi, std::max(2 * p.cap, 4096));
+    const int ncap = std::max(p.ra.hi, std::max(2 * p.cap, p.minCap));
     HIPCHK(hipStreamSynchronize(b->stream));
     for (auto& a : p.arrs) {
         void* np = nullptr;
@@ -328,14 +337,10 @@ vx3h_status readReal(vx3h_batch* b, const void* src, size_t n, std::vector<doubl
 
 char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx; }
 
+StepOpts stepOpts(const vx3h_batch* b);
+
 vx3h_status launchSteps(vx3h_batch* b, int n) {
-    StepOpts o;
-    o.contacts = b->anyColl;
-    o.extras = b->anyExtras;
-    o.prestep = b->anyDetach;
-    o.signals = b->anySignals;
-    o.attach = b->anyAttach;
-    o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
+    const StepOpts o = stepOpts(b);
     for (int i = 0; i < n; i++) {
         if (b->realSize == 4) launchStep(b->db, b->stream, o);
         else launchStep(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, o);
@@ -447,6 +452,7 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; FAILV(VX3H_EDEVICE, "stream creation failed"); }
     memset(&b->db, 0, sizeof(b->db));
+    b->db.solo = -1;
     initSpecs(b);
     {
         ResidentShapeInfo tmp[16];
@@ -491,7 +497,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims, (void*)b->db.hist}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
-    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool})
+    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool, &b->cellPool, &b->tgtPool})
         for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
     for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
     for (hipEvent_t e : b->evPool) hipEventDestroy(e);
@@ -617,13 +623,20 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         if ((st = copyReal(b, at(d.pz, rs, v0), pz))) return st;
         for (void* p : {(void*)d.mx, (void*)d.my, (void*)d.mz, (void*)d.ox, (void*)d.oy, (void*)d.oz, (void*)d.ax, (void*)d.ay, (void*)d.az})
             if ((st = copyReal(b, at(p, rs, v0), zero))) return st;
+        if (desc->init_velocity) {   // linear momentum = mass * velocity (the voxel material's float mass)
+            void* mom[3] = {(void*)d.mx, (void*)d.my, (void*)d.mz};
+            std::vector<double> lm(NV);
+            for (int k = 0; k < 3; k++) {
+                for (int i = 0; i < NV; i++) lm[i] = desc->init_velocity[3 * i + k] * (double)H.vmats[H.vmat[i]].mass;
+                if ((st = copyReal(b, at(mom[k], rs, v0), lm))) return st;
+            }
+        }
         if ((st = copyReal(b, at(d.ow, rs, v0), one))) return st;
         std::vector<int> vm(NV), vs(NV, s), ve(NV);
         for (int i = 0; i < NV; i++) { vm[i] = vmIdx[H.vmat[i]]; ve[i] = H.vext[i] >= 0 ? ext0 + H.vext[i] : -1; }
         std::vector<int> vf(H.vflags);
-        if (H.collisions)   // target materials (handle_collision_attachment counts their contacts)
-            for (int i = 0; i < NV; i++)
-                if (desc->materials[desc->material[i]].is_target) vf[i] |= VF_TARGET;
+        for (int i = 0; i < NV; i++)   // target materials: contacts counted as hits, close pairs
+            if (desc->materials[desc->material[i]].is_target) vf[i] |= VF_TARGET;
         if (attach)   // VX3_Voxel::enableAttach starts true (VX3_Voxel.h:302)
             for (int i = 0; i < NV; i++) vf[i] |= VF_ATTACH;
         HIPCHK(hipMemcpy(d.vflags + v0, vf.data(), NV * 4, hipMemcpyHostToDevice));
@@ -681,9 +694,10 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     const int nPhases = (int)(std::unique(phases.begin(), phases.end()) - phases.begin());
     // VoxCAD stop conditions and equilibrium mode are evaluated by the stream path's control pass
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
+    const bool closePairs = desc->max_dist_close_pairs > 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
                        nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach &&
-                       !desc->secondary_experiment && !desc->enable_signals && !attach;
+                       !desc->secondary_experiment && !desc->enable_signals && !attach && !closePairs;
     if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
@@ -701,8 +715,9 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
-    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals || attach) && b->cfg.path == VX3H_PATH_RESIDENT)
-        FAILV(VX3H_EUNSUPPORTED, "detach, attach, secondary experiments and signals run on the stream path");
+    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals || attach || closePairs) &&
+        b->cfg.path == VX3H_PATH_RESIDENT)
+        FAILV(VX3H_EUNSUPPORTED, "detach, attach, secondary experiments, signals and close pairs run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -790,13 +805,28 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         const std::vector<double> unset(3 * VX3_HIST, -1.0);   // ClearHistories
         HIPCHK(hipMemcpy(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    int tg0 = 0, nTg = 0;
+    if (closePairs && NV > 0) {   // registerTargets (VX3_VoxelyzeKernel.cu:536-543): target voxels in voxel order
+        std::vector<int> tg;
+        for (int i = 0; i < NV; i++)
+            if (desc->materials[desc->material[i]].is_target) tg.push_back(v0 + i);
+        nTg = (int)tg.size();
+        if (nTg) {
+            vx3h_status st = poolAlloc(b, b->tgtPool, nTg, tg0);
+            if (st) return st;
+            HIPCHK(hipMemcpy(b->db.tgtList + tg0, tg.data(), (size_t)nTg * 4, hipMemcpyHostToDevice));
+        }
+        c.closeR = desc->max_dist_close_pairs * H.voxSize;   // MaxDistInVoxelLengthsToCountAsPair * voxSize
+        c.tg0 = tg0;
+        c.nTg = nTg;
+    }
     if (NV == 0) c.state = VX3H_SIM_DONE;   // "No links and no voxels" (VX3_SimulationManager.cu:15-19)
     else if (!H.stopTok.empty() &&
              evalMathTree(H.stopTok.data(), (int)H.stopTok.size(), c.com[0], c.com[1], c.com[2], 0, 0.0, 0, 0, 0, NV) > 0)
         c.state = VX3H_SIM_DONE;
     // collisions: thresholds of CVoxelyze::updateCollisions (Voxelyze.cpp:673-676, float arithmetic),
     // nearby lists of CVX_Voxel::generateNearby (Voxelyze.cpp:394-417) and device pool ranges
-    int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0;
+    int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0, c0 = 0;
     if (H.collisions && NV > 0) {
         const float br = (float)H.boundingRadius, wd = (float)H.watchDistance;
         const float wrv = 2 * br + wd;
@@ -835,15 +865,17 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
             nearCount[v] = (int)tmp.size();
             nearList.insert(nearList.end(), tmp.begin(), tmp.end());
         }
-        nbN = std::max(64, 64 * nSurf);
-        int hb = 6;
+        nbN = b->cfg.collision_table_entries > 0 ? b->cfg.collision_table_entries : std::max(64, 64 * nSurf);
+        int hb = 10;   // whole blocks of VX3_COL_HBLOCK buckets, at least two buckets per surface voxel
         while ((1 << hb) < 2 * nSurf) hb++;
         hN = 1 << hb;
         nearN = std::max(1, (int)nearList.size());
         vx3h_status st;
-        if ((st = poolAlloc(b, b->nbPool, nbN, nb0)) || (st = poolAlloc(b, b->hashPool, hN, h0)) ||
-            (st = poolAlloc(b, b->nearPool, nearN, near0)))
+        int hblk0 = 0;
+        if ((st = poolAlloc(b, b->nbPool, nbN, nb0)) || (st = poolAlloc(b, b->hashPool, hN / VX3_COL_HBLOCK, hblk0)) ||
+            (st = poolAlloc(b, b->nearPool, nearN, near0)) || (st = poolAlloc(b, b->cellPool, NV, c0)))
             return st;
+        h0 = hblk0 * VX3_COL_HBLOCK;
         for (int& x : nearStart) x += near0;
         if (!nearList.empty()) HIPCHK(hipMemcpy(b->db.nearList + near0, nearList.data(), nearList.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b->db.nearStart + v0, nearStart.data(), NV * 4, hipMemcpyHostToDevice));
@@ -856,10 +888,11 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         c.nbCap = nbN;
         c.h0 = h0;
         c.hBits = hb;
+        c.c0 = c0;
         c.flags |= SC_REBUILD;
         b->anyColl = true;
         b->db.Ccap = b->nbPool.cap;
-        b->db.hashTot = b->hashPool.ra.hi;
+        b->db.hashTot = b->hashPool.ra.hi * VX3_COL_HBLOCK;
     }
     HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(c), hipMemcpyHostToDevice));
 
@@ -874,7 +907,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.fit = H.fitTok;
     sh.bytes = need;
     sh.shape = shape;
-    sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN;
+    sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN; sh.c0 = c0;
+    sh.tg0 = tg0; sh.nTg = nTg;
     if (shape >= 0) { b->resLists[shape].push_back(s); b->resDirty = true; }
     else b->nStreamSims++;
     sh.vlinks = H.voxelLinks;
@@ -938,6 +972,88 @@ extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
         if (js) return js;
         done += c;
     }
+    return VX3H_OK;
+}
+
+namespace {
+StepOpts stepOpts(const vx3h_batch* b) {
+    StepOpts o;
+    o.contacts = b->anyColl;
+    o.extras = b->anyExtras;
+    o.prestep = b->anyDetach;
+    o.signals = b->anySignals;
+    o.attach = b->anyAttach;
+    o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
+    return o;
+}
+}  // namespace
+
+// VX3_VoxelyzeKernel::doTimeStep(float dt) of one simulation (VX3_VoxelyzeKernel.cu:237-359).
+extern "C" vx3h_status vx3h_sim_step(vx3h_batch* b, int slot, float dt, int* ok) {
+    if (!b || !ok) FAILV(VX3H_EINVAL, "null argument");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    if (dt != dt) FAILV(VX3H_EINVAL, "dt is NaN");
+    HIPCHK(hipSetDevice(b->device));
+    *ok = 1;
+    if (b->dtHi > b->dtLo) {   // recommended step of newly admitted simulations (the first doTimeStep's)
+        if (b->realSize == 4) launchDt(b->db, b->stream, b->dtLo, b->dtHi);
+        else launchDt(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, b->dtLo, b->dtHi);
+        HIPCHK(hipGetLastError());
+        b->dtLo = 1 << 30;
+        b->dtHi = -1;
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    SimCtl c;
+    HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(c), hipMemcpyDeviceToHost));
+    if (c.state == VX3H_SIM_DIVERGED) { *ok = 0; return VX3H_OK; }
+    if (c.state == VX3H_SIM_EMPTY || c.state == VX3H_SIM_FAILED || b->slots[slot].nv == 0) return VX3H_OK;
+    if (c.flags & SC_COL_GROW) {   // a collision table to enlarge first (vx3h_batch_poll does it)
+        int n = 0;
+        vx3h_status st = vx3h_batch_poll(b, nullptr, 0, &n);
+        if (st) return st;
+        HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(c), hipMemcpyDeviceToHost));
+    }
+    if (dt == 0) {   // CurStepCount++ and return: the temperature for the unchanged time is already in place
+        c.steps++;
+        if (c.maxSteps > 0 && c.steps >= c.maxSteps) c.state = VX3H_SIM_DONE;
+        HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
+        return VX3H_OK;
+    }
+    // dt < 0: DtFrac * OptimalDt (or the fixed step), the simulation's own dt
+    const SimCtl saved = c;
+    const float stepDt = dt < 0 ? c.dt : dt;
+    c.dt = stepDt;
+    // CycleStep = int(TempPeriod / dt) of this call's dt (VX3_VoxelyzeKernel.cu:314-316)
+    c.cycleStep = c.tempPeriod / (double)stepDt >= 1.0 ? (int)(c.tempPeriod / (double)stepDt) : 0;
+    c.flags &= ~SC_RESIDENT;   // the per-step kernels take it, whichever path it runs on
+    if (c.state == VX3H_SIM_DONE) c.state = VX3H_SIM_RUNNING;   // stepping past the stop condition is the caller's call
+    HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
+    StepOpts o = stepOpts(b);
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if (b->realSize == 4) {
+            DevBatch<float> v = b->db;
+            v.solo = slot;
+            launchStep(v, b->stream, o);
+        } else {
+            DevBatch<double> v = *reinterpret_cast<DevBatch<double>*>(&b->db);
+            v.solo = slot;
+            launchStep(v, b->stream, o);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipMemcpy(&c, b->db.sims + slot, sizeof(c), hipMemcpyDeviceToHost));
+        if (!(c.flags & SC_COL_GROW)) break;
+        int n = 0;   // the collision table was too small: enlarge it, then take the step again
+        vx3h_status st = vx3h_batch_poll(b, nullptr, 0, &n);
+        if (st) return st;
+    }
+    if (c.flags & SC_COL_GROW) FAILV(VX3H_EDEVICE, "collision table did not grow");
+    // the simulation's own step size and sampling period stay for the next call
+    c.dt = saved.dt;
+    c.cycleStep = saved.cycleStep;
+    c.flags = (c.flags & ~SC_RESIDENT) | (saved.flags & SC_RESIDENT);
+    HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
+    *ok = c.state == VX3H_SIM_DIVERGED ? 0 : 1;
     return VX3H_OK;
 }
 
@@ -1107,7 +1223,10 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
     o.total_distance_of_all_voxels = dist;
     double off[3] = {o.current_com[0] - c.com0[0], o.current_com[1] - c.com0[1], o.current_com[2] - c.com0[2]};
     // computeFitness (VX3_VoxelyzeKernel.cu:530-534)
-    o.fitness = evalMathTree(sh.fit.data(), (int)sh.fit.size(), off[0], off[1], off[2], c.hits, c.t, c.recentAngle, 0, 0, sh.nv);
+    o.num_close_pairs = c.closePairs;
+    o.target_closeness = c.closeness;
+    o.fitness = evalMathTree(sh.fit.data(), (int)sh.fit.size(), off[0], off[1], off[2], c.hits, c.t, c.recentAngle, c.closeness,
+                             c.closePairs, sh.nv);
     if (posOut) {
         posOut->resize(3 * sh.nv);
         for (int i = 0; i < sh.nv; i++) { (*posOut)[3 * i] = x[i]; (*posOut)[3 * i + 1] = y[i]; (*posOut)[3 * i + 2] = z[i]; }
@@ -1116,12 +1235,44 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
 }
 }  // namespace vx3
 
+namespace {
+// Simulations whose collision partners outgrew their neighbour table stand still
+// (SC_COL_GROW: no pass has changed their state since).  Each gets a table with room
+// for half as many entries again as its last rebuild needed (at least twice the old
+// one) and rebuilds its lists at its next step, which then runs as it would have.
+// The batch's streams are idle.
+vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
+    for (int s = 0; s < (int)ctl.size(); s++) {
+        if (!b->slots[s].used || !(ctl[s].flags & SC_COL_GROW)) continue;
+        SlotHost& sh = b->slots[s];
+        const int need = std::max(2 * sh.nbN, ctl[s].nbNeed + ctl[s].nbNeed / 2);
+        b->nbPool.ra.release(sh.nb0, sh.nbN);
+        int nb0 = 0;
+        vx3h_status st = poolAlloc(b, b->nbPool, need, nb0);
+        if (st) return st;
+        sh.nb0 = nb0;
+        sh.nbN = need;
+        ctl[s].nb0 = nb0;
+        ctl[s].nbCap = need;
+        ctl[s].flags = (ctl[s].flags & ~SC_COL_GROW) | SC_REBUILD;
+        HIPCHK(hipMemcpy(b->db.sims + s, &ctl[s], sizeof(SimCtl), hipMemcpyHostToDevice));
+        b->db.Ccap = b->nbPool.cap;
+        b->tableGrowths++;
+    }
+    return VX3H_OK;
+}
+}  // namespace
+
 extern "C" vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int cap, int* n) {
     if (!b || !n) FAILV(VX3H_EINVAL, "null argument");
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
     std::vector<SimCtl> ctl(b->slots.size());
     if (!ctl.empty()) HIPCHK(hipMemcpy(ctl.data(), b->db.sims, ctl.size() * sizeof(SimCtl), hipMemcpyDeviceToHost));
+    if (b->anyColl) {
+        vx3h_status gs = growCollisionTables(b, ctl);
+        if (gs) return gs;
+    }
     int k = 0;
     for (int s = 0; s < (int)b->slots.size(); s++) {
         if (!b->slots[s].used) continue;
@@ -1151,11 +1302,13 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
     b->la.release(sh.l0, sh.nlCap);
     if (sh.hN) {
         HIPCHK(hipMemset(b->db.hashSim + sh.h0, 0xFF, (size_t)sh.hN * 4));
-        b->hashPool.ra.release(sh.h0, sh.hN);
+        b->hashPool.ra.release(sh.h0 / VX3_COL_HBLOCK, sh.hN / VX3_COL_HBLOCK);
         b->nbPool.ra.release(sh.nb0, sh.nbN);
         b->nearPool.ra.release(sh.near0, sh.nearN);
-        b->db.hashTot = b->hashPool.ra.hi;
+        b->cellPool.ra.release(sh.c0, sh.nv);
+        b->db.hashTot = b->hashPool.ra.hi * VX3_COL_HBLOCK;
     }
+    if (sh.nTg) b->tgtPool.ra.release(sh.tg0, sh.nTg);
     if (sh.shape >= 0) {
         auto& L = b->resLists[sh.shape];
         L.erase(std::remove(L.begin(), L.end(), slot), L.end());

@@ -32,7 +32,7 @@ struct DescCopy {
     std::vector<double> phase;
     std::vector<vx3h_external> exts;
     std::vector<vx3h_token> stop, fit, ff[3], att[5];
-    std::vector<double> baseCilia, shiftCilia;
+    std::vector<double> baseCilia, shiftCilia, initVel;
     void take(const vx3h_sim_desc& s) {
         d = s;
         mats.assign(s.materials, s.materials + s.n_materials);
@@ -53,6 +53,8 @@ struct DescCopy {
         if (s.shift_cilia_force) shiftCilia.assign(s.shift_cilia_force, s.shift_cilia_force + 3 * (size_t)s.n_voxels);
         d.base_cilia_force = s.base_cilia_force ? baseCilia.data() : nullptr;
         d.shift_cilia_force = s.shift_cilia_force ? shiftCilia.data() : nullptr;
+        if (s.init_velocity) initVel.assign(s.init_velocity, s.init_velocity + 3 * (size_t)s.n_voxels);
+        d.init_velocity = s.init_velocity ? initVel.data() : nullptr;
         for (int k = 0; k < 3; k++) {
             if (s.n_force_field_tokens[k] > 0)
                 ff[k].assign(s.force_field_tokens[k], s.force_field_tokens[k] + s.n_force_field_tokens[k]);

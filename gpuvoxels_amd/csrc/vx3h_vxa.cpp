@@ -903,6 +903,7 @@ vx3h_status readVxa(const XNode& doc, vx3h_vxa& o, std::string& err) {
     d.ijk = o.ijk.data();
     d.material = o.mat.data();
     d.phase = o.phase.data();
+    d.max_dist_close_pairs = in.max_dist_close_pairs;   // VX3_SimulationManager.cu:358
     d.n_externals = (int)o.exts.size();
     d.externals = o.exts.data();
     d.gravity_mult = (float)(-(double)(float)gravAcc / 9.80665);
@@ -1030,7 +1031,7 @@ extern "C" vx3h_status vx3h_report_xml(const char* input_dir_name, const vx3h_re
         body += "<num_voxel>" + std::to_string(s.num_voxel) + "</num_voxel>";
         body += "<num_measured_voxel>" + std::to_string(s.num_measured_voxel) + "</num_measured_voxel>";
         body += "<voxSize>" + fmtDouble(e.voxel_size) + "</voxSize>";
-        body += "<numClosePairs>0</numClosePairs>";
+        body += "<numClosePairs>" + std::to_string(s.num_close_pairs) + "</numClosePairs>";
         body += "<initialCenterOfMass><x>" + fmtDouble(s.initial_com[0]) + "</x><y>" + fmtDouble(s.initial_com[1]) + "</y><z>" +
                 fmtDouble(s.initial_com[2]) + "</z></initialCenterOfMass>";
         body += "<currentCenterOfMass><x>" + fmtDouble(s.current_com[0]) + "</x><y>" + fmtDouble(s.current_com[1]) + "</y><z>" +

@@ -39,6 +39,7 @@ def lib():
         "vx3h_sim_admit": (S, [P, C.POINTER(_abi.SimDesc), C.POINTER(C.c_int)]),
         "vx3h_sim_footprint": (S, [C.POINTER(_abi.SimDesc), C.c_int, C.POINTER(C.c_size_t)]),
         "vx3h_batch_step": (S, [P, C.c_int]),
+        "vx3h_sim_step": (S, [P, C.c_int, C.c_float, C.POINTER(C.c_int)]),
         "vx3h_batch_sync": (S, [P]),
         "vx3h_batch_poll": (S, [P, C.POINTER(_abi.SimStatus), C.c_int, C.POINTER(C.c_int)]),
         "vx3h_sim_retire": (S, [P, C.c_int]),
@@ -211,7 +212,7 @@ class Batch:
     """A device-resident batch of simulations (vx3h_batch)."""
 
     def __init__(self, device=0, precision=_abi.PREC_F32, path=_abi.PATH_STREAM, max_device_bytes=0, block_size=0,
-                 steps_per_launch=0, use_graph=1, collision_rebuild_interval=0):
+                 steps_per_launch=0, use_graph=1, collision_rebuild_interval=0, collision_table_entries=0):
         cfg = _abi.BatchCfg()
         cfg.max_device_bytes = max_device_bytes
         cfg.precision = precision
@@ -219,6 +220,7 @@ class Batch:
         cfg.block_size = block_size
         cfg.steps_per_launch = steps_per_launch
         cfg.collision_rebuild_interval = collision_rebuild_interval
+        cfg.collision_table_entries = collision_table_entries
         cfg.use_graph = use_graph
         self.h = C.c_void_p()
         check(lib().vx3h_batch_create(device, C.byref(cfg), C.byref(self.h)))
@@ -249,6 +251,13 @@ class Batch:
 
     def step(self, n):
         check(lib().vx3h_batch_step(self.h, int(n)))
+
+    def do_time_step(self, slot, dt=-1.0):
+        """VX3_VoxelyzeKernel::doTimeStep(dt) of one simulation (vx3h_sim_step): False when
+        a link diverged."""
+        ok = C.c_int(0)
+        check(lib().vx3h_sim_step(self.h, int(slot), float(dt), C.byref(ok)))
+        return bool(ok.value)
 
     def sync(self):
         check(lib().vx3h_batch_sync(self.h))

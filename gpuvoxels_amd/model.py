@@ -177,6 +177,8 @@ class Model:
     enable_attach: int = 0                    # touching sticky voxels get new links (VX3 attach)
     attach_conditions: list = field(default_factory=lambda: [[], [], [], [], []])   # MathTrees, all > 0
     safety_guard: int = 500                   # steps a new link carries 1/100 of its force
+    init_velocity: Optional[np.ndarray] = None   # (V, 3) m/s: initial linear velocity per voxel
+    max_dist_close_pairs: float = 0.0         # MaxDistInVoxelLengthsToCountAsPair (target pairs)
     name: str = "sim"
 
     @property
@@ -266,4 +268,11 @@ class Model:
                 d.n_attach_tokens[k] = len(toks)
                 d.attach_tokens[k] = C.cast(arr, C.POINTER(_abi.Token))
         d.safety_guard = self.safety_guard
+        if self.init_velocity is not None:
+            a = np.ascontiguousarray(self.init_velocity, dtype=np.float64).reshape(-1)
+            if a.size != 3 * self.n_voxels:
+                raise ValueError("init_velocity needs 3 values per voxel")
+            keep.append(a)
+            d.init_velocity = a.ctypes.data_as(C.POINTER(C.c_double))
+        d.max_dist_close_pairs = self.max_dist_close_pairs
         return d, keep

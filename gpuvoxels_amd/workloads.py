@@ -11,6 +11,7 @@
                 Experiments/framework/voxelyze/workflow.py:45-60 (array [z, y, x],
                 x fastest in the structure order).
 * ``slab``      64x64x32 bilinear slab with a four-phase checkerboard.
+* ``arena_balls`` 2048 radius-4 balls thrown into a 400x400 arena (collisions).
 """
 import numpy as np
 
@@ -167,3 +168,93 @@ def balls(n_side: int = 4, layers: int = 2, radius: int = 4, gap: float = 1.5, s
     m.vary_temp_enabled = 0
     m.collisions_enabled = 1
     return m
+
+
+def ball_offsets(radius: int = 4) -> np.ndarray:
+    """Lattice offsets within `radius` of a ball's centre cell (257 for radius 4)."""
+    d = np.arange(-radius, radius + 1)
+    z, y, x = np.meshgrid(d, d, d, indexing="ij")
+    inside = x * x + y * y + z * z <= radius * radius
+    return np.stack([x[inside], y[inside], z[inside]], axis=1)
+
+
+def arena_balls(n_balls: int = 2048, radius: int = 4, arena: int = 400, gap: float = 1.5, sigma_v: float = 0.05,
+                steps: int = 3000, voxel_size: float = 0.01, seed: int = 0, z_layers: int = 3) -> Model:
+    """The collision-heavy multi-body configuration: `n_balls` detached balls of the
+    given radius in one simulation, centres on lattice cells drawn at random (seeded)
+    inside an arena x arena footprint and z_layers ball diameters of height, accepted
+    when at least 2*radius + 1 + gap voxels from every ball placed before (so balls
+    keep >= gap voxels between their surfaces); every voxel of a ball starts with that
+    ball's velocity, drawn per ball from N(0, sigma_v) m/s per axis.  Gravity and floor
+    on, no actuation; the broadphase period is a batch setting (8 steps for this
+    configuration).  Voxels are listed in structure-index order (x fastest)."""
+    rng = np.random.default_rng(seed)
+    r = radius
+    dmin = 2 * r + 1 + gap
+    lo, hi = r, arena - 1 - r
+    zlo, zhi = r, r + z_layers * (2 * r + 1)
+    cell = int(np.ceil(dmin))
+    grid = {}
+    centres = []
+    tries = 0
+    while len(centres) < n_balls:
+        tries += 1
+        if tries > 200 * n_balls:
+            raise RuntimeError("arena too small for %d balls" % n_balls)
+        c = (int(rng.integers(lo, hi + 1)), int(rng.integers(lo, hi + 1)), int(rng.integers(zlo, zhi + 1)))
+        k = (c[0] // cell, c[1] // cell, c[2] // cell)
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for q in grid.get((k[0] + dx, k[1] + dy, k[2] + dz), ()):
+                        if (q[0] - c[0]) ** 2 + (q[1] - c[1]) ** 2 + (q[2] - c[2]) ** 2 < dmin * dmin:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if ok:
+            grid.setdefault(k, []).append(c)
+            centres.append(c)
+    vel = rng.normal(0.0, sigma_v, size=(n_balls, 3))
+    off = ball_offsets(r)
+    cells = (np.array(centres)[:, None, :] + off[None, :, :]).reshape(-1, 3)
+    v = np.repeat(vel, len(off), axis=0)
+    order = np.lexsort((cells[:, 0], cells[:, 1], cells[:, 2]))    # structure-index order
+    m = Model(voxel_size=voxel_size, name="arena_balls")
+    m.materials = [base_material(cte=0.0)]
+    m.ijk = cells[order].astype(np.int32)
+    m.init_velocity = v[order].copy()
+    m.mat = np.zeros(len(cells), np.int32)
+    _physics(m, steps)
+    m.vary_temp_enabled = 0
+    m.collisions_enabled = 1
+    return m
+
+
+def serving_request(rng: np.random.Generator, v_min: int = 100, v_max: int = 50000, h_min: int = 2000,
+                    h_max: int = 20000, name: str = "req"):
+    """One request of the serving stream: a robot built as in the 512-robot workload on a
+    grid scaled so that its expected voxel count (47.5% of the cells survive the largest
+    component rule) is log-uniform in [v_min, v_max], and a horizon uniform in
+    [h_min, h_max] steps.  Returns (model, horizon)."""
+    target = float(np.exp(rng.uniform(np.log(v_min), np.log(v_max))))
+    grid = max(5, int(round((target / 0.475) ** (1.0 / 3.0))))
+    m = robot(rng, grid=grid, steps=0, name=name)
+    return m, int(rng.integers(h_min, h_max + 1))
+
+
+def serving_stream(seconds: float, rate: float, seed: int = 4, **kw):
+    """Poisson arrivals at `rate` requests/s over `seconds`: [(arrival time, model, horizon)]."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        m, h = serving_request(rng, name="req%04d" % len(out), **kw)
+        out.append((t, m, h))

@@ -20,6 +20,8 @@
  *   vx3h_batch_step       CUDA_Simulation loop body calling
  *                         VX3_VoxelyzeKernel::doTimeStep
  *                         (VX3_SimulationManager.cu:62-69, VX3_VoxelyzeKernel.cu:237-359)
+ *   vx3h_sim_step         VX3_VoxelyzeKernel::doTimeStep(float dt) of one simulation
+ *                         (VX3_VoxelyzeKernel.cuh:30, VX3_VoxelyzeKernel.cu:237-359)
  *   vx3h_batch_poll       VX3_SimulationManager::collectResults
  *                         (VX3_SimulationManager.cu:416-458)
  *   vx3h_sim_read_state   direct reads of the device voxel/link arrays done by
@@ -211,6 +213,18 @@ typedef struct vx3h_sim_desc {
     int    n_attach_tokens[5];
     const vx3h_token* attach_tokens[5];
     int    safety_guard;          /* SafetyGuard [500] */
+    /* Initial linear velocity of every voxel (3 per voxel, m/s, global frame), or NULL
+     * for bodies at rest: the voxel starts with linear momentum mass * velocity.  No VXA
+     * key carries it (CVX_Voxel::reset zeroes linMom, old/VX_Voxel.cpp:48-57); it is
+     * for synthetic workloads such as bodies thrown into an arena. */
+    const double* init_velocity;
+    /* MaxDistInVoxelLengthsToCountAsPair [0] (VX3_SimulationManager.cu:358): when > 0,
+     * at every center-of-mass sample the voxels of isTarget materials are paired, and
+     * numClosePairs counts the pairs closer than this many voxel sizes while
+     * targetCloseness sums 1/distance over all pairs (computeTargetCloseness,
+     * VX3_VoxelyzeKernel.cu:545-562); both feed the stop, fitness, force-field and
+     * attach-condition programs and the report. */
+    double max_dist_close_pairs;
 } vx3h_sim_desc;
 
 enum {                         /* StopCondition (Voxelyze3/src/old/VX_Enums.h:55-63) */
@@ -266,6 +280,10 @@ typedef struct vx3h_batch_cfg {
     int    steps_per_launch;  /* resident kernel steps per launch; 0 = tuned default */
     int    collision_rebuild_interval; /* broadphase rebuild period in steps; 0 = on staleness */
     int    use_graph;         /* capture the stream step sequence in a hipGraph */
+    int    collision_table_entries; /* initial collision partner table of each simulation, in entries;
+                                 0 = 64 per surface voxel.  A rebuild that needs more stops that
+                                 simulation in place (no state change) until the next
+                                 vx3h_batch_poll enlarges its table */
 } vx3h_batch_cfg;
 
 typedef struct vx3h_batch vx3h_batch;
@@ -290,6 +308,8 @@ typedef struct vx3h_sim_status {
                                      0 until the first step */
     int    collision_count;       /* target contacts so far (collisionCount, the MathTree variable hit) */
     int    n_links;               /* links so far: the admitted ones plus those made by attach */
+    int    num_close_pairs;       /* numClosePairs at the last center-of-mass sample (report; MathTree) */
+    double target_closeness;      /* targetCloseness at the last sample: sum of 1/distance over target pairs */
 } vx3h_sim_status;
 
 vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, vx3h_batch** out);
@@ -302,6 +322,18 @@ vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* desc, int precision, size_t*
  * step budget, stop condition or divergence).  Asynchronous on the batch stream. */
 vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps);
 vx3h_status vx3h_batch_sync(vx3h_batch* b);
+/* One time step of one simulation of the batch, synchronously:
+ * VX3_VoxelyzeKernel::doTimeStep(float dt = -1) (VX3_VoxelyzeKernel.cuh:30, semantics at
+ * VX3_VoxelyzeKernel.cu:237-359).  dt < 0 steps by DtFrac x the recommended step, cached
+ * at the simulation's first step (or by its fixed dt); dt == 0 only counts a step
+ * (CurStepCount++) and returns; dt > 0 steps by dt, with the center-of-mass sampling
+ * period int(TempPeriod / dt) of that dt.  *ok receives 0 when a link diverged (strain
+ * > 100): the reference's false return, after which the simulation is
+ * VX3H_SIM_DIVERGED and further calls return 0 without stepping; 1 otherwise.  The
+ * step is taken by the per-step kernels for this simulation alone, whichever path it
+ * was admitted to, and regardless of its stop condition: as in the reference, the
+ * caller's loop tests the stop condition (vx3h_batch_poll: state VX3H_SIM_DONE). */
+vx3h_status vx3h_sim_step(vx3h_batch* b, int slot, float dt, int* ok);
 /* Status of every admitted sim (synchronizes). */
 vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int cap, int* n);
 vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot);

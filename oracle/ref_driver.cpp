@@ -59,6 +59,7 @@ struct vxo_sim {
     std::vector<vx3h_token> stopTok, fitTok;
     float lastDt = 0;
     VoxcadStop voxcad;               // VoxCAD stop-condition family and equilibrium mode
+    int colInterval = 0;             // > 0: collision lists also rebuilt every colInterval steps
 };
 
 static int matIndex(vxo_sim* h, CVX_MaterialVoxel* m) {
@@ -148,6 +149,10 @@ vxo_sim* vxo_create(const vx3h_sim_desc* d) {
         for (int i = 0; i < Vx.voxelCount(); i++) Vx.voxel(i)->setTemperature(t0);
     }
     for (int i = 0; i < d->n_materials; i++) Vx.material(i)->setPoissonsRatio(d->volume_effects ? (float)d->materials[i].poissons_ratio : 0.0f);
+    if (d->init_velocity)   // starting momentum (CVX_Voxel::linMom is a public member)
+        for (int i = 0; i < Vx.voxelCount(); i++)
+            Vx.voxel(i)->linMom = Vec3D<double>(d->init_velocity[3 * i], d->init_velocity[3 * i + 1], d->init_velocity[3 * i + 2]) *
+                                  (double)Vx.voxel(i)->material()->mass();
     if (d->collisions_enabled) {
         Vx.boundingRadius = (float)d->bounding_radius;
         Vx.watchDistance = (float)d->watch_distance;
@@ -376,6 +381,8 @@ static bool stepVX3(vxo_sim* h) {
         dt = h->DtFrac * h->OptimalDt;
     }
     h->lastDt = dt;
+    // an optional fixed broadphase period on top of CVoxelyze's staleness rule
+    if (h->colInterval > 0 && (h->CurStepCount - 1) % h->colInterval == 0) h->vx->collisionsStale = true;
     if (!h->vx->doTimeStep(dt)) return false;
     int CycleStep = int(h->TempPeriod / dt);
     if (CycleStep > 0 && h->CurStepCount % CycleStep == 0) {
@@ -479,6 +486,7 @@ void vxo_corner_offset(vxo_sim* h, int v, int corner, float* out) {
     Vec3D<float> r = h->vx->voxel(v)->cornerOffset((CVX_Voxel::voxelCorner)corner);
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
+void vxo_set_collision_interval(vxo_sim* h, int n) { h->colInterval = n > 0 ? n : 0; }
 int vxo_collision_pairs(vxo_sim* h, int* pairs, int cap) {
     const std::vector<CVX_Collision*>* cl = h->vx->collisionList();
     int n = (int)cl->size();

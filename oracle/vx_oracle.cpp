@@ -1036,6 +1036,8 @@ struct Engine {
     std::vector<Link*> linksList;
     std::vector<Collision*> collisionsList;
     bool collisionsStale = true, nearbyStale = true;
+    int colInterval = 0;        // > 0: also regenerate at every step index that is a multiple of it
+    long colStepIndex = 0;      // zero-based index of the step being run (set by the VX3 driver)
 
     ~Engine() {
         for (Collision* c : collisionsList) delete c;
@@ -1181,10 +1183,17 @@ struct Engine {
         for (Voxel* v : voxelsList) v->reset();
         for (Link* l : linksList) l->reset();
     }
-    // Voxelyze.cpp:394-417
+    // Voxelyze.cpp:394-417.  The breadth-first order and the resulting set are the
+    // reference's; membership is tracked with a per-voxel stamp instead of a linear
+    // search of the list (same result, linear time).
+    std::vector<int> nearStamp;
+    int nearTick = 0;
     void generateNearby(Voxel* self, int linkDepth) {
+        if (nearStamp.size() != voxelsList.size()) { nearStamp.assign(voxelsList.size(), 0); nearTick = 0; }
+        const int tick = ++nearTick;
         std::vector<Voxel*> allNearby;
         allNearby.push_back(self);
+        nearStamp[self->index] = tick;
         int iCurrent = 0;
         for (int k = 0; k < linkDepth; k++) {
             int iPassEnd = (int)allNearby.size();
@@ -1192,29 +1201,53 @@ struct Engine {
                 Voxel* pV = allNearby[iCurrent++];
                 for (int i = 0; i < 6; i++) {
                     Voxel* pV2 = pV->adjacentVoxel(i);
-                    if (pV2 && std::find(allNearby.begin(), allNearby.end(), pV2) == allNearby.end()) allNearby.push_back(pV2);
+                    if (pV2 && nearStamp[pV2->index] != tick) { nearStamp[pV2->index] = tick; allNearby.push_back(pV2); }
                 }
             }
         }
         self->nearby->clear();
         for (Voxel* pV : allNearby) if (pV->isSurface() && pV != self) self->nearby->push_back(pV);
     }
-    // Voxelyze.cpp:726-751
+    // Voxelyze.cpp:726-751.  The reference tests every later voxel b of every surface
+    // voxel a; here the candidates b come from a uniform grid of cell edge slightly
+    // above the threshold radius (every pair within the radius lies in neighbouring
+    // cells, whatever the rounding of pos/cell), and each a takes its partners in
+    // ascending list order -- the same pairs, in the same order, on collisionsList and
+    // on every voxel's colWatch list.
     void regenerateCollisions(float threshRadiusSq) {
         clearCollisions();
+        const double cell = std::sqrt((double)threshRadiusSq) * 1.0001 + 1e-300;
+        std::map<std::tuple<long, long, long>, std::vector<int>> grid;
+        auto key = [&](const Vd& p) {
+            return std::make_tuple((long)std::floor(p.x / cell), (long)std::floor(p.y / cell), (long)std::floor(p.z / cell));
+        };
+        for (size_t a = 0; a < voxelsList.size(); a++)
+            if (!voxelsList[a]->isInterior()) grid[key(voxelsList[a]->pos)].push_back((int)a);
+        if (nearStamp.size() != voxelsList.size()) { nearStamp.assign(voxelsList.size(), 0); nearTick = 0; }
+        std::vector<int> cand;
         for (size_t a = 0; a < voxelsList.size(); a++) {
             Voxel* pV1 = voxelsList[a];
             if (pV1->isInterior()) continue;
             *pV1->lastColWatchPosition = (Vf)pV1->pos;
-            for (size_t b = a + 1; b < voxelsList.size(); b++) {
+            const int tick = ++nearTick;
+            for (Voxel* n : *pV1->nearby) nearStamp[n->index] = tick;
+            cand.clear();
+            const auto c = key(pV1->pos);
+            for (long dz = -1; dz <= 1; dz++)
+                for (long dy = -1; dy <= 1; dy++)
+                    for (long dx = -1; dx <= 1; dx++) {
+                        auto it = grid.find(std::make_tuple(std::get<0>(c) + dx, std::get<1>(c) + dy, std::get<2>(c) + dz));
+                        if (it == grid.end()) continue;
+                        for (int b : it->second) if (b > (int)a) cand.push_back(b);
+                    }
+            std::sort(cand.begin(), cand.end());
+            for (int b : cand) {
                 Voxel* pV2 = voxelsList[b];
-                if (pV2->isInterior() || (pV1->pos - pV2->pos).Length2() > threshRadiusSq ||
-                    std::find(pV1->nearby->begin(), pV1->nearby->end(), pV2) != pV1->nearby->end())
-                    continue;
-                Collision* c = new Collision(pV1, pV2);
-                collisionsList.push_back(c);
-                pV1->colWatch->push_back(c);
-                pV2->colWatch->push_back(c);
+                if ((pV1->pos - pV2->pos).Length2() > threshRadiusSq || nearStamp[pV2->index] == tick) continue;
+                Collision* col = new Collision(pV1, pV2);
+                collisionsList.push_back(col);
+                pV1->colWatch->push_back(col);
+                pV2->colWatch->push_back(col);
             }
         }
         collisionsStale = false;
@@ -1232,6 +1265,7 @@ struct Engine {
         for (Voxel* pV : voxelsList) {
             if (pV->isSurface() && (pV->pos - *pV->lastColWatchPosition).Length2() > recalcDist * recalcDist) collisionsStale = true;
         }
+        if (colInterval > 0 && colStepIndex % colInterval == 0) collisionsStale = true;
         if (collisionsStale) regenerateCollisions(watchRadiusMm * watchRadiusMm);
         for (Collision* c : collisionsList) c->updateContactForce();
     }
@@ -1357,6 +1391,23 @@ struct Sim {
     float lastDt = 0;
     VoxcadStop voxcad;   // VoxCAD stop-condition family and equilibrium mode
     int hits = 0;        // collisionCount (the MathTree variable hit)
+    // computeTargetCloseness (VX3_VoxelyzeKernel.cu:545-562) over the voxels of isTarget
+    // materials in voxel order (registerTargets, :536-543), at every center-of-mass sample
+    double closeR = 0, targetCloseness = 0;
+    int numClosePairs = 0;
+    std::vector<Voxel*> closeTargets;
+    void computeTargetCloseness() {
+        if (closeR == 0) return;
+        double ret = 0;
+        numClosePairs = 0;
+        for (size_t i = 0; i < closeTargets.size(); i++)
+            for (size_t j = i + 1; j < closeTargets.size(); j++) {
+                const double distance = closeTargets[i]->pos.Dist(closeTargets[j]->pos);
+                if (distance < closeR) numClosePairs++;
+                ret += 1 / distance;
+            }
+        targetCloseness = ret;
+    }
     std::vector<vx3h_token> attachTok[5];   // VX3 attach conditions
 
     // handle_collision_attachment (VX3_VoxelyzeKernel.cu:682-722) over the watched pairs:
@@ -1425,7 +1476,7 @@ struct Sim {
             for (int k = 0; k < 3; k++)
                 f[k] = ffTok[k].empty() ? 0.0
                                         : mathtree_eval(ffTok[k].data(), (int)ffTok[k].size(), v->pos.x, v->pos.y, v->pos.z, hits,
-                                                        currentTime, recentAngle, 0, 0, nv);
+                                                        currentTime, recentAngle, targetCloseness, numClosePairs, nv);
             v->fieldForce = Vd(f[0], f[1], f[2]);
         }
     }
@@ -1479,7 +1530,8 @@ struct Sim {
     bool stopConditionMet() {
         if (maxSteps > 0 && CurStepCount >= maxSteps) return true;
         if (!stopTok.empty() &&
-            mathtree_eval(stopTok.data(), (int)stopTok.size(), currentCoM.x, currentCoM.y, currentCoM.z, hits, currentTime, recentAngle, 0, 0,
+            mathtree_eval(stopTok.data(), (int)stopTok.size(), currentCoM.x, currentCoM.y, currentCoM.z, hits, currentTime, recentAngle,
+                          targetCloseness, numClosePairs,
                           (int)vx.voxelsList.size()) > 0)
             return true;
         return voxcad.type != 0 && voxcad.met(CurStepCount, currentTime, TempPeriod, (int)vx.voxelsList.size());
@@ -1499,7 +1551,13 @@ struct Sim {
         if (vx.surfaceChanged) {
             vx.surfaceChanged = false;
             for (Voxel* v : vx.voxelsList) v->updateSurface();
+            // VX3 topology changes (attach, detach, voxel removal) of the last step: with
+            // collisions, the nearby lists follow the links present now (generateNearby on
+            // the current topology, then a collision regeneration), as CVoxelyze does when
+            // voxels are added or removed (Voxelyze.cpp:442,466,678-684)
+            if (vx.collisions) vx.nearbyStale = true;
         }
+        vx.colStepIndex = CurStepCount - 1;
         if (!vx.doTimeStep(dt)) return false;   // target contacts and extra forces run inside, after the contact update
         if (vx.attach)   // VX3_VoxelyzeKernel.cu:608-621, after each voxel's update; a missing condition does not restrict
             for (Voxel* v : vx.voxelsList) {
@@ -1508,7 +1566,8 @@ struct Sim {
                 for (int k = 0; k < 5 && on; k++)
                     if (!attachTok[k].empty())
                         on = mathtree_eval(attachTok[k].data(), (int)attachTok[k].size(), v->pos.x, v->pos.y, v->pos.z, hits,
-                                           currentTime, recentAngle, 0, 0, (int)vx.voxelsList.size()) > 0;
+                                           currentTime, recentAngle, targetCloseness, numClosePairs,
+                                           (int)vx.voxelsList.size()) > 0;
                 v->enableAttach = on;
             }
         if (signals) {   // every moving voxel on its own state, then every voxel takes its neighbours' posts
@@ -1533,6 +1592,7 @@ struct Sim {
             Vd A = comHist[0], B = comHist[1], C = currentCoM;
             if (B == C || A == B || angleSampleTimes < 3) recentAngle = 0;
             else recentAngle = acos((B - A).Dot(C - B) / (B.Dist(A) * C.Dist(B)));
+            computeTargetCloseness();
         }
         if (secondary) {
             removeVoxels();
@@ -1633,6 +1693,11 @@ static Sim* build(const vx3h_sim_desc* d) {
     // EnableVolumeEffects (VX_Sim.cpp:445-458)
     for (int i = 0; i < d->n_materials; i++)
         vx.voxelMats[i]->setPoissonsRatio(d->volume_effects ? (float)d->materials[i].poissons_ratio : 0.0f);
+    if (d->init_velocity)   // starting momentum mass * velocity (no VXA key; synthetic workloads)
+        for (size_t i = 0; i < vx.voxelsList.size(); i++) {
+            Voxel* v = vx.voxelsList[i];
+            v->linMom = Vd(d->init_velocity[3 * i], d->init_velocity[3 * i + 1], d->init_velocity[3 * i + 2]) * (double)v->mat->_mass;
+        }
     if (d->collisions_enabled) {
         vx.boundingRadius = (float)d->bounding_radius;
         vx.watchDistance = (float)d->watch_distance;
@@ -1668,6 +1733,9 @@ static Sim* build(const vx3h_sim_desc* d) {
     if (vx.attach)
         for (int k = 0; k < 5; k++)
             if (d->n_attach_tokens[k] > 0) s->attachTok[k].assign(d->attach_tokens[k], d->attach_tokens[k] + d->n_attach_tokens[k]);
+    s->closeR = d->max_dist_close_pairs * d->voxel_size;
+    for (size_t i = 0; i < vx.voxelsList.size(); i++)
+        if (d->materials[d->material[i]].is_target) s->closeTargets.push_back(vx.voxelsList[i]);
     s->secondary = d->secondary_experiment != 0;
     s->reinitAfter = d->reinit_after;
     for (int i = 0; i < d->n_materials; i++) s->matRemoveAfter.push_back(d->materials[i].remove_after);
@@ -1809,7 +1877,10 @@ void vxo_vx3_status(vxo_sim* h, vx3h_sim_status* st) {
     st->collision_count = s->hits;
     st->initial_com[0] = s->initialCoM.x; st->initial_com[1] = s->initialCoM.y; st->initial_com[2] = s->initialCoM.z;
     st->current_com[0] = s->currentCoM.x; st->current_com[1] = s->currentCoM.y; st->current_com[2] = s->currentCoM.z;
-    st->fitness = mathtree_eval(s->fitTok.data(), (int)s->fitTok.size(), off.x, off.y, off.z, s->hits, s->currentTime, s->recentAngle, 0, 0,
+    st->num_close_pairs = s->numClosePairs;
+    st->target_closeness = s->targetCloseness;
+    st->fitness = mathtree_eval(s->fitTok.data(), (int)s->fitTok.size(), off.x, off.y, off.z, s->hits, s->currentTime, s->recentAngle,
+                                s->targetCloseness, s->numClosePairs,
                                 (int)s->vx.voxelsList.size());
     st->recent_angle = s->recentAngle;
     int nm = 0;
@@ -1863,6 +1934,7 @@ void vxo_corner_offset(vxo_sim* h, int v, int corner, float* out) {
     Vf r = h->s->vx.voxelsList[v]->cornerOffset(corner);
     out[0] = r.x; out[1] = r.y; out[2] = r.z;
 }
+void vxo_set_collision_interval(vxo_sim* h, int n) { h->s->vx.colInterval = n > 0 ? n : 0; }
 int vxo_collision_pairs(vxo_sim* h, int* pairs, int cap) {
     Engine& e = h->s->vx;
     int n = (int)e.collisionsList.size();

@@ -71,6 +71,10 @@ void vxo_voxcad_history(vxo_sim* s, double* kinetic, double* total, double* max_
 /* VX3 local signal of each voxel (0 where the engine has no signals) */
 void vxo_local_signal(vxo_sim* s, double* out);
 int  vxo_collision_pairs(vxo_sim* s, int* pairs, int cap);        /* returns count */
+/* n > 0: the VX3 driver also regenerates the collision lists at every step whose
+ * zero-based index is a multiple of n, besides CVoxelyze's staleness rule (a fixed
+ * broadphase period; the engine's collision_rebuild_interval) */
+void vxo_set_collision_interval(vxo_sim* s, int n);
 /* derived material constants, for the host builder's parity checks.
  * voxel mats: 14 floats each; link mats (list order): 20 floats each */
 int  vxo_voxel_material_consts(vxo_sim* s, float* out, int cap);

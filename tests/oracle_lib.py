@@ -49,6 +49,7 @@ def _bind(lib):
         "vxo_voxcad_history": (None, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "vxo_local_signal": (None, [P, C.POINTER(C.c_double)]),
         "vxo_collision_pairs": (I, [P, pI, I]),
+        "vxo_set_collision_interval": (None, [P, I]),
         "vxo_voxel_material_consts": (I, [P, pF, I]),
         "vxo_link_material_consts": (I, [P, pI, pF, I]),
         "vxo_mathtree_eval": (D, [C.POINTER(_abi.Token), I, D, D, D, D, D, D, D, I, I]),
@@ -183,6 +184,10 @@ class OSim:
         out = np.zeros(2 * max(n, 1), np.int32)
         self.lib.vxo_collision_pairs(self.h, _p(out, C.c_int), n)
         return out[: 2 * n].reshape(-1, 2)
+
+    def set_collision_interval(self, n):
+        """Also regenerate the collision lists every n steps (the engine's collision_rebuild_interval)."""
+        self.lib.vxo_set_collision_interval(self.h, int(n))
 
     def voxel_material_consts(self):
         n = self.lib.vxo_voxel_material_consts(self.h, None, 0)

@@ -140,3 +140,15 @@ def test_footprint_and_errors():
     d, keep = bad.desc()
     assert lib.vx3h_sim_footprint(C.byref(d), _abi.PREC_F32, C.byref(b32)) == _abi.EINVAL
     assert b"duplicate voxel position" in lib.vx3h_last_error()
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """VX3_VoxelyzeKernel (doTimeStep/StopConditionMet) and VX3_SimulationManager, the
+    reference's class names over the C ABI, compile and link against libvx3h.so."""
+    import subprocess
+    from test_gpu_facade import CPP, ROOT as R
+    src = tmp_path / "facade.cpp"
+    src.write_text(CPP)
+    lib = os.path.join(R, "gpuvoxels_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-o", str(tmp_path / "facade"), str(src), "-I" + R, "-L" + lib, "-lvx3h",
+                    "-Wl,-rpath," + lib], check=True)

@@ -1,0 +1,64 @@
+"""Parity of the 512-robot workload's robots over the full 20,000-step horizon and of
+the FP32 state over the first 1,000 steps, against the CPU oracle (FP64, the
+reference's precision).  Robots: the first eight of the seed-0 batch (12^3 grid,
+Bernoulli(0.5) occupancy, largest face-connected component, four materials)."""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from gpuvoxels_amd import _abi, engine, workloads
+from oracle_lib import OSim
+
+pytestmark = pytest.mark.gpu
+
+SPAN = 0.12   # m: the 12-voxel grid, the position scale of the relative error
+
+
+def oracle_runs(models, steps):
+    def one(m):
+        o = OSim(m)
+        o.step(steps)
+        return o
+    with ThreadPoolExecutor(max_workers=8) as ex:   # the oracle's calls release the GIL
+        return list(ex.map(one, models))
+
+
+def test_fitness_within_half_percent_over_20000_steps_f64():
+    rs = workloads.robots(8, seed=0, steps=20000)
+    os_ = oracle_runs(rs, 20000)
+    b = engine.Batch(precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
+    slots = [b.admit(r) for r in rs]
+    b.step(20000)
+    st = b.poll()
+    errs = []
+    for o, s in zip(os_, slots):
+        ost, gst = o.status(), st[s]
+        assert gst.state == ost.state == _abi.SIM_DONE and gst.steps == ost.steps == 20000
+        errs.append(abs(gst.fitness - ost.fitness) / abs(ost.fitness))
+    print("max relative fitness error after 20000 steps: %.3e" % max(errs))
+    assert max(errs) < 0.005
+    # the FP64 state follows the CPU engine far more closely than the 0.5% bar
+    assert max(errs) < 1e-5
+    b.close()
+
+
+@pytest.mark.parametrize("path", [_abi.PATH_RESIDENT, _abi.PATH_STREAM])
+def test_f32_positions_and_orientations_first_1000_steps(path):
+    """FP32 state against the FP64 oracle after 1000 steps.  The bounds are the
+    measured ones (max over the eight robots, both paths), with headroom: positions
+    within 1e-4 of the 12-voxel span; orientations (unit quaternions) within 5e-4."""
+    rs = workloads.robots(8, seed=0, steps=0)
+    os_ = oracle_runs(rs, 1000)
+    b = engine.Batch(precision=_abi.PREC_F32, path=path)
+    slots = [b.admit(r) for r in rs]
+    b.step(1000)
+    pe = oe = 0.0
+    for o, s in zip(os_, slots):
+        g, c = b.read_state(s), o.state()
+        pe = max(pe, float(np.max(np.abs(g["pos"] - c["pos"]))) / SPAN)
+        oe = max(oe, float(np.max(np.abs(g["orient"] - c["orient"]))))
+    print("FP32 after 1000 steps: max |dpos|/span %.3e, max |dquat| %.3e" % (pe, oe))
+    assert pe < 1e-4
+    assert oe < 5e-4
+    b.close()

@@ -2,7 +2,10 @@
 import numpy as np
 import pytest
 
+from concurrent.futures import ThreadPoolExecutor
+
 from gpuvoxels_amd import _abi, engine, workloads
+from oracle_lib import OSim
 
 pytestmark = pytest.mark.gpu
 
@@ -54,3 +57,65 @@ def test_results_same_alone_and_in_mix(precision):
         assert list(r.status.current_com) == list(ref.current_com)
         assert r.t_submit <= r.t_admit <= r.t_done
     sv.close()
+
+
+def full_size_requests():
+    """Requests spanning the serving stream's range: robots of ~100 to ~50,000 voxels
+    (grids 7 to 47), horizons of 2,000 steps and more."""
+    rng = np.random.default_rng(7)
+    out = []
+    for i, g in enumerate([7, 11, 19, 30, 40, 47]):
+        m = workloads.robot(rng, grid=g, steps=0, name="full%d" % i)
+        out.append((m, 2000 + 97 * i))
+    return out
+
+
+def serve(reqs, precision, quantum=700, max_sims=4):
+    sv = engine.Server(precision=precision, quantum_steps=quantum, max_sims=max_sims)
+    tickets = {sv.submit(m, h): i for i, (m, h) in enumerate(reqs)}
+    results, rounds = {}, 0
+    while True:
+        active = sv.round()
+        rounds += 1
+        for r in sv.results():
+            results[tickets[r.ticket]] = r
+        if active == 0:
+            break
+        assert rounds < 500
+    sv.close()
+    return results
+
+
+@pytest.mark.parametrize("precision", [_abi.PREC_F64, _abi.PREC_F32])
+def test_full_size_requests_same_alone_and_in_mix(precision):
+    reqs = full_size_requests()
+    assert max(m.n_voxels for m, _ in reqs) > 45000 and min(m.n_voxels for m, _ in reqs) < 250
+    results = serve(reqs, precision)
+    assert len(results) == len(reqs)
+    for i, (m, h) in enumerate(reqs):
+        r = results[i]
+        assert r.status.state == _abi.SIM_DONE and r.status.steps == h
+        ref = solo(m, h, precision)
+        assert r.status.current_time == ref.current_time
+        assert r.status.fitness == ref.fitness
+        assert list(r.status.current_com) == list(ref.current_com)
+
+
+def test_full_size_requests_match_oracle_f64():
+    reqs = full_size_requests()
+    picked = [reqs[0], reqs[2], reqs[5]]   # ~100, ~3,000 and ~50,000 voxels
+    results = serve(picked, _abi.PREC_F64)
+
+    def oracle(mh):
+        o = OSim(mh[0])
+        o.step(mh[1])
+        return o.status()
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        ost = list(ex.map(oracle, picked))
+    for i, ((m, h), o) in enumerate(zip(picked, ost)):
+        g = results[i].status
+        assert g.steps == o.steps == h
+        err = abs(g.fitness - o.fitness) / abs(o.fitness)
+        print("%d voxels, %d steps: relative fitness error %.3e" % (m.n_voxels, h, err))
+        assert err < 1e-6
+        assert np.max(np.abs(np.array(g.current_com) - np.array(o.current_com))) < 1e-9

@@ -835,3 +835,38 @@ def test_new_link_carries_no_force_then_one_hundredth():
     X, Y, G = [a["lin_mom"][1, 2] - b["lin_mom"][1, 2] for a, b in zip(after, now)]
     assert abs(X - G) > 1e3 * abs(G) * 1e-12
     assert (Y - G) == pytest.approx(0.01 * (X - G), rel=1e-9)
+
+
+def _close_targets():
+    """Three material-fixed target voxels on the x axis at 0, 2 and 5 voxel sizes and a
+    free non-target voxel far away; MaxDistInVoxelLengthsToCountAsPair = 3, center of
+    mass sampled every 10 steps (TempPeriod / dt = 10)."""
+    s = 0.01
+    m = Model(voxel_size=s)
+    m.materials = [Material(elastic_mod=1e6, density=1000.0, fixed=1, is_target=1, matid=1),
+                   Material(elastic_mod=1e6, density=1000.0, matid=2)]
+    m.ijk = np.array([[0, 0, 0], [2, 0, 0], [5, 0, 0], [9, 9, 9]], np.int32)
+    m.mat = np.array([0, 0, 0, 1], np.int32)
+    m.dt_fixed = 1e-4
+    m.temp_period = 1e-3
+    m.max_dist_close_pairs = 3.0
+    return m
+
+
+def test_target_closeness_and_close_pairs_closed_form():
+    """computeTargetCloseness (VX3_VoxelyzeKernel.cu:545-562): at the sample, pairs of
+    target voxels closer than 3 voxel sizes are counted (only 0-2: the 3-size pair 2-5
+    is not closer than 3) and 1/distance is summed over all three pairs; the stop
+    program numClosePairs > 0 stops the run after the sample step."""
+    m = _close_targets()
+    s = m.voxel_size
+    o = OSim(m)
+    o.step(9)
+    assert o.status().num_close_pairs == 0 and o.status().target_closeness == 0
+    o.step(1)
+    st = o.status()
+    assert st.num_close_pairs == 1
+    assert st.target_closeness == pytest.approx(1 / (2 * s) + 1 / (5 * s) + 1 / (3 * s), rel=1e-12)
+    m.stop_tokens = [(MT["mtCONST"], 0.0), (MT["mtVAR"], 7.0), (MT["mtSUB"], 0.0), (MT["mtEND"], 0.0)]
+    o2 = OSim(m)
+    assert o2.step(100) == 10 and o2.status().state == 1

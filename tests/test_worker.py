@@ -1,4 +1,4 @@
-"""vx3_node_worker and the Voxelyze3 launcher: command-line contracts (CPU) and full
+"""vx3_node_worker: command-line contracts (CPU) and full
 task runs checked against the CPU oracle (GPU)."""
 import os
 import subprocess
@@ -64,64 +64,3 @@ def test_worker_task_matches_oracle(tmp_path):
         fits.append((f, name))
     best = max(fits)
     assert rep.find("bestfit").findtext("filename") == best[1]
-
-
-LAUNCHER = os.path.join(os.path.dirname(WORKER), "Voxelyze3")
-
-
-def launch(args, cwd, timeout=300):
-    return subprocess.run([LAUNCHER] + args, cwd=str(cwd), stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
-                          timeout=timeout)
-
-
-def test_launcher_argument_errors(tmp_path):
-    vxt, inp, _, _ = make_task(str(tmp_path))
-    assert launch([], tmp_path).returncode == 1
-    assert launch(["-i", str(tmp_path / "nope")], tmp_path).returncode == 1
-    empty = tmp_path / "empty"
-    empty.mkdir()
-    r = launch(["-i", str(empty), "-w", WORKER], tmp_path)
-    assert r.returncode == 1 and "No base.vxa" in r.stdout
-    r = launch(["-i", inp, "-w", str(tmp_path / "no_worker")], tmp_path)
-    assert r.returncode == 1 and "worker" in r.stdout
-    out = tmp_path / "out.xml"
-    out.write_text("keep")
-    r = launch(["-i", inp, "-w", WORKER, "-o", str(out), "-l"], tmp_path)
-    assert r.returncode == 1 and out.read_text() == "keep"
-    assert launch(["-i", inp, "-n", "2", "-l"], tmp_path).returncode == 1
-
-
-def test_launcher_task_file_and_copy(tmp_path):
-    """With a stand-in worker that copies its task file to its output, the report path
-    receives the task file the launcher wrote: base.vxa, the input directory and every
-    .vxd (any case) in name order."""
-    vxt, inp, _, names = make_task(str(tmp_path))
-    with open(os.path.join(inp, "extra.VXD"), "w") as f:
-        f.write("<VXD/>")
-    with open(os.path.join(inp, "ignored.txt"), "w") as f:
-        f.write("x")
-    fake = tmp_path / "fake_worker"
-    fake.write_text('#!/bin/sh\ncp "$2" "$4"\n')
-    fake.chmod(0o755)
-    out = tmp_path / "report.xml"
-    r = launch(["-i", inp, "-w", str(fake), "-o", str(out), "-lf"], tmp_path)
-    assert r.returncode == 0, r.stdout + r.stderr
-    root = ET.fromstring("<t>%s</t>" % out.read_text().split("?>", 1)[1])
-    assert root.findtext("vxa") == os.path.join(inp, "base.vxa")
-    assert root.findtext("input_dir") == inp
-    assert [e.text for e in root.find("vxd")] == sorted(names + ["extra.VXD"])
-    assert len(os.listdir(tmp_path / "workspace" / "locally")) == 2      # .vxt and .vxr
-
-
-@pytest.mark.gpu
-def test_launcher_runs_worker(tmp_path):
-    vxt, inp, _, names = make_task(str(tmp_path), n=2, stop_time=0.005)
-    direct = tmp_path / "direct.xml"
-    assert run(["-i", vxt, "-o", str(direct)]).returncode == 0
-    out = tmp_path / "report.xml"
-    r = launch(["-i", inp, "-w", WORKER, "-o", str(out), "-lf"], tmp_path)
-    assert r.returncode == 0, r.stdout + r.stderr
-    a, b = ET.parse(str(direct)).getroot(), ET.parse(str(out)).getroot()
-    for name in names:
-        key = name.split(".")[0]
-        assert a.find("detail").find(key).findtext("fitness_score") == b.find("detail").find(key).findtext("fitness_score")
