@@ -4,8 +4,9 @@
 
 HIP translation units are compiled by hipcc for gfx950 with packed FP32
 instructions disabled; the kernel units without machine loop-invariant code motion
-(KERNEL_FLAGS), the FP64 kernel unit also without floating-point contraction so
-that it follows the CPU engine's rounding.  Host-only units are
+(KERNEL_FLAGS) and without floating-point contraction (the FP64 unit so that it
+follows the CPU engine's rounding, both so that the per-step and resident kernels
+give identical results).  Host-only units are
 compiled by g++.
 """
 import os
@@ -34,7 +35,9 @@ KERNEL_FLAGS = ["-mllvm", "-disable-machine-licm"]
 
 UNITS = [
     # (source, compiler, extra flags)
-    ("vx3_kernels_f32.hip", "hip", KERNEL_FLAGS),
+    # both precisions without floating-point contraction: the per-step and resident kernels
+    # then round every operation alike and give identical results on either path
+    ("vx3_kernels_f32.hip", "hip", KERNEL_FLAGS + ["-ffp-contract=off"]),
     ("vx3_kernels_f64.hip", "hip", KERNEL_FLAGS + ["-ffp-contract=off"]),
     ("vx3h_engine.cpp", "hip", ["-x", "hip"]),
     ("vx3h_build.cpp", "cxx", []),

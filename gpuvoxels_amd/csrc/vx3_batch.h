@@ -60,6 +60,8 @@ template <typename T> struct DevBatch {
     int* cellVox;             // surface voxels sorted by bucket, a range of nv entries per sim
     int *vbk, *vrank;         // per voxel: its bucket and its rank within the bucket (last rebuild)
     int* tgtList;             // voxels of isTarget materials per sim (close pairs), a range per sim
+    int* attCand;             // attach scratch per sim: attachable voxels, then their pair offsets
+    unsigned long long* attPair;   // attach scratch per sim: candidate pairs (higher << 32 | lower), ascending
     // tables
     const VoxMatDev* vmatTab;
     const LinkMatDev* lmatTab;

@@ -178,6 +178,7 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     // before their update; their surface state follows at the next step's start
     if ((sc.flags & SC_DETACH) && mat.epsilonFail != -1.0f && st.maxStrain > mat.epsilonFail) {
         b.lflags[l] = st.flags | LF_GONE;
+        if (sc.flags & SC_TOPOCOL) atomicOr(&b.sims[s].flags, SC_TOPO_CHG);
         atomicAnd(&b.vflags[vn], ~(1 << (VF_LINKMASK_SHIFT + dn)));
         atomicAnd(&b.vflags[vp], ~(1 << (VF_LINKMASK_SHIFT + dp)));
     }
@@ -228,7 +229,7 @@ __device__ __forceinline__ bool colWatched(const DevBatch<real>& b, const SimCtl
     // lattice-close voxels can be nearby
     const int li = abs(b.vijk[3 * v] - b.vijk[3 * u]) + abs(b.vijk[3 * v + 1] - b.vijk[3 * u + 1]) +
                    abs(b.vijk[3 * v + 2] - b.vijk[3 * u + 2]);
-    if (li > sc.colNearHops) return true;
+    if (li > sc.colNearHops && !(sc.flags & SC_ATTACH)) return true;   // attached links shortcut the lattice
     const int local = u - sc.v0;
     int lo = b.nearStart[v], hi = lo + b.nearCount[v];
     while (lo < hi) {
@@ -238,6 +239,123 @@ __device__ __forceinline__ bool colWatched(const DevBatch<real>& b, const SimCtl
         if (x < local) lo = mid + 1; else hi = mid;
     }
     return true;
+}
+
+// ---- nearby lists after a topology change (CVX_Voxel::generateNearby, Voxelyze.cpp:394-417,
+// on the links present now).  A step that attaches, detaches or removes voxels marks the
+// simulation (SC_TOPO_CHG); the next step regenerates the nearby list of every surface
+// voxel before its collision rebuild (the CPU engine's nearbyStale -> collisionsStale).
+// One thread per voxel runs the breadth-first search of generateNearby with its queue in
+// LDS (VX3_NEAR_MAX voxels); count, scan and emit passes fill nearList.  A search or a
+// table too small for the result stops the simulation in place (SC_COL_GROW) until the
+// host builds the lists itself or enlarges the table.
+#define VX3_NEAR_MAX 384
+#define VX3_NEAR_BS 64
+
+__device__ __forceinline__ bool nearDirty(const DevBatch<real>& b, int s) {
+    return simRunning(b, s) && (b.sims[s].flags & SC_NEARDIRTY) && !(b.sims[s].flags & SC_DIVERGED_PENDING);
+}
+
+// Queues v and the voxels within `hops` links of it, level by level, into q; returns
+// how many, or -1 when they do not fit.
+__device__ int nearBfs(const DevBatch<real>& b, int v, int hops, int* q) {
+    const size_t V = (size_t)b.Vcap;
+    int n = 0, cur = 0;
+    q[n++] = v;
+    for (int k = 0; k < hops; k++) {
+        const int end = n;
+        while (cur < end) {
+            const int u = q[cur++];
+            const int fl = b.vflags[u];
+            for (int d = 0; d < 6; d++) {
+                if (!(fl & (1 << (VF_LINKMASK_SHIFT + d)))) continue;
+                const int w = b.nbr[d * V + u];
+                bool seen = false;
+                for (int i = 0; i < n && !seen; i++) seen = q[i] == w;
+                if (seen) continue;
+                if (n == VX3_NEAR_MAX) return -1;
+                q[n++] = w;
+            }
+        }
+    }
+    return n;
+}
+
+__global__ void __launch_bounds__(VX3_NEAR_BS) k_near_count(DevBatch<real> b) {
+    __shared__ int q[VX3_NEAR_BS][VX3_NEAR_MAX];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!nearDirty(b, s)) return;
+    if (b.vflags[v] & VF_INTERIOR) { b.nearCount[v] = 0; return; }
+    int* Q = q[threadIdx.x];
+    const int n = nearBfs(b, v, b.sims[s].colNearHops, Q);
+    if (n < 0) {
+        b.nearCount[v] = 0;
+        atomicMax(&b.sims[s].nearNeed, 1 << 30);   // the host builds this simulation's lists itself
+        atomicOr(&b.sims[s].flags, SC_COL_GROW);
+        return;
+    }
+    int c = 0;
+    for (int i = 1; i < n; i++) c += !(b.vflags[Q[i]] & VF_INTERIOR);
+    b.nearCount[v] = c;
+}
+
+__global__ void __launch_bounds__(1024) k_near_scan(DevBatch<real> b) {
+    __shared__ int part[1024 / 64];
+    __shared__ int carry;
+    const int s = blockIdx.x;
+    if (s >= b.nSims || !nearDirty(b, s) || (b.sims[s].flags & SC_COL_GROW)) return;
+    SimCtl& sc = b.sims[s];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int base = 0; base < sc.nv; base += 1024) {
+        const int v = sc.v0 + base + threadIdx.x;
+        const bool in = base + (int)threadIdx.x < sc.nv;
+        const int x = in && !(b.vflags[v] & VF_INTERIOR) ? b.nearCount[v] : 0;
+        int incl = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) part[wid] = incl;
+        __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wid; w++) before += part[w];
+        if (in) b.nearStart[v] = sc.near0 + before + incl - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry = before + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (carry > sc.nearCap) {
+            sc.nearNeed = carry;
+            atomicOr(&sc.flags, SC_COL_GROW);
+        } else atomicOr(&sc.flags, SC_REBUILD);   // new nearby lists: the collision lists are stale
+    }
+}
+
+__global__ void __launch_bounds__(VX3_NEAR_BS) k_near_emit(DevBatch<real> b) {
+    __shared__ int q[VX3_NEAR_BS][VX3_NEAR_MAX];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= b.vHi) return;
+    const int s = b.vsim[v];
+    if (!nearDirty(b, s) || (b.sims[s].flags & SC_COL_GROW)) return;
+    if (b.vflags[v] & VF_INTERIOR) { b.nearCount[v] = 0; return; }
+    int* Q = q[threadIdx.x];
+    const int n = nearBfs(b, v, b.sims[s].colNearHops, Q);
+    int* out = b.nearList + b.nearStart[v];
+    const int v0 = b.sims[s].v0;
+    int c = 0;
+    for (int i = 1; i < n; i++)
+        if (!(b.vflags[Q[i]] & VF_INTERIOR)) out[c++] = Q[i] - v0;
+    for (int i = 1; i < c; i++) {   // ascending, for the binary search of colWatched
+        const int x = out[i];
+        int j = i - 1;
+        while (j >= 0 && out[j] > x) { out[j + 1] = out[j]; j--; }
+        out[j + 1] = x;
+    }
 }
 
 // decides, per simulation, whether its lists are rebuilt this step: the first step, every
@@ -758,14 +876,17 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
 //
 // One workgroup per simulation.  Only voxels that can attach take part: on the surface,
 // not removed, their attach conditions held at their last update, of a sticky material
-// that is not fixed.  Those are compacted in voxel order; the pairs (higher, lower)
-// within the envelope and not directly linked are collected in parallel and sorted;
-// one lane then takes them in ascending order, as the reference's threads would if
-// they ran one after another: a pair within five links (counting the links made
-// before it) is skipped, the dominant axis of the offset in the first voxel's frame
-// picks the faces, and the link is made when both faces are free.
-#define VX3_ATT_MAXV 4096   // attachable voxels per simulation and step
-#define VX3_ATT_MAXP 1024   // candidate pairs per simulation and step
+// that is not fixed.  Those are compacted in voxel order into the simulation's attCand
+// range; then the candidate pairs (higher, lower) -- within the envelope, same material,
+// not within five links -- are counted per higher voxel, the counts scanned, and each
+// higher voxel writes its pairs in ascending order of the lower one, so the list in
+// attPair comes out sorted with no sort pass.  The five-link test is exact here
+// because links are only made later in this pass (a link only shortens paths).  One
+// lane then takes the pairs in order, as the reference's threads would if they ran one
+// after another: a pair within five links (counting the links made before it) is
+// skipped, the dominant axis of the offset in the first voxel's frame picks the faces,
+// and the link is made when both faces are free.  A pair table too small for this step
+// stops the simulation in place (SC_COL_GROW) until the host enlarges it.
 
 // is_neighbor (VX3_VoxelyzeKernel.cu:651-680): t is reachable from a within depth links,
 // never stepping straight back over the link just taken
@@ -786,16 +907,47 @@ __device__ bool linkedWithin(const DevBatch<real>& b, size_t V, int a, int t, in
     return false;
 }
 
+// the candidate test of one (higher, lower) pair of attachable voxels
+__device__ __forceinline__ bool attachCandidate(const DevBatch<real>& b, size_t V, double nom, int v1, int v2) {
+    if (b.vmat[v2] != b.vmat[v1]) return false;
+    const double b1 = nom * (double)b.tfac[v1], b2 = nom * (double)b.tfac[v2];
+    const double wd = ((b1 + b1 + b1) / 3.0f + (b2 + b2 + b2) / 3.0f) * 0.625;   // COLLISION_ENVELOPE_RADIUS
+    const double dx = (double)b.px[v1] - (double)b.px[v2], dy = (double)b.py[v1] - (double)b.py[v2],
+                 dz = (double)b.pz[v1] - (double)b.pz[v2];
+    if (dx > wd || dx < -wd || dy > wd || dy < -wd || dz > wd || dz < -wd) return false;
+    if (sqrt(dx * dx + dy * dy + dz * dz) > wd) return false;
+    return !linkedWithin(b, V, v1, v2, 5);   // covers is_neighbor depth 1 (directly linked)
+}
+
+// block-wide exclusive scan of x (256 threads); returns this thread's offset, total in *tot
+__device__ __forceinline__ int blockScan256(int x, int* part, int* tot) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) part[w] = incl;
+    __syncthreads();
+    int before = incl - x;
+    for (int i = 0; i < w; i++) before += part[i];
+    const int t = part[0] + part[1] + part[2] + part[3];
+    __syncthreads();
+    *tot = t;
+    return before;
+}
+
 __global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
     const int s = blockIdx.x;
     if (s >= b.nSims || !simRunning(b, s)) return;
     SimCtl& sc = b.sims[s];
     if (!(sc.flags & SC_ATTACH) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
-    __shared__ int cand[VX3_ATT_MAXV];
-    __shared__ unsigned long long pairs[VX3_ATT_MAXP], sorted[VX3_ATT_MAXP];
-    __shared__ int wcnt[4], nc, np;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ int part[4];
+    __shared__ int nc, np;
+    const int tid = threadIdx.x;
     const size_t V = (size_t)b.Vcap;
+    int* cand = b.attCand + sc.at0;   // attachable voxels (sim-local), then per candidate its pair offset
+    int* poff = b.attCand + sc.at0 + sc.nv;
     if (tid == 0) { nc = 0; np = 0; }
     __syncthreads();
     for (int base = 0; base < sc.nv; base += 256) {
@@ -806,54 +958,39 @@ __global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
             const VoxMatDev& m = b.vmatTab[b.vmat[v]];
             ok = !(fl & (VF_INTERIOR | VF_REMOVED)) && (fl & VF_ATTACH) && m.sticky && !m.fixed;
         }
-        const unsigned long long bal = __ballot(ok);
-        if (lane == 0) wcnt[wv] = __popcll(bal);
+        int tot;
+        const int off = nc + blockScan256(ok ? 1 : 0, part, &tot);
+        if (ok) cand[off] = i;
+        if (tid == 0) nc += tot;
         __syncthreads();
-        int off = nc + __popcll(bal & ((1ull << lane) - 1));
-        for (int w = 0; w < wv; w++) off += wcnt[w];
-        if (ok && off < VX3_ATT_MAXV) cand[off] = i;
-        __syncthreads();
-        if (tid == 0) nc += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
-        __syncthreads();
-    }
-    if (nc > VX3_ATT_MAXV) {   // more attachable voxels than the pass holds: the simulation fails
-        if (tid == 0) atomicOr(&sc.flags, SC_COL_OVERFLOW);
-        return;
     }
     const int n = nc;
     const double nom = sc.voxSize;
-    for (int x = tid; x < n; x += 256) {
-        const int v1 = sc.v0 + cand[x];
-        const double b1 = nom * (double)b.tfac[v1];
-        const int m1 = b.vmat[v1];
-        for (int y = 0; y < x; y++) {
-            const int v2 = sc.v0 + cand[y];
-            if (b.vmat[v2] != m1) continue;
-            const double b2 = nom * (double)b.tfac[v2];
-            const double wd = ((b1 + b1 + b1) / 3.0f + (b2 + b2 + b2) / 3.0f) * 0.625;   // COLLISION_ENVELOPE_RADIUS
-            const double dx = (double)b.px[v1] - (double)b.px[v2], dy = (double)b.py[v1] - (double)b.py[v2],
-                         dz = (double)b.pz[v1] - (double)b.pz[v2];
-            if (dx > wd || dx < -wd || dy > wd || dy < -wd || dz > wd || dz < -wd) continue;
-            if (sqrt(dx * dx + dy * dy + dz * dz) > wd) continue;
-            bool linked = false;   // is_neighbor depth 1
-            for (int d = 0; d < 6; d++)
-                if ((b.vflags[v1] & (1 << (VF_LINKMASK_SHIFT + d))) && b.nbr[d * V + v1] == v2) linked = true;
-            if (linked) continue;
-            const int k = atomicAdd(&np, 1);
-            if (k < VX3_ATT_MAXP) pairs[k] = ((unsigned long long)cand[x] << 32) | (unsigned)cand[y];
+    // pairs per higher candidate x, then their offsets (chunks of 256 candidates)
+    for (int base = 0; base < n; base += 256) {
+        const int x = base + tid;
+        int c = 0;
+        if (x < n) {
+            const int v1 = sc.v0 + cand[x];
+            for (int y = 0; y < x; y++) c += attachCandidate(b, V, nom, v1, sc.v0 + cand[y]);
         }
-    }
-    __syncthreads();
-    if (np > VX3_ATT_MAXP) {
-        if (tid == 0) atomicOr(&sc.flags, SC_COL_OVERFLOW);
-        return;
+        int tot;
+        const int off = np + blockScan256(c, part, &tot);
+        if (x < n) poff[x] = off;
+        if (tid == 0) np += tot;
+        __syncthreads();
     }
     const int p = np;
-    for (int k = tid; k < p; k += 256) {   // rank sort: the keys are distinct
-        const unsigned long long key = pairs[k];
-        int r = 0;
-        for (int j = 0; j < p; j++) r += pairs[j] < key;
-        sorted[r] = key;
+    if (p > sc.apCap) {   // the pair table is too small: stand still until the host enlarges it
+        if (tid == 0) { sc.apNeed = p; atomicOr(&sc.flags, SC_COL_GROW); }
+        return;
+    }
+    unsigned long long* sorted = b.attPair + sc.ap0;
+    for (int x = tid; x < n; x += 256) {
+        const int v1 = sc.v0 + cand[x];
+        int k = poff[x];
+        for (int y = 0; y < x; y++)
+            if (attachCandidate(b, V, nom, v1, sc.v0 + cand[y])) sorted[k++] = ((unsigned long long)cand[x] << 32) | (unsigned)cand[y];
     }
     __syncthreads();
     if (tid != 0) return;
@@ -901,7 +1038,10 @@ __global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
         b.vflags[vpos] |= 1 << (VF_LINKMASK_SHIFT + dp);
         made = true;
     }
-    if (made) sc.nl = nl;
+    if (made) {
+        sc.nl = nl;
+        sc.flags |= SC_TOPO_CHG;
+    }
 }
 
 // Target contacts (handle_collision_attachment, VX3_VoxelyzeKernel.cu:682-722): each
@@ -1113,6 +1253,7 @@ __device__ bool secondaryExperiment(const DevBatch<real>& b, SimCtl& sc) {
     }
     __syncthreads();
     if (any) {
+        if (threadIdx.x == 0) atomicOr(&sc.flags, SC_TOPO_CHG);
         for (int l = sc.l0 + threadIdx.x; l < sc.l0 + sc.nl; l += blockDim.x) {
             const int lf = b.lflags[l];
             if (lf & LF_GONE) continue;
@@ -1197,6 +1338,10 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     if (threadIdx.x != 0) return;
     sc.steps = steps;
     sc.flags &= ~SC_REBUILD;
+    if (sc.flags & SC_TOPOCOL) {   // a change made during this step: nearby lists regenerated at the next
+        sc.flags &= ~SC_NEARDIRTY;
+        if (sc.flags & SC_TOPO_CHG) sc.flags = (sc.flags & ~SC_TOPO_CHG) | SC_NEARDIRTY;
+    }
     if (sc.flags & SC_COL_OVERFLOW) {
         sc.state = 4;
         return;

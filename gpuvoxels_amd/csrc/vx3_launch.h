@@ -11,6 +11,7 @@ struct StepOpts {
     bool prestep;      // some simulation has cilia, a force field or detach: run the step-start pass
     bool signals;      // some simulation has signals: run the signal-delivery pass
     bool attach;       // some simulation attaches sticky voxels: run the attach pass and conditions
+    bool topo;         // some simulation has collisions and topology changes: run the nearby-list passes
     int block;         // workgroup size of the element passes
 };
 // Kernel passes of one step, in launch order; ev (optional) receives a start/stop

@@ -4,8 +4,14 @@
 namespace vx3 {
 
 namespace {
-void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs) {
+void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool topo) {
     const int gv = (b.vHi + bs - 1) / bs;
+    if (topo) {   // nearby lists of simulations whose topology changed in the last step
+        const int gn = (b.vHi + VX3_NEAR_BS - 1) / VX3_NEAR_BS;
+        hipLaunchKernelGGL(VX_NS::k_near_count, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
+        hipLaunchKernelGGL(VX_NS::k_near_scan, dim3(b.nSims), dim3(1024), 0, s, b);
+        hipLaunchKernelGGL(VX_NS::k_near_emit, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
+    }
     hipLaunchKernelGGL(VX_NS::k_col_stale, dim3(gv), dim3(bs), 0, s, b);
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_clear, dim3((b.hashTot + 255) / 256), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_insert, dim3(gv), dim3(bs), 0, s, b);
@@ -35,7 +41,7 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     // link pass (which moves no voxel): the reference rebuilds them between the link and
     // voxel passes from the same positions.  A simulation whose neighbour table proves too
     // small stands still for this step, link pass included.
-    if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs));
+    if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs, o.topo));
     if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
     if (o.contacts && o.extras) hipLaunchKernelGGL(VX_NS::k_hits, dim3(gv), dim3(bs), 0, s, b);
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 1);

@@ -46,8 +46,12 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_SIGNALS = 8192,       // signals between linked voxels (runs with SC_EXTRAS)
        SC_TARGETS = 16384,      // target contacts are counted (runs with SC_EXTRAS)
        SC_ATTACH = 32768,       // touching sticky surface voxels get new links
-       SC_COL_GROW = 65536 };   // the collision neighbour table is too small: the simulation stands
-                                // still (no pass changes it) until the host enlarges the table
+       SC_COL_GROW = 65536,     // a collision table is too small: the simulation stands still
+                                // (no pass changes it) until the host enlarges the table
+       SC_TOPOCOL = 1 << 17,    // collisions with topology changes (attach, detach, voxel removal):
+                                // the nearby lists are regenerated on the device after a change
+       SC_TOPO_CHG = 1 << 18,   // the topology changed during this step
+       SC_NEARDIRTY = 1 << 19 };  // this step regenerates the nearby lists from the current links
 #define VX3_COL_HBLOCK 1024     // hash buckets per scan block; a simulation's buckets are whole blocks
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
@@ -117,6 +121,9 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int nbNeed;                      // entries the last rebuild needed when it set SC_COL_GROW
     int h0, hBits;                   // hash-table range: (1 << hBits) buckets from bucket h0 (whole blocks)
     int c0;                          // cell-sorted surface voxels: range [c0, c0 + nv) of cellVox
+    int near0, nearCap, nearNeed;    // nearby-list range in nearList, and the size a rebuild needed
+    int at0;                         // attach: attachable voxels and their pair offsets, attCand[at0 .. at0 + 2 nv)
+    int ap0, apCap, apNeed;          // attach: candidate pair table attPair[ap0 .. ap0 + apCap), size last needed
     int rebuilds;                    // neighbour-list rebuilds so far
     // VoxCAD stop-condition family and equilibrium mode (VoxCAD/src/VX1/VX_Sim.cpp:862-891,1015-1160)
     int scType;                      // VX3H_SC_*

@@ -117,6 +117,7 @@ struct SlotHost {
     // collision pools: neighbour lists, hash buckets, nearby lists
     int nb0 = 0, nbN = 0, h0 = 0, hN = 0, near0 = 0, nearN = 0, c0 = 0;   // hN in buckets (whole blocks)
     int tg0 = 0, nTg = 0;        // close pairs: range of the sim's target voxels in tgtList
+    int at0 = 0, ap0 = 0, apN = 0;   // attach scratch ranges (attCand: 2*nv; attPair: apN)
     // history records: topology and link constants of the sim, palette in order
     std::vector<int> vlinks;     // 6 per voxel, sim-local link index or -1
     std::vector<char> surface;   // per voxel: not all six links present
@@ -169,12 +170,13 @@ struct vx3h_batch {
     void* dExt = nullptr; size_t capExt = 0;
     void* dTok = nullptr; size_t capTok = 0;
     void* dSimMats = nullptr; size_t capSimMats = 0;
-    DevPool nbPool, hashPool, nearPool, cellPool, tgtPool;
+    DevPool nbPool, hashPool, nearPool, cellPool, tgtPool, attPool, pairPool;
     bool anyColl = false;
     bool anyExtras = false;             // some simulation has cilia or a force field
     bool anyDetach = false;             // some simulation detaches failed links
     bool anyAttach = false;             // some simulation attaches sticky voxels
     bool anySignals = false;            // some simulation has signals
+    bool anyTopoCol = false;            // some simulation has collisions and attach, detach or voxel removal
     int dtLo = 1 << 30, dtHi = -1;
     size_t bytesUsed = 0, budget = 0;
     // resident path: kernel shapes and, per shape, the sims it runs
@@ -237,6 +239,8 @@ void initSpecs(vx3h_batch* b) {
     b->cellPool.arrs = {{(void**)&d.cellVox, 4}};
     b->hashPool.minCap = 16;
     b->tgtPool.arrs = {{(void**)&d.tgtList, 4}};
+    b->attPool.arrs = {{(void**)&d.attCand, 4}};
+    b->pairPool.arrs = {{(void**)&d.attPair, 8}};
     b->nearPool.arrs = {{(void**)&d.nearList, 4}};
     L(&d.lneg, 4, 1, false); L(&d.lpos, 4, 1, false); L(&d.lmeta, 4, 1, false); L(&d.lsim, 4, 1, false); L(&d.lflags, 4, 1, false);
     L(&d.q2x, R, 1, false); L(&d.q2y, R, 1, false); L(&d.q2z, R, 1, false);
@@ -497,7 +501,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims, (void*)b->db.hist}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
-    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool, &b->cellPool, &b->tgtPool})
+    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool, &b->cellPool, &b->tgtPool, &b->attPool, &b->pairPool})
         for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
     for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
     for (hipEvent_t e : b->evPool) hipEventDestroy(e);
@@ -520,12 +524,6 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (!b || !desc || !slot) FAILV(VX3H_EINVAL, "null argument");
     if (desc->stop_condition_type < VX3H_SC_NONE || desc->stop_condition_type > VX3H_SC_MIN_MAXMOVE)
         FAILV(VX3H_EINVAL, "unknown stop condition type");
-    if (desc->enable_detach && desc->collisions_enabled)
-        FAILV(VX3H_EUNSUPPORTED, "detach with collisions is not supported (the contact exclusion lists are built once)");
-    if (desc->secondary_experiment && desc->collisions_enabled)
-        FAILV(VX3H_EUNSUPPORTED, "voxel removal with collisions is not supported (the contact exclusion lists are built once)");
-    if (desc->enable_attach && desc->collisions_enabled)
-        FAILV(VX3H_EUNSUPPORTED, "attach with collisions is not supported (the contact exclusion lists are built once)");
     HIPCHK(hipSetDevice(b->device));
     HostModel H;
     if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
@@ -755,6 +753,15 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         c.flags |= SC_TARGETS | SC_EXTRAS;
         b->anyExtras = true;
     }
+    int at0 = 0, ap0 = 0, apN = 0;
+    if (attach) {   // scratch of the attach pass: 2 ints per voxel, and a pair table that grows on demand
+        apN = 2 * NV + 64;
+        vx3h_status st;
+        if ((st = poolAlloc(b, b->attPool, 2 * NV, at0)) || (st = poolAlloc(b, b->pairPool, apN, ap0))) return st;
+        c.at0 = at0;
+        c.ap0 = ap0;
+        c.apCap = apN;
+    }
     if (attach) {
         c.flags |= SC_ATTACH | SC_EXTRAS;
         b->anyAttach = b->anyExtras = true;
@@ -763,7 +770,8 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         for (int k = 0; k < 5; k++) { c.atOff[k] = atOff[k]; c.atLen[k] = (int)H.atTok[k].size(); }
     }
     c.nlMax = NLcap;
-    if ((desc->enable_signals || attach) && NV) {   // neighbour per link direction
+    const bool topoCol = H.collisions && (desc->enable_detach || desc->secondary_experiment || attach);
+    if ((desc->enable_signals || attach || topoCol) && NV) {   // neighbour per link direction
         std::vector<int> nb(NV);
         for (int k = 0; k < 6; k++) {
             for (int i = 0; i < NV; i++) {
@@ -870,6 +878,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         while ((1 << hb) < 2 * nSurf) hb++;
         hN = 1 << hb;
         nearN = std::max(1, (int)nearList.size());
+        if (topoCol) nearN = std::max(nearN + nearN / 2, 64);   // lists change with the topology
         vx3h_status st;
         int hblk0 = 0;
         if ((st = poolAlloc(b, b->nbPool, nbN, nb0)) || (st = poolAlloc(b, b->hashPool, hN / VX3_COL_HBLOCK, hblk0)) ||
@@ -889,6 +898,12 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         c.h0 = h0;
         c.hBits = hb;
         c.c0 = c0;
+        c.near0 = near0;
+        c.nearCap = nearN;
+        if (topoCol) {   // nearby lists regenerated on the device after topology changes
+            c.flags |= SC_TOPOCOL;
+            b->anyTopoCol = true;
+        }
         c.flags |= SC_REBUILD;
         b->anyColl = true;
         b->db.Ccap = b->nbPool.cap;
@@ -909,6 +924,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.shape = shape;
     sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN; sh.c0 = c0;
     sh.tg0 = tg0; sh.nTg = nTg;
+    sh.at0 = at0; sh.ap0 = ap0; sh.apN = apN;
     if (shape >= 0) { b->resLists[shape].push_back(s); b->resDirty = true; }
     else b->nStreamSims++;
     sh.vlinks = H.voxelLinks;
@@ -983,6 +999,7 @@ StepOpts stepOpts(const vx3h_batch* b) {
     o.prestep = b->anyDetach;
     o.signals = b->anySignals;
     o.attach = b->anyAttach;
+    o.topo = b->anyTopoCol;
     o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
     return o;
 }
@@ -1236,26 +1253,95 @@ vx3h_status pollOne(vx3h_batch* b, int s, const SimCtl& c, vx3h_sim_status& o, s
 }  // namespace vx3
 
 namespace {
-// Simulations whose collision partners outgrew their neighbour table stand still
-// (SC_COL_GROW: no pass has changed their state since).  Each gets a table with room
-// for half as many entries again as its last rebuild needed (at least twice the old
-// one) and rebuilds its lists at its next step, which then runs as it would have.
-// The batch's streams are idle.
+// A fresh range of `need` entries for one simulation in a pool (the old range is released).
+vx3h_status regrow(vx3h_batch* b, DevPool& p, int& start, int& len, int need) {
+    p.ra.release(start, len);
+    vx3h_status st = poolAlloc(b, p, need, start);
+    if (st) return st;
+    len = need;
+    return VX3H_OK;
+}
+
+// The nearby lists of one simulation from its current links (the device search did not
+// fit in its queue): generateNearby (Voxelyze.cpp:394-417) on the host, uploaded.
+vx3h_status hostNearby(vx3h_batch* b, int s, SimCtl& c) {
+    SlotHost& sh = b->slots[s];
+    const int NV = sh.nv;
+    const size_t V = (size_t)b->db.Vcap;
+    std::vector<int> fl(NV), nbr(6 * (size_t)NV);
+    HIPCHK(hipMemcpy(fl.data(), b->db.vflags + sh.v0, NV * 4, hipMemcpyDeviceToHost));
+    for (int d = 0; d < 6; d++) HIPCHK(hipMemcpy(nbr.data() + (size_t)d * NV, b->db.nbr + d * V + sh.v0, NV * 4, hipMemcpyDeviceToHost));
+    std::vector<int> start(NV, 0), count(NV, 0), list, mark(NV, -1), q;
+    for (int v = 0; v < NV; v++) {
+        if (fl[v] & VF_INTERIOR) continue;
+        q.assign(1, v);
+        mark[v] = v;
+        size_t cur = 0;
+        for (int k = 0; k < c.colNearHops; k++) {
+            const size_t end = q.size();
+            while (cur < end) {
+                const int u = q[cur++];
+                for (int d = 0; d < 6; d++) {
+                    if (!(fl[u] & (1 << (VF_LINKMASK_SHIFT + d)))) continue;
+                    const int w = nbr[(size_t)d * NV + u] - sh.v0;
+                    if (w < 0 || w >= NV || mark[w] == v) continue;
+                    mark[w] = v;
+                    q.push_back(w);
+                }
+            }
+        }
+        std::vector<int> t;
+        for (size_t i = 1; i < q.size(); i++) if (!(fl[q[i]] & VF_INTERIOR)) t.push_back(q[i]);
+        std::sort(t.begin(), t.end());
+        start[v] = (int)list.size();
+        count[v] = (int)t.size();
+        list.insert(list.end(), t.begin(), t.end());
+    }
+    if ((int)list.size() > sh.nearN) {
+        vx3h_status st = regrow(b, b->nearPool, sh.near0, sh.nearN, (int)list.size() * 3 / 2 + 64);
+        if (st) return st;
+        c.near0 = sh.near0;
+        c.nearCap = sh.nearN;
+    }
+    for (int& x : start) x += sh.near0;
+    if (!list.empty()) HIPCHK(hipMemcpy(b->db.nearList + sh.near0, list.data(), list.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->db.nearStart + sh.v0, start.data(), NV * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->db.nearCount + sh.v0, count.data(), NV * 4, hipMemcpyHostToDevice));
+    return VX3H_OK;
+}
+
+// Simulations whose collision tables proved too small stand still (SC_COL_GROW: no pass
+// has changed their state since).  A neighbour or nearby table gets room for half as
+// many entries again as the rebuild needed (at least twice the old one); nearby lists
+// whose search outgrew the device queue are built here.  The simulation then rebuilds
+// at its next step, which runs as it would have.  The batch's streams are idle.
 vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
     for (int s = 0; s < (int)ctl.size(); s++) {
         if (!b->slots[s].used || !(ctl[s].flags & SC_COL_GROW)) continue;
         SlotHost& sh = b->slots[s];
-        const int need = std::max(2 * sh.nbN, ctl[s].nbNeed + ctl[s].nbNeed / 2);
-        b->nbPool.ra.release(sh.nb0, sh.nbN);
-        int nb0 = 0;
-        vx3h_status st = poolAlloc(b, b->nbPool, need, nb0);
-        if (st) return st;
-        sh.nb0 = nb0;
-        sh.nbN = need;
-        ctl[s].nb0 = nb0;
-        ctl[s].nbCap = need;
-        ctl[s].flags = (ctl[s].flags & ~SC_COL_GROW) | SC_REBUILD;
-        HIPCHK(hipMemcpy(b->db.sims + s, &ctl[s], sizeof(SimCtl), hipMemcpyHostToDevice));
+        SimCtl& c = ctl[s];
+        vx3h_status st;
+        if (c.nbNeed > c.nbCap) {
+            if ((st = regrow(b, b->nbPool, sh.nb0, sh.nbN, std::max(2 * sh.nbN, c.nbNeed + c.nbNeed / 2)))) return st;
+            c.nb0 = sh.nb0;
+            c.nbCap = sh.nbN;
+        }
+        if (c.nearNeed == (1 << 30)) {
+            if ((st = hostNearby(b, s, c))) return st;
+            c.flags &= ~SC_NEARDIRTY;
+        } else if (c.nearNeed > c.nearCap) {   // the device rebuilds the lists again, in the larger table
+            if ((st = regrow(b, b->nearPool, sh.near0, sh.nearN, std::max(2 * sh.nearN, c.nearNeed + c.nearNeed / 2)))) return st;
+            c.near0 = sh.near0;
+            c.nearCap = sh.nearN;
+        }
+        if (c.apNeed > c.apCap) {
+            if ((st = regrow(b, b->pairPool, sh.ap0, sh.apN, std::max(2 * sh.apN, c.apNeed + c.apNeed / 2)))) return st;
+            c.ap0 = sh.ap0;
+            c.apCap = sh.apN;
+        }
+        c.nbNeed = c.nearNeed = c.apNeed = 0;
+        c.flags = (c.flags & ~SC_COL_GROW) | SC_REBUILD;
+        HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(SimCtl), hipMemcpyHostToDevice));
         b->db.Ccap = b->nbPool.cap;
         b->tableGrowths++;
     }
@@ -1269,7 +1355,7 @@ extern "C" vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int 
     HIPCHK(hipStreamSynchronize(b->stream));
     std::vector<SimCtl> ctl(b->slots.size());
     if (!ctl.empty()) HIPCHK(hipMemcpy(ctl.data(), b->db.sims, ctl.size() * sizeof(SimCtl), hipMemcpyDeviceToHost));
-    if (b->anyColl) {
+    if (b->anyColl || b->anyAttach) {
         vx3h_status gs = growCollisionTables(b, ctl);
         if (gs) return gs;
     }
@@ -1309,6 +1395,10 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
         b->db.hashTot = b->hashPool.ra.hi * VX3_COL_HBLOCK;
     }
     if (sh.nTg) b->tgtPool.ra.release(sh.tg0, sh.nTg);
+    if (sh.apN) {
+        b->attPool.ra.release(sh.at0, 2 * sh.nv);
+        b->pairPool.ra.release(sh.ap0, sh.apN);
+    }
     if (sh.shape >= 0) {
         auto& L = b->resLists[sh.shape];
         L.erase(std::remove(L.begin(), L.end(), slot), L.end());

@@ -68,10 +68,15 @@ def test_refusals():
         b.admit(_sticky_bars(3))
     assert e.value.status == _abi.EUNSUPPORTED
     b.close()
-    m = _sticky_bars(3)
-    m.collisions_enabled = 1
-    b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
-    with pytest.raises(engine.VX3Error) as e:
-        b.admit(m)
-    assert e.value.status == _abi.EUNSUPPORTED
-    b.close()
+
+
+def test_attach_with_collisions_like_the_oracle():
+    """Sticky bars with collisions: contacts push the touching voxels while links are
+    made; after each attachment the nearby lists (and so the contact pairs) follow the
+    new links."""
+    models = []
+    for v in (1, 3):
+        m = _sticky_bars(v)
+        m.collisions_enabled = 1
+        models.append(m)
+    check(models, 4000)

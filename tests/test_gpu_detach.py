@@ -60,11 +60,41 @@ def test_ciliated_robots_with_fragile_links():
     b.close()
 
 
-def test_detach_with_collisions_refused():
-    m = workloads.drop(3, steps=0)
-    m.enable_detach = 1
+def test_detach_with_collisions_like_the_oracle():
+    """Detach with collisions: after the centre voxel's links fail, the nearby lists are
+    rebuilt on the device from the links present, and the free centre voxel is watched
+    against its 26 neighbours (the oracle's known answer) -- pair lists equal, positions
+    within 1e-9 m."""
+    m = _shrinking_core(1)
+    m.collisions_enabled = 1
+    b, s, g, o = run_pair(m, 2000)
+    r = o.state()
+    assert np.max(np.abs(g["pos"] - r["pos"])) < 1e-9
+    assert np.array_equal(g["flags"] & 2, r["flags"] & 2)
+    op, gp = o.collision_pairs(), b.collision_pairs(s)
+    assert np.array_equal(op, gp)
+    assert sum(13 in p for p in gp) == 26
+    b.close()
+
+
+def test_fragile_robots_with_collisions_and_cilia():
+    """Dense ciliated robots whose links fail, with collisions: every failure changes the
+    nearby lists the contact pairs exclude."""
+    models = []
+    for i in range(2):
+        m = workloads.robot(np.random.default_rng(31 + i), grid=7, p=0.85, steps=0)
+        for mt in m.materials:
+            mt.mat_model, mt.fail_stress = _abi.MDL_LINEAR_FAIL, 500.0
+        m.enable_detach = 1
+        m.collisions_enabled = 1
+        models.append(ciliated(m, seed=i, field=False))
     b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
-    with pytest.raises(engine.VX3Error) as e:
-        b.admit(m)
-    assert e.value.status == _abi.EUNSUPPORTED
+    slots = [b.admit(m) for m in models]
+    b.step(1500)
+    for m, s in zip(models, slots):
+        o = OSim(m)
+        o.step(1500)
+        g, r = b.read_state(s, nl=o.nl), o.state()
+        assert np.max(np.abs(g["pos"] - r["pos"])) < 1e-9
+        assert np.array_equal(o.collision_pairs(), b.collision_pairs(s))
     b.close()

@@ -112,3 +112,21 @@ def test_resident_stop_condition_and_budget_match_oracle():
     assert b2.poll()[s2].steps == 123
     b.close()
     b2.close()
+
+
+def test_resident_f32_equals_stream_f32_on_robots():
+    # the FP32 kernels are built without contraction too: identical results on either path
+    rs = workloads.robots(4, seed=12, grid=9, steps=0)
+    bs = engine.Batch(precision=_abi.PREC_F32, path=_abi.PATH_STREAM)
+    br = engine.Batch(precision=_abi.PREC_F32, path=_abi.PATH_RESIDENT, steps_per_launch=300)
+    ss = [bs.admit(r) for r in rs]
+    sr = [br.admit(r) for r in rs]
+    bs.step(900)
+    br.step(900)
+    for a, c in zip(ss, sr):
+        ga, gc = bs.read_state(a), br.read_state(c)
+        assert np.array_equal(ga["pos"], gc["pos"])
+        assert np.array_equal(ga["orient"], gc["orient"])
+        assert np.array_equal(ga["flags"], gc["flags"])
+    bs.close()
+    br.close()

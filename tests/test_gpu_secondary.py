@@ -56,11 +56,11 @@ def test_robots_losing_a_material():
     b.close()
 
 
-def test_removal_with_collisions_refused():
-    m = workloads.drop(3, steps=0)
-    m.secondary_experiment = 1
-    b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
-    with pytest.raises(engine.VX3Error) as e:
-        b.admit(m)
-    assert e.value.status == _abi.EUNSUPPORTED
+def test_removal_with_collisions_like_the_oracle():
+    """Voxel removal with collisions: the removed layer's voxels lose their links and the
+    nearby lists are rebuilt from the links left."""
+    m = _layered_cube(0.003)
+    m.collisions_enabled = 1
+    b, slots = check([m], 1500)
+    assert (b.read_state(slots[0])["flags"] & REMOVED).any()
     b.close()

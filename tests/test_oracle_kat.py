@@ -870,3 +870,90 @@ def test_target_closeness_and_close_pairs_closed_form():
     m.stop_tokens = [(MT["mtCONST"], 0.0), (MT["mtVAR"], 7.0), (MT["mtSUB"], 0.0), (MT["mtEND"], 0.0)]
     o2 = OSim(m)
     assert o2.step(100) == 10 and o2.status().state == 1
+
+
+# ---- VX3 topology changes with collisions: the nearby lists follow the current links
+
+def _brute_pairs(pos, flags, voxel_links, voxel_size, hops=5, br=0.75, wd=1.0):
+    """CVoxelyze::regenerateCollisions (Voxelyze.cpp:726-751) by brute force: surface
+    voxel pairs i < j within the watch radius, unless j is within `hops` links of i over
+    the links present (generateNearby on the current topology)."""
+    INTERIOR = 2
+    n = len(pos)
+    adj = [[] for _ in range(n)]
+    for v in range(n):
+        for d in range(6):
+            l = voxel_links[v, d]
+            if l >= 0:
+                adj[v].append(l)
+    # voxel_links holds link indices; map each to its two voxels
+    ends = {}
+    for v in range(n):
+        for d in range(6):
+            l = voxel_links[v, d]
+            if l >= 0:
+                ends.setdefault(l, []).append(v)
+    nb = [[u for l in adj[v] for u in ends[l] if u != v] for v in range(n)]
+    wr = np.float32(voxel_size * np.float32(2 * np.float32(br) + np.float32(wd)))
+    thr = float(np.float32(wr * wr))
+    out = []
+    for i in range(n):
+        if flags[i] & INTERIOR:
+            continue
+        near, frontier = {i}, [i]
+        for _ in range(hops):
+            frontier = [u for v in frontier for u in nb[v] if u not in near and not near.add(u)]
+        for j in range(i + 1, n):
+            if flags[j] & INTERIOR or j in near:
+                continue
+            if float(np.sum((pos[i] - pos[j]) ** 2)) <= thr:
+                out.append((i, j))
+    return np.array(out, np.int32).reshape(-1, 2)
+
+
+def test_detach_with_collisions_rebuilds_nearby_from_current_links():
+    """The shrinking core with collisions: once the centre voxel's links fail it is a
+    free surface voxel, no longer within five links of the shell, so at the next step
+    it is watched against its 26 neighbours; the pair list equals the brute-force
+    rule on the positions of the regeneration and the links present then."""
+    m = _shrinking_core(1)
+    m.collisions_enabled = 1
+    o = OSim(m)
+    fail = 1e4 / 1e6
+    for n in range(1, 3000):
+        o.step(1)
+        if (o.link_state()["max_strain"] > fail).any():
+            break
+    before = o.collision_pairs()
+    assert not any(13 in p for p in before)              # still interior, still linked
+    st = o.state()                                       # positions the next step regenerates from
+    o.step(1)
+    vl = o.topology()[3]
+    got = o.collision_pairs()
+    want = _brute_pairs(st["pos"], o.state()["flags"], vl, m.voxel_size)
+    assert np.array_equal(got, want)
+    assert sum(13 in p for p in got) == 26
+
+
+def test_kinetic_friction_is_discontinuous_at_rest():
+    """CVX_Voxel::floorForce (old/VX_Voxel.cpp:272-297): off the static latch, a voxel in
+    floor contact is slowed by mu_k * N along its normalized horizontal velocity, so the
+    force does not shrink with the velocity.  A voxel at rest gets none; the same voxel
+    started at 1e-15 m/s gets the full kinetic force and within a few steps moves at
+    more than 1e6 times that speed.  Rounding differences of that size are why FP32
+    state (or FMA-contracted FP64) cannot follow these trajectories closely."""
+    m = Model(voxel_size=0.01)
+    m.materials = [Material(elastic_mod=1e6, density=1000.0, u_static=1.0, u_dynamic=0.5, zeta_internal=1.0,
+                            zeta_global=0.01, zeta_collision=0.8)]
+    m.ijk = np.array([[0, 0, 0]], np.int32)
+    m.mat = np.zeros(1, np.int32)
+    m.gravity_mult = float(np.float32(1.0))
+    m.floor_enabled = 1
+    a = OSim(m)
+    a.step(10)
+    assert a.state()["lin_mom"][0, 0] == 0.0
+    m.init_velocity = np.array([[1e-15, 0.0, 0.0]])
+    b = OSim(m)
+    b.step(10)
+    mass = 1000.0 * 0.01 ** 3
+    assert abs(b.state()["lin_mom"][0, 0]) / mass > 1e6 * 1e-15
