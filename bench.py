@@ -246,30 +246,34 @@ def serving_legs(precision, seconds, burst, seed=4):
 
 
 def config_leg(model, precision, steps, interval=0, kernel_steps=50, warmup=10):
-    """One configuration as a single simulation: admitted, `warmup` steps, then `steps`
-    timed steps bracketed by stream synchronization; per-pass kernel durations from HIP
-    events over `kernel_steps` further steps, with the link and voxel passes priced at
-    their algorithmic bytes (FP64: 196 B/link, 232 B/voxel; FP32: 124 B each)."""
+    """One configuration (a model, or a list of models batched together): admitted,
+    `warmup` steps, then `steps` timed steps bracketed by stream synchronization;
+    per-pass kernel durations from HIP events over `kernel_steps` further steps, with
+    the link and voxel passes priced at their algorithmic bytes (FP64: 196 B/link,
+    232 B/voxel; FP32: 124 B each)."""
     from gpuvoxels_amd import _abi, engine
     f64 = precision == _abi.PREC_F64
+    models = model if isinstance(model, list) else [model]
     b = engine.Batch(precision=precision, path=_abi.PATH_AUTO, collision_rebuild_interval=interval)
     t0 = time.perf_counter()
-    s = b.admit(model)
+    slots = [b.admit(m) for m in models]
+    s = slots[0]
     admit_s = time.perf_counter() - t0
-    V = model.n_voxels
-    L = int(len(engine.topology(model)[0]))
+    V = sum(m.n_voxels for m in models)
+    L = int(sum(len(engine.topology(m)[0]) for m in models))
     b.step(warmup)
     b.sync()
     t0 = time.perf_counter()
     b.step(steps)
     b.sync()
     dt = time.perf_counter() - t0
-    st = b.poll()[s]
+    sts = b.poll()
+    st = sts[s]
     kt = b.kernel_times(kernel_steps) if kernel_steps else {}
     b.close()
     out = {"value": V * steps / dt, "unit": "voxel-steps/s", "voxels": V, "links": L, "steps": steps,
-           "ms_per_step": dt / steps * 1e3, "dtype": "f64" if f64 else "f32", "admit_s": admit_s,
-           "state": int(st.state), "steps_done": int(st.steps), "kernel_ms": kt,
+           "sims": len(models), "ms_per_step": dt / steps * 1e3, "dtype": "f64" if f64 else "f32", "admit_s": admit_s,
+           "state": int(st.state), "steps_done": int(min(sts[x].steps for x in slots)), "kernel_ms": kt,
            "alg_frac_per_step": bytes_per_step(V, L, f64) / (dt / steps) / 1e9 / HBM_PEAK_GBS}
     if "links" in kt:
         out["links_hbm_frac"] = (196 if f64 else 124) * L / (kt["links"] * 1e-3) / 1e9 / HBM_PEAK_GBS
@@ -277,6 +281,33 @@ def config_leg(model, precision, steps, interval=0, kernel_steps=50, warmup=10):
     if "voxels" in kt:
         out["voxels_hbm_frac"] = (232 if f64 else 124) * V / (kt["voxels"] * 1e-3) / 1e9 / HBM_PEAK_GBS
     return out
+
+
+def config_legs(args):
+    """The configuration legs named in args.leg_names, each a batch of its own."""
+    from gpuvoxels_amd import _abi, workloads
+    legs = {}
+    names = set(args.leg_names.split(","))
+    if "C0" in names:
+        m0 = workloads.cube(6, steps=0)
+        legs["C0_single_cube_f64"] = config_leg(m0, _abi.PREC_F64, args.leg_steps or 10000, kernel_steps=0)
+    if "C2" in names:
+        m2 = workloads.slab(64, 64, 32, steps=0)
+        legs["C2_slab_f64"] = config_leg(m2, _abi.PREC_F64, args.leg_steps or 5000)
+    if "C1c" in names:
+        rc = workloads.robots(args.robots, seed=args.seed, steps=0)
+        for r in rc:   # the Voxelyze3 default: self-collision on
+            r.collisions_enabled = 1
+        legs["C1_robots_with_collisions_f64"] = config_leg(rc, _abi.PREC_F64, args.leg_steps or 2000, kernel_steps=100)
+    if "C3" in names or "C3f32" in names:
+        m3 = workloads.arena_balls(steps=0)
+        if "C3" in names:
+            legs["C3_arena_balls_f64"] = config_leg(m3, _abi.PREC_F64, args.leg_steps or 3000, interval=8)
+        if "C3f32" in names:
+            legs["C3_arena_balls_f32"] = config_leg(m3, _abi.PREC_F32, args.leg_steps or 3000, interval=8)
+    for k, v in legs.items():
+        log("leg %s: %.3e voxel-steps/s, %.4f ms/step, kernels %s" % (k, v["value"], v["ms_per_step"], v["kernel_ms"]))
+    return legs
 
 
 def precision_horizon(models, horizons, cpu_robots, threads, kind):
@@ -391,6 +422,14 @@ def main():
                          "engine, FP64 under tiny initial-velocity perturbations) on the robot batch")
     ap.add_argument("--no-legs", dest="legs", action="store_false",
                     help="skip the single-cube, slab and arena-balls configuration legs")
+    ap.add_argument("--legs", dest="leg_names", default="C0,C2,C1c,C3,C3f32",
+                    help="comma-separated configuration legs to run: C0 (single cube), C2 (slab), C1c (the robots "
+                         "with self-collision), C3 (arena balls, FP64), C3f32 (arena balls, FP32)")
+    ap.add_argument("--legs-only", action="store_true",
+                    help="run only the configuration legs named by --legs (one JSON object of legs, no headline): "
+                         "one batch per leg, for profiling a leg in a process of its own")
+    ap.add_argument("--leg-steps", type=int, default=0,
+                    help="timed steps of every configuration leg (0 = each leg's own: 10000, 5000, 2000, 3000)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (two launches: warm-up + timed), a kernel of "
                          "known HBM traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -413,6 +452,10 @@ def main():
         ms = engine.calib_copy(nb, reps=1)
         copy_gbs = 2 * nb / (ms * 1e-3) / 1e9
         log("plain copy of %d MiB: %.3f ms, %.0f GB/s read+write" % (args.calib_copy_mib, ms, copy_gbs))
+    if args.legs_only:
+        legs = config_legs(args)
+        print(json.dumps({"config_legs": legs}))
+        return
     models = workloads.robots(args.robots, seed=args.seed, steps=0)
     if args.precision_horizon:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -514,17 +557,7 @@ def main():
                              "dtype": args.precision, "batched_speedup": value / pv}
         log("plain path %.3e voxel-steps/s (%.2f s) -> batched speedup %.1fx" % (pv, pdt, value / pv))
     if args.legs:
-        legs = {}
-        m0 = workloads.cube(6, steps=0)
-        legs["C0_single_cube_f64"] = config_leg(m0, _abi.PREC_F64, 10000, kernel_steps=0)
-        m2 = workloads.slab(64, 64, 32, steps=0)
-        legs["C2_slab_f64"] = config_leg(m2, _abi.PREC_F64, 5000)
-        m3 = workloads.arena_balls(steps=0)
-        legs["C3_arena_balls_f64"] = config_leg(m3, _abi.PREC_F64, 3000, interval=8)
-        legs["C3_arena_balls_f32"] = config_leg(m3, _abi.PREC_F32, 3000, interval=8)
-        for k, v in legs.items():
-            log("leg %s: %.3e voxel-steps/s, %.4f ms/step, kernels %s" % (k, v["value"], v["ms_per_step"], v["kernel_ms"]))
-        out["config_legs"] = legs
+        out["config_legs"] = config_legs(args)
     if args.serving_seconds > 0:
         out["serving"] = serving_legs(prec, args.serving_seconds, args.serving_burst)
     if not args.no_cpu:

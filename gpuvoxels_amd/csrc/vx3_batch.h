@@ -19,7 +19,8 @@ template <typename T> struct DevBatch {
     int* vext;                // index into extTab or -1
     int* vijk;                // 3*Vcap lattice index (externals / reporting)
     float *psx, *psy, *psz;   // Poisson strain of the voxel (CVX_Voxel::poissonsStrain)
-    T* F;                     // 36*Vcap link end loads: F[(d*6+c)*Vcap + v]
+    T* F;                     // link end loads of the step: F[(d*6+c)*Fs + v], 36 per voxel
+    size_t Fs;                // stride of F: Vcap for the batch's own buffer
     double* cil;              // 8*Vcap cilia: base force xyz, shift force xyz (voxel frame), Cilia scale,
                               // switch-on time; cil[k*Vcap + v]
     T* xf;                    // 6*Vcap this step's global cilia force xyz and force-field force xyz
@@ -74,7 +75,8 @@ template <typename T> struct DevBatch {
     int hashTot;              // buckets in use (high-water mark, whole blocks)
     int vHi, lHi, nSims;      // high-water marks: kernels cover [0, vHi) / [0, lHi) / [0, nSims)
     int solo;                 // >= 0: only this simulation takes the step (on the per-step kernels,
-                              // whichever path it was admitted to); -1: every stream-path simulation
+                              // whichever path it was admitted to); -1: every stream-path simulation;
+                              // -2: every simulation, resident ones included
 };
 
 }  // namespace vx3

@@ -26,6 +26,7 @@ __device__ __forceinline__ bool simRunning(const DevBatch<real>& b, int s) {
     if (s < 0 || b.sims[s].state != 0) return false;
     const int f = b.sims[s].flags;
     if (b.solo >= 0) return s == b.solo && !(f & SC_COL_GROW);
+    if (b.solo == -2) return !(f & SC_COL_GROW);   // every simulation, resident ones included
     return !(f & (SC_RESIDENT | SC_COL_GROW));
 }
 
@@ -162,11 +163,11 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     // slot of the negative end: the link lies in direction 2*axis (X_POS/Y_POS/Z_POS) from it
     const int dn = 2 * axis, dp = 2 * axis + 1;
     real* F = b.F;
-    const size_t V = (size_t)b.Vcap;
-    F[(dn * 6 + 0) * V + vn] = fN.x; F[(dn * 6 + 1) * V + vn] = fN.y; F[(dn * 6 + 2) * V + vn] = fN.z;
-    F[(dn * 6 + 3) * V + vn] = mN.x; F[(dn * 6 + 4) * V + vn] = mN.y; F[(dn * 6 + 5) * V + vn] = mN.z;
-    F[(dp * 6 + 0) * V + vp] = fP.x; F[(dp * 6 + 1) * V + vp] = fP.y; F[(dp * 6 + 2) * V + vp] = fP.z;
-    F[(dp * 6 + 3) * V + vp] = mP.x; F[(dp * 6 + 4) * V + vp] = mP.y; F[(dp * 6 + 5) * V + vp] = mP.z;
+    const size_t V = (size_t)b.Vcap, FS = b.Fs;
+    F[(dn * 6 + 0) * FS + vn] = fN.x; F[(dn * 6 + 1) * FS + vn] = fN.y; F[(dn * 6 + 2) * FS + vn] = fN.z;
+    F[(dn * 6 + 3) * FS + vn] = mN.x; F[(dn * 6 + 4) * FS + vn] = mN.y; F[(dn * 6 + 5) * FS + vn] = mN.z;
+    F[(dp * 6 + 0) * FS + vp] = fP.x; F[(dp * 6 + 1) * FS + vp] = fP.y; F[(dp * 6 + 2) * FS + vp] = fP.z;
+    F[(dp * 6 + 3) * FS + vp] = mP.x; F[(dp * 6 + 4) * FS + vp] = mP.y; F[(dp * 6 + 5) * FS + vp] = mP.z;
     if (sc.flags & SC_POISSON) {
         // CVX_Link::axialStrain(positiveEnd) (VX_Link.cpp:115-118)
         const float r = mat.strainRatio;
@@ -241,6 +242,35 @@ __device__ __forceinline__ bool colWatched(const DevBatch<real>& b, const SimCtl
     return true;
 }
 
+// Hands each active lane a range of n entries from its simulation's counter (*counter of
+// sim s), one atomic per simulation present in the wave; returns the range's start (an
+// offset in the simulation's table).  Ranges are contiguous per lane; their order across
+// voxels is arbitrary (only each voxel's own list order matters).
+template <typename CounterOf>
+__device__ __forceinline__ int waveAlloc(bool active, int s, int n, CounterOf counter) {
+    const int lane = threadIdx.x & 63;
+    int start = 0;
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        const int lead = __ffsll((unsigned long long)pending) - 1;
+        const int sl = __shfl(s, lead, 64);
+        const bool mine = active && s == sl;
+        const int x = mine ? n : 0;
+        int incl = x;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const int tot = __shfl(incl, 63, 64);
+        int base = 0;
+        if (lane == lead) base = atomicAdd(counter(sl), tot);
+        base = __shfl(base, lead, 64);
+        if (mine) start = base + incl - x;
+        pending &= ~__ballot(mine);
+    }
+    return start;
+}
+
 // ---- nearby lists after a topology change (CVX_Voxel::generateNearby, Voxelyze.cpp:394-417,
 // on the links present now).  A step that attaches, detaches or removes voxels marks the
 // simulation (SC_TOPO_CHG); the next step regenerates the nearby list of every surface
@@ -284,55 +314,26 @@ __device__ int nearBfs(const DevBatch<real>& b, int v, int hops, int* q) {
 __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_count(DevBatch<real> b) {
     __shared__ int q[VX3_NEAR_BS][VX3_NEAR_MAX];
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= b.vHi) return;
-    const int s = b.vsim[v];
-    if (!nearDirty(b, s)) return;
-    if (b.vflags[v] & VF_INTERIOR) { b.nearCount[v] = 0; return; }
-    int* Q = q[threadIdx.x];
-    const int n = nearBfs(b, v, b.sims[s].colNearHops, Q);
-    if (n < 0) {
-        b.nearCount[v] = 0;
+    const int s = v < b.vHi ? b.vsim[v] : -1;
+    const bool act = v < b.vHi && nearDirty(b, s) && !(b.vflags[v] & VF_INTERIOR);
+    int c = 0;
+    bool over = false;
+    if (act) {
+        int* Q = q[threadIdx.x];
+        const int n = nearBfs(b, v, b.sims[s].colNearHops, Q);
+        if (n < 0) over = true;
+        else for (int i = 1; i < n; i++) c += !(b.vflags[Q[i]] & VF_INTERIOR);
+    }
+    const int start = waveAlloc(act, s, c, [&](int sl) { return &b.sims[sl].nearUsed; });
+    if (v >= b.vHi || !nearDirty(b, s)) return;
+    b.nearCount[v] = act ? c : 0;
+    b.nearStart[v] = b.sims[s].near0 + start;
+    if (over) {
         atomicMax(&b.sims[s].nearNeed, 1 << 30);   // the host builds this simulation's lists itself
         atomicOr(&b.sims[s].flags, SC_COL_GROW);
-        return;
-    }
-    int c = 0;
-    for (int i = 1; i < n; i++) c += !(b.vflags[Q[i]] & VF_INTERIOR);
-    b.nearCount[v] = c;
-}
-
-__global__ void __launch_bounds__(1024) k_near_scan(DevBatch<real> b) {
-    __shared__ int part[1024 / 64];
-    __shared__ int carry;
-    const int s = blockIdx.x;
-    if (s >= b.nSims || !nearDirty(b, s) || (b.sims[s].flags & SC_COL_GROW)) return;
-    SimCtl& sc = b.sims[s];
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int base = 0; base < sc.nv; base += 1024) {
-        const int v = sc.v0 + base + threadIdx.x;
-        const bool in = base + (int)threadIdx.x < sc.nv;
-        const int x = in && !(b.vflags[v] & VF_INTERIOR) ? b.nearCount[v] : 0;
-        int incl = x;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        if (lane == 63) part[wid] = incl;
-        __syncthreads();
-        int before = carry;
-        for (int w = 0; w < wid; w++) before += part[w];
-        if (in) b.nearStart[v] = sc.near0 + before + incl - x;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = before + incl;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        if (carry > sc.nearCap) {
-            sc.nearNeed = carry;
-            atomicOr(&sc.flags, SC_COL_GROW);
-        } else atomicOr(&sc.flags, SC_REBUILD);   // new nearby lists: the collision lists are stale
+    } else if (start + c > b.sims[s].nearCap) {
+        atomicMax(&b.sims[s].nearNeed, start + c);
+        atomicOr(&b.sims[s].flags, SC_COL_GROW);
     }
 }
 
@@ -367,7 +368,11 @@ __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
     if (!colVoxel(b, v, s)) return;
     SimCtl& sc = b.sims[s];
     if (sc.flags & SC_REBUILD) return;
-    if (sc.colInterval > 0 && sc.steps % sc.colInterval == 0) { atomicOr(&sc.flags, SC_REBUILD); return; }
+    // new nearby lists (CVoxelyze: nearbyStale -> collisionsStale), or the fixed period
+    if ((sc.flags & SC_NEARDIRTY) || (sc.colInterval > 0 && sc.steps % sc.colInterval == 0)) {
+        atomicOr(&sc.flags, SC_REBUILD);
+        return;
+    }
     const real dx = b.px[v] - (real)b.lastWatch[3 * v], dy = b.py[v] - (real)b.lastWatch[3 * v + 1],
                dz = b.pz[v] - (real)b.lastWatch[3 * v + 2];
     if (dx * dx + dy * dy + dz * dz > (real)sc.colRecalcSq) atomicOr(&sc.flags, SC_REBUILD);
@@ -437,7 +442,11 @@ __global__ void __launch_bounds__(256) k_col_boff(DevBatch<real> b) {
     __shared__ int carry;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int B0 = sc.h0 / VX3_COL_HBLOCK, nB = (1 << sc.hBits) / VX3_COL_HBLOCK;
-    if (tid == 0) carry = sc.c0;
+    if (tid == 0) {
+        carry = sc.c0;
+        b.sims[s].nbUsed = 0;   // the partner ranges of this rebuild are handed out from 0
+        b.sims[s].rebuilds++;
+    }
     __syncthreads();
     for (int base = 0; base < nB; base += 256) {
         const int i = base + tid;
@@ -496,52 +505,22 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
     }
 }
 
+// counts the partners of every surface voxel being rebuilt and hands it a range of the
+// neighbour table; a table too small stops the simulation in place until the host grows it
 __global__ void __launch_bounds__(256) k_col_count(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= b.vHi) return;
-    int s;
-    if (!colVoxel(b, v, s)) return;
-    const SimCtl& sc = b.sims[s];
-    if (!(sc.flags & SC_REBUILD)) return;
+    int s = -1;
+    const bool act = v < b.vHi && colVoxel(b, v, s) && (b.sims[s].flags & SC_REBUILD);
     int n = 0;
-    colForPartners(b, sc, v, [&](int) { n++; });
-    b.nbCount[v] = n;
-}
-
-// per rebuilt simulation: exclusive scan of the partner counts over its voxels; a total
-// beyond the neighbour table stops the simulation in place until the host grows it
-__global__ void __launch_bounds__(1024) k_col_scan(DevBatch<real> b) {
-    __shared__ int part[1024 / 64];
-    __shared__ int carry;
-    const int s = blockIdx.x;
-    if (s >= b.nSims || !colRebuilding(b, s)) return;
+    if (act) colForPartners(b, b.sims[s], v, [&](int) { n++; });
+    const int start = waveAlloc(act, s, n, [&](int sl) { return &b.sims[sl].nbUsed; });
+    if (!act) return;
     SimCtl& sc = b.sims[s];
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int base = 0; base < sc.nv; base += 1024) {
-        const int v = sc.v0 + base + threadIdx.x;
-        const bool in = base + (int)threadIdx.x < sc.nv;
-        const int x = in && !(b.vflags[v] & VF_INTERIOR) ? b.nbCount[v] : 0;
-        int incl = x;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
-        if (lane == 63) part[wid] = incl;
-        __syncthreads();
-        int before = carry;
-        for (int w = 0; w < wid; w++) before += part[w];
-        if (in) b.nbStart[v] = sc.nb0 + before + incl - x;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry = before + incl;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        if (carry > sc.nbCap) {
-            sc.nbNeed = carry;
-            atomicOr(&sc.flags, SC_COL_GROW);
-        } else sc.rebuilds++;
+    b.nbCount[v] = n;
+    b.nbStart[v] = sc.nb0 + start;
+    if (start + n > sc.nbCap) {
+        atomicMax(&sc.nbNeed, start + n);
+        atomicOr(&sc.flags, SC_COL_GROW);
     }
 }
 
@@ -576,9 +555,35 @@ __global__ void __launch_bounds__(256) k_col_emit(DevBatch<real> b) {
     }
 }
 
+// Contact force of the pair (i, j), i the lower index, on voxel i: CVX_Collision::
+// updateContactForce (VX_Collision.cpp:41-56) from both voxels' pre-step position,
+// momentum and thermal factor.  Shared by the per-step and resident kernels.
+__device__ __forceinline__ void pairContact(const VoxMatDev& m1, const VoxMatDev& m2, real pix, real piy, real piz, real pjx,
+                                            real pjy, real pjz, real mix, real miy, real miz, real mjx, real mjy, real mjz,
+                                            float tfi, float tfj, real nom, float& fx, float& fy, float& fz) {
+    const float kp = 2.0f / (1.0f / m1.penetrationStiffness + 1.0f / m2.penetrationStiffness);
+    const float cd = 0.5f * (m1.zetaCollision * m1.c2xSqMxExS + m2.zetaCollision * m2.c2xSqMxExS);
+    const float ox = (float)(pjx - pix), oy = (float)(pjy - piy), oz = (float)(pjz - piz);
+    const real b1 = nom * (real)tfi, b2 = nom * (real)tfj;
+    const float nomDist = (float)(((b1 + b1 + b1) / (real)3.0f + (b2 + b2 + b2) / (real)3.0f) * (real)0.625f);
+    const float len = sqrtf(ox * ox + oy * oy + oz * oz);
+    const float rel = nomDist - len;
+    fx = fy = fz = 0;
+    if (rel > 0) {
+        float ux = ox, uy = oy, uz = oz;
+        if (len > 0) { const float inv = 1.0f / len; ux = inv * ox; uy = inv * oy; uz = inv * oz; }
+        const real mi1 = (real)m1.massInverse, mi2 = (real)m2.massInverse;
+        const real v1 = (mix * mi1) * (real)ux + (miy * mi1) * (real)uy + (miz * mi1) * (real)uz;
+        const real v2 = (mjx * mi2) * (real)ux + (mjy * mi2) * (real)uy + (mjz * mi2) * (real)uz;
+        const float relVel = (float)(v1 - v2);
+        const float mag = kp * rel + cd * relVel;
+        fx = ux * mag; fy = uy * mag; fz = uz * mag;
+    }
+}
+
 // contact force on voxel a from its pair with c, from the pair's first voxel
 // min(a, c): CVX_Collision::updateContactForce (VX_Collision.cpp:41-56) and
-// contactForce(pVoxel) (VX_Collision.h:44)
+// contactForce(pVoxel) (VX_Collision.h:44), subtracted in the voxel's partner order
 struct Contacts {
     const DevBatch<real>* b;
     int v, start, count;
@@ -587,27 +592,11 @@ struct Contacts {
         for (int k = 0; k < count; k++) {
             const int u = b->nbList[start + k];
             const int i = v < u ? v : u, j = v < u ? u : v;
-            const VoxMatDev& m1 = b->vmatTab[b->vmat[i]];
-            const VoxMatDev& m2 = b->vmatTab[b->vmat[j]];
-            const float kp = 2.0f / (1.0f / m1.penetrationStiffness + 1.0f / m2.penetrationStiffness);
-            const float cd = 0.5f * (m1.zetaCollision * m1.c2xSqMxExS + m2.zetaCollision * m2.c2xSqMxExS);
+            float fx, fy, fz;
             // both voxels' state from the contact snapshot (pre-step values)
-            const float ox = (float)(b->cpx[j] - b->cpx[i]), oy = (float)(b->cpy[j] - b->cpy[i]), oz = (float)(b->cpz[j] - b->cpz[i]);
-            const real b1 = nom * (real)b->ctf[i], b2 = nom * (real)b->ctf[j];
-            const float nomDist = (float)(((b1 + b1 + b1) / (real)3.0f + (b2 + b2 + b2) / (real)3.0f) * (real)0.625f);
-            const float len = sqrtf(ox * ox + oy * oy + oz * oz);
-            const float rel = nomDist - len;
-            float fx = 0, fy = 0, fz = 0;
-            if (rel > 0) {
-                float ux = ox, uy = oy, uz = oz;
-                if (len > 0) { const float inv = 1.0f / len; ux = inv * ox; uy = inv * oy; uz = inv * oz; }
-                const real mi1 = (real)m1.massInverse, mi2 = (real)m2.massInverse;
-                const real v1 = (b->cmx[i] * mi1) * (real)ux + (b->cmy[i] * mi1) * (real)uy + (b->cmz[i] * mi1) * (real)uz;
-                const real v2 = (b->cmx[j] * mi2) * (real)ux + (b->cmy[j] * mi2) * (real)uy + (b->cmz[j] * mi2) * (real)uz;
-                const float relVel = (float)(v1 - v2);
-                const float mag = kp * rel + cd * relVel;
-                fx = ux * mag; fy = uy * mag; fz = uz * mag;
-            }
+            pairContact(b->vmatTab[b->vmat[i]], b->vmatTab[b->vmat[j]], b->cpx[i], b->cpy[i], b->cpz[i], b->cpx[j], b->cpy[j],
+                        b->cpz[j], b->cmx[i], b->cmy[i], b->cmz[i], b->cmx[j], b->cmy[j], b->cmz[j], b->ctf[i], b->ctf[j], nom,
+                        fx, fy, fz);
             if (v != i) { fx = -fx; fy = -fy; fz = -fz; }
             force.x -= (T)fx; force.y -= (T)fy; force.z -= (T)fz;
         }
@@ -690,11 +679,12 @@ template <bool EXTRAS> __device__ __forceinline__ void voxelPass(const DevBatch<
         V3<real> f(0, 0, 0), mo(0, 0, 0);
         const size_t V = (size_t)b.Vcap;
         const real* F = b.F;
+        const size_t FS = b.Fs;
 #pragma unroll
         for (int d = 0; d < 6; d++) {
             if (fl & (1 << (VF_LINKMASK_SHIFT + d))) {
-                f.add(V3<real>(F[(d * 6 + 0) * V + v], F[(d * 6 + 1) * V + v], F[(d * 6 + 2) * V + v]));
-                mo.add(V3<real>(F[(d * 6 + 3) * V + v], F[(d * 6 + 4) * V + v], F[(d * 6 + 5) * V + v]));
+                f.add(V3<real>(F[(d * 6 + 0) * FS + v], F[(d * 6 + 1) * FS + v], F[(d * 6 + 2) * FS + v]));
+                mo.add(V3<real>(F[(d * 6 + 3) * FS + v], F[(d * 6 + 4) * FS + v], F[(d * 6 + 5) * FS + v]));
             }
         }
         const int e = b.vext[v];
@@ -1027,8 +1017,8 @@ __global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
         b.lsim[l] = s;
         const int dn = 2 * axis, dp = 2 * axis + 1;
         for (int q = 0; q < 6; q++) {   // no force from the new link until its first update
-            b.F[(dn * 6 + q) * V + vneg] = 0;
-            b.F[(dp * 6 + q) * V + vpos] = 0;
+            b.F[(dn * 6 + q) * b.Fs + vneg] = 0;
+            b.F[(dp * 6 + q) * b.Fs + vpos] = 0;
         }
         b.es[dn * V + vneg] = 0;
         b.es[dp * V + vpos] = 0;
@@ -1163,16 +1153,16 @@ __device__ void voxcadStats(const DevBatch<real>& b, SimCtl& sc, int s) {
         v2 = fmax(v2, vx * vx + vy * vy + vz * vz);
     }
     if (strainE) {
-        const size_t V = (size_t)b.Vcap;
         for (int l = sc.l0 + threadIdx.x; l < sc.l0 + sc.nl; l += blockDim.x) {
             // CVX_Link::strainEnergy (old/VX_Link.cpp:243-249) from the loads the link pass
             // left in the slots of its two voxels
             const int meta = b.lmeta[l], axis = meta & 3;
             const LinkMatDev& mat = b.lmatTab[meta >> 2];
             const int vn = b.lneg[l], vp = b.lpos[l], dn = 2 * axis, dp = 2 * axis + 1;
-            const double fNx = b.F[(dn * 6 + 0) * V + vn];
-            const double mNx = b.F[(dn * 6 + 3) * V + vn], mNy = b.F[(dn * 6 + 4) * V + vn], mNz = b.F[(dn * 6 + 5) * V + vn];
-            const double mPy = b.F[(dp * 6 + 4) * V + vp], mPz = b.F[(dp * 6 + 5) * V + vp];
+            const size_t FS = b.Fs;
+            const double fNx = b.F[(dn * 6 + 0) * FS + vn];
+            const double mNx = b.F[(dn * 6 + 3) * FS + vn], mNy = b.F[(dn * 6 + 4) * FS + vn], mNz = b.F[(dn * 6 + 5) * FS + vn];
+            const double mPy = b.F[(dp * 6 + 4) * FS + vp], mPz = b.F[(dp * 6 + 5) * FS + vp];
             se += (float)(fNx * fNx / (2.0f * mat.a1) + mNx * mNx / (2.0 * mat.a2) +
                           (mNz * mNz - mNz * mPz + mPz * mPz) / (3.0 * mat.b3) + (mNy * mNy - mNy * mPy + mPy * mPy) / (3.0 * mat.b3));
         }
@@ -1340,7 +1330,10 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     sc.flags &= ~SC_REBUILD;
     if (sc.flags & SC_TOPOCOL) {   // a change made during this step: nearby lists regenerated at the next
         sc.flags &= ~SC_NEARDIRTY;
-        if (sc.flags & SC_TOPO_CHG) sc.flags = (sc.flags & ~SC_TOPO_CHG) | SC_NEARDIRTY;
+        if (sc.flags & SC_TOPO_CHG) {
+            sc.flags = (sc.flags & ~SC_TOPO_CHG) | SC_NEARDIRTY;
+            sc.nearUsed = 0;
+        }
     }
     if (sc.flags & SC_COL_OVERFLOW) {
         sc.state = 4;

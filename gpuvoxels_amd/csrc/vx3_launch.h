@@ -24,18 +24,19 @@ void launchStep(const DevBatch<double>& b, hipStream_t s, const StepOpts& o, hip
 // A sim fits a shape when V <= block*vpt, each axis has <= block*lpa links, and it uses at
 // most RES_MAX_VMATS voxel and RES_MAX_LMATS link material variants and RES_MAX_PHASES
 // distinct actuation phase offsets.
-struct ResidentShapeInfo { int block, vpt, lpa; size_t lds; };
+struct ResidentShapeInfo { int block, vpt, lpa; size_t lds, ldsCol; };   // ldsCol: the variant with collisions
 enum { RES_MAX_VMATS = 8, RES_MAX_LMATS = 16, RES_MAX_PHASES = 64 };
 int residentShapes_f32(ResidentShapeInfo* out, int cap);
 int residentShapes_f64(ResidentShapeInfo* out, int cap);
 inline int residentShapes(bool f64, ResidentShapeInfo* out, int cap) {
     return f64 ? residentShapes_f64(out, cap) : residentShapes_f32(out, cap);
 }
-// Launches the resident kernel of shape `shape` over the sims in simList (device array).
-hipError_t launchResident(const DevBatch<float>& b, hipStream_t s, int shape, const int* simList, int nSims, int nSteps,
-                          hipEvent_t* ev = nullptr);
-hipError_t launchResident(const DevBatch<double>& b, hipStream_t s, int shape, const int* simList, int nSims, int nSteps,
-                          hipEvent_t* ev = nullptr);
+// Launches the resident kernel of shape `shape` (with collisions or without) over the sims
+// in simList (device array).
+hipError_t launchResident(const DevBatch<float>& b, hipStream_t s, int shape, bool collisions, const int* simList, int nSims,
+                          int nSteps, hipEvent_t* ev = nullptr);
+hipError_t launchResident(const DevBatch<double>& b, hipStream_t s, int shape, bool collisions, const int* simList, int nSims,
+                          int nSteps, hipEvent_t* ev = nullptr);
 void launchDt(const DevBatch<float>& b, hipStream_t s, int s0, int s1);
 void launchDt(const DevBatch<double>& b, hipStream_t s, int s0, int s1);
 }  // namespace vx3

@@ -18,6 +18,20 @@
 // Global memory is read once at launch start and written once at the end in the
 // stream path's layout, so the two paths can hand simulations to each other.
 //
+// Collisions (COL): a simulation with self-collision keeps its contact partner lists
+// in LDS (sNb, ascending per voxel, the stream layout's order) across the launch.  At a
+// step that the staleness rule (a surface voxel moved more than recalcDist since the
+// last rebuild; checked after each voxel update) or the fixed period marks, the lists
+// are rebuilt before the link pass: surface voxels are counting-sorted into LDS hash
+// buckets of their cell (edge = watch radius), and every surface voxel tests the
+// voxels of its 27 neighbouring cells against the CPU engine's pair rule (watch
+// radius, link-hop exclusion), as the per-step kernels do.  After the link passes the
+// load buffer holds a snapshot of positions and momenta at the step's start, from which
+// the voxel pass computes each contact (the thermal factor of the next step is written
+// only after every voxel has used this step's).  Lists that do not fit the LDS table
+// stop the simulation in place (SC_COL_GROW); the host then moves it to the per-step
+// kernels.
+//
 // Included by the per-precision kernel units after vx3_kernels.inl (VX_NS, real).
 
 namespace vx3 {
@@ -29,7 +43,9 @@ namespace VX_NS {
 #define VX3_RES_LMATS 16     // link material variants a resident sim may use
 #endif
 
-template <int BS, int VPT, int LPA>
+#define VX3_RES_PAIRS 8192   // contact partner entries (both directions) a resident sim may hold
+
+template <int BS, int VPT, int LPA, bool COL = false>
 struct ResidentShape {
     static constexpr int VMAX = BS * VPT;   // voxels per sim
     static constexpr int LAX = BS * LPA;    // links per axis per sim
@@ -38,18 +54,90 @@ struct ResidentShape {
                                   + (size_t)VMAX * 7 * sizeof(real)      // pose
                                   + (size_t)LAX * 12 * sizeof(real)      // loads of one axis
                                   + (size_t)VMAX * 4 * sizeof(float)     // thermal factor, Poisson strain
-                                  + (size_t)LAX * 6 * sizeof(float);     // end strains (axis, end, slot)
+                                  + (size_t)LAX * 6 * sizeof(float)      // end strains (axis, end, slot)
+                                  + (COL ? (size_t)VX3_RES_PAIRS * 2 : 0);  // contact partners
     // setup tables alias the load buffer: voxel -> slot per direction, and (axis, slot) -> link
     static_assert((size_t)VMAX * 6 * 2 + (size_t)LAX * 3 * 2 <= (size_t)LAX * 12 * sizeof(real), "setup alias");
+    // with collisions the load buffer also holds the rebuild's buckets (2 ints per bucket, one
+    // bucket per voxel) and sorted voxels, and later the step-start snapshot (6 reals per voxel)
+    static_assert(!COL || (size_t)VMAX * 10 <= (size_t)LAX * 12 * sizeof(real), "rebuild alias");
+    static_assert(!COL || (size_t)VMAX * 6 <= (size_t)LAX * 12, "snapshot alias");
+    static_assert((VMAX & (VMAX - 1)) == 0, "bucket count is a power of two");
 };
+
+// block-wide exclusive scan of one int per thread; *total receives the sum.  part holds
+// BS/64 ints.  Ends with the block synchronized.
+template <int BS>
+__device__ __forceinline__ int resBlockScan(int x, int* part, int* total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int incl = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) part[w] = incl;
+    __syncthreads();
+    int before = incl - x, t = 0;
+    for (int i = 0; i < BS / 64; i++) {
+        if (i < w) before += part[i];
+        t += part[i];
+    }
+    __syncthreads();
+    *total = t;
+    return before;
+}
+
+// contact force of a resident simulation's voxel from its partners (sim-local indices in
+// nb[start, start + count), ascending), from the step-start snapshot snap = [6][VMAX]
+// (position, momentum) and this step's thermal factors tf
+struct ResContacts {
+    const DevBatch<real>* b;
+    const real* snap;
+    const float* tf;
+    const unsigned short* nb;
+    int vmax, v, v0, start, count;
+    real nom;
+    template <typename T> __device__ __forceinline__ void subtract(V3<T>& force) const {
+        for (int k = 0; k < count; k++) {
+            const int u = nb[start + k];
+            const int i = v < u ? v : u, j = v < u ? u : v;
+            float fx, fy, fz;
+            pairContact(b->vmatTab[b->vmat[v0 + i]], b->vmatTab[b->vmat[v0 + j]], snap[i], snap[vmax + i], snap[2 * vmax + i],
+                        snap[j], snap[vmax + j], snap[2 * vmax + j], snap[3 * vmax + i], snap[4 * vmax + i], snap[5 * vmax + i],
+                        snap[3 * vmax + j], snap[4 * vmax + j], snap[5 * vmax + j], tf[i], tf[j], nom, fx, fy, fz);
+            if (v != i) { fx = -fx; fy = -fy; fz = -fz; }
+            force.x -= (T)fx; force.y -= (T)fy; force.z -= (T)fz;
+        }
+    }
+};
+
+// the CPU pair rule between surface voxels v and u (sim-local) of a resident simulation,
+// positions from the LDS pose cache (colWatched of the per-step kernels)
+__device__ __forceinline__ bool resWatched(const DevBatch<real>& b, const SimCtl& sc, const real* sPx, const real* sPy,
+                                           const real* sPz, int v, int u) {
+    const real dx = sPx[v] - sPx[u], dy = sPy[v] - sPy[u], dz = sPz[v] - sPz[u];
+    if (dx * dx + dy * dy + dz * dz > (real)sc.colThreshSq) return false;
+    const int gv = sc.v0 + v, gu = sc.v0 + u;
+    const int li = abs(b.vijk[3 * gv] - b.vijk[3 * gu]) + abs(b.vijk[3 * gv + 1] - b.vijk[3 * gu + 1]) +
+                   abs(b.vijk[3 * gv + 2] - b.vijk[3 * gu + 2]);
+    if (li > sc.colNearHops) return true;
+    int lo = b.nearStart[gv], hi = lo + b.nearCount[gv];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int x = b.nearList[mid];
+        if (x == u) return false;
+        if (x < u) lo = mid + 1; else hi = mid;
+    }
+    return true;
+}
 
 __device__ __forceinline__ void ldsCopyWords(int* dst, const int* src, int words, int tid, int bs) {
     for (int i = tid; i < words; i += bs) dst[i] = src[i];
 }
 
-template <int BS, int VPT, int LPA>
+template <int BS, int VPT, int LPA, bool COL>
 __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* simList, int nSteps) {
-    typedef ResidentShape<BS, VPT, LPA> Shape;
+    typedef ResidentShape<BS, VPT, LPA, COL> Shape;
     constexpr int VMAX = Shape::VMAX, LAX = Shape::LAX;
     extern __shared__ __align__(16) unsigned char smem[];
     VoxMatDev* sVM = (VoxMatDev*)smem;
@@ -61,6 +149,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     float* sTf = (float*)(sF + 12 * LAX);
     float* sPsx = sTf + VMAX; float* sPsy = sPsx + VMAX; float* sPsz = sPsy + VMAX;
     float* sEs = sPsz + VMAX;                   // [(axis*2 + end)][LAX]
+    unsigned short* sNb = (unsigned short*)(sEs + 6 * LAX);   // contact partners (COL)
     unsigned short* sVL = (unsigned short*)sF;  // [6][VMAX] slot of the link in direction d (setup only)
     unsigned short* sSL = sVL + 6 * VMAX;       // [3][LAX]  link of (axis, slot) (setup only)
     __shared__ int sRun, sDiv, sCnt[3], sExp[3], sExpAt[3];
@@ -71,11 +160,13 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     __shared__ float sPhT[VX3_RES_PHASES];      // their temperatures for the step being run
     __shared__ int sNph;                        // table size; -1: more than VX3_RES_PHASES (admission prevents it)
     __shared__ float sDm[VX3_RES_VMATS];        // CVX_Voxel::dampingMultiplier per voxel material (dt is fixed)
+    __shared__ int sStale[2], sColOvf;          // COL: a rebuild is due (per step parity); the pair table overflowed
+    __shared__ int sScan[BS / 64];
 
     const int s = simList[blockIdx.x];
     SimCtl& scg = b.sims[s];
     const int tid = threadIdx.x;
-    if (scg.state != 0) return;
+    if (scg.state != 0 || (scg.flags & SC_COL_GROW)) return;   // finished, or waiting to move to the per-step kernels
     const int v0 = scg.v0, nv = scg.nv, l0 = scg.l0, nl = scg.nl;
     const double dt = scg.dt;
     const float dtf = scg.dt;
@@ -299,6 +390,39 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             }
         }
     }
+    // ---- contact lists (COL): the stream layout's lists into LDS, last-rebuild positions into
+    // registers, and the staleness rule for the launch's first step
+    const bool colFirst = COL && (flags0 & SC_REBUILD) != 0;
+    float lw[VPT][3];
+    int nbS[VPT], nbC[VPT];
+    int rebuilds = 0;
+    bool colOvf = false;
+    if (COL) {
+        if (tid < 2) sStale[tid] = 0;
+        if (tid == 0) sColOvf = 0;
+        __syncthreads();
+        const int nb0 = scg.nb0;
+        const real recalcSq = (real)scg.colRecalcSq;
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            nbS[k] = nbC[k] = 0;
+            lw[k][0] = lw[k][1] = lw[k][2] = 0;
+            const int v = tid + k * BS;
+            if (vmat[k] < 0 || (vf[k] & VF_INTERIOR)) continue;
+            const int g = v0 + v;
+            lw[k][0] = b.lastWatch[3 * g]; lw[k][1] = b.lastWatch[3 * g + 1]; lw[k][2] = b.lastWatch[3 * g + 2];
+            if (colFirst) continue;
+            const int st0 = b.nbStart[g] - nb0, c = b.nbCount[g];
+            if (c > 0) {
+                if (st0 < 0 || st0 + c > VX3_RES_PAIRS) { sColOvf = 1; continue; }
+                for (int i = 0; i < c; i++) sNb[st0 + i] = (unsigned short)(b.nbList[nb0 + st0 + i] - v0);
+                nbS[k] = st0;
+                nbC[k] = c;
+            }
+            const real dx = sPx[v] - (real)lw[k][0], dy = sPy[v] - (real)lw[k][1], dz = sPz[v] - (real)lw[k][2];
+            if (dx * dx + dy * dy + dz * dz > recalcSq) sStale[0] = 1;
+        }
+    }
     double t = scg.t;
     int steps = scg.steps;
     int angleSamples = scg.angleSamples;
@@ -310,7 +434,90 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     }
     __syncthreads();
 
-    for (int it = 0; it < nSteps; it++) {
+    if (COL && sColOvf) colOvf = true;   // lists larger than the LDS table: no step here
+    for (int it = 0; it < nSteps && !colOvf; it++) {
+        // ---- contact lists rebuilt when stale or at the fixed period (COL)
+        if (COL && ((it == 0 && colFirst) || sStale[it & 1] != 0 || (scg.colInterval > 0 && steps % scg.colInterval == 0))) {
+            int* cnt = (int*)sF;                        // [VMAX] voxels per bucket
+            int* bst = cnt + VMAX;                      // [VMAX] first sorted voxel of each bucket
+            unsigned short* cvox = (unsigned short*)(bst + VMAX);   // [VMAX] surface voxels by bucket
+            constexpr unsigned mask = VMAX - 1;
+            const double cellE = (double)scg.colCell;
+            for (int i = tid; i < VMAX; i += BS) cnt[i] = 0;
+            __syncthreads();
+            int bk[VPT], rk[VPT];
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                bk[k] = -1;
+                rk[k] = 0;
+                const int v = tid + k * BS;
+                if (vmat[k] < 0 || (vf[k] & VF_INTERIOR)) continue;
+                bk[k] = (int)(colHash((int)floor((double)sPx[v] / cellE), (int)floor((double)sPy[v] / cellE),
+                                      (int)floor((double)sPz[v] / cellE)) & mask);
+                rk[k] = atomicAdd(&cnt[bk[k]], 1);
+                lw[k][0] = (float)sPx[v]; lw[k][1] = (float)sPy[v]; lw[k][2] = (float)sPz[v];   // lastColWatchPosition
+            }
+            __syncthreads();
+            int loc = 0;   // thread tid owns buckets [tid*VPT, tid*VPT + VPT)
+#pragma unroll
+            for (int k = 0; k < VPT; k++) loc += cnt[tid * VPT + k];
+            int tot;
+            int off = resBlockScan<BS>(loc, sScan, &tot);
+#pragma unroll
+            for (int k = 0; k < VPT; k++) { bst[tid * VPT + k] = off; off += cnt[tid * VPT + k]; }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < VPT; k++)
+                if (bk[k] >= 0) cvox[bst[bk[k]] + rk[k]] = (unsigned short)(tid + k * BS);
+            __syncthreads();
+            // every watched partner of surface voxel v, from the voxels of its 27 neighbouring cells
+            auto forPartners = [&](int v, auto f) {
+                const int cx = (int)floor((double)sPx[v] / cellE), cy = (int)floor((double)sPy[v] / cellE),
+                          cz = (int)floor((double)sPz[v] / cellE);
+                int hs[27];
+#pragma unroll
+                for (int q = 0; q < 27; q++) hs[q] = (int)(colHash(cx + q % 3 - 1, cy + (q / 3) % 3 - 1, cz + q / 9 - 1) & mask);
+                for (int q = 0; q < 27; q++) {
+                    const int h = hs[q];
+                    bool seen = false;   // a bucket shared by several of the 27 cells is read once
+                    for (int j = 0; j < q; j++) seen |= hs[j] == h;
+                    if (seen) continue;
+                    const int n = cnt[h], b0 = bst[h];
+                    for (int i = 0; i < n; i++) {
+                        const int u = cvox[b0 + i];
+                        if (u != v && resWatched(b, scg, sPx, sPy, sPz, v, u)) f(u);
+                    }
+                }
+            };
+            int pc[VPT], mine = 0;
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                pc[k] = 0;
+                if (bk[k] >= 0) forPartners(tid + k * BS, [&](int) { pc[k]++; });
+                mine += pc[k];
+            }
+            int total;
+            int o = resBlockScan<BS>(mine, sScan, &total);
+            if (total > VX3_RES_PAIRS || total > scg.nbCap) { colOvf = true; break; }   // uniform: the block's total
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                nbS[k] = o;
+                nbC[k] = pc[k];
+                o += pc[k];
+                if (!pc[k]) continue;
+                unsigned short* out = sNb + nbS[k];
+                int n = 0;
+                forPartners(tid + k * BS, [&](int u) { out[n++] = (unsigned short)u; });
+                for (int i = 1; i < n; i++) {   // ascending partner order
+                    const unsigned short x = out[i];
+                    int j = i - 1;
+                    while (j >= 0 && out[j] > x) { out[j + 1] = out[j]; j--; }
+                    out[j + 1] = x;
+                }
+            }
+            __syncthreads();
+            rebuilds++;
+        }
         // keep per-step LDS address arithmetic inside the loop (register pressure)
 #pragma unroll
         for (int a = 0; a < 3; a++)
@@ -375,6 +582,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                 if (ls[a][k].strain > 100) sDiv = 1;
             }
             __syncthreads();
+            if (COL && a == 0 && tid == 0) sStale[it & 1] = 0;   // every thread has read it
             // ---- the voxels add this axis' two directions: 2a (negative end) then 2a+1 (positive end)
 #pragma unroll
             for (int k = 0; k < VPT; k++) {
@@ -390,11 +598,25 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             }
             if (a < 2) __syncthreads();   // the next axis overwrites the load buffer
         }
+        if (COL) {   // the load buffer becomes the step-start snapshot the contacts read
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < VPT; k++) {
+                if (vmat[k] < 0) continue;
+                const int v = tid + k * BS;
+                sF[v] = sPx[v]; sF[VMAX + v] = sPy[v]; sF[2 * VMAX + v] = sPz[v];
+                sF[3 * VMAX + v] = vm[k].x; sF[4 * VMAX + v] = vm[k].y; sF[5 * VMAX + v] = vm[k].z;
+            }
+            __syncthreads();
+        }
         const bool diverged = sDiv != 0;   // written before the last link-pass barrier
         steps++;
         const bool sample = !diverged && cycleStep > 0 && steps % cycleStep == 0;
         double psx = 0, psy = 0, psz = 0, psm = 0;
         const double tn = t + dt;
+        float newTf[VPT];
+#pragma unroll
+        for (int k = 0; k < VPT; k++) newTf[k] = -1.0f;
         // ---- voxel pass
         if (!diverged) {
 #pragma unroll
@@ -408,12 +630,23 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     int ix = 0, iy = 0, iz = 0;
                     if (ext) { ix = ext->ix; iy = ext->iy; iz = ext->iz; }
                     Q4<real> vo(sOw[v], sOx[v], sOy[v], sOz[v]);
-                    voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], NoContacts(), vp, vm[k], vo, va[k], vf[k]);
+                    if (COL && nbC[k] > 0) {
+                        const ResContacts con{&b, sF, sTf, sNb, VMAX, v, v0, nbS[k], nbC[k], nom};
+                        voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], con, vp, vm[k], vo, va[k], vf[k]);
+                    } else {
+                        voxelStep<real>(m, ext, dtf, nom, ix, iy, iz, sTf[v], fs[k], ms[k], NoContacts(), vp, vm[k], vo, va[k], vf[k]);
+                    }
                     sPx[v] = vp.x; sPy[v] = vp.y; sPz[v] = vp.z;
                     sOw[v] = vo.w; sOx[v] = vo.x; sOy[v] = vo.y; sOz[v] = vo.z;
                 }
+                if (COL && !(vf[k] & VF_INTERIOR)) {   // CVoxelyze::updateCollisions' staleness rule, for the next step
+                    const real dx = vp.x - (real)lw[k][0], dy = vp.y - (real)lw[k][1], dz = vp.z - (real)lw[k][2];
+                    if (dx * dx + dy * dy + dz * dz > (real)scg.colRecalcSq) sStale[(it + 1) & 1] = 1;
+                }
                 if (vary && !m.fixed && !(m.thermalOnAfter > tn)) {
-                    sTf[v] = 1 + sPhT[vpi[k]] * m.alphaCTE;
+                    // with contacts, every voxel reads this step's factors first (written below)
+                    if (COL) newTf[k] = 1 + sPhT[vpi[k]] * m.alphaCTE;
+                    else sTf[v] = 1 + sPhT[vpi[k]] * m.alphaCTE;
                 }
                 if (poisson) {
                     // CVX_Voxel::strain(true) for the next step from this step's end strains:
@@ -445,6 +678,12 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     psm += mass;
                 }
             }
+        }
+        if (COL && !diverged) {
+            __syncthreads();   // every contact has read this step's thermal factors
+#pragma unroll
+            for (int k = 0; k < VPT; k++)
+                if (newTf[k] >= 0.0f) sTf[tid + k * BS] = newTf[k];
         }
         if (sample) {
             // FP64 center of mass of the measured voxels (VX3_VoxelyzeKernel.cu:477-493)
@@ -518,7 +757,26 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         b.tfac[g] = sTf[v];
         b.psx[g] = sPsx[v]; b.psy[g] = sPsy[v]; b.psz[g] = sPsz[v];
     }
+    if (COL && !colOvf) {   // contact lists and last-rebuild positions in the stream layout
+        const int nb0 = scg.nb0;
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            if (vmat[k] < 0 || (vf[k] & VF_INTERIOR)) continue;
+            const int g = v0 + tid + k * BS;
+            b.lastWatch[3 * g] = lw[k][0]; b.lastWatch[3 * g + 1] = lw[k][1]; b.lastWatch[3 * g + 2] = lw[k][2];
+            b.nbStart[g] = nb0 + nbS[k];
+            b.nbCount[g] = nbC[k];
+            for (int i = 0; i < nbC[k]; i++) b.nbList[nb0 + nbS[k] + i] = v0 + sNb[nbS[k] + i];
+        }
+    }
     if (tid == 0) {
+        if (COL) {
+            int f = scg.flags;
+            if (rebuilds > 0 && !colOvf) f &= ~SC_REBUILD;
+            if (colOvf) f |= SC_COL_GROW | SC_REBUILD;   // the host moves it to the per-step kernels
+            scg.flags = f;
+            scg.rebuilds += rebuilds;
+        }
         if (steps != scg.steps) scg.prevDt = dtf;
         scg.t = t;
         scg.steps = steps;

@@ -9,7 +9,6 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
     if (topo) {   // nearby lists of simulations whose topology changed in the last step
         const int gn = (b.vHi + VX3_NEAR_BS - 1) / VX3_NEAR_BS;
         hipLaunchKernelGGL(VX_NS::k_near_count, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
-        hipLaunchKernelGGL(VX_NS::k_near_scan, dim3(b.nSims), dim3(1024), 0, s, b);
         hipLaunchKernelGGL(VX_NS::k_near_emit, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
     }
     hipLaunchKernelGGL(VX_NS::k_col_stale, dim3(gv), dim3(bs), 0, s, b);
@@ -19,7 +18,6 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
     hipLaunchKernelGGL(VX_NS::k_col_boff, dim3(b.nSims), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_place, dim3(gv), dim3(bs), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_count, dim3(gv), dim3(bs), 0, s, b);
-    hipLaunchKernelGGL(VX_NS::k_col_scan, dim3(b.nSims), dim3(1024), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_emit, dim3(gv), dim3(bs), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_snap, dim3(gv), dim3(bs), 0, s, b);
 }

@@ -119,6 +119,7 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int colNearHops;                 // link-hop radius of the excluded "nearby" voxels (int(2*watchRadiusVx))
     int nb0, nbCap;                  // neighbour-list range in nbList
     int nbNeed;                      // entries the last rebuild needed when it set SC_COL_GROW
+    int nbUsed, nearUsed;            // entries handed out so far by the current rebuild of each table
     int h0, hBits;                   // hash-table range: (1 << hBits) buckets from bucket h0 (whole blocks)
     int c0;                          // cell-sorted surface voxels: range [c0, c0 + nv) of cellVox
     int near0, nearCap, nearNeed;    // nearby-list range in nearList, and the size a rebuild needed

@@ -274,6 +274,7 @@ vx3h_status growArrays(vx3h_batch* b, std::vector<ArrSpec>& arrs, int& cap, int 
     if (ncap > used) HIPCHK(hipMemsetAsync(owner + used, 0xFF, (size_t)(ncap - used) * 4, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     cap = ncap;
+    if (isVoxel) b->db.Fs = (size_t)ncap;
     b->layout++;
     return VX3H_OK;
 }
@@ -422,9 +423,12 @@ vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
             evp = pair;
         }
         hipStream_t rs = b->resStreams[i];
+        const int ns = (int)b->shapes.size();
+        const int shape = (int)i % ns;
+        const bool col = (int)i >= ns;
         hipError_t e = b->realSize == 4
-            ? launchResident(b->db, rs, (int)i, b->dResLists[i], n, nSteps, evp)
-            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), rs, (int)i, b->dResLists[i], n, nSteps, evp);
+            ? launchResident(b->db, rs, shape, col, b->dResLists[i], n, nSteps, evp)
+            : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), rs, shape, col, b->dResLists[i], n, nSteps, evp);
         if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("resident launch failed: ") + hipGetErrorString(e));
     }
     return VX3H_OK;
@@ -462,12 +466,13 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
         ResidentShapeInfo tmp[16];
         int ns = residentShapes(b->realSize == 8, tmp, 16);
         b->shapes.assign(tmp, tmp + ns);
-        b->resLists.assign(ns, {});
-        b->dResLists.assign(ns, nullptr);
-        b->capRes.assign(ns, 0);
-        b->resStreams.assign(ns, nullptr);
-        b->joinEv.assign(ns, nullptr);
-        b->joinPending.assign(ns, false);
+        // list i < ns: shape i without collisions; ns + i: shape i with collisions
+        b->resLists.assign(2 * ns, {});
+        b->dResLists.assign(2 * ns, nullptr);
+        b->capRes.assign(2 * ns, 0);
+        b->resStreams.assign(2 * ns, nullptr);
+        b->joinEv.assign(2 * ns, nullptr);
+        b->joinPending.assign(2 * ns, false);
         // default preference: smallest voxel capacity first (more workgroups per CU), table order on ties
         for (int i = 0; i < ns; i++) b->shapePref.push_back(i);
         std::stable_sort(b->shapePref.begin(), b->shapePref.end(), [&](int x, int y) {
@@ -693,10 +698,13 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     // VoxCAD stop conditions and equilibrium mode are evaluated by the stream path's control pass
     const bool voxcad = desc->stop_condition_type != VX3H_SC_NONE || desc->equilibrium_mode != 0;
     const bool closePairs = desc->max_dist_close_pairs > 0;
+    bool targets = false;   // target contacts (hit counts) are counted by the per-step kernels
+    if (H.collisions)
+        for (int i = 0; i < desc->n_materials; i++) targets |= desc->materials[i].is_target != 0;
     const bool resOk = (int)H.vmats.size() <= RES_MAX_VMATS && (int)H.lmats.size() <= RES_MAX_LMATS &&
                        nPhases <= RES_MAX_PHASES && !voxcad && !H.extras() && !desc->enable_detach &&
-                       !desc->secondary_experiment && !desc->enable_signals && !attach && !closePairs;
-    if (b->cfg.path != VX3H_PATH_STREAM && !H.collisions && NV > 0 && resOk) {
+                       !desc->secondary_experiment && !desc->enable_signals && !attach && !closePairs && !targets;
+    if (b->cfg.path != VX3H_PATH_STREAM && NV > 0 && resOk) {
         for (int si : b->shapePref) {
             const ResidentShapeInfo& r = b->shapes[si];
             const int lax = r.block * r.lpa;
@@ -705,7 +713,6 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         if (shape < 0 && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "simulation too large for the resident kernel");
     }
     if (shape < 0 && NV > 0 && b->cfg.path == VX3H_PATH_RESIDENT) {   // a forced resident path takes only what fits
-        if (H.collisions) FAILV(VX3H_EUNSUPPORTED, "collisions run on the stream path");
         if (!resOk && (int)H.vmats.size() > RES_MAX_VMATS) FAILV(VX3H_EUNSUPPORTED, "too many voxel materials for the resident kernel");
         if (!resOk && (int)H.lmats.size() > RES_MAX_LMATS) FAILV(VX3H_EUNSUPPORTED, "too many link materials for the resident kernel");
         if (!resOk && nPhases > RES_MAX_PHASES) FAILV(VX3H_EUNSUPPORTED, "too many distinct phase offsets for the resident kernel");
@@ -713,9 +720,10 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (voxcad && b->cfg.path == VX3H_PATH_RESIDENT)
         FAILV(VX3H_EUNSUPPORTED, "VoxCAD stop conditions and equilibrium mode run on the stream path");
     if (H.extras() && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "cilia and force fields run on the stream path");
-    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals || attach || closePairs) &&
+    if ((desc->enable_detach || desc->secondary_experiment || desc->enable_signals || attach || closePairs || targets) &&
         b->cfg.path == VX3H_PATH_RESIDENT)
-        FAILV(VX3H_EUNSUPPORTED, "detach, attach, secondary experiments, signals and close pairs run on the stream path");
+        FAILV(VX3H_EUNSUPPORTED,
+              "detach, attach, secondary experiments, signals, target contacts and close pairs run on the stream path");
     // control block
     SimCtl c;
     memset(&c, 0, sizeof(c));
@@ -746,9 +754,6 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (H.extras()) { c.flags |= SC_EXTRAS; b->anyExtras = true; }
     if (!H.cilia.empty()) c.flags |= SC_CILIA;
     if (desc->enable_detach) { c.flags |= SC_DETACH; b->anyDetach = true; }
-    bool targets = false;
-    if (H.collisions)
-        for (int i = 0; i < desc->n_materials; i++) targets |= desc->materials[i].is_target != 0;
     if (targets) {
         c.flags |= SC_TARGETS | SC_EXTRAS;
         b->anyExtras = true;
@@ -925,7 +930,11 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN; sh.c0 = c0;
     sh.tg0 = tg0; sh.nTg = nTg;
     sh.at0 = at0; sh.ap0 = ap0; sh.apN = apN;
-    if (shape >= 0) { b->resLists[shape].push_back(s); b->resDirty = true; }
+    if (shape >= 0) {   // the kernel variant with contacts for a simulation with collisions
+        sh.shape = shape + (H.collisions ? (int)b->shapes.size() : 0);
+        b->resLists[sh.shape].push_back(s);
+        b->resDirty = true;
+    }
     else b->nStreamSims++;
     sh.vlinks = H.voxelLinks;
     sh.surface.resize(NV);
@@ -1321,6 +1330,15 @@ vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
         SlotHost& sh = b->slots[s];
         SimCtl& c = ctl[s];
         vx3h_status st;
+        if (sh.shape >= 0) {   // contact lists beyond the resident kernel's LDS table: the per-step kernels take it
+            auto& L = b->resLists[sh.shape];
+            L.erase(std::remove(L.begin(), L.end(), s), L.end());
+            b->resDirty = true;
+            sh.shape = -1;
+            b->nStreamSims++;
+            c.flags &= ~SC_RESIDENT;
+            b->layout++;
+        }
         if (c.nbNeed > c.nbCap) {
             if ((st = regrow(b, b->nbPool, sh.nb0, sh.nbN, std::max(2 * sh.nbN, c.nbNeed + c.nbNeed / 2)))) return st;
             c.nb0 = sh.nb0;
@@ -1340,6 +1358,7 @@ vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
             c.apCap = sh.apN;
         }
         c.nbNeed = c.nearNeed = c.apNeed = 0;
+        c.nearUsed = 0;
         c.flags = (c.flags & ~SC_COL_GROW) | SC_REBUILD;
         HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(SimCtl), hipMemcpyHostToDevice));
         b->db.Ccap = b->nbPool.cap;

@@ -172,7 +172,9 @@ typedef struct vx3h_sim_desc {
     /* VX3 detach (Simulator.AttachDetach.EnableDetach; gpu_update_detach,
      * VX3_VoxelyzeKernel.cu:946-967): a link whose maximum strain passes its material's
      * failure strain leaves both voxels after the link update of that step, before the
-     * voxel update.  Not with collisions. */
+     * voxel update.  With collisions, the nearby lists (voxels within the link-hop radius,
+     * never paired) follow the links present at the next step, as after every topology
+     * change (attach, detach, voxel removal). */
     int    enable_detach;
     /* VX3 secondary experiment (Simulator.SecondaryExperiment; VX3_VoxelyzeKernel.cu:
      * 336-350, 365-397): at the end of each step, every material whose remove_after is
@@ -180,7 +182,7 @@ typedef struct vx3h_sim_desc {
      * place, still counted in the center of mass) and their links; once the start time
      * passes reinit_after (ReinitializeInitialPositionAfterThisManySeconds), the
      * initial center of mass becomes the last sampled one and the initial positions
-     * the current ones.  Not with collisions. */
+     * the current ones. */
     int    secondary_experiment;
     double reinit_after;
     /* VX3 signals (Simulator.EnableSignals; VX3_Voxel.cu:271-348): after its update each
@@ -208,7 +210,7 @@ typedef struct vx3h_sim_desc {
      * taken in ascending (higher, lower) voxel order, each seeing the links made before
      * it.  Attach conditions: up to five MathTrees of the voxel position, hit, t and
      * angle, evaluated after each voxel update; all must be > 0 (0 tokens = no
-     * condition).  Not with collisions. */
+     * condition). */
     int    enable_attach;
     int    n_attach_tokens[5];
     const vx3h_token* attach_tokens[5];

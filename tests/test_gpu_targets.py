@@ -59,18 +59,20 @@ def test_close_pairs_and_closeness_match_oracle():
 
 def test_close_pairs_of_moving_robots_match_oracle():
     """Robots whose actuated material is a target: close pairs and closeness of the
-    voxels' positions at the first center-of-mass sample (one temperature cycle)."""
+    voxels' positions at the center-of-mass samples, taken once per temperature period
+    (0.1 s here, about 11,000 steps of these robots: two samples in 24,000 steps)."""
     from gpuvoxels_amd import workloads
     rs = workloads.robots(3, seed=21, grid=7, steps=0)
     for r in rs:
         r.materials[2].is_target = 1
         r.max_dist_close_pairs = 1.5
+        r.temp_period = 0.1
     os_ = [OSim(r) for r in rs]
     b = engine.Batch(device=0, precision=_abi.PREC_F64, path=_abi.PATH_AUTO)
     slots = [b.admit(r) for r in rs]
     for o in os_:
-        o.step(36000)
-    b.step(36000)
+        o.step(24000)
+    b.step(24000)
     st = b.poll()
     for o, s in zip(os_, slots):
         so, g = o.status(), st[s]
