@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1013,6 +1014,71 @@ StepOpts stepOpts(const vx3h_batch* b) {
     return o;
 }
 }  // namespace
+
+// ---- the entry point that allocates nothing (vx3h_batch_step_noalloc)
+//
+// The per-step kernels' one scratch buffer, the link end loads of a step (36 values per
+// voxel in the state precision), is a file-scope device array.  Every GPU the library
+// runs on holds its own copy until the process ends; host code uses the address
+// hipGetSymbolAddress returns for the current device, never the array's name.
+static __device__ __align__(256) unsigned char scratch[VX3H_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratchMu;
+void* g_scratchPtr[128] = {};
+
+// The device address of `scratch` on `device` (current), looked up once per device.
+vx3h_status noallocScratch(int device, void** out) {
+    if (device < 0 || device >= 128) FAILV(VX3H_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) FAILV(VX3H_EDEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratchPtr[device] = p;
+    }
+    *out = g_scratchPtr[device];
+    return VX3H_OK;
+}
+}  // namespace
+
+extern "C" vx3h_status vx3h_batch_step_noalloc(vx3h_batch* b, int n_steps) {
+    if (!b || n_steps < 0) FAILV(VX3H_EINVAL, "bad argument");
+    HIPCHK(hipSetDevice(b->device));
+    void* scr = nullptr;
+    vx3h_status st = noallocScratch(b->device, &scr);
+    if (st) return st;
+    const size_t need = (size_t)36 * (size_t)b->db.vHi * b->realSize;
+    if (need > (size_t)VX3H_NOALLOC_SCRATCH_BYTES)
+        FAILV(VX3H_EINVAL, "batch too large for the no-allocation step: its end loads need " + std::to_string(need) +
+                               " bytes of the " + std::to_string((size_t)VX3H_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
+    if (b->slots.empty() || n_steps == 0 || b->db.vHi == 0) return VX3H_OK;
+    if (b->dtHi > b->dtLo) {   // recommended step of newly admitted simulations (a kernel; nothing allocated)
+        if (b->realSize == 4) launchDt(b->db, b->stream, b->dtLo, b->dtHi);
+        else launchDt(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, b->dtLo, b->dtHi);
+        HIPCHK(hipGetLastError());
+        b->dtLo = 1 << 30;
+        b->dtHi = -1;
+    }
+    const StepOpts o = stepOpts(b);
+    for (int i = 0; i < n_steps; i++) {
+        if (b->realSize == 4) {
+            DevBatch<float> v = b->db;
+            v.solo = -2;
+            v.F = (float*)scr;
+            v.Fs = (size_t)v.vHi;
+            launchStep(v, b->stream, o);
+        } else {
+            DevBatch<double> v = *reinterpret_cast<DevBatch<double>*>(&b->db);
+            v.solo = -2;
+            v.F = (double*)scr;
+            v.Fs = (size_t)v.vHi;
+            launchStep(v, b->stream, o);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return VX3H_OK;
+}
 
 // VX3_VoxelyzeKernel::doTimeStep(float dt) of one simulation (VX3_VoxelyzeKernel.cu:237-359).
 extern "C" vx3h_status vx3h_sim_step(vx3h_batch* b, int slot, float dt, int* ok) {

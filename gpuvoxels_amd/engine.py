@@ -40,6 +40,7 @@ def lib():
         "vx3h_sim_footprint": (S, [C.POINTER(_abi.SimDesc), C.c_int, C.POINTER(C.c_size_t)]),
         "vx3h_batch_step": (S, [P, C.c_int]),
         "vx3h_sim_step": (S, [P, C.c_int, C.c_float, C.POINTER(C.c_int)]),
+        "vx3h_batch_step_noalloc": (S, [P, C.c_int]),
         "vx3h_batch_sync": (S, [P]),
         "vx3h_batch_poll": (S, [P, C.POINTER(_abi.SimStatus), C.c_int, C.POINTER(C.c_int)]),
         "vx3h_sim_retire": (S, [P, C.c_int]),
@@ -251,6 +252,11 @@ class Batch:
 
     def step(self, n):
         check(lib().vx3h_batch_step(self.h, int(n)))
+
+    def step_noalloc(self, n):
+        """vx3h_batch_step_noalloc: n steps of every simulation on the per-step kernels with
+        no device allocation (the step's scratch is a static device array)."""
+        check(lib().vx3h_batch_step_noalloc(self.h, int(n)))
 
     def do_time_step(self, slot, dt=-1.0):
         """VX3_VoxelyzeKernel::doTimeStep(dt) of one simulation (vx3h_sim_step): False when

@@ -336,6 +336,19 @@ vx3h_status vx3h_batch_sync(vx3h_batch* b);
  * was admitted to, and regardless of its stop condition: as in the reference, the
  * caller's loop tests the stop condition (vx3h_batch_poll: state VX3H_SIM_DONE). */
 vx3h_status vx3h_sim_step(vx3h_batch* b, int slot, float dt, int* ok);
+/* No counterpart in the reference: vx3h_batch_step for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Every running simulation of the batch, resident ones included,
+ * takes n_steps steps on the per-step kernels (the same results as vx3h_batch_step),
+ * asynchronously on the batch's stream; no device memory, graph, event or stream is
+ * created.  The kernels' one scratch buffer, the link end loads of a step (36 values per
+ * voxel in the state precision), is a static device array of VX3H_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends: a batch whose voxels (up to the highest one
+ * admitted) need more is refused with VX3H_EINVAL -- 466,033 voxels in FP64, 932,067 in
+ * FP32.  Every call on one GPU uses that GPU's array and leaves its contents behind:
+ * calls that overlap on one GPU (other batches, streams or threads) race. */
+#define VX3H_NOALLOC_SCRATCH_BYTES (128u << 20)
+vx3h_status vx3h_batch_step_noalloc(vx3h_batch* b, int n_steps);
 /* Status of every admitted sim (synchronizes). */
 vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int cap, int* n);
 vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot);
