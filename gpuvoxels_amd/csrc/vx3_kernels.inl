@@ -57,10 +57,11 @@ __device__ __forceinline__ float actuationTemp(double amp, double period, double
     }
 }
 
-// CVX_Voxel::transverseArea / transverseStrainSum (VX_Voxel.cpp:345-373) from the cached Poisson strain
-__device__ __forceinline__ void transverse(const DevBatch<real>& b, int v, int axis, float size, bool nuZero, float& area, float& sum) {
+// CVX_Voxel::transverseArea / transverseStrainSum (VX_Voxel.cpp:345-373) from the cached
+// Poisson strain ps = (x, y, z) of the voxel
+__device__ __forceinline__ void transverseOf(const float* ps, int axis, float size, bool nuZero, float& area, float& sum) {
     if (nuZero) { area = size * size; sum = 0.0f; return; }
-    float sx = b.psx[v], sy = b.psy[v], sz = b.psz[v];
+    float sx = ps[0], sy = ps[1], sz = ps[2];
     real s2 = (real)(size * size);
     if (axis == 0) { area = (float)(s2 * ((real)1 + (real)sy) * ((real)1 + (real)sz)); sum = sy + sz; }
     else if (axis == 1) { area = (float)(s2 * ((real)1 + (real)sx) * ((real)1 + (real)sz)); sum = sx + sz; }
@@ -102,30 +103,54 @@ __device__ __forceinline__ void poissonStrain(const DevBatch<real>& b, int v) {
 __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= b.lHi) return;
+    // The loads come in three rounds of independent requests, so that a link waits on
+    // memory three times rather than once per table it visits: the link's own indices,
+    // then its simulation's control block, then everything the force needs (link state,
+    // both end voxels' pose, thermal factor and Poisson strain, their materials).
     const int s = b.lsim[l];
-    if (!simRunning(b, s)) return;
-    const SimCtl& sc = b.sims[s];
     const int vn = b.lneg[l], vp = b.lpos[l];
     const int meta = b.lmeta[l];
-    const int axis = meta & 3;
-    const LinkMatDev& mat = b.lmatTab[meta >> 2];
-    const VoxMatDev& mn = b.vmatTab[b.vmat[vn]];
-    const VoxMatDev& mp = b.vmatTab[b.vmat[vp]];
-    if (mn.fixed && mp.fixed) return;   // VX3_VoxelyzeKernel.cu:572
-    if (b.lflags[l] & LF_GONE) return;  // detached (VX3_VoxelyzeKernel.cu:574)
-    if (b.lflags[l] & LF_FRESH) {       // made by attach after this step's link pass in the reference
-        b.lflags[l] &= ~LF_FRESH;
+    const int lf0 = b.lflags[l];
+    if (!simRunning(b, s)) return;
+    const SimCtl& sc = b.sims[s];
+    if (sc.flags & SC_LINKS_DONE) return;   // resuming after this step's link pass
+    if (lf0 & LF_GONE) return;              // detached (VX3_VoxelyzeKernel.cu:574)
+    if (lf0 & LF_FRESH) {                   // made by attach after this step's link pass in the reference
+        b.lflags[l] = lf0 & ~LF_FRESH;
         return;
     }
+    const bool poisson = (sc.flags & SC_POISSON) != 0;
+    const int vmn = b.vmat[vn], vmp = b.vmat[vp];
+    LinkState<real> st;
+    st.pos2 = V3<real>(b.q2x[l], b.q2y[l], b.q2z[l]);
+    st.a1v = V3<real>(b.a1x[l], b.a1y[l], b.a1z[l]);
+    st.a2v = V3<real>(b.a2x[l], b.a2y[l], b.a2z[l]);
+    st.strain = b.strain[l];
+    st.maxStrain = b.maxStrain[l];
+    st.strainOffset = b.strainOffset[l];
+    st.flags = lf0;
+    const V3<real> pN(b.px[vn], b.py[vn], b.pz[vn]), pP(b.px[vp], b.py[vp], b.pz[vp]);
+    const Q4<real> oN(b.ow[vn], b.ox[vn], b.oy[vn], b.oz[vn]), oP(b.ow[vp], b.ox[vp], b.oy[vp], b.oz[vp]);
+    const float tfN = b.tfac[vn], tfP = b.tfac[vp];
+    float psN[3] = {0, 0, 0}, psP[3] = {0, 0, 0};
+    if (poisson) {
+        psN[0] = b.psx[vn]; psN[1] = b.psy[vn]; psN[2] = b.psz[vn];
+        psP[0] = b.psx[vp]; psP[1] = b.psy[vp]; psP[2] = b.psz[vp];
+    }
+    const int axis = meta & 3;
+    const LinkMatDev& mat = b.lmatTab[meta >> 2];
+    const VoxMatDev& mn = b.vmatTab[vmn];
+    const VoxMatDev& mp = b.vmatTab[vmp];
+    if (mn.fixed && mp.fixed) return;   // VX3_VoxelyzeKernel.cu:572
     const real nom = (real)sc.voxSize;
     // CVX_Link::updateRestLength: 0.5*(baseSize_neg + baseSize_pos)
-    const real restLength = (real)0.5 * (nom * (real)b.tfac[vn] + nom * (real)b.tfac[vp]);
+    const real restLength = (real)0.5 * (nom * (real)tfN + nom * (real)tfP);
     float area, tss;
     const float size = (float)sc.voxSize;
     if (mat.nu != 0.0f) {
         float an, sn, ap, sp;
-        transverse(b, vn, axis, size, mn.nu == 0.0f, an, sn);
-        transverse(b, vp, axis, size, mp.nu == 0.0f, ap, sp);
+        transverseOf(psN, axis, size, mn.nu == 0.0f, an, sn);
+        transverseOf(psP, axis, size, mp.nu == 0.0f, ap, sp);
         area = 0.5f * (an + ap);
         tss = 0.5f * (sn + sp);
     } else {
@@ -136,16 +161,6 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     // voxel's last update; material-fixed voxels never step
     const float dmN = 2 * mn.sqrtMass * mn.zetaInternal / (mn.fixed ? 0.0f : sc.prevDt);
     const float dmP = 2 * mp.sqrtMass * mp.zetaInternal / (mp.fixed ? 0.0f : sc.prevDt);
-    LinkState<real> st;
-    st.pos2 = V3<real>(b.q2x[l], b.q2y[l], b.q2z[l]);
-    st.a1v = V3<real>(b.a1x[l], b.a1y[l], b.a1z[l]);
-    st.a2v = V3<real>(b.a2x[l], b.a2y[l], b.a2z[l]);
-    st.strain = b.strain[l];
-    st.maxStrain = b.maxStrain[l];
-    st.strainOffset = b.strainOffset[l];
-    st.flags = b.lflags[l];
-    const V3<real> pN(b.px[vn], b.py[vn], b.pz[vn]), pP(b.px[vp], b.py[vp], b.pz[vp]);
-    const Q4<real> oN(b.ow[vn], b.ox[vn], b.oy[vn], b.oz[vn]), oP(b.ow[vp], b.ox[vp], b.oy[vp], b.oz[vp]);
     V3<real> fN, mN, fP, mP;
     linkForces<real>(mat, axis, pN, oN, pP, oP, restLength, area, tss, mat.nu != 0.0f, dmN, dmP, st, fN, mN, fP, mP);
     if ((sc.flags & SC_ATTACH) && b.lnew[l] > 0) {   // isNewLink (VX3_Link.cu:206-213)
@@ -203,7 +218,8 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
 // k_col_place scatters the voxels into the simulation's cell-ordered range of cellVox.
 // No bucket has a capacity.  The partner lists are counted, scanned and emitted; if a
 // simulation's partners exceed its neighbour table, it sets SC_COL_GROW and stands
-// still (no pass changes its state) until the host enlarges the table.
+// still after its link pass (SC_LINKS_DONE: no later pass changes its state) until the
+// host enlarges the table; its next step then resumes after the link pass.
 
 __device__ __forceinline__ bool colVoxel(const DevBatch<real>& b, int v, int& s) {
     s = b.vsim[v];
@@ -274,11 +290,13 @@ __device__ __forceinline__ int waveAlloc(bool active, int s, int n, CounterOf co
 // ---- nearby lists after a topology change (CVX_Voxel::generateNearby, Voxelyze.cpp:394-417,
 // on the links present now).  A step that attaches, detaches or removes voxels marks the
 // simulation (SC_TOPO_CHG); the next step regenerates the nearby list of every surface
-// voxel before its collision rebuild (the CPU engine's nearbyStale -> collisionsStale).
-// One thread per voxel runs the breadth-first search of generateNearby with its queue in
-// LDS (VX3_NEAR_MAX voxels); count, scan and emit passes fill nearList.  A search or a
-// table too small for the result stops the simulation in place (SC_COL_GROW) until the
-// host builds the lists itself or enlarges the table.
+// voxel before its collision rebuild (the CPU engine's nearbyStale -> collisionsStale),
+// after its link pass, so that the links that pass detached are already gone, as in
+// CVoxelyze::updateCollisions.  One thread per voxel runs the breadth-first search of
+// generateNearby with its queue in LDS (VX3_NEAR_MAX voxels); count, scan and emit
+// passes fill nearList.  A search or a table too small for the result stops the
+// simulation after its link pass (SC_COL_GROW | SC_LINKS_DONE) until the host builds
+// the lists itself or enlarges the table.
 #define VX3_NEAR_MAX 384
 #define VX3_NEAR_BS 64
 
@@ -330,10 +348,10 @@ __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_count(DevBatch<real> b) {
     b.nearStart[v] = b.sims[s].near0 + start;
     if (over) {
         atomicMax(&b.sims[s].nearNeed, 1 << 30);   // the host builds this simulation's lists itself
-        atomicOr(&b.sims[s].flags, SC_COL_GROW);
+        atomicOr(&b.sims[s].flags, SC_COL_GROW | SC_LINKS_DONE);
     } else if (start + c > b.sims[s].nearCap) {
         atomicMax(&b.sims[s].nearNeed, start + c);
-        atomicOr(&b.sims[s].flags, SC_COL_GROW);
+        atomicOr(&b.sims[s].flags, SC_COL_GROW | SC_LINKS_DONE);
     }
 }
 
@@ -363,19 +381,29 @@ __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_emit(DevBatch<real> b) {
 // colInterval steps, or once a surface voxel moved more than recalcDist since the last rebuild
 __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= b.vHi) return;
-    int s;
-    if (!colVoxel(b, v, s)) return;
-    SimCtl& sc = b.sims[s];
-    if (sc.flags & SC_REBUILD) return;
-    // new nearby lists (CVoxelyze: nearbyStale -> collisionsStale), or the fixed period
-    if ((sc.flags & SC_NEARDIRTY) || (sc.colInterval > 0 && sc.steps % sc.colInterval == 0)) {
-        atomicOr(&sc.flags, SC_REBUILD);
-        return;
+    int s = -1;
+    bool need = false;
+    if (v < b.vHi && colVoxel(b, v, s)) {
+        const SimCtl& sc = b.sims[s];
+        if (!(sc.flags & SC_REBUILD)) {
+            // new nearby lists (CVoxelyze: nearbyStale -> collisionsStale), or the fixed period
+            // (both normally set by the control pass of the step before)
+            need = (sc.flags & SC_NEARDIRTY) || (sc.colInterval > 0 && sc.steps % sc.colInterval == 0);
+            if (!need) {
+                const real dx = b.px[v] - (real)b.lastWatch[3 * v], dy = b.py[v] - (real)b.lastWatch[3 * v + 1],
+                           dz = b.pz[v] - (real)b.lastWatch[3 * v + 2];
+                need = dx * dx + dy * dy + dz * dz > (real)sc.colRecalcSq;
+            }
+        }
     }
-    const real dx = b.px[v] - (real)b.lastWatch[3 * v], dy = b.py[v] - (real)b.lastWatch[3 * v + 1],
-               dz = b.pz[v] - (real)b.lastWatch[3 * v + 2];
-    if (dx * dx + dy * dy + dz * dz > (real)sc.colRecalcSq) atomicOr(&sc.flags, SC_REBUILD);
+    // one atomic per simulation present in the wave, not one per voxel
+    unsigned long long pending = __ballot(need);
+    while (pending) {
+        const int lead = __ffsll((unsigned long long)pending) - 1;
+        const int sl = __shfl(s, lead, 64);
+        if ((int)(threadIdx.x & 63) == lead) atomicOr(&b.sims[sl].flags, SC_REBUILD);
+        pending &= ~__ballot(need && s == sl);
+    }
 }
 
 __device__ __forceinline__ bool colRebuilding(const DevBatch<real>& b, int s) {
@@ -505,14 +533,24 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
     }
 }
 
-// counts the partners of every surface voxel being rebuilt and hands it a range of the
-// neighbour table; a table too small stops the simulation in place until the host grows it
-__global__ void __launch_bounds__(256) k_col_count(DevBatch<real> b) {
+// Partner lists of every surface voxel being rebuilt, in one pass over the candidate
+// cells: the partners are gathered into an LDS column of the voxel's thread (up to
+// VX3_COL_STAGE; beyond that the cells are walked a second time straight into the table),
+// the voxel receives a range of the neighbour table, and the list is sorted there.  A
+// table too small stops the simulation after its link pass until the host grows it.
+#define VX3_COL_STAGE 48
+__global__ void __launch_bounds__(256) k_col_list(DevBatch<real> b) {
+    __shared__ int stage[VX3_COL_STAGE][256];   // [entry][thread]: a thread's column, conflict-free
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = threadIdx.x;
     int s = -1;
     const bool act = v < b.vHi && colVoxel(b, v, s) && (b.sims[s].flags & SC_REBUILD);
     int n = 0;
-    if (act) colForPartners(b, b.sims[s], v, [&](int) { n++; });
+    if (act)
+        colForPartners(b, b.sims[s], v, [&](int u) {
+            if (n < VX3_COL_STAGE) stage[n][t] = u;
+            n++;
+        });
     const int start = waveAlloc(act, s, n, [&](int sl) { return &b.sims[sl].nbUsed; });
     if (!act) return;
     SimCtl& sc = b.sims[s];
@@ -520,7 +558,27 @@ __global__ void __launch_bounds__(256) k_col_count(DevBatch<real> b) {
     b.nbStart[v] = sc.nb0 + start;
     if (start + n > sc.nbCap) {
         atomicMax(&sc.nbNeed, start + n);
-        atomicOr(&sc.flags, SC_COL_GROW);
+        atomicOr(&sc.flags, SC_COL_GROW | SC_LINKS_DONE);
+        return;
+    }
+    int* out = b.nbList + sc.nb0 + start;
+    if (n <= VX3_COL_STAGE) {
+        for (int i = 1; i < n; i++) {   // ascending partner order
+            const int x = stage[i][t];
+            int j = i - 1;
+            while (j >= 0 && stage[j][t] > x) { stage[j + 1][t] = stage[j][t]; j--; }
+            stage[j + 1][t] = x;
+        }
+        for (int i = 0; i < n; i++) out[i] = stage[i][t];
+    } else {
+        int m = 0;
+        colForPartners(b, sc, v, [&](int u) { out[m++] = u; });
+        for (int i = 1; i < n; i++) {
+            const int x = out[i];
+            int j = i - 1;
+            while (j >= 0 && out[j] > x) { out[j + 1] = out[j]; j--; }
+            out[j + 1] = x;
+        }
     }
 }
 
@@ -535,24 +593,6 @@ __global__ void __launch_bounds__(256) k_col_snap(DevBatch<real> b) {
     b.cpx[v] = b.px[v]; b.cpy[v] = b.py[v]; b.cpz[v] = b.pz[v];
     b.cmx[v] = b.mx[v]; b.cmy[v] = b.my[v]; b.cmz[v] = b.mz[v];
     b.ctf[v] = b.tfac[v];
-}
-
-__global__ void __launch_bounds__(256) k_col_emit(DevBatch<real> b) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= b.vHi) return;
-    int s;
-    if (!colVoxel(b, v, s)) return;
-    const SimCtl& sc = b.sims[s];
-    if (!(sc.flags & SC_REBUILD)) return;
-    int* out = b.nbList + b.nbStart[v];
-    int n = 0;
-    colForPartners(b, sc, v, [&](int u) { out[n++] = u; });
-    for (int i = 1; i < n; i++) {   // ascending partner order
-        const int x = out[i];
-        int j = i - 1;
-        while (j >= 0 && out[j] > x) { out[j + 1] = out[j]; j--; }
-        out[j + 1] = x;
-    }
 }
 
 // Contact force of the pair (i, j), i the lower index, on voxel i: CVX_Collision::
@@ -729,6 +769,15 @@ template <bool EXTRAS> __device__ __forceinline__ void voxelPass(const DevBatch<
     if (sc.flags & SC_POISSON) poissonStrain(b, v);
 }
 
+// the 36 end-load slots of voxels [0, nv) from one buffer to another (their strides may differ)
+__global__ void __launch_bounds__(256) k_copy_loads(const real* __restrict__ src, size_t srcFs, real* __restrict__ dst,
+                                                    size_t dstFs, int nv) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+#pragma unroll
+    for (int k = 0; k < 36; k++) dst[k * dstFs + v] = src[k * srcFs + v];
+}
+
 // at most 128 VGPRs (four waves per SIMD): the voxel pass is a gather over the force
 // slots, latency-bound at the three waves its unconstrained allocation allows
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_voxels(DevBatch<real> b) { voxelPass<false>(b); }
@@ -816,7 +865,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBatch<real> b, int forces) {
     if (!simRunning(b, s)) return;
     const SimCtl& sc = b.sims[s];
     if (!forces) {   // before the link pass: CVX_Voxel::updateSurface, interior iff all six links remain
-        if (!(sc.flags & (SC_DETACH | SC_SECONDARY | SC_ATTACH))) return;
+        if (!(sc.flags & (SC_DETACH | SC_SECONDARY | SC_ATTACH)) || (sc.flags & SC_LINKS_DONE)) return;
         const int fl = b.vflags[v];
         const int all = 0x3F << VF_LINKMASK_SHIFT;
         const int nf = (fl & all) == all ? (fl | VF_INTERIOR) : (fl & ~VF_INTERIOR);
@@ -931,7 +980,7 @@ __global__ void __launch_bounds__(256) k_attach(DevBatch<real> b) {
     const int s = blockIdx.x;
     if (s >= b.nSims || !simRunning(b, s)) return;
     SimCtl& sc = b.sims[s];
-    if (!(sc.flags & SC_ATTACH) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW))) return;
+    if (!(sc.flags & SC_ATTACH) || (sc.flags & (SC_DIVERGED_PENDING | SC_COL_OVERFLOW | SC_LINKS_DONE))) return;
     __shared__ int part[4];
     __shared__ int nc, np;
     const int tid = threadIdx.x;
@@ -1327,7 +1376,7 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     if ((sc.flags & SC_SECONDARY) && !diverged && !(sc.flags & SC_COL_OVERFLOW)) reinit = secondaryExperiment(b, sc);
     if (threadIdx.x != 0) return;
     sc.steps = steps;
-    sc.flags &= ~SC_REBUILD;
+    sc.flags &= ~(SC_REBUILD | SC_LINKS_DONE);
     if (sc.flags & SC_TOPOCOL) {   // a change made during this step: nearby lists regenerated at the next
         sc.flags &= ~SC_NEARDIRTY;
         if (sc.flags & SC_TOPO_CHG) {
@@ -1335,6 +1384,9 @@ __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
             sc.nearUsed = 0;
         }
     }
+    // the next step rebuilds its contact lists after new nearby lists, or at the fixed period
+    if ((sc.flags & SC_COLLISIONS) && ((sc.flags & SC_NEARDIRTY) || (sc.colInterval > 0 && steps % sc.colInterval == 0)))
+        sc.flags |= SC_REBUILD;
     if (sc.flags & SC_COL_OVERFLOW) {
         sc.state = 4;
         return;
@@ -1400,8 +1452,9 @@ __global__ void __launch_bounds__(256) k_dt(DevBatch<real> b, int s0, int s1) {
         else {
             const int axis = b.lmeta[l] & 3;
             float an, sn, ap, sp;
-            transverse(b, vn, axis, size, mn.nu == 0.0f, an, sn);
-            transverse(b, vp, axis, size, mp.nu == 0.0f, ap, sp);
+            const float psN[3] = {b.psx[vn], b.psy[vn], b.psz[vn]}, psP[3] = {b.psx[vp], b.psy[vp], b.psz[vp]};
+            transverseOf(psN, axis, size, mn.nu == 0.0f, an, sn);
+            transverseOf(psP, axis, size, mp.nu == 0.0f, ap, sp);
             const float area = 0.5f * (an + ap);
             const double rl = 0.5 * ((double)nom * (double)b.tfac[vn] + (double)nom * (double)b.tfac[vp]);
             k = (float)(mat.eHat * area / ((b.strain[l] + 1) * rl));

@@ -6,13 +6,13 @@
 
 namespace vx3 {
 struct StepOpts {
-    bool contacts;     // some simulation has collisions: run the collision passes
-    bool extras;       // some simulation has cilia or a force field: run its voxel pass
-    bool prestep;      // some simulation has cilia, a force field or detach: run the step-start pass
-    bool signals;      // some simulation has signals: run the signal-delivery pass
-    bool attach;       // some simulation attaches sticky voxels: run the attach pass and conditions
-    bool topo;         // some simulation has collisions and topology changes: run the nearby-list passes
-    int block;         // workgroup size of the element passes
+    bool contacts = false;     // some simulation has collisions: run the collision passes
+    bool extras = false;       // some simulation has cilia or a force field: run its voxel pass
+    bool prestep = false;      // some simulation has cilia, a force field or detach: run the step-start pass
+    bool signals = false;      // some simulation has signals: run the signal-delivery pass
+    bool attach = false;       // some simulation attaches sticky voxels: run the attach pass and conditions
+    bool topo = false;         // some simulation has collisions and topology changes: run the nearby-list passes
+    int block = 256;   // workgroup size of the element passes
 };
 // Kernel passes of one step, in launch order; ev (optional) receives a start/stop
 // event pair per pass: ev[2*k], ev[2*k+1].  PASS_POISSON no longer runs on its own
@@ -39,5 +39,9 @@ hipError_t launchResident(const DevBatch<double>& b, hipStream_t s, int shape, b
                           int nSteps, hipEvent_t* ev = nullptr);
 void launchDt(const DevBatch<float>& b, hipStream_t s, int s0, int s1);
 void launchDt(const DevBatch<double>& b, hipStream_t s, int s0, int s1);
+// copies the link end loads of voxels [0, nv) from src (slot stride srcFs) to dst (slot
+// stride dstFs), in the state precision
+void launchCopyLoads(const DevBatch<float>& b, hipStream_t s, const void* src, size_t srcFs, void* dst, size_t dstFs, int nv);
+void launchCopyLoads(const DevBatch<double>& b, hipStream_t s, const void* src, size_t srcFs, void* dst, size_t dstFs, int nv);
 }  // namespace vx3
 #endif

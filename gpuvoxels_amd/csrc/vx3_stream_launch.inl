@@ -17,8 +17,7 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_bscan, dim3(b.hashTot / VX3_COL_HBLOCK), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_boff, dim3(b.nSims), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_place, dim3(gv), dim3(bs), 0, s, b);
-    hipLaunchKernelGGL(VX_NS::k_col_count, dim3(gv), dim3(bs), 0, s, b);
-    hipLaunchKernelGGL(VX_NS::k_col_emit, dim3(gv), dim3(bs), 0, s, b);
+    hipLaunchKernelGGL(VX_NS::k_col_list, dim3((b.vHi + 255) / 256), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_snap, dim3(gv), dim3(bs), 0, s, b);
 }
 }  // namespace
@@ -35,12 +34,13 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     } while (0)
     if (o.prestep) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 0);
     if (o.attach) hipLaunchKernelGGL(VX_NS::k_attach, dim3(b.nSims), dim3(256), 0, s, b);
-    // The collision lists are rebuilt from the positions at the step's start, before the
-    // link pass (which moves no voxel): the reference rebuilds them between the link and
-    // voxel passes from the same positions.  A simulation whose neighbour table proves too
-    // small stands still for this step, link pass included.
-    if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs, o.topo));
+    // The collision lists are rebuilt between the link and voxel passes, as the reference
+    // does (CVoxelyze::doTimeStep, Voxelyze.cpp:251-284): the link pass moves no voxel, and
+    // nearby lists regenerated after a topology change follow the links this step's link
+    // pass left (its detaches included).  A simulation whose table proves too small stands
+    // still after its link pass (SC_LINKS_DONE) and resumes there once the table is grown.
     if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
+    if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs, o.topo));
     if (o.contacts && o.extras) hipLaunchKernelGGL(VX_NS::k_hits, dim3(gv), dim3(bs), 0, s, b);
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 1);
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_voxels_extras, dim3(gv), dim3(bs), 0, s, b);
@@ -49,6 +49,12 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     if (o.attach) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 2);
     VX3_EV(PASS_CONTROL, hipLaunchKernelGGL(VX_NS::k_control, dim3(b.nSims), dim3(256), 0, s, b));
 #undef VX3_EV
+}
+
+void launchCopyLoads(const DevBatch<VX_REAL>& b, hipStream_t s, const void* src, size_t srcFs, void* dst, size_t dstFs, int nv) {
+    if (nv > 0)
+        hipLaunchKernelGGL(VX_NS::k_copy_loads, dim3((nv + 255) / 256), dim3(256), 0, s, (const VX_REAL*)src, srcFs,
+                           (VX_REAL*)dst, dstFs, nv);
 }
 
 void launchDt(const DevBatch<VX_REAL>& b, hipStream_t s, int s0, int s1) {

@@ -47,11 +47,13 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
        SC_TARGETS = 16384,      // target contacts are counted (runs with SC_EXTRAS)
        SC_ATTACH = 32768,       // touching sticky surface voxels get new links
        SC_COL_GROW = 65536,     // a collision table is too small: the simulation stands still
-                                // (no pass changes it) until the host enlarges the table
+                                // (no further pass changes it) until the host enlarges the table
        SC_TOPOCOL = 1 << 17,    // collisions with topology changes (attach, detach, voxel removal):
                                 // the nearby lists are regenerated on the device after a change
        SC_TOPO_CHG = 1 << 18,   // the topology changed during this step
-       SC_NEARDIRTY = 1 << 19 };  // this step regenerates the nearby lists from the current links
+       SC_NEARDIRTY = 1 << 19,  // this step regenerates the nearby lists from the current links
+       SC_LINKS_DONE = 1 << 20 };  // stood still for a collision table after this step's link pass:
+                                   // the step resumes after the link pass (its end loads are kept)
 #define VX3_COL_HBLOCK 1024     // hash buckets per scan block; a simulation's buckets are whole blocks
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)

@@ -206,6 +206,14 @@ struct vx3h_batch {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evUsed;
     long long timedSteps = 0;
     int tableGrowths = 0;                        // collision neighbour tables enlarged so far
+    // A simulation that stood still for a collision table after its link pass resumes at
+    // its next step from the end loads that link pass left, in the buffer the stepping
+    // entry point of that step used: the batch's own (false) or the no-allocation
+    // entry point's static array (true).
+    bool loadsInScratch = false;
+    size_t loadsFs = 0;   // slot stride of that buffer at that step
+    int loadsNv = 0;      // voxels [0, loadsNv) it holds
+    bool resumePending = false;
 };
 
 namespace {
@@ -344,6 +352,7 @@ vx3h_status readReal(vx3h_batch* b, const void* src, size_t n, std::vector<doubl
 char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx; }
 
 StepOpts stepOpts(const vx3h_batch* b);
+vx3h_status placeLoads(vx3h_batch* b, bool scratch);
 
 vx3h_status launchSteps(vx3h_batch* b, int n) {
     const StepOpts o = stepOpts(b);
@@ -980,6 +989,8 @@ extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
         b->dtHi = -1;
     }
     const int chunk = b->cfg.steps_per_launch > 0 ? b->cfg.steps_per_launch : 1000;
+    if (vx3h_status ps = placeLoads(b, false)) return ps;
+    b->resumePending = false;   // every waiting simulation resumes in the first step below
     for (int done = 0; done < n_steps;) {
         const int c = std::min(chunk, n_steps - done);
         vx3h_status st = launchResidentAll(b, c, nullptr);
@@ -1040,6 +1051,31 @@ vx3h_status noallocScratch(int device, void** out) {
     *out = g_scratchPtr[device];
     return VX3H_OK;
 }
+
+// Before a stepping entry point launches its steps with the end loads in the batch's
+// buffer (scratch false) or in the static array (true): a simulation resuming after its
+// link pass finds the loads that pass left, copied over if the last step used the other
+// buffer.
+vx3h_status placeLoads(vx3h_batch* b, bool scratch) {
+    const size_t fs = scratch ? (size_t)b->db.vHi : (size_t)b->db.Fs;   // the stride the coming steps use
+    const int nv = std::min(b->loadsNv, b->db.vHi);
+    if (b->resumePending && scratch != b->loadsInScratch && nv > 0) {
+        void* scr = nullptr;
+        vx3h_status st = noallocScratch(b->device, &scr);
+        if (st) return st;
+        const void* src = scratch ? (const void*)b->db.F : scr;
+        void* dst = scratch ? scr : (void*)b->db.F;
+        // the batch's buffer keeps its contents at its current stride when it grows
+        const size_t srcFs = scratch ? (size_t)b->db.Fs : b->loadsFs;
+        if (b->realSize == 4) launchCopyLoads(b->db, b->stream, src, srcFs, dst, fs, nv);
+        else launchCopyLoads(*reinterpret_cast<DevBatch<double>*>(&b->db), b->stream, src, srcFs, dst, fs, nv);
+        HIPCHK(hipGetLastError());
+    }
+    b->loadsInScratch = scratch;
+    b->loadsFs = fs;
+    b->loadsNv = b->db.vHi;
+    return VX3H_OK;
+}
 }  // namespace
 
 extern "C" vx3h_status vx3h_batch_step_noalloc(vx3h_batch* b, int n_steps) {
@@ -1060,6 +1096,8 @@ extern "C" vx3h_status vx3h_batch_step_noalloc(vx3h_batch* b, int n_steps) {
         b->dtLo = 1 << 30;
         b->dtHi = -1;
     }
+    if ((st = placeLoads(b, true))) return st;
+    b->resumePending = false;   // every waiting simulation resumes in the first step below
     const StepOpts o = stepOpts(b);
     for (int i = 0; i < n_steps; i++) {
         if (b->realSize == 4) {
@@ -1120,6 +1158,7 @@ extern "C" vx3h_status vx3h_sim_step(vx3h_batch* b, int slot, float dt, int* ok)
     c.flags &= ~SC_RESIDENT;   // the per-step kernels take it, whichever path it runs on
     if (c.state == VX3H_SIM_DONE) c.state = VX3H_SIM_RUNNING;   // stepping past the stop condition is the caller's call
     HIPCHK(hipMemcpy(b->db.sims + slot, &c, sizeof(c), hipMemcpyHostToDevice));
+    if (vx3h_status ps = placeLoads(b, false)) return ps;
     StepOpts o = stepOpts(b);
     for (int attempt = 0; attempt < 8; attempt++) {
         if (b->realSize == 4) {
@@ -1156,13 +1195,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     if (b->slots.empty()) return VX3H_OK;
     std::vector<hipEvent_t> ev(2 * PASS_COUNT * n_steps, nullptr);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    StepOpts o;
-    o.contacts = b->anyColl;
-    o.extras = b->anyExtras;
-    o.prestep = b->anyDetach;
-    o.signals = b->anySignals;
-    o.attach = b->anyAttach;
-    o.block = b->cfg.block_size > 0 && b->cfg.block_size <= 256 ? b->cfg.block_size : 256;
+    const StepOpts o = stepOpts(b);
     for (int i = 0; i < n_steps; i++) {
         hipEvent_t* e = &ev[2 * PASS_COUNT * i];
         if (b->realSize == 4) launchStep(b->db, b->stream, o, e);
@@ -1386,10 +1419,12 @@ vx3h_status hostNearby(vx3h_batch* b, int s, SimCtl& c) {
 }
 
 // Simulations whose collision tables proved too small stand still (SC_COL_GROW: no pass
-// has changed their state since).  A neighbour or nearby table gets room for half as
-// many entries again as the rebuild needed (at least twice the old one); nearby lists
-// whose search outgrew the device queue are built here.  The simulation then rebuilds
-// at its next step, which runs as it would have.  The batch's streams are idle.
+// has changed their state since; with SC_LINKS_DONE, since the link pass of the step
+// that stopped, whose end loads are kept).  A neighbour or nearby table gets room for
+// half as many entries again as the rebuild needed (at least twice the old one); nearby
+// lists whose search outgrew the device queue are built here.  The simulation then
+// rebuilds at its next step, which runs (or, after its link pass, resumes) as it would
+// have.  The batch's streams are idle.
 vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
     for (int s = 0; s < (int)ctl.size(); s++) {
         if (!b->slots[s].used || !(ctl[s].flags & SC_COL_GROW)) continue;
@@ -1425,6 +1460,7 @@ vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
         }
         c.nbNeed = c.nearNeed = c.apNeed = 0;
         c.nearUsed = 0;
+        if (c.flags & SC_LINKS_DONE) b->resumePending = true;
         c.flags = (c.flags & ~SC_COL_GROW) | SC_REBUILD;
         HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(SimCtl), hipMemcpyHostToDevice));
         b->db.Ccap = b->nbPool.cap;

@@ -45,3 +45,40 @@ def test_noalloc_refuses_batches_beyond_its_scratch():
         b.step_noalloc(1)
     assert e.value.status == _abi.EINVAL
     b.close()
+
+
+@pytest.mark.parametrize("first_noalloc", [False, True])
+def test_resume_after_link_pass_across_entry_points(first_noalloc):
+    """Touching balls with a 16-entry neighbour table: the first step call stands the
+    simulation still after the link pass of its first step until a poll enlarges the
+    table; the next call, through the other entry point, resumes it (its end loads
+    carried between the batch's buffer and the static array).  It ends where
+    vx3h_batch_step with a large table from the start ends."""
+    m = workloads.arena_balls(n_balls=64, arena=80, gap=0.0, steps=0, seed=2)
+
+    def run(alternate, table):
+        b = engine.Batch(precision=_abi.PREC_F64, path=_abi.PATH_AUTO, collision_table_entries=table,
+                         collision_rebuild_interval=8)
+        s = b.admit(m)
+        calls, after_first = 0, None
+        while True:
+            st = b.poll()[s]
+            if calls == 1:
+                after_first = st.steps
+            if st.steps >= 17:
+                break
+            if alternate and (calls % 2 == 0) == first_noalloc:
+                b.step_noalloc(17 - st.steps)
+            else:
+                b.step(17 - st.steps)
+            calls += 1
+            assert calls < 200, "simulation does not advance"
+        out = b.read_state(s), b.collision_pairs(s), calls, after_first
+        b.close()
+        return out
+
+    (ga, pa, ca, fa), (gc, pc, _, _) = run(True, 16), run(False, 0)
+    assert fa == 0 and ca >= 2, "the first call did not stop for the table"
+    assert np.array_equal(pa, pc)
+    for k in ("pos", "orient", "lin_mom", "ang_mom"):
+        assert np.array_equal(ga[k], gc[k]), k
