@@ -100,7 +100,7 @@ __device__ __forceinline__ void poissonStrain(const DevBatch<real>& b, int v) {
     b.psx[v] = st[0]; b.psy[v] = st[1]; b.psz[v] = st[2];
 }
 
-__global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_links(DevBatch<real> b) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= b.lHi) return;
     // The loads come in three rounds of independent requests, so that a link waits on
@@ -382,10 +382,11 @@ __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_emit(DevBatch<real> b) {
 __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     int s = -1;
-    bool need = false;
+    bool need = false, rebuilding = false;
     if (v < b.vHi && colVoxel(b, v, s)) {
         const SimCtl& sc = b.sims[s];
-        if (!(sc.flags & SC_REBUILD)) {
+        rebuilding = (sc.flags & SC_REBUILD) != 0;
+        if (!rebuilding) {
             // new nearby lists (CVoxelyze: nearbyStale -> collisionsStale), or the fixed period
             // (both normally set by the control pass of the step before)
             need = (sc.flags & SC_NEARDIRTY) || (sc.colInterval > 0 && sc.steps % sc.colInterval == 0);
@@ -396,6 +397,8 @@ __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
             }
         }
     }
+    // the rest of the rebuild passes run only if some simulation rebuilds this step
+    if (__ballot(need || rebuilding) && (threadIdx.x & 63) == 0) *b.colAny = 1;
     // one atomic per simulation present in the wave, not one per voxel
     unsigned long long pending = __ballot(need);
     while (pending) {
@@ -413,12 +416,14 @@ __device__ __forceinline__ bool colRebuilding(const DevBatch<real>& b, int s) {
 }
 
 __global__ void __launch_bounds__(256) k_col_clear(DevBatch<real> b) {
+    if (!*b.colAny) return;
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= b.hashTot) return;
     if (colRebuilding(b, b.hashSim[h])) b.hashCnt[h] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_col_insert(DevBatch<real> b) {
+    if (!*b.colAny) return;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     int s;
@@ -438,6 +443,7 @@ __global__ void __launch_bounds__(256) k_col_insert(DevBatch<real> b) {
 // exclusive scan of the bucket counts within each block of VX3_COL_HBLOCK buckets
 // (a simulation's buckets are whole blocks); the block's total goes to hashBlk
 __global__ void __launch_bounds__(256) k_col_bscan(DevBatch<real> b) {
+    if (!*b.colAny) return;
     const int B = blockIdx.x;
     const int h0 = B * VX3_COL_HBLOCK;
     if (h0 >= b.hashTot || !colRebuilding(b, b.hashSim[h0])) return;
@@ -463,6 +469,7 @@ __global__ void __launch_bounds__(256) k_col_bscan(DevBatch<real> b) {
 
 // per rebuilt simulation: offsets of its bucket blocks in its cellVox range
 __global__ void __launch_bounds__(256) k_col_boff(DevBatch<real> b) {
+    if (!*b.colAny) return;
     const int s = blockIdx.x;
     if (s >= b.nSims || !colRebuilding(b, s)) return;
     const SimCtl& sc = b.sims[s];
@@ -500,6 +507,7 @@ __device__ __forceinline__ int colBucketStart(const DevBatch<real>& b, int h) {
 }
 
 __global__ void __launch_bounds__(256) k_col_place(DevBatch<real> b) {
+    if (!*b.colAny) return;
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= b.vHi) return;
     int s;
@@ -540,6 +548,7 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
 // table too small stops the simulation after its link pass until the host grows it.
 #define VX3_COL_STAGE 48
 __global__ void __launch_bounds__(256) k_col_list(DevBatch<real> b) {
+    if (!*b.colAny) return;
     __shared__ int stage[VX3_COL_STAGE][256];   // [entry][thread]: a thread's column, conflict-free
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = threadIdx.x;
@@ -1360,6 +1369,7 @@ __device__ void blockCloseness(const DevBatch<real>& b, const SimCtl& sc, double
 __global__ void __launch_bounds__(256) k_control(DevBatch<real> b) {
     __shared__ double sQ[VX3_DEV_MT_MAX + 1];
     const int s = blockIdx.x;
+    if (s == 0 && threadIdx.x == 0) *b.colAny = 0;   // every rebuild pass of this step is done
     if (s >= b.nSims) return;
     SimCtl& sc = b.sims[s];
     if (!simRunning(b, s)) return;

@@ -73,9 +73,6 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
     };
     auto describe = [&](vx3h_vxa* v, const vx3h_vxa_info& info) {
         vx3h_sim_desc d = *vx3h_vxa_desc(v);
-        if (d.enable_detach && d.collisions_enabled) d.enable_detach = 0;   // reached only with allow_unsupported
-        if (d.enable_attach && d.collisions_enabled) d.enable_attach = 0;
-        if (d.secondary_experiment && d.collisions_enabled) d.secondary_experiment = 0;
         if (opt.voxcad_stop) {
             d.stop_condition_type = info.stop_condition_type;
             d.stop_condition_value = info.stop_condition_value;
@@ -93,14 +90,6 @@ void SimulationManager::runDevice(int device, const std::vector<std::string>& fi
             const char* raw = nullptr;
             check(vx3h_vxa_get_info(v, &info, &raw), f);
             if (raw && *raw) printf("%s\n", raw);   // RawPrint (VX3_SimulationManager.cu:319-321)
-            std::string missing;
-            if (info.enable_attach && vx3h_vxa_desc(v)->collisions_enabled) missing += " attach-with-collisions";
-            if (info.enable_detach && vx3h_vxa_desc(v)->collisions_enabled) missing += " detach-with-collisions";
-            if (info.secondary_experiment && vx3h_vxa_desc(v)->collisions_enabled) missing += " secondary-experiment-with-collisions";
-            if (!missing.empty()) {
-                if (!opt.allow_unsupported) { vx3h_vxa_free(v); throw std::runtime_error(f + ": not supported by this engine:" + missing); }
-                fprintf(stderr, "warning: %s: ignoring%s\n", f.c_str(), missing.c_str());
-            }
             if (info.record_step_size > 0) {   // recorded simulations run alone, after the batch
                 history.push_back({baseName(f), -1, v, info, describe(v, info)});
                 continue;

@@ -41,7 +41,7 @@ struct ManagerOptions {
     int precision = VX3H_PREC_F64;      // state precision of the device engine
     int path = VX3H_PATH_AUTO;
     int steps_per_poll = 2000;          // steps between host polls of the batch
-    bool allow_unsupported = false;     // run VXAs that enable features this engine ignores
+    bool allow_unsupported = false;     // accepted for callers that set it: nothing is left unsimulated
     bool verbose = true;
     FILE* history_out = nullptr;        // playback stream of RecordHistory simulations [stdout]
     bool voxcad_stop = false;           // apply the VXA's VoxCAD stop condition and equilibrium mode

@@ -8,9 +8,8 @@
 // read from input_dir and spread over the devices round-robin; the report is written
 // to the output path.  Exit status 1 on bad arguments, an existing output without -f,
 // a missing task file or no GPU; 0 otherwise.  Additions: --f32 runs the FP32 state
-// engine (default FP64), --allow-unsupported runs VXAs that enable features this
-// engine does not simulate (attach/detach, signals) without them, and
-// --voxcad-stop honours the VXA's StopConditionType/Value and EquilibriumMode the way
+// engine (default FP64), --allow-unsupported is accepted for scripts that pass it
+// (every feature the loader reads is simulated), and --voxcad-stop honours the VXA's StopConditionType/Value and EquilibriumMode the way
 // VoxCAD does (the Voxelyze3 worker reads and ignores them).
 #include <sys/stat.h>
 
@@ -35,7 +34,7 @@ const char* USAGE =
     "  -o [ --output ] arg    output file path for the report (e.g. report_1.xml)\n"
     "  -f [ --force ]         overwrite output file if exists\n"
     "  --f32                  FP32 state engine (default FP64)\n"
-    "  --allow-unsupported    ignore features this engine does not simulate\n"
+    "  --allow-unsupported    accepted; every feature the loader reads is simulated\n"
     "  --voxcad-stop          honour StopConditionType/Value and EquilibriumMode as VoxCAD does\n";
 
 bool isFile(const std::string& p) {

@@ -471,6 +471,11 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
     if (e != hipSuccess) { delete b; FAILV(VX3H_EDEVICE, "stream creation failed"); }
     memset(&b->db, 0, sizeof(b->db));
     b->db.solo = -1;
+    if (hipMalloc(&b->db.colAny, 256) != hipSuccess || hipMemset(b->db.colAny, 0, 256) != hipSuccess) {
+        hipStreamDestroy(b->stream);
+        delete b;
+        FAILV(VX3H_EDEVICE, "device allocation failed");
+    }
     initSpecs(b);
     {
         ResidentShapeInfo tmp[16];
@@ -516,6 +521,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         for (ArrSpec& a : *arrs) if (*a.slot) hipFree(*a.slot);
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims, (void*)b->db.hist}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
+    if (b->db.colAny) hipFree(b->db.colAny);
     for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool, &b->cellPool, &b->tgtPool, &b->attPool, &b->pairPool})
         for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
     for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
