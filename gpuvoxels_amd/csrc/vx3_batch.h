@@ -63,6 +63,8 @@ template <typename T> struct DevBatch {
     int* tgtList;             // voxels of isTarget materials per sim (close pairs), a range per sim
     int* attCand;             // attach scratch per sim: attachable voxels, then their pair offsets
     unsigned long long* attPair;   // attach scratch per sim: candidate pairs (higher << 32 | lower), ascending
+    unsigned* nearMask;       // [VX3_NEAR_WORDS][Vcap]: each surface voxel's nearby set over lattice offsets
+                              // (simulations with SC_NEARMASK), kept with nearStart/nearCount/nearList
     int* colAny;              // one word: some simulation rebuilds its contact lists this step (set by the
                               // staleness pass, cleared by the control pass)
     // tables

@@ -100,7 +100,7 @@ __device__ __forceinline__ void poissonStrain(const DevBatch<real>& b, int v) {
     b.psx[v] = st[0]; b.psy[v] = st[1]; b.psz[v] = st[2];
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_links(DevBatch<real> b) {
+__global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= b.lHi) return;
     // The loads come in three rounds of independent requests, so that a link waits on
@@ -238,6 +238,28 @@ __device__ __forceinline__ unsigned colHash(int x, int y, int z) {
     return ((unsigned)x * 73856093u) ^ ((unsigned)y * 19349663u) ^ ((unsigned)z * 83492791u);
 }
 
+__constant__ NearBitTable kNearBits = NearBitTable();
+
+// whether global voxel u is in global voxel v's nearby set: its bit in v's mask (SC_NEARMASK),
+// else a binary search of v's sorted nearby list (sim-local indices)
+__device__ __forceinline__ bool nearbyOf(const DevBatch<real>& b, const SimCtl& sc, int v, int u) {
+    if (sc.flags & SC_NEARMASK) {
+        const int dx = b.vijk[3 * u] - b.vijk[3 * v], dy = b.vijk[3 * u + 1] - b.vijk[3 * v + 1],
+                  dz = b.vijk[3 * u + 2] - b.vijk[3 * v + 2];
+        const int bit = kNearBits.bit[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)];   // |d| <= 5: checked by the caller
+        return bit >= 0 && ((b.nearMask[(size_t)(bit >> 5) * b.Vcap + v] >> (bit & 31)) & 1u);
+    }
+    const int local = u - sc.v0;
+    int lo = b.nearStart[v], hi = lo + b.nearCount[v];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int x = b.nearList[mid];
+        if (x == local) return true;
+        if (x < local) lo = mid + 1; else hi = mid;
+    }
+    return false;
+}
+
 // the CPU pair criterion between two surface voxels of one simulation
 __device__ __forceinline__ bool colWatched(const DevBatch<real>& b, const SimCtl& sc, int v, int u) {
     const real dx = b.px[v] - b.px[u], dy = b.py[v] - b.py[u], dz = b.pz[v] - b.pz[u];
@@ -247,15 +269,7 @@ __device__ __forceinline__ bool colWatched(const DevBatch<real>& b, const SimCtl
     const int li = abs(b.vijk[3 * v] - b.vijk[3 * u]) + abs(b.vijk[3 * v + 1] - b.vijk[3 * u + 1]) +
                    abs(b.vijk[3 * v + 2] - b.vijk[3 * u + 2]);
     if (li > sc.colNearHops && !(sc.flags & SC_ATTACH)) return true;   // attached links shortcut the lattice
-    const int local = u - sc.v0;
-    int lo = b.nearStart[v], hi = lo + b.nearCount[v];
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int x = b.nearList[mid];
-        if (x == local) return false;
-        if (x < local) lo = mid + 1; else hi = mid;
-    }
-    return true;
+    return !nearbyOf(b, sc, v, u);
 }
 
 // Hands each active lane a range of n entries from its simulation's counter (*counter of
@@ -361,14 +375,31 @@ __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_emit(DevBatch<real> b) {
     if (v >= b.vHi) return;
     const int s = b.vsim[v];
     if (!nearDirty(b, s) || (b.sims[s].flags & SC_COL_GROW)) return;
-    if (b.vflags[v] & VF_INTERIOR) { b.nearCount[v] = 0; return; }
+    if (b.vflags[v] & VF_INTERIOR) {
+        b.nearCount[v] = 0;
+        if (b.sims[s].flags & SC_NEARMASK)
+            for (int w = 0; w < VX3_NEAR_WORDS; w++) b.nearMask[(size_t)w * b.Vcap + v] = 0;
+        return;
+    }
     int* Q = q[threadIdx.x];
     const int n = nearBfs(b, v, b.sims[s].colNearHops, Q);
     int* out = b.nearList + b.nearStart[v];
     const int v0 = b.sims[s].v0;
+    const bool mask = (b.sims[s].flags & SC_NEARMASK) != 0;
+    unsigned mk[VX3_NEAR_WORDS] = {0, 0, 0, 0, 0, 0, 0, 0};
     int c = 0;
     for (int i = 1; i < n; i++)
-        if (!(b.vflags[Q[i]] & VF_INTERIOR)) out[c++] = Q[i] - v0;
+        if (!(b.vflags[Q[i]] & VF_INTERIOR)) {
+            out[c++] = Q[i] - v0;
+            if (mask) {   // within the hop radius (<= 5), so within the table's offsets
+                const int u = Q[i];
+                const int bit = kNearBits.bit[(b.vijk[3 * u] - b.vijk[3 * v] + 5) * 121 + (b.vijk[3 * u + 1] - b.vijk[3 * v + 1] + 5) * 11 +
+                                              (b.vijk[3 * u + 2] - b.vijk[3 * v + 2] + 5)];
+                mk[bit >> 5] |= 1u << (bit & 31);
+            }
+        }
+    if (mask)
+        for (int w = 0; w < VX3_NEAR_WORDS; w++) b.nearMask[(size_t)w * b.Vcap + v] = mk[w];
     for (int i = 1; i < c; i++) {   // ascending, for the binary search of colWatched
         const int x = out[i];
         int j = i - 1;
@@ -377,14 +408,22 @@ __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_emit(DevBatch<real> b) {
     }
 }
 
-// decides, per simulation, whether its lists are rebuilt this step: the first step, every
-// colInterval steps, or once a surface voxel moved more than recalcDist since the last rebuild
+// Takes the contact snapshot of every surface voxel and decides, per simulation,
+// whether its lists are rebuilt this step: the first step, every colInterval steps,
+// after new nearby lists, or once a surface voxel moved more than recalcDist since the
+// last rebuild
 __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     int s = -1;
     bool need = false, rebuilding = false;
     if (v < b.vHi && colVoxel(b, v, s)) {
         const SimCtl& sc = b.sims[s];
+        // contact snapshot (see DevBatch::cpx): the reference computes every contact force
+        // from the state before any voxel moves this step (CVoxelyze::updateCollisions ahead
+        // of the voxel loop, Voxelyze.cpp:263-279); the link pass before this one moves none
+        b.cpx[v] = b.px[v]; b.cpy[v] = b.py[v]; b.cpz[v] = b.pz[v];
+        b.cmx[v] = b.mx[v]; b.cmy[v] = b.my[v]; b.cmz[v] = b.mz[v];
+        b.ctf[v] = b.tfac[v];
         rebuilding = (sc.flags & SC_REBUILD) != 0;
         if (!rebuilding) {
             // new nearby lists (CVoxelyze: nearbyStale -> collisionsStale), or the fixed period
@@ -591,18 +630,6 @@ __global__ void __launch_bounds__(256) k_col_list(DevBatch<real> b) {
     }
 }
 
-// Contact snapshot (see DevBatch::cpx): the reference computes every contact force
-// from the state before any voxel moves this step (CVoxelyze::updateCollisions ahead of
-// the voxel loop, Voxelyze.cpp:263-279).
-__global__ void __launch_bounds__(256) k_col_snap(DevBatch<real> b) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= b.vHi) return;
-    const int s = b.vsim[v];
-    if (!simRunning(b, s) || !(b.sims[s].flags & SC_COLLISIONS) || (b.vflags[v] & VF_INTERIOR)) return;
-    b.cpx[v] = b.px[v]; b.cpy[v] = b.py[v]; b.cpz[v] = b.pz[v];
-    b.cmx[v] = b.mx[v]; b.cmy[v] = b.my[v]; b.cmz[v] = b.mz[v];
-    b.ctf[v] = b.tfac[v];
-}
 
 // Contact force of the pair (i, j), i the lower index, on voxel i: CVX_Collision::
 // updateContactForce (VX_Collision.cpp:41-56) from both voxels' pre-step position,

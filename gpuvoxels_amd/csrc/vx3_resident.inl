@@ -121,14 +121,7 @@ __device__ __forceinline__ bool resWatched(const DevBatch<real>& b, const SimCtl
     const int li = abs(b.vijk[3 * gv] - b.vijk[3 * gu]) + abs(b.vijk[3 * gv + 1] - b.vijk[3 * gu + 1]) +
                    abs(b.vijk[3 * gv + 2] - b.vijk[3 * gu + 2]);
     if (li > sc.colNearHops) return true;
-    int lo = b.nearStart[gv], hi = lo + b.nearCount[gv];
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int x = b.nearList[mid];
-        if (x == u) return false;
-        if (x < u) lo = mid + 1; else hi = mid;
-    }
-    return true;
+    return !nearbyOf(b, sc, gv, gu);
 }
 
 __device__ __forceinline__ void ldsCopyWords(int* dst, const int* src, int words, int tid, int bs) {

@@ -18,7 +18,6 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
     hipLaunchKernelGGL(VX_NS::k_col_boff, dim3(b.nSims), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_place, dim3(gv), dim3(bs), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_list, dim3((b.vHi + 255) / 256), dim3(256), 0, s, b);
-    hipLaunchKernelGGL(VX_NS::k_col_snap, dim3(gv), dim3(bs), 0, s, b);
 }
 }  // namespace
 

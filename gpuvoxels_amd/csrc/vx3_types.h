@@ -52,9 +52,29 @@ enum { SC_VARY = 1, SC_EXPANSION = 2, SC_DIVERGED_PENDING = 4, SC_POISSON = 8, S
                                 // the nearby lists are regenerated on the device after a change
        SC_TOPO_CHG = 1 << 18,   // the topology changed during this step
        SC_NEARDIRTY = 1 << 19,  // this step regenerates the nearby lists from the current links
-       SC_LINKS_DONE = 1 << 20 };  // stood still for a collision table after this step's link pass:
+       SC_LINKS_DONE = 1 << 20,    // stood still for a collision table after this step's link pass:
                                    // the step resumes after the link pass (its end loads are kept)
+       SC_NEARMASK = 1 << 21 };    // nearby sets also held as lattice-offset bit masks (nearMask)
 #define VX3_COL_HBLOCK 1024     // hash buckets per scan block; a simulation's buckets are whole blocks
+
+// A voxel's nearby set (the surface voxels within the link-hop radius) as a bit mask over
+// lattice offsets: the voxels of one simulation sit at distinct lattice points and a voxel
+// h hops away is at most h lattice steps away, so with a radius of at most 5 hops every
+// member is at one of the 230 offsets (dx, dy, dz) with 1 <= |dx|+|dy|+|dz| <= 5.
+#define VX3_NEAR_RADIUS 5
+#define VX3_NEAR_WORDS 8        // 230 bits
+struct NearBitTable {
+    short bit[11 * 11 * 11];    // by (dx+5)*121 + (dy+5)*11 + (dz+5): the offset's bit, or -1
+    constexpr NearBitTable() : bit() {
+        int n = 0;
+        for (int dx = -5; dx <= 5; dx++)
+            for (int dy = -5; dy <= 5; dy++)
+                for (int dz = -5; dz <= 5; dz++) {
+                    const int l = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy) + (dz < 0 ? -dz : dz);
+                    bit[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)] = (l == 0 || l > 5) ? (short)-1 : (short)n++;
+                }
+    }
+};
 
 struct VoxMatDev {                  // CVX_MaterialVoxel derived constants (VX_MaterialVoxel.cpp:57-79)
     float mass, massInverse, sqrtMass, momentInertiaInverse;

@@ -238,7 +238,7 @@ void initSpecs(vx3h_batch* b) {
     V(&d.vbk, 4, 1, false); V(&d.vrank, 4, 1, false);
     V(&d.cpx, R, 1, false); V(&d.cpy, R, 1, false); V(&d.cpz, R, 1, false);
     V(&d.cmx, R, 1, false); V(&d.cmy, R, 1, false); V(&d.cmz, R, 1, false); V(&d.ctf, 4, 1, false);
-    V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false);
+    V(&d.nearStart, 4, 1, false); V(&d.nearCount, 4, 1, false); V(&d.nearMask, 4, VX3_NEAR_WORDS, true);
     V(&d.cil, 8, 8, true); V(&d.xf, R, 6, true); V(&d.ip, 8, 3, true); V(&d.rt, 8, 1, false);
     V(&d.sg, 8, 8, true); V(&d.sp, 8, 5, true); V(&d.nbr, 4, 6, true);
     b->nbPool.arrs = {{(void**)&d.nbList, 4}};
@@ -350,6 +350,30 @@ vx3h_status readReal(vx3h_batch* b, const void* src, size_t n, std::vector<doubl
 }
 
 char* at(void* base, size_t elem, size_t idx) { return (char*)base + elem * idx; }
+
+constexpr NearBitTable kHostNearBits;
+
+// Uploads the nearby sets of voxels [v0, v0 + nv) as lattice-offset bit masks (SC_NEARMASK):
+// voxel v's set is list[start[v], start[v] + count[v]) of sim-local indices, ijk the
+// sim's lattice indices (3 per voxel).
+vx3h_status uploadNearMasks(vx3h_batch* b, int v0, int nv, const int* ijk, const std::vector<int>& start,
+                            const std::vector<int>& count, const std::vector<int>& list) {
+    std::vector<unsigned> m((size_t)VX3_NEAR_WORDS * nv, 0u);
+    for (int v = 0; v < nv; v++)
+        for (int i = 0; i < count[v]; i++) {
+            const int u = list[start[v] + i];
+            const int dx = ijk[3 * u] - ijk[3 * v], dy = ijk[3 * u + 1] - ijk[3 * v + 1], dz = ijk[3 * u + 2] - ijk[3 * v + 2];
+            if (dx < -5 || dx > 5 || dy < -5 || dy > 5 || dz < -5 || dz > 5) FAILV(VX3H_EINVAL, "nearby voxel beyond the mask's offsets");
+            const int bit = kHostNearBits.bit[(dx + 5) * 121 + (dy + 5) * 11 + (dz + 5)];
+            if (bit < 0) FAILV(VX3H_EINVAL, "nearby voxel beyond the mask's offsets");
+            m[(size_t)(bit >> 5) * nv + v] |= 1u << (bit & 31);
+        }
+    if (nv > 0)
+        for (int w = 0; w < VX3_NEAR_WORDS; w++)
+            HIPCHK(hipMemcpy(b->db.nearMask + (size_t)w * b->db.Vcap + v0, m.data() + (size_t)w * nv, (size_t)nv * 4,
+                             hipMemcpyHostToDevice));
+    return VX3H_OK;
+}
 
 StepOpts stepOpts(const vx3h_batch* b);
 vx3h_status placeLoads(vx3h_batch* b, bool scratch);
@@ -906,6 +930,12 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
             (st = poolAlloc(b, b->nearPool, nearN, near0)) || (st = poolAlloc(b, b->cellPool, NV, c0)))
             return st;
         h0 = hblk0 * VX3_COL_HBLOCK;
+        // nearby sets as lattice-offset masks too, where every member is within the mask's
+        // offsets: a hop radius of at most 5 and only lattice links (attach makes others)
+        if (hops <= VX3_NEAR_RADIUS && !attach) {
+            if ((st = uploadNearMasks(b, v0, NV, H.ijk.data(), nearStart, nearCount, nearList))) return st;
+            c.flags |= SC_NEARMASK;
+        }
         for (int& x : nearStart) x += near0;
         if (!nearList.empty()) HIPCHK(hipMemcpy(b->db.nearList + near0, nearList.data(), nearList.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b->db.nearStart + v0, nearStart.data(), NV * 4, hipMemcpyHostToDevice));
@@ -1416,6 +1446,11 @@ vx3h_status hostNearby(vx3h_batch* b, int s, SimCtl& c) {
         if (st) return st;
         c.near0 = sh.near0;
         c.nearCap = sh.nearN;
+    }
+    if (c.flags & SC_NEARMASK) {
+        std::vector<int> ijk(3 * (size_t)NV);
+        HIPCHK(hipMemcpy(ijk.data(), b->db.vijk + 3 * (size_t)sh.v0, (size_t)NV * 12, hipMemcpyDeviceToHost));
+        if (vx3h_status st = uploadNearMasks(b, sh.v0, NV, ijk.data(), start, count, list)) return st;
     }
     for (int& x : start) x += sh.near0;
     if (!list.empty()) HIPCHK(hipMemcpy(b->db.nearList + sh.near0, list.data(), list.size() * 4, hipMemcpyHostToDevice));
