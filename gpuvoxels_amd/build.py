@@ -44,6 +44,7 @@ UNITS = [
     ("vx3h_vxa.cpp", "cxx", []),
     ("vx3h_server.cpp", "cxx", []),
     ("vx3h_tune.cpp", "cxx", []),
+    ("vx3h_tile.cpp", "cxx", []),
 ]
 
 

@@ -83,5 +83,21 @@ template <typename T> struct DevBatch {
                               // -2: every simulation, resident ones included
 };
 
+// Tables of a tiled resident launch (one workgroup per tile, TileDesc); unused by
+// one-workgroup launches.
+struct TileArgs {
+    const TileDesc* tiles;   // per workgroup
+    const int* vox;          // per tile slot: sim-local voxel; owned slots carry bit 31 when another tile reads them
+    const int* link;         // per tile link: sim-local link; bit 31 = this tile writes it back
+    const int* ends;         // per tile link: LDS slots of its ends, negative | positive << 16
+    const int* xidx;         // per tile slot: its exchange record (written if owned, read if halo), -1 = none
+    void* xch;               // exchange areas, in the state precision
+    int* sync;               // per simulation: arrivals, divergence word (zero at launch start)
+    double* part;            // per simulation: centre-of-mass partial sums
+    long long spin;          // wall-clock ticks a tile waits at a barrier before failing the simulation
+    int repair;              // 1: run only the simulations whose last launch diverged (divergence word
+                             // = 1 + the launch step), up to that step's link pass
+};
+
 }  // namespace vx3
 #endif

@@ -814,6 +814,12 @@ __global__ void __launch_bounds__(256) k_copy_loads(const real* __restrict__ src
     for (int k = 0; k < 36; k++) dst[k * dstFs + v] = src[k * srcFs + v];
 }
 
+// p[i] = x for i in [0, n) step `stride` (a kernel, not a memset: callers may capture it in a graph)
+__global__ void __launch_bounds__(256) k_fill_ints(int* p, int n, int stride, int x) {
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) * stride;
+    if (i < n) p[i] = x;
+}
+
 // at most 128 VGPRs (four waves per SIMD): the voxel pass is a gather over the force
 // slots, latency-bound at the three waves its unconstrained allocation allows
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_voxels(DevBatch<real> b) { voxelPass<false>(b); }

@@ -37,6 +37,14 @@ hipError_t launchResident(const DevBatch<float>& b, hipStream_t s, int shape, bo
                           int nSteps, hipEvent_t* ev = nullptr);
 hipError_t launchResident(const DevBatch<double>& b, hipStream_t s, int shape, bool collisions, const int* simList, int nSims,
                           int nSteps, hipEvent_t* ev = nullptr);
+// Launches the tiled resident kernel of shape `shape` (no collisions) over the tiles
+// ta.tiles[0, nTiles) after clearing ta.sync[0, nSync); every tile is resident at once.
+hipError_t launchTiled(const DevBatch<float>& b, hipStream_t s, int shape, const TileArgs& ta, int nTiles, int nSync,
+                       int nSteps, hipEvent_t* ev = nullptr);
+hipError_t launchTiled(const DevBatch<double>& b, hipStream_t s, int shape, const TileArgs& ta, int nTiles, int nSync,
+                       int nSteps, hipEvent_t* ev = nullptr);
+hipError_t tiledOccupancy(const DevBatch<float>& b, int shape, int* perCU);
+hipError_t tiledOccupancy(const DevBatch<double>& b, int shape, int* perCU);
 void launchDt(const DevBatch<float>& b, hipStream_t s, int s0, int s1);
 void launchDt(const DevBatch<double>& b, hipStream_t s, int s0, int s1);
 // copies the link end loads of voxels [0, nv) from src (slot stride srcFs) to dst (slot

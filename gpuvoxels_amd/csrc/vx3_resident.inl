@@ -32,6 +32,22 @@
 // stop the simulation in place (SC_COL_GROW); the host then moves it to the per-step
 // kernels.
 //
+// Tiled (TILED): a simulation too large for one workgroup runs on several, one per tile
+// (TileDesc).  A tile holds in LDS the poses of the voxels it owns and of its halo (the
+// other ends of its owned voxels' links), computes every link with an owned end and
+// integrates only its owned voxels; a link between two tiles is computed by both from
+// the same inputs, bit for bit, so no loads cross tiles.  After each step's voxel pass
+// a tile writes its boundary voxels' poses to the simulation's exchange area (one of
+// two, by step parity), arrives at the simulation's barrier and, once every tile has,
+// reads its halo back.  The centre-of-mass partial sums and a divergence word travel
+// through the same barrier, so every tile takes the same control decisions.  A link
+// that diverges in one tile is seen by the others only after the step's voxel pass:
+// the tiles then leave without writing anything back (global state is untouched until
+// the end of a launch), and a repair launch right after runs those simulations again
+// from the same start to the link pass of that step, where the one-workgroup kernel
+// stops.  A tile that waits at a barrier longer than TileArgs::spin wall-clock ticks
+// fails the simulation instead of waiting on.
+//
 // Included by the per-precision kernel units after vx3_kernels.inl (VX_NS, real).
 
 namespace vx3 {
@@ -45,7 +61,7 @@ namespace VX_NS {
 
 #define VX3_RES_PAIRS 8192   // contact partner entries (both directions) a resident sim may hold
 
-template <int BS, int VPT, int LPA, bool COL = false>
+template <int BS, int VPT, int LPA, bool COL = false, bool TILED = false>
 struct ResidentShape {
     static constexpr int VMAX = BS * VPT;   // voxels per sim
     static constexpr int LAX = BS * LPA;    // links per axis per sim
@@ -55,7 +71,8 @@ struct ResidentShape {
                                   + (size_t)LAX * 12 * sizeof(real)      // loads of one axis
                                   + (size_t)VMAX * 4 * sizeof(float)     // thermal factor, Poisson strain
                                   + (size_t)LAX * 6 * sizeof(float)      // end strains (axis, end, slot)
-                                  + (COL ? (size_t)VX3_RES_PAIRS * 2 : 0);  // contact partners
+                                  + (COL ? (size_t)VX3_RES_PAIRS * 2 : 0)   // contact partners
+                                  + (TILED ? (size_t)VMAX * 4 : 0);         // exchange voxel per slot
     // setup tables alias the load buffer: voxel -> slot per direction, and (axis, slot) -> link
     static_assert((size_t)VMAX * 6 * 2 + (size_t)LAX * 3 * 2 <= (size_t)LAX * 12 * sizeof(real), "setup alias");
     // with collisions the load buffer also holds the rebuild's buckets (2 ints per bucket, one
@@ -128,9 +145,38 @@ __device__ __forceinline__ void ldsCopyWords(int* dst, const int* src, int words
     for (int i = tid; i < words; i += bs) dst[i] = src[i];
 }
 
-template <int BS, int VPT, int LPA, bool COL>
-__global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* simList, int nSteps) {
-    typedef ResidentShape<BS, VPT, LPA, COL> Shape;
+// Data that crosses tiles is written and read with relaxed agent-scope atomics, which the
+// compiler issues coherent across the device's L2 caches; so no fence has to write back
+// or invalidate a whole L2 per step.  Ordering comes from waiting for every outstanding
+// memory operation (s_waitcnt 0: the stores have reached their point of coherence)
+// before the arrival is counted.
+template <typename T> __device__ __forceinline__ void xput(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T> __device__ __forceinline__ T xget(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One thread of each tile, after every thread of the tile has written its exchange data
+// and waited for it (s_waitcnt 0) and the workgroup has synchronized: arrive at the
+// simulation's barrier number `gen` (0-based) and wait until all n tiles have.  False
+// when the wait exceeded `spin` ticks.
+__device__ __forceinline__ bool tileBarrier(int* ctr, int n, unsigned gen, long long spin) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int target = n * (int)(gen + 1);
+    const long long t0 = wall_clock64();
+    while (xget(ctr) < target) {
+        if (wall_clock64() - t0 > spin) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+template <int BS, int VPT, int LPA, bool COL, bool TILED = false>
+__global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* simList, int nSteps, TileArgs ta) {
+    static_assert(!(COL && TILED), "tiled simulations have no collisions");
+    typedef ResidentShape<BS, VPT, LPA, COL, TILED> Shape;
     constexpr int VMAX = Shape::VMAX, LAX = Shape::LAX;
     extern __shared__ __align__(16) unsigned char smem[];
     VoxMatDev* sVM = (VoxMatDev*)smem;
@@ -143,6 +189,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     float* sPsx = sTf + VMAX; float* sPsy = sPsx + VMAX; float* sPsz = sPsy + VMAX;
     float* sEs = sPsz + VMAX;                   // [(axis*2 + end)][LAX]
     unsigned short* sNb = (unsigned short*)(sEs + 6 * LAX);   // contact partners (COL)
+    int* sXv = (int*)(sEs + 6 * LAX);   // TILED: per slot its exchange record (owned: another tile reads
+                                        // it; halo: its owner's), else -1
     unsigned short* sVL = (unsigned short*)sF;  // [6][VMAX] slot of the link in direction d (setup only)
     unsigned short* sSL = sVL + 6 * VMAX;       // [3][LAX]  link of (axis, slot) (setup only)
     __shared__ int sRun, sDiv, sCnt[3], sExp[3], sExpAt[3];
@@ -155,12 +203,26 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     __shared__ float sDm[VX3_RES_VMATS];        // CVX_Voxel::dampingMultiplier per voxel material (dt is fixed)
     __shared__ int sStale[2], sColOvf;          // COL: a rebuild is due (per step parity); the pair table overflowed
     __shared__ int sScan[BS / 64];
+    __shared__ int sTile;                       // TILED: 1 = some tile diverged this step, 2 = a barrier timed out
 
-    const int s = simList[blockIdx.x];
+    TileDesc td{};
+    if (TILED) td = ta.tiles[blockIdx.x];
+    const int s = TILED ? td.sim : simList[blockIdx.x];
     SimCtl& scg = b.sims[s];
     const int tid = threadIdx.x;
+    int divAt = -1;      // TILED: the launch's step at which a link diverged (a repair launch stops there)
+    unsigned gen = 0;    // TILED: barriers passed in this launch
     if (scg.state != 0 || (scg.flags & SC_COL_GROW)) return;   // finished, or waiting to move to the per-step kernels
+    if (TILED && ta.repair) {   // only simulations whose previous launch saw a divergence run again
+        const int dw = ta.sync[td.cOff + 1];
+        if (!dw) return;
+        divAt = dw - 1;
+    }
     const int v0 = scg.v0, nv = scg.nv, l0 = scg.l0, nl = scg.nl;
+    // the voxels this workgroup integrates, the LDS pose slots it fills, the links it computes
+    const int nvOwn = TILED ? td.nOwn : nv, nvAll = TILED ? td.nOwn + td.nHalo : nv, nlT = TILED ? td.nLinks : nl;
+    auto gvox = [&](int slot) { return v0 + (TILED ? ta.vox[td.vOff + slot] & 0x7fffffff : slot); };
+    auto glink = [&](int j) { return l0 + (TILED ? ta.link[td.lOff + j] & 0x7fffffff : j); };
     const double dt = scg.dt;
     const float dtf = scg.dt;
     const real nom = (real)scg.voxSize;
@@ -192,8 +254,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         const real x = b.a2x[g], y = b.a2y[g], z = b.a2z[g];
         return !(b.lflags[g] & LF_SMALL_ANGLE) || x * x + y * y + z * z > (real)0.009;
     };
-    for (int l = tid; l < nl; l += BS) {
-        const int g = l0 + l;
+    for (int l = tid; l < nlT; l += BS) {
+        const int g = glink(l);
         if (expensive(g)) atomicAdd(&sExp[b.lmeta[g] & 3], 1);
     }
     __syncthreads();
@@ -208,12 +270,12 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         __shared__ int sWc[NW][6];
         const int lane = tid & 63, w = tid >> 6;
         const unsigned long long below = (1ull << lane) - 1;
-        for (int base = 0; base < nl; base += BS) {
+        for (int base = 0; base < nlT; base += BS) {
             const int l = base + tid;
             int ax = 0, cls = -1;
             bool ex = false;
-            if (l < nl) {
-                const int g = l0 + l;
+            if (l < nlT) {
+                const int g = glink(l);
                 ax = b.lmeta[g] & 3;
                 ex = expensive(g);
                 cls = 2 * ax + (ex ? 0 : 1);
@@ -229,10 +291,11 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             if (cls >= 0) {
                 for (int i = 0; i < w; i++) rank += sWc[i][cls];
                 const int slot = ex ? sExpAt[ax] + rank : sExp[ax] + sCnt[ax] + rank;
-                const int g = l0 + l;
+                const int g = glink(l);
+                const int e = TILED ? ta.ends[td.lOff + l] : (b.lneg[g] - v0) | ((b.lpos[g] - v0) << 16);
                 sSL[ax * LAX + slot] = (unsigned short)l;
-                sVL[(2 * ax) * VMAX + (b.lneg[g] - v0)] = (unsigned short)slot;
-                sVL[(2 * ax + 1) * VMAX + (b.lpos[g] - v0)] = (unsigned short)slot;
+                sVL[(2 * ax) * VMAX + (e & 0xFFFF)] = (unsigned short)slot;
+                sVL[(2 * ax + 1) * VMAX + (e >> 16)] = (unsigned short)slot;
             }
             __syncthreads();
             if (tid < 3) {
@@ -270,11 +333,10 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             lEnds[a][k] = -1;
             if (slot < sCnt[a]) {
                 const int l = sSL[a * LAX + slot];
-                const int g = l0 + l;
-                const int vn = b.lneg[g] - v0, vq = b.lpos[g] - v0;
+                const int g = glink(l);
                 lIdx[a][k] = l;
-                lEnds[a][k] = vn | (vq << 16);
-                lMats[a][k] = localLmat(b.lmeta[g] >> 2) | (localVmat(b.vmat[v0 + vn]) << 8) | (localVmat(b.vmat[v0 + vq]) << 16);
+                lEnds[a][k] = TILED ? ta.ends[td.lOff + l] : (b.lneg[g] - v0) | ((b.lpos[g] - v0) << 16);
+                lMats[a][k] = localLmat(b.lmeta[g] >> 2) | (localVmat(b.vmat[b.lneg[g]]) << 8) | (localVmat(b.vmat[b.lpos[g]]) << 16);
                 ls[a][k].pos2 = V3<real>(b.q2x[g], b.q2y[g], b.q2z[g]);
                 ls[a][k].a1v = V3<real>(b.a1x[g], b.a1y[g], b.a1z[g]);
                 ls[a][k].a2v = V3<real>(b.a2x[g], b.a2y[g], b.a2z[g]);
@@ -296,10 +358,10 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         long long tab = 0;
         int n = 0;
         bool over = false;
-        for (int base = 0; base < nv && !over; base += 64) {
+        for (int base = 0; base < nvOwn && !over; base += 64) {
             const int v = base + tid;
-            const long long ph = v < nv ? __double_as_longlong(b.phase[v0 + v]) : 0;
-            bool pending = v < nv;
+            const long long ph = v < nvOwn ? __double_as_longlong(b.phase[gvox(v)]) : 0;
+            bool pending = v < nvOwn;
             while (!over && __any(pending)) {
                 const int src = __ffsll((unsigned long long)__ballot(pending)) - 1;   // lowest pending lane
                 const long long cand = __shfl(ph, src, 64);
@@ -335,8 +397,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     for (int k = 0; k < VPT; k++) {
         const int v = tid + k * BS;
         vmat[k] = -1;
-        if (v < nv) {
-            const int g = v0 + v;
+        if (v < nvOwn) {
+            const int g = gvox(v);
             vm[k] = V3<real>(b.mx[g], b.my[g], b.mz[g]);
             va[k] = V3<real>(b.ax[g], b.ay[g], b.az[g]);
             vf[k] = b.vflags[g];
@@ -359,11 +421,13 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         }
     }
     __syncthreads();   // setup tables alias the load buffer: done with them
+    if (TILED)
+        for (int v = tid; v < VMAX; v += BS) sXv[v] = v < nvAll ? ta.xidx[td.vOff + v] : -1;
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         const int v = tid + k * BS;
-        if (v < nv) {
-            const int g = v0 + v;
+        if (v < nvAll) {
+            const int g = gvox(v);
             sPx[v] = b.px[g]; sPy[v] = b.py[g]; sPz[v] = b.pz[g];
             sOw[v] = b.ow[g]; sOx[v] = b.ox[g]; sOy[v] = b.oy[g]; sOz[v] = b.oz[g];
             sTf[v] = b.tfac[g];
@@ -378,8 +442,9 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                 if (lEnds[a][k] < 0) continue;
                 const int slot = tid + k * BS;
                 const int vn = lEnds[a][k] & 0xFFFF, vq = lEnds[a][k] >> 16;
-                sEs[(2 * a) * LAX + slot] = b.es[(2 * a) * VC + v0 + vn];
-                sEs[(2 * a + 1) * LAX + slot] = b.es[(2 * a + 1) * VC + v0 + vq];
+                const int gn = TILED ? b.lneg[glink(lIdx[a][k])] : v0 + vn, gq = TILED ? b.lpos[glink(lIdx[a][k])] : v0 + vq;
+                sEs[(2 * a) * LAX + slot] = b.es[(2 * a) * VC + gn];
+                sEs[(2 * a + 1) * LAX + slot] = b.es[(2 * a + 1) * VC + gq];
             }
         }
     }
@@ -602,7 +667,9 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             }
             __syncthreads();
         }
-        const bool diverged = sDiv != 0;   // written before the last link-pass barrier
+        // written before the last link-pass barrier; a tile learns of its simulation's
+        // divergence only at the step's barrier and stops at that step on a second run
+        const bool diverged = TILED ? it == divAt : sDiv != 0;
         steps++;
         const bool sample = !diverged && cycleStep > 0 && steps % cycleStep == 0;
         double psx = 0, psy = 0, psz = 0, psm = 0;
@@ -685,12 +752,83 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                 psz += __shfl_xor(psz, o, 64); psm += __shfl_xor(psm, o, 64);
             }
             if ((tid & 63) == 0) { sRed[0][tid >> 6] = psx; sRed[1][tid >> 6] = psy; sRed[2][tid >> 6] = psz; sRed[3][tid >> 6] = psm; }
+        }
+        if (TILED && !diverged) {
+            // ---- the simulation's barrier: boundary poses out, halo poses in (TILED)
+            const int par = gen & 1;
+            real* X = (real*)ta.xch + td.xOff + (size_t)par * td.xPar;
+#pragma unroll 1
+            for (int v = tid; v < nvOwn; v += BS) {   // a tile's records are consecutive, in slot order
+                const int x = sXv[v];
+                if (x < 0) continue;
+                // all values into registers first: the scoped stores keep their order, and a
+                // load between two of them would wait for the LDS each time
+                real q[VX3_XCH_STRIDE - 1] = {sPx[v], sPy[v], sPz[v], sOw[v], sOx[v], sOy[v], sOz[v],
+                                              (real)sTf[v], (real)sPsx[v], (real)sPsy[v], (real)sPsz[v]};
+                real* r = X + (size_t)x * VX3_XCH_STRIDE;
+#pragma unroll
+                for (int f = 0; f < VX3_XCH_STRIDE - 1; f++) xput(r + f, q[f]);
+            }
+            __builtin_amdgcn_s_waitcnt(0);   // this thread's exchange stores are complete
             __syncthreads();
+            if (tid == 0) {
+                double* P = ta.part + td.pOff + (size_t)par * td.nTiles * 4;
+                if (sample) {
+                    double tot[4] = {0, 0, 0, 0};
+                    for (int w = 0; w < BS / 64; w++)
+                        for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
+                    for (int q = 0; q < 4; q++) xput(P + 4 * td.tile + q, tot[q]);
+                }
+                int* sync = ta.sync + td.cOff;
+                if (sDiv) __hip_atomic_fetch_max(sync + 1, it + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                int flag = 0;
+                if (!tileBarrier(sync, td.nTiles, gen, ta.spin)) flag = 2;
+                else if (divAt < 0 && xget(sync + 1)) flag = 1;
+                sTile = flag;
+            }
+            gen++;
+            __syncthreads();
+            if (sTile == 2) {   // a tile did not arrive: the simulation fails, its state stays as it was
+                if (tid == 0) scg.state = 4;
+                return;
+            }
+            if (sTile == 1) return;   // state untouched: the repair launch runs it to this step's link pass
+            if (sample) {   // the tiles' partial sums, read in parallel, added in tile order by one thread
+                const double* P = ta.part + td.pOff + (size_t)par * td.nTiles * 4;
+                double* sP = (double*)sF;   // the load buffer is free between the voxel and link passes
+                for (int j = tid; j < td.nTiles; j += BS) {
+                    const double a0 = xget(P + 4 * j), a1 = xget(P + 4 * j + 1), a2 = xget(P + 4 * j + 2),
+                                 a3 = xget(P + 4 * j + 3);
+                    sP[4 * j] = a0; sP[4 * j + 1] = a1; sP[4 * j + 2] = a2; sP[4 * j + 3] = a3;
+                }
+                __syncthreads();
+                if (tid == 0) {   // the same totals in every tile
+                    for (int q = 0; q < 4; q++) sRed[q][0] = 0;
+                    for (int j = 0; j < td.nTiles; j++)
+                        for (int q = 0; q < 4; q++) sRed[q][0] += sP[4 * j + q];
+                }
+            }
+#pragma unroll 1
+            for (int v = nvOwn + tid; v < nvAll; v += BS) {
+                // every load issued before the first LDS write: one memory latency, not eleven
+                const real* r = X + (size_t)sXv[v] * VX3_XCH_STRIDE;
+                real q[VX3_XCH_STRIDE - 1];
+#pragma unroll
+                for (int f = 0; f < VX3_XCH_STRIDE - 1; f++) q[f] = xget(r + f);
+                sPx[v] = q[0]; sPy[v] = q[1]; sPz[v] = q[2];
+                sOw[v] = q[3]; sOx[v] = q[4]; sOy[v] = q[5]; sOz[v] = q[6];
+                sTf[v] = (float)q[7]; sPsx[v] = (float)q[8]; sPsy[v] = (float)q[9]; sPsz[v] = (float)q[10];
+            }
+        } else if (sample) __syncthreads();
+        if (sample) {
             angleSamples++;
             if (tid == 0) {
                 double tot[4] = {0, 0, 0, 0};
-                for (int w = 0; w < BS / 64; w++) {
-                    for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
+                if (TILED) {
+                    for (int q = 0; q < 4; q++) tot[q] = sRed[q][0];
+                } else {
+                    for (int w = 0; w < BS / 64; w++)
+                        for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
                 }
                 double* A = sComH0; double* B = sComH1; double* C = sCom;
                 for (int q = 0; q < 3; q++) { A[q] = B[q]; B[q] = C[q]; C[q] = tot[3] == 0 ? 0.0 : (1.0 / tot[3]) * tot[q]; }
@@ -722,7 +860,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
 #pragma unroll
         for (int k = 0; k < LPA; k++) {
             if (lEnds[a][k] < 0) continue;
-            const int g = l0 + lIdx[a][k];
+            if (TILED && !(ta.link[td.lOff + lIdx[a][k]] & 0x80000000)) continue;   // the other tile writes it
+            const int g = glink(lIdx[a][k]);
             b.q2x[g] = ls[a][k].pos2.x; b.q2y[g] = ls[a][k].pos2.y; b.q2z[g] = ls[a][k].pos2.z;
             b.a1x[g] = ls[a][k].a1v.x; b.a1y[g] = ls[a][k].a1v.y; b.a1z[g] = ls[a][k].a1v.z;
             b.a2x[g] = ls[a][k].a2v.x; b.a2y[g] = ls[a][k].a2v.y; b.a2z[g] = ls[a][k].a2v.z;
@@ -733,15 +872,16 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             if (poisson) {
                 const int slot = tid + k * BS;
                 const int vn = lEnds[a][k] & 0xFFFF, vq = lEnds[a][k] >> 16;
-                b.es[(2 * a) * VC + v0 + vn] = sEs[(2 * a) * LAX + slot];
-                b.es[(2 * a + 1) * VC + v0 + vq] = sEs[(2 * a + 1) * LAX + slot];
+                const int gn = TILED ? b.lneg[g] : v0 + vn, gq = TILED ? b.lpos[g] : v0 + vq;
+                b.es[(2 * a) * VC + gn] = sEs[(2 * a) * LAX + slot];
+                b.es[(2 * a + 1) * VC + gq] = sEs[(2 * a + 1) * LAX + slot];
             }
         }
     }
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         if (vmat[k] < 0) continue;
-        const int v = tid + k * BS, g = v0 + v;
+        const int v = tid + k * BS, g = gvox(v);
         b.px[g] = sPx[v]; b.py[g] = sPy[v]; b.pz[g] = sPz[v];
         b.mx[g] = vm[k].x; b.my[g] = vm[k].y; b.mz[g] = vm[k].z;
         b.ax[g] = va[k].x; b.ay[g] = va[k].y; b.az[g] = va[k].z;
@@ -762,7 +902,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             for (int i = 0; i < nbC[k]; i++) b.nbList[nb0 + nbS[k] + i] = v0 + sNb[nbS[k] + i];
         }
     }
-    if (tid == 0) {
+    if (tid == 0 && (!TILED || td.tile == 0)) {   // every tile holds the same control values
         if (COL) {
             int f = scg.flags;
             if (rebuilds > 0 && !colOvf) f &= ~SC_REBUILD;

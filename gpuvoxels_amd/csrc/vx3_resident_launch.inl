@@ -27,9 +27,60 @@ hipError_t launchResidentAs(const DevBatch<VX_REAL>& b, hipStream_t s, int shape
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
         if (e != hipSuccess) return e;                                                                           \
         if (ev) hipEventRecord(ev[0], s);                                                                        \
-        hipLaunchKernelGGL(kern, dim3(nSims), dim3(B), lds, s, b, simList, nSteps);                              \
+        hipLaunchKernelGGL(kern, dim3(nSims), dim3(B), lds, s, b, simList, nSteps, TileArgs{});                              \
         if (ev) hipEventRecord(ev[1], s);                                                                        \
         return hipGetLastError();                                                                                \
+    }
+    VX3_RES_SHAPES(VX3_CASE)
+#undef VX3_CASE
+    return hipErrorInvalidValue;
+}
+
+// Tiled launches: one workgroup per tile of the simulations in ta.tiles[0, nTiles), all
+// resident at once (a cooperative launch), after the simulations' sync words
+// ta.sync[0, nSync) are cleared.
+hipError_t launchTiled(const DevBatch<VX_REAL>& b, hipStream_t s, int shape, const TileArgs& ta, int nTiles, int nSync,
+                       int nSteps, hipEvent_t* ev) {
+    if (nTiles <= 0 || nSteps <= 0) return hipSuccess;
+    int i = 0;
+#define VX3_CASE(B, V, L)                                                                                        \
+    if (shape == i++) {                                                                                          \
+        auto kern = VX_NS::k_resident<B, V, L, false, true>;                                                     \
+        const size_t lds = VX_NS::ResidentShape<B, V, L, false, true>::bytes;                                    \
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        if (e != hipSuccess) return e;                                                                           \
+        DevBatch<VX_REAL> bb = b;                                                                                \
+        const int* none = nullptr;                                                                               \
+        int steps = nSteps;                                                                                      \
+        TileArgs t = ta;                                                                                         \
+        void* args[] = {(void*)&bb, (void*)&none, (void*)&steps, (void*)&t};                                     \
+        for (int pass = 0; pass < 2; pass++) {   /* the launch, then its repair launch */                        \
+            t.repair = pass;                                                                                     \
+            /* arrivals (and, before the launch, divergence words) start at zero */                             \
+            hipLaunchKernelGGL(VX_NS::k_fill_ints, dim3((nSync + 255) / 256), dim3(256), 0, s, ta.sync, nSync,   \
+                               pass ? 2 : 1, 0);                                                                 \
+            if (ev && !pass) hipEventRecord(ev[0], s);                                                           \
+            e = hipLaunchCooperativeKernel((const void*)kern, dim3(nTiles), dim3(B), args, (unsigned)lds, s);    \
+            if (ev && !pass) hipEventRecord(ev[1], s);                                                           \
+            if (e != hipSuccess) return e;                                                                       \
+        }                                                                                                        \
+        return hipGetLastError();                                                                                \
+    }
+    VX3_RES_SHAPES(VX3_CASE)
+#undef VX3_CASE
+    return hipErrorInvalidValue;
+}
+
+// workgroups of shape `shape`'s tiled kernel that one compute unit holds at once
+hipError_t tiledOccupancy(const DevBatch<VX_REAL>&, int shape, int* perCU) {
+    int i = 0;
+#define VX3_CASE(B, V, L)                                                                                        \
+    if (shape == i++) {                                                                                          \
+        auto kern = VX_NS::k_resident<B, V, L, false, true>;                                                     \
+        const size_t lds = VX_NS::ResidentShape<B, V, L, false, true>::bytes;                                    \
+        hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+        if (e != hipSuccess) return e;                                                                           \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(perCU, (const void*)kern, B, lds);                   \
     }
     VX3_RES_SHAPES(VX3_CASE)
 #undef VX3_CASE

@@ -170,6 +170,23 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     int atOff[5], atLen[5];
 };
 
+// One tile of a simulation that the resident kernel runs on several workgroups (tiled
+// launch): the tile owns a compact block of the simulation's voxels and computes every
+// link with an end among them; the other ends (its halo) are read from the exchange
+// area after each step's barrier.  A link whose ends two tiles own is computed by both
+// from the same inputs, so both keep the same state; one of them writes it back.
+struct TileDesc {
+    int sim;                         // simulation slot
+    int tile, nTiles;                // index of this tile in its simulation, and their number
+    int vOff, nOwn, nHalo;           // tileVox[vOff ..): nOwn owned voxels then nHalo halo voxels
+    int lOff, nLinks;                // tileLink / tileEnds[lOff .. lOff + nLinks)
+    int xOff, xPar;                  // exchange area from xOff: two step parities of xPar reals, each
+                                     // VX3_XCH_STRIDE reals per voxel another tile reads
+    int cOff;                        // sync words [cOff, cOff + 2): arrivals, divergence
+    int pOff;                        // centre-of-mass partial sums: [2][nTiles][4] doubles from pOff
+};
+#define VX3_XCH_STRIDE 12            // position (3), orientation (4), thermal factor, Poisson strains (3), pad
+
 #define VX3_HIST 500                 // HISTORY_SIZE (VoxCAD/src/VX1/VX_Sim.h)
 
 struct TokenDev { int op; int pad; double value; };

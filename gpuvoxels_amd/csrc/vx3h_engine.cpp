@@ -23,6 +23,7 @@
 #include "vx3_launch.h"
 #include "vx3h_build.h"
 #include "vx3h_internal.h"
+#include "vx3h_tile.h"
 
 namespace vx3 {
 thread_local std::string g_err;
@@ -106,7 +107,9 @@ struct SlotHost {
     int v0 = 0, nv = 0, l0 = 0, nl = 0;
     int nlCap = 0;               // link range size: nl, or room for the links attach makes
     int nlSeen = 0;              // links known to the host shadows below (nl + attached ones read back)
-    int shape = -1;              // resident kernel shape, -1 = stream path
+    int shape = -1;              // resident kernel shape, -1 = stream path (or tiled)
+    int tiles = 0;               // > 0: runs on the tiled resident kernel as this many workgroups
+    TilePlan tplan;              // its tiles (sim-local)
     std::vector<int> ijk;
     std::vector<int> matid;
     std::vector<int> measured;   // per voxel
@@ -188,6 +191,27 @@ struct vx3h_batch {
     std::vector<size_t> capRes;
     bool resDirty = false;
     int nStreamSims = 0;
+    // tiled resident path: simulations too large for one workgroup run as several tiles,
+    // packed into cooperative launches of at most tileCap workgroups (whole simulations),
+    // one after another on tileStream beside the other paths
+    int tiledShape = -1;                         // shape of the tiled kernel: the largest voxel capacity
+    int tileCap = 0;                             // workgroups of it the device holds at once (0: no tiling)
+    long long spinTicks = 0;                     // a tile's longest barrier wait, in wall-clock ticks
+    std::vector<int> tiledSims;                  // slots, in admission order
+    bool tileDirty = false;
+    struct TileGroup { int first, count, sync0, nSync; };
+    std::vector<TileGroup> tileGroups;
+    TileDesc* dTiles = nullptr; size_t capTiles = 0;
+    int* dTileVox = nullptr; size_t capTileVox = 0;
+    int* dTileLink = nullptr; size_t capTileLink = 0;
+    int* dTileEnds = nullptr; size_t capTileEnds = 0;
+    int* dTileXidx = nullptr; size_t capTileXidx = 0;
+    void* dXch = nullptr; size_t capXch = 0;     // bytes
+    int* dSync = nullptr; size_t capSync = 0;
+    double* dPart = nullptr; size_t capPart = 0;
+    hipStream_t tileStream = nullptr;
+    hipEvent_t tileJoin = nullptr;
+    bool tilePending = false;
     // one stream per resident shape: each shape's launches run beside the stream-path
     // steps of the main stream (disjoint simulations) and beside the other shapes;
     // fork/join events order them with the main stream around every chunk of steps
@@ -421,6 +445,91 @@ vx3h_status syncResLists(vx3h_batch* b) {
     return VX3H_OK;
 }
 
+// grows a device array to at least n elements (contents not kept)
+template <typename T> vx3h_status growDev(T** p, size_t* cap, size_t n) {
+    if (n <= *cap) return VX3H_OK;
+    if (*p) HIPCHK(hipFree(*p));
+    *p = nullptr;
+    const size_t c = std::max(n, *cap + *cap / 2);
+    HIPCHK(hipMalloc((void**)p, c * sizeof(T)));
+    *cap = c;
+    return VX3H_OK;
+}
+
+// The tiled simulations' tables on the device, after a simulation joined or left: whole
+// simulations packed into launch groups of at most tileCap tiles (first fit, largest
+// first: a launch lasts as long per step whether full or not); per simulation an
+// exchange area (two step parities of VX3_XCH_STRIDE reals per record), two sync words
+// (per group, from the group's base) and the partial sums.  None of it carries state between
+// launches.
+vx3h_status syncTiles(vx3h_batch* b) {
+    if (!b->tileDirty) return VX3H_OK;
+    std::vector<int> order = b->tiledSims;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->slots[x].tiles > b->slots[y].tiles; });
+    std::vector<std::vector<int>> bins;
+    std::vector<int> fill;
+    for (int s : order) {
+        size_t i = 0;
+        while (i < bins.size() && fill[i] + b->slots[s].tiles > b->tileCap) i++;
+        if (i == bins.size()) { bins.emplace_back(); fill.push_back(0); }
+        bins[i].push_back(s);
+        fill[i] += b->slots[s].tiles;
+    }
+    std::vector<TileDesc> T;
+    std::vector<int> vox, link, ends, xidx;
+    b->tileGroups.clear();
+    size_t xoff = 0, poff = 0;
+    int syncTot = 0;
+    for (const auto& bin : bins) {
+        vx3h_batch::TileGroup g{(int)T.size(), 0, syncTot, 0};
+        for (int s : bin) {
+            const SlotHost& sh = b->slots[s];
+            const int n = sh.tiles;
+            const int vb = (int)vox.size(), lb = (int)link.size();
+            for (TileDesc d : sh.tplan.tiles) {
+                d.sim = s;
+                d.vOff += vb;
+                d.lOff += lb;
+                d.xOff = (int)xoff;
+                d.xPar = VX3_XCH_STRIDE * sh.tplan.nPub;
+                d.cOff = g.nSync;
+                d.pOff = (int)poff;
+                T.push_back(d);
+            }
+            vox.insert(vox.end(), sh.tplan.vox.begin(), sh.tplan.vox.end());
+            link.insert(link.end(), sh.tplan.link.begin(), sh.tplan.link.end());
+            ends.insert(ends.end(), sh.tplan.ends.begin(), sh.tplan.ends.end());
+            xidx.insert(xidx.end(), sh.tplan.xidx.begin(), sh.tplan.xidx.end());
+            xoff += (size_t)2 * VX3_XCH_STRIDE * sh.tplan.nPub;
+            poff += (size_t)8 * n;
+            g.count += n;
+            g.nSync += 2;
+            syncTot += 2;
+        }
+        b->tileGroups.push_back(g);
+    }
+    if (xoff > (size_t)INT32_MAX) FAILV(VX3H_ENOMEM, "tiled simulations' exchange areas exceed 2^31 values");
+    vx3h_status st;
+    if ((st = growDev(&b->dTiles, &b->capTiles, T.size())) || (st = growDev(&b->dTileVox, &b->capTileVox, vox.size())) ||
+        (st = growDev(&b->dTileLink, &b->capTileLink, link.size())) ||
+        (st = growDev(&b->dTileEnds, &b->capTileEnds, ends.size())) ||
+        (st = growDev(&b->dTileXidx, &b->capTileXidx, xidx.size())) ||
+        (st = growDev((unsigned char**)&b->dXch, &b->capXch, xoff * b->realSize)) ||
+        (st = growDev(&b->dSync, &b->capSync, (size_t)syncTot)) || (st = growDev(&b->dPart, &b->capPart, poff)))
+        return st;
+    if (!T.empty()) HIPCHK(hipMemcpy(b->dTiles, T.data(), T.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
+    if (!vox.empty()) {
+        HIPCHK(hipMemcpy(b->dTileVox, vox.data(), vox.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->dTileXidx, xidx.data(), xidx.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (!link.empty()) {
+        HIPCHK(hipMemcpy(b->dTileLink, link.data(), link.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->dTileEnds, ends.data(), ends.size() * 4, hipMemcpyHostToDevice));
+    }
+    b->tileDirty = false;
+    return VX3H_OK;
+}
+
 vx3h_status takeEvent(vx3h_batch* b, hipEvent_t& e) {
     if (b->evPool.empty()) { HIPCHK(hipEventCreate(&e)); }
     else { e = b->evPool.back(); b->evPool.pop_back(); }
@@ -432,7 +541,7 @@ vx3h_status takeEvent(vx3h_batch* b, hipEvent_t& e) {
 // them.  Between the two the caller enqueues the stream-path steps on the main stream.
 vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
     vx3h_status st = syncResLists(b);
-    if (st) return st;
+    if (st || (st = syncTiles(b))) return st;
     bool forked = false;
     for (size_t i = 0; i < b->resLists.size(); i++) {
         const int n = (int)b->resLists[i].size();
@@ -465,6 +574,45 @@ vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
             : launchResident(*reinterpret_cast<DevBatch<double>*>(&b->db), rs, shape, col, b->dResLists[i], n, nSteps, evp);
         if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("resident launch failed: ") + hipGetErrorString(e));
     }
+    // tiled simulations: their launch groups one after another on the tile stream
+    if (!b->tileGroups.empty()) {
+        if (!forked) {
+            if (!b->forkEv) HIPCHK(hipEventCreateWithFlags(&b->forkEv, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(b->forkEv, b->stream));
+        }
+        if (!b->tileStream) {
+            HIPCHK(hipStreamCreateWithFlags(&b->tileStream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&b->tileJoin, hipEventDisableTiming));
+        }
+        HIPCHK(hipStreamWaitEvent(b->tileStream, b->forkEv, 0));
+        b->tilePending = true;
+        for (size_t gi = 0; gi < b->tileGroups.size(); gi++) {
+            const vx3h_batch::TileGroup& g = b->tileGroups[gi];
+            TileArgs ta{};
+            ta.tiles = b->dTiles + g.first;
+            ta.vox = b->dTileVox;
+            ta.link = b->dTileLink;
+            ta.ends = b->dTileEnds;
+            ta.xidx = b->dTileXidx;
+            ta.xch = b->dXch;
+            ta.sync = b->dSync + g.sync0;
+            ta.part = b->dPart;
+            ta.spin = b->spinTicks;
+            hipEvent_t pair[2];
+            hipEvent_t* evp = ev ? ev + 2 * (b->resLists.size() + gi) : nullptr;
+            if (!ev && b->timing) {
+                if ((st = takeEvent(b, pair[0])) || (st = takeEvent(b, pair[1]))) return st;
+                b->evUsed.push_back({pair[0], pair[1]});
+                b->timedSteps += nSteps;
+                evp = pair;
+            }
+            const hipError_t e = b->realSize == 4
+                ? launchTiled(b->db, b->tileStream, b->tiledShape, ta, g.count, g.nSync, nSteps, evp)
+                : launchTiled(*reinterpret_cast<DevBatch<double>*>(&b->db), b->tileStream, b->tiledShape, ta, g.count, g.nSync,
+                              nSteps, evp);
+            if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("tiled resident launch failed: ") + hipGetErrorString(e));
+        }
+    }
     return VX3H_OK;
 }
 
@@ -474,6 +622,11 @@ vx3h_status joinResident(vx3h_batch* b) {
         HIPCHK(hipEventRecord(b->joinEv[i], b->resStreams[i]));
         HIPCHK(hipStreamWaitEvent(b->stream, b->joinEv[i], 0));
         b->joinPending[i] = false;
+    }
+    if (b->tilePending) {
+        HIPCHK(hipEventRecord(b->tileJoin, b->tileStream));
+        HIPCHK(hipStreamWaitEvent(b->stream, b->tileJoin, 0));
+        b->tilePending = false;
     }
     return VX3H_OK;
 }
@@ -522,6 +675,20 @@ extern "C" vx3h_status vx3h_batch_create(int device, const vx3h_batch_cfg* cfg, 
             std::stable_sort(b->shapePref.begin(), b->shapePref.end(), [&](int x, int y) {
                 return (b->shapes[x].block == b->cfg.block_size) > (b->shapes[y].block == b->cfg.block_size);
             });
+        // tiles take the preferred shape of the largest voxel capacity; a launch holds as many
+        // tiles as the device keeps resident at once (a cooperative launch)
+        int vmaxBest = 0;
+        for (int si : b->shapePref) vmaxBest = std::max(vmaxBest, b->shapes[si].block * b->shapes[si].vpt);
+        for (int si : b->shapePref)
+            if (b->shapes[si].block * b->shapes[si].vpt == vmaxBest) { b->tiledShape = si; break; }
+        int coop = 0, cus = 0, perCU = 0, khz = 0;
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
+        const hipError_t oe = b->realSize == 4 ? tiledOccupancy(b->db, b->tiledShape, &perCU)
+                                               : tiledOccupancy(*reinterpret_cast<DevBatch<double>*>(&b->db), b->tiledShape, &perCU);
+        b->tileCap = (coop && oe == hipSuccess && khz > 0) ? perCU * cus : 0;
+        b->spinTicks = (long long)khz * 1000 * 4;   // 4 s
     }
     size_t freeB = 0, totB = 0;
     if (hipMemGetInfo(&freeB, &totB) != hipSuccess) freeB = 0;
@@ -538,6 +705,11 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         if (b->resStreams[i]) { hipStreamSynchronize(b->resStreams[i]); hipStreamDestroy(b->resStreams[i]); }
         if (b->joinEv[i]) hipEventDestroy(b->joinEv[i]);
     }
+    if (b->tileStream) { hipStreamSynchronize(b->tileStream); hipStreamDestroy(b->tileStream); }
+    if (b->tileJoin) hipEventDestroy(b->tileJoin);
+    for (void* p : {(void*)b->dTiles, (void*)b->dTileVox, (void*)b->dTileLink, (void*)b->dTileEnds, (void*)b->dTileXidx,
+                    b->dXch, (void*)b->dSync, (void*)b->dPart})
+        if (p) hipFree(p);
     if (b->forkEv) hipEventDestroy(b->forkEv);
     if (b->graphExec) hipGraphExecDestroy(b->graphExec);
     if (b->graph) hipGraphDestroy(b->graph);
@@ -750,8 +922,21 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
             const int lax = r.block * r.lpa;
             if (NV <= r.block * r.vpt && perAxis[0] <= lax && perAxis[1] <= lax && perAxis[2] <= lax) { shape = si; break; }
         }
-        if (shape < 0 && b->cfg.path == VX3H_PATH_RESIDENT) FAILV(VX3H_EUNSUPPORTED, "simulation too large for the resident kernel");
     }
+    // too large for one workgroup: tiles of the tiled resident kernel, when they fit one launch
+    TilePlan tplan;
+    int tiles = 0;
+    if (b->cfg.path != VX3H_PATH_STREAM && NV > 0 && resOk && shape < 0 && !H.collisions && b->tiledShape >= 0 &&
+        b->tileCap > 1) {
+        const ResidentShapeInfo& r = b->shapes[b->tiledShape];
+        if (planTiles(NV, H.ijk.data(), NL, H.lneg.data(), H.lpos.data(), H.laxis.data(), r.block * r.vpt, r.block * r.lpa,
+                      b->tileCap, &tplan) &&
+            tplan.tiles.size() > 1)
+            tiles = (int)tplan.tiles.size();
+    }
+    if (b->cfg.path == VX3H_PATH_RESIDENT && NV > 0 && resOk && shape < 0 && !tiles)
+        FAILV(VX3H_EUNSUPPORTED, H.collisions ? "simulation with collisions too large for one workgroup of the resident kernel"
+                                              : "simulation too large for the resident kernel");
     if (shape < 0 && NV > 0 && b->cfg.path == VX3H_PATH_RESIDENT) {   // a forced resident path takes only what fits
         if (!resOk && (int)H.vmats.size() > RES_MAX_VMATS) FAILV(VX3H_EUNSUPPORTED, "too many voxel materials for the resident kernel");
         if (!resOk && (int)H.lmats.size() > RES_MAX_LMATS) FAILV(VX3H_EUNSUPPORTED, "too many link materials for the resident kernel");
@@ -778,7 +963,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     c.tInit = H.tInit;
     c.state = VX3H_SIM_RUNNING;
     c.maxSteps = H.maxSteps;
-    c.flags = H.flags | (shape >= 0 ? SC_RESIDENT : 0);
+    c.flags = H.flags | (shape >= 0 || tiles ? SC_RESIDENT : 0);
     c.v0 = v0; c.nv = NV; c.l0 = l0; c.nl = NL;
     c.stopOff = tok0;
     c.mat0 = mat0;
@@ -976,7 +1161,12 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN; sh.c0 = c0;
     sh.tg0 = tg0; sh.nTg = nTg;
     sh.at0 = at0; sh.ap0 = ap0; sh.apN = apN;
-    if (shape >= 0) {   // the kernel variant with contacts for a simulation with collisions
+    if (tiles) {
+        sh.tiles = tiles;
+        sh.tplan = std::move(tplan);
+        b->tiledSims.push_back(s);
+        b->tileDirty = true;
+    } else if (shape >= 0) {   // the kernel variant with contacts for a simulation with collisions
         sh.shape = shape + (H.collisions ? (int)b->shapes.size() : 0);
         b->resLists[sh.shape].push_back(s);
         b->resDirty = true;
@@ -1251,19 +1441,31 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
         avg_ms[k] = tot / n_steps;
     }
     for (auto& e : ev) hipEventDestroy(e);
-    // resident kernels: one launch of n_steps steps per shape in use; reports the launch durations' sum
-    std::vector<hipEvent_t> rev(2 * b->resLists.size(), nullptr);
+    // resident kernels: one launch of n_steps steps per shape in use and per tiled launch
+    // group; reports the launch durations' sum
+    vx3h_status st = syncTiles(b);
+    if (st) return st;
+    const size_t nl = b->resLists.size() + b->tileGroups.size();
+    std::vector<hipEvent_t> rev(2 * nl, nullptr);
     for (auto& e : rev) HIPCHK(hipEventCreate(&e));
-    vx3h_status st = launchResidentAll(b, n_steps, rev.data());
+    st = launchResidentAll(b, n_steps, rev.data());
     if (st || (st = joinResident(b))) return st;
     HIPCHK(hipStreamSynchronize(b->stream));
-    for (size_t i = 0; i < b->resLists.size(); i++) {
-        if (b->resLists[i].empty()) continue;
+    for (size_t i = 0; i < nl; i++) {
+        if (i < b->resLists.size() && b->resLists[i].empty()) continue;
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, rev[2 * i], rev[2 * i + 1]));
         avg_ms[5] += ms;
     }
     for (auto& e : rev) hipEventDestroy(e);
+    return VX3H_OK;
+}
+
+extern "C" vx3h_status vx3h_sim_workgroups(vx3h_batch* b, int slot, int* workgroups) {
+    if (!b || !workgroups) FAILV(VX3H_EINVAL, "null argument");
+    if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used) FAILV(VX3H_ENOTFOUND, "no such slot");
+    const SlotHost& sh = b->slots[slot];
+    *workgroups = sh.tiles ? sh.tiles : sh.shape >= 0 ? 1 : 0;
     return VX3H_OK;
 }
 
@@ -1561,7 +1763,10 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
         b->attPool.ra.release(sh.at0, 2 * sh.nv);
         b->pairPool.ra.release(sh.ap0, sh.apN);
     }
-    if (sh.shape >= 0) {
+    if (sh.tiles) {
+        b->tiledSims.erase(std::remove(b->tiledSims.begin(), b->tiledSims.end(), slot), b->tiledSims.end());
+        b->tileDirty = true;
+    } else if (sh.shape >= 0) {
         auto& L = b->resLists[sh.shape];
         L.erase(std::remove(L.begin(), L.end(), slot), L.end());
         b->resDirty = true;

@@ -1,0 +1,164 @@
+// vx3h_tile.cpp -- cutting a simulation into tiles for the tiled resident kernel.
+//
+// A tile's workgroup holds the poses of its owned voxels and of their link partners in
+// other tiles (the halo) in LDS, and computes every link with an owned end.  Compact
+// blocks keep the halo small: recursive bisection across the longest lattice extent,
+// splitting at the quantile that leaves each side a whole number of target blocks.
+#include "vx3h_tile.h"
+
+#include <algorithm>
+#include <climits>
+
+namespace vx3 {
+namespace {
+
+struct Planner {
+    int nv = 0, nl = 0;
+    const int *ijk = nullptr, *lneg = nullptr, *lpos = nullptr, *laxis = nullptr;
+    int vmax = 0, lax = 0;
+    std::vector<int> inc0, inc;   // links incident to each voxel (CSR)
+    std::vector<int> vmark, lmark;
+    int epoch = 0;
+    bool failed = false;
+    std::vector<std::vector<int>> parts;
+
+    void index() {
+        inc0.assign(nv + 1, 0);
+        for (int l = 0; l < nl; l++) { inc0[lneg[l] + 1]++; inc0[lpos[l] + 1]++; }
+        for (int v = 0; v < nv; v++) inc0[v + 1] += inc0[v];
+        inc.resize(inc0[nv]);
+        std::vector<int> at(inc0.begin(), inc0.end() - 1);
+        for (int l = 0; l < nl; l++) { inc[at[lneg[l]]++] = l; inc[at[lpos[l]]++] = l; }
+        vmark.assign(nv, -1);
+        lmark.assign(nl, -1);
+    }
+
+    int other(int l, int v) const { return lneg[l] == v ? lpos[l] : lneg[l]; }
+
+    // owned + halo voxels within vmax, links with an owned end within lax per axis
+    bool fits(const std::vector<int>& part) {
+        if ((int)part.size() > vmax) return false;
+        ++epoch;
+        const int own = 2 * epoch, halo = own + 1;
+        for (int v : part) vmark[v] = own;
+        int nh = 0, per[3] = {0, 0, 0};
+        for (int v : part)
+            for (int i = inc0[v]; i < inc0[v + 1]; i++) {
+                const int l = inc[i];
+                if (lmark[l] == epoch) continue;
+                lmark[l] = epoch;
+                per[laxis[l]]++;
+                const int u = other(l, v);
+                if (vmark[u] != own && vmark[u] != halo) { vmark[u] = halo; nh++; }
+            }
+        return (int)part.size() + nh <= vmax && per[0] <= lax && per[1] <= lax && per[2] <= lax;
+    }
+
+    // k: the number of blocks this part is meant to become
+    void cut(std::vector<int>& part, int k) {
+        if (failed || part.empty()) return;
+        if (k <= 1) {
+            if (fits(part)) { parts.push_back(part); return; }
+            if (part.size() == 1) { failed = true; return; }
+            k = 2;
+        }
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+        for (int v : part)
+            for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], ijk[3 * v + a]); hi[a] = std::max(hi[a], ijk[3 * v + a]); }
+        int ax = 0;
+        for (int a = 1; a < 3; a++) if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
+        std::sort(part.begin(), part.end(), [&](int x, int y) {
+            const int cx = ijk[3 * x + ax], cy = ijk[3 * y + ax];
+            return cx != cy ? cx < cy : x < y;
+        });
+        const int kl = k / 2;
+        size_t n = (size_t)((double)part.size() * kl / k + 0.5);
+        n = std::max<size_t>(1, std::min(n, part.size() - 1));
+        std::vector<int> left(part.begin(), part.begin() + n), right(part.begin() + n, part.end());
+        cut(left, kl);
+        cut(right, k - kl);
+    }
+};
+
+}  // namespace
+
+bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos, const int* laxis, int vmax, int lax,
+               int maxTiles, TilePlan* out) {
+    if (nv <= 0 || vmax <= 0 || lax <= 0 || maxTiles <= 0 || vmax > 0xFFFF) return false;
+    Planner p;
+    p.nv = nv; p.nl = nl; p.ijk = ijk; p.lneg = lneg; p.lpos = lpos; p.laxis = laxis; p.vmax = vmax; p.lax = lax;
+    p.index();
+    std::vector<std::vector<int>> best;
+    for (double frac : {0.8, 0.72, 0.64, 0.56, 0.48, 0.4}) {
+        const int target = std::max(1, (int)(frac * vmax));
+        std::vector<int> all(nv);
+        for (int v = 0; v < nv; v++) all[v] = v;
+        p.parts.clear();
+        p.failed = false;
+        p.cut(all, (nv + target - 1) / target);
+        if (p.failed) continue;
+        if (best.empty() || p.parts.size() < best.size()) best = p.parts;
+    }
+    if (best.empty() || (int)best.size() > maxTiles) return false;
+
+    const int T = (int)best.size();
+    std::vector<int> owner(nv, -1);
+    for (int t = 0; t < T; t++) {
+        std::sort(best[t].begin(), best[t].end());
+        for (int v : best[t]) owner[v] = t;
+    }
+    out->tiles.clear(); out->vox.clear(); out->link.clear(); out->ends.clear(); out->xidx.clear();
+    std::vector<int> slot(nv, -1);
+    std::vector<int> rec(nv, -1);   // exchange record of each voxel another tile reads
+    int nPub = 0;
+    for (int t = 0; t < T; t++) {
+        const std::vector<int>& own = best[t];
+        std::vector<int> halo, links;
+        for (int v : own)
+            for (int i = p.inc0[v]; i < p.inc0[v + 1]; i++) {
+                const int l = p.inc[i];
+                links.push_back(l);
+                const int u = p.other(l, v);
+                if (owner[u] != t) halo.push_back(u);
+            }
+        std::sort(halo.begin(), halo.end());
+        halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+        std::sort(links.begin(), links.end());
+        links.erase(std::unique(links.begin(), links.end()), links.end());
+        TileDesc d{};
+        d.tile = t;
+        d.nTiles = T;
+        d.vOff = (int)out->vox.size();
+        d.nOwn = (int)own.size();
+        d.nHalo = (int)halo.size();
+        d.lOff = (int)out->link.size();
+        d.nLinks = (int)links.size();
+        for (int i = 0; i < d.nOwn; i++) {
+            const int v = own[i];
+            bool shared = false;
+            for (int j = p.inc0[v]; j < p.inc0[v + 1]; j++) shared |= owner[p.other(p.inc[j], v)] != t;
+            out->vox.push_back(v | (shared ? (int)0x80000000u : 0));
+            if (shared) rec[v] = nPub++;
+            out->xidx.push_back(shared ? rec[v] : -1);
+            slot[v] = i;
+        }
+        for (int i = 0; i < d.nHalo; i++) {
+            out->vox.push_back(halo[i]);
+            out->xidx.push_back(-2 - halo[i]);   // the owner's record, filled in below
+            slot[halo[i]] = d.nOwn + i;
+        }
+        for (int l : links) {
+            out->link.push_back(l | (owner[lneg[l]] == t ? (int)0x80000000u : 0));
+            out->ends.push_back(slot[lneg[l]] | (slot[lpos[l]] << 16));
+        }
+        for (int v : own) slot[v] = -1;
+        for (int v : halo) slot[v] = -1;
+        out->tiles.push_back(d);
+    }
+    for (int& x : out->xidx)
+        if (x <= -2) x = rec[-2 - x];   // every halo voxel is linked to its reader: its owner publishes it
+    out->nPub = nPub;
+    return true;
+}
+
+}  // namespace vx3

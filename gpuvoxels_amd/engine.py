@@ -48,6 +48,7 @@ def lib():
         "vx3h_build_topology": (S, [C.POINTER(_abi.SimDesc), C.POINTER(_abi.TopologyView)]),
         "vx3h_batch_kernel_times": (S, [P, C.c_int, C.POINTER(C.c_double)]),
         "vx3h_batch_set_timing": (S, [P, C.c_int]),
+        "vx3h_sim_workgroups": (S, [P, C.c_int, C.POINTER(C.c_int)]),
         "vx3h_calib_copy": (S, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
         "vx3h_sim_collision_pairs": (S, [P, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
         "vx3h_server_create": (S, [C.c_int, C.POINTER(_abi.ServerCfg), C.POINTER(P)]),
@@ -283,6 +284,13 @@ class Batch:
         out = (C.c_double * len(self.PASSES))()
         check(lib().vx3h_batch_kernel_times(self.h, int(n_steps), out))
         return {k: out[i] for i, k in enumerate(self.PASSES) if out[i] > 0}
+
+    def workgroups(self, slot):
+        """Workgroups that step the simulation in `slot`: 0 = per-step kernels, 1 = one
+        workgroup of the resident kernel, n > 1 = n tiles of the tiled resident kernel."""
+        n = C.c_int(0)
+        check(lib().vx3h_sim_workgroups(self.h, int(slot), C.byref(n)))
+        return n.value
 
     def set_timing(self, enable=True):
         check(lib().vx3h_batch_set_timing(self.h, int(bool(enable))))

@@ -267,8 +267,10 @@ vx3h_status vx3h_vxa_get_info(const vx3h_vxa* v, vx3h_vxa_info* info, const char
 /* ---------------------------------------------------------------- batch engine */
 enum {
     VX3H_PATH_STREAM = 0,    /* state in HBM; link pass + voxel pass per step over every resident sim */
-    VX3H_PATH_RESIDENT = 1,  /* one workgroup per sim, state held in LDS/registers across many steps;
-                                admitting a sim it cannot take fails with VX3H_EUNSUPPORTED */
+    VX3H_PATH_RESIDENT = 1,  /* one workgroup per sim, state held in LDS/registers across many steps
+                                (a sim without collisions too large for one workgroup runs as several
+                                tiles, one workgroup each, with a barrier per step); admitting a sim
+                                it cannot take fails with VX3H_EUNSUPPORTED */
     VX3H_PATH_AUTO = 2       /* resident where the sim fits, stream otherwise */
 };
 enum { VX3H_PREC_F32 = 0, VX3H_PREC_F64 = 1 };
@@ -291,7 +293,8 @@ typedef struct vx3h_batch_cfg {
 typedef struct vx3h_batch vx3h_batch;
 
 enum { VX3H_SIM_RUNNING = 0, VX3H_SIM_DONE = 1, VX3H_SIM_DIVERGED = 2, VX3H_SIM_EMPTY = 3,
-       VX3H_SIM_FAILED = 4   /* a collision table overflowed its capacity; the results are not valid */ };
+       VX3H_SIM_FAILED = 4   /* a collision table overflowed its capacity, or a tiled simulation's
+                                workgroups were not all running at once; the results are not valid */ };
 
 typedef struct vx3h_sim_status {
     int    slot;
@@ -393,6 +396,11 @@ vx3h_status vx3h_build_topology(const vx3h_sim_desc* desc, vx3h_topology_view* v
  * did not run). */
 #define VX3H_N_PASSES 6
 vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, double* avg_ms);
+
+/* Which kernels step a simulation (no counterpart in the reference, which has one
+ * path): *workgroups = 0 for the per-step kernels, 1 for one workgroup of the resident
+ * kernel, n > 1 for n tiles of the tiled resident kernel. */
+vx3h_status vx3h_sim_workgroups(vx3h_batch* b, int slot, int* workgroups);
 
 /* Launch timing of ordinary stepping (no counterpart in the reference): while
  * enabled, vx3h_batch_step records a HIP event pair on the batch's stream around
