@@ -157,13 +157,15 @@ template <typename T> __device__ __forceinline__ T xget(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One thread of each tile, after every thread of the tile has written its exchange data
-// and waited for it (s_waitcnt 0) and the workgroup has synchronized: arrive at the
-// simulation's barrier number `gen` (0-based) and wait until all n tiles have.  False
-// when the wait exceeded `spin` ticks.
-__device__ __forceinline__ bool tileBarrier(int* ctr, int n, unsigned gen, long long spin) {
+// The simulation's barrier, in two halves.  One thread of each tile arrives, after every
+// thread of the tile has written its exchange data and waited for it (s_waitcnt 0) and
+// the workgroup has synchronized; later it waits until all n tiles have arrived at
+// barrier number `gen` (0-based).  False when the wait exceeded `spin` ticks.
+__device__ __forceinline__ void tileArrive(int* ctr) {
     __builtin_amdgcn_s_waitcnt(0);
     __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool tileWait(int* ctr, int n, unsigned gen, long long spin) {
     const int target = n * (int)(gen + 1);
     const long long t0 = wall_clock64();
     while (xget(ctr) < target) {
@@ -493,6 +495,60 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     __syncthreads();
 
     if (COL && sColOvf) colOvf = true;   // lists larger than the LDS table: no step here
+    // TILED: the second half of the simulation's barrier `gen`, whose arrival is pending:
+    // wait for every tile, then (sampleNow) the tiles' centre-of-mass partial sums in tile
+    // order into sRed[.][0], then (loadHalo) the halo poses into LDS.  1: some tile saw a
+    // link diverge (leave; the repair launch takes over), 2: a tile did not arrive (the
+    // simulation fails).
+    bool pend = false;
+    auto finishBarrier = [&](bool loadHalo, bool sampleNow) -> int {
+        if (tid == 0) {
+            int* sync = ta.sync + td.cOff;
+            int flag = 0;
+            if (!tileWait(sync, td.nTiles, gen, ta.spin)) flag = 2;
+            else if (divAt < 0 && xget(sync + 1)) flag = 1;
+            sTile = flag;
+        }
+        const int par = gen & 1;
+        gen++;
+        pend = false;
+        __syncthreads();
+        if (sTile == 2) {   // state stays as it was
+            if (tid == 0) scg.state = 4;
+            return 2;
+        }
+        if (sTile == 1) return 1;   // state untouched: the repair launch runs it to that step's link pass
+        if (sampleNow) {   // the partial sums read in parallel, added in tile order by one thread
+            const double* P = ta.part + td.pOff + (size_t)par * td.nTiles * 4;
+            double* sP = (double*)sF;   // the load buffer is free between the voxel and link passes
+            for (int j = tid; j < td.nTiles; j += BS) {
+                const double a0 = xget(P + 4 * j), a1 = xget(P + 4 * j + 1), a2 = xget(P + 4 * j + 2), a3 = xget(P + 4 * j + 3);
+                sP[4 * j] = a0; sP[4 * j + 1] = a1; sP[4 * j + 2] = a2; sP[4 * j + 3] = a3;
+            }
+            __syncthreads();
+            if (tid == 0) {   // the same totals in every tile
+                for (int q = 0; q < 4; q++) sRed[q][0] = 0;
+                for (int j = 0; j < td.nTiles; j++)
+                    for (int q = 0; q < 4; q++) sRed[q][0] += sP[4 * j + q];
+            }
+        }
+        if (loadHalo) {
+            const real* X = (const real*)ta.xch + td.xOff + (size_t)par * td.xPar;
+#pragma unroll 1
+            for (int v = nvOwn + tid; v < nvAll; v += BS) {
+                // every load issued before the first LDS write: one memory latency, not eleven
+                const real* r = X + (size_t)sXv[v] * VX3_XCH_STRIDE;
+                real q[VX3_XCH_STRIDE - 1];
+#pragma unroll
+                for (int f = 0; f < VX3_XCH_STRIDE - 1; f++) q[f] = xget(r + f);
+                sPx[v] = q[0]; sPy[v] = q[1]; sPz[v] = q[2];
+                sOw[v] = q[3]; sOx[v] = q[4]; sOy[v] = q[5]; sOz[v] = q[6];
+                sTf[v] = (float)q[7]; sPsx[v] = (float)q[8]; sPsy[v] = (float)q[9]; sPsz[v] = (float)q[10];
+            }
+            __syncthreads();
+        }
+        return 0;
+    };
     for (int it = 0; it < nSteps && !colOvf; it++) {
         // ---- contact lists rebuilt when stale or at the fixed period (COL)
         if (COL && ((it == 0 && colFirst) || sStale[it & 1] != 0 || (scg.colInterval > 0 && steps % scg.colInterval == 0))) {
@@ -590,16 +646,16 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         for (int k = 0; k < VPT; k++) { fs[k] = V3<real>(0, 0, 0); ms[k] = V3<real>(0, 0, 0); }
 #pragma unroll
         for (int a = 0; a < 3; a++) {
-            // ---- link pass of axis a
-#pragma unroll
-            for (int k = 0; k < LPA; k++) {
-                if (lEnds[a][k] < 0) continue;
+            // ---- link pass of axis a.  TILED with the last step's barrier pending: the
+            // axis-0 links between owned voxels first, then the wait and the halo, then the
+            // axis-0 links with a halo end
+            auto linkAt = [&](int k) {
                 const int slot = tid + k * BS;
                 const int vn = lEnds[a][k] & 0xFFFF, vq = lEnds[a][k] >> 16;
                 const LinkMatDev& mat = sLM[lMats[a][k] & 0xFF];
                 const VoxMatDev& mn = sVM[(lMats[a][k] >> 8) & 0xFF];
                 const VoxMatDev& mp = sVM[lMats[a][k] >> 16];
-                if (mn.fixed && mp.fixed) continue;   // its slots are never read: both ends are fixed
+                if (mn.fixed && mp.fixed) return;   // its slots are never read: both ends are fixed
                 const real restLength = (real)0.5 * (nom * (real)sTf[vn] + nom * (real)sTf[vq]);
                 float area, tss;
                 if (mat.nu != 0.0f) {
@@ -638,6 +694,19 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     sEs[(2 * a + 1) * LAX + slot] = 2.0f * ls[a][k].strain * r / (1.0f + r);
                 }
                 if (ls[a][k].strain > 100) sDiv = 1;
+            };
+            if (TILED && a == 0 && pend) {
+#pragma unroll
+                for (int k = 0; k < LPA; k++)
+                    if (lEnds[a][k] >= 0 && (lEnds[a][k] & 0xFFFF) < nvOwn && (lEnds[a][k] >> 16) < nvOwn) linkAt(k);
+                if (finishBarrier(true, false)) return;
+#pragma unroll
+                for (int k = 0; k < LPA; k++)
+                    if (lEnds[a][k] >= 0 && ((lEnds[a][k] & 0xFFFF) >= nvOwn || (lEnds[a][k] >> 16) >= nvOwn)) linkAt(k);
+            } else {
+#pragma unroll
+                for (int k = 0; k < LPA; k++)
+                    if (lEnds[a][k] >= 0) linkAt(k);
             }
             __syncthreads();
             if (COL && a == 0 && tid == 0) sStale[it & 1] = 0;   // every thread has read it
@@ -737,6 +806,15 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                     psx += (double)vp.x * mass; psy += (double)vp.y * mass; psz += (double)vp.z * mass;
                     psm += mass;
                 }
+                if (TILED && sXv[v] >= 0) {
+                    // a voxel another tile reads: its record out as soon as it is final (such
+                    // voxels take the lowest slots, so the stores drain while the rest integrate)
+                    real q[VX3_XCH_STRIDE - 1] = {sPx[v], sPy[v], sPz[v], sOw[v], sOx[v], sOy[v], sOz[v],
+                                                  (real)sTf[v], (real)sPsx[v], (real)sPsy[v], (real)sPsz[v]};
+                    real* r = (real*)ta.xch + td.xOff + (size_t)(gen & 1) * td.xPar + (size_t)sXv[v] * VX3_XCH_STRIDE;
+#pragma unroll
+                    for (int f = 0; f < VX3_XCH_STRIDE - 1; f++) xput(r + f, q[f]);
+                }
             }
         }
         if (COL && !diverged) {
@@ -754,71 +832,25 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             if ((tid & 63) == 0) { sRed[0][tid >> 6] = psx; sRed[1][tid >> 6] = psy; sRed[2][tid >> 6] = psz; sRed[3][tid >> 6] = psm; }
         }
         if (TILED && !diverged) {
-            // ---- the simulation's barrier: boundary poses out, halo poses in (TILED)
-            const int par = gen & 1;
-            real* X = (real*)ta.xch + td.xOff + (size_t)par * td.xPar;
-#pragma unroll 1
-            for (int v = tid; v < nvOwn; v += BS) {   // a tile's records are consecutive, in slot order
-                const int x = sXv[v];
-                if (x < 0) continue;
-                // all values into registers first: the scoped stores keep their order, and a
-                // load between two of them would wait for the LDS each time
-                real q[VX3_XCH_STRIDE - 1] = {sPx[v], sPy[v], sPz[v], sOw[v], sOx[v], sOy[v], sOz[v],
-                                              (real)sTf[v], (real)sPsx[v], (real)sPsy[v], (real)sPsz[v]};
-                real* r = X + (size_t)x * VX3_XCH_STRIDE;
-#pragma unroll
-                for (int f = 0; f < VX3_XCH_STRIDE - 1; f++) xput(r + f, q[f]);
-            }
+            // ---- arrival at the simulation's barrier (TILED): the records are out; the wait
+            // comes in the next step's link pass, or now on a sampling step (the centre of
+            // mass needs every tile's partial sum)
             __builtin_amdgcn_s_waitcnt(0);   // this thread's exchange stores are complete
             __syncthreads();
             if (tid == 0) {
-                double* P = ta.part + td.pOff + (size_t)par * td.nTiles * 4;
+                int* sync = ta.sync + td.cOff;
                 if (sample) {
+                    double* P = ta.part + td.pOff + (size_t)(gen & 1) * td.nTiles * 4;
                     double tot[4] = {0, 0, 0, 0};
                     for (int w = 0; w < BS / 64; w++)
                         for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
                     for (int q = 0; q < 4; q++) xput(P + 4 * td.tile + q, tot[q]);
                 }
-                int* sync = ta.sync + td.cOff;
                 if (sDiv) __hip_atomic_fetch_max(sync + 1, it + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                int flag = 0;
-                if (!tileBarrier(sync, td.nTiles, gen, ta.spin)) flag = 2;
-                else if (divAt < 0 && xget(sync + 1)) flag = 1;
-                sTile = flag;
+                tileArrive(sync);
             }
-            gen++;
-            __syncthreads();
-            if (sTile == 2) {   // a tile did not arrive: the simulation fails, its state stays as it was
-                if (tid == 0) scg.state = 4;
-                return;
-            }
-            if (sTile == 1) return;   // state untouched: the repair launch runs it to this step's link pass
-            if (sample) {   // the tiles' partial sums, read in parallel, added in tile order by one thread
-                const double* P = ta.part + td.pOff + (size_t)par * td.nTiles * 4;
-                double* sP = (double*)sF;   // the load buffer is free between the voxel and link passes
-                for (int j = tid; j < td.nTiles; j += BS) {
-                    const double a0 = xget(P + 4 * j), a1 = xget(P + 4 * j + 1), a2 = xget(P + 4 * j + 2),
-                                 a3 = xget(P + 4 * j + 3);
-                    sP[4 * j] = a0; sP[4 * j + 1] = a1; sP[4 * j + 2] = a2; sP[4 * j + 3] = a3;
-                }
-                __syncthreads();
-                if (tid == 0) {   // the same totals in every tile
-                    for (int q = 0; q < 4; q++) sRed[q][0] = 0;
-                    for (int j = 0; j < td.nTiles; j++)
-                        for (int q = 0; q < 4; q++) sRed[q][0] += sP[4 * j + q];
-                }
-            }
-#pragma unroll 1
-            for (int v = nvOwn + tid; v < nvAll; v += BS) {
-                // every load issued before the first LDS write: one memory latency, not eleven
-                const real* r = X + (size_t)sXv[v] * VX3_XCH_STRIDE;
-                real q[VX3_XCH_STRIDE - 1];
-#pragma unroll
-                for (int f = 0; f < VX3_XCH_STRIDE - 1; f++) q[f] = xget(r + f);
-                sPx[v] = q[0]; sPy[v] = q[1]; sPz[v] = q[2];
-                sOw[v] = q[3]; sOx[v] = q[4]; sOy[v] = q[5]; sOz[v] = q[6];
-                sTf[v] = (float)q[7]; sPsx[v] = (float)q[8]; sPsy[v] = (float)q[9]; sPsz[v] = (float)q[10];
-            }
+            pend = true;
+            if (sample && finishBarrier(true, true)) return;
         } else if (sample) __syncthreads();
         if (sample) {
             angleSamples++;
@@ -851,8 +883,13 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         if (!stop && stopLen > 0 && tid == 0) sRun = mathTreeEval(b.tokens + stopOff, stopLen, sCom[0], sCom[1], sCom[2], 0, t, sAngle, 0, 0, nv, sQ) > 0 ? 0 : 1;
         if (it == 0 && prevDt0 != dtf && tid < nvm) sDm[tid] = 2 * sVM[tid].sqrtMass * sVM[tid].zetaInternal / (sVM[tid].fixed ? 0.0f : dtf);
         __syncthreads();   // next step's link pass reads the new poses
-        if (stop || !sRun) { state = 1; break; }
+        if (stop || !sRun) {
+            if (TILED && pend && finishBarrier(false, false)) return;
+            state = 1;
+            break;
+        }
     }
+    if (TILED && pend && finishBarrier(false, false)) return;   // the launch's last step
 
     // ---- write back in the stream layout
 #pragma unroll

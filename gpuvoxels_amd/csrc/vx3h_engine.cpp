@@ -1713,6 +1713,27 @@ vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
 }
 }  // namespace
 
+namespace vx3 {
+vx3h_status pollFinished(vx3h_batch* b, std::vector<vx3h_sim_status>& finished) {
+    finished.clear();
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    std::vector<SimCtl> ctl(b->slots.size());
+    if (!ctl.empty()) HIPCHK(hipMemcpy(ctl.data(), b->db.sims, ctl.size() * sizeof(SimCtl), hipMemcpyDeviceToHost));
+    if (b->anyColl || b->anyAttach) {
+        vx3h_status gs = growCollisionTables(b, ctl);
+        if (gs) return gs;
+    }
+    for (int s = 0; s < (int)b->slots.size(); s++) {
+        if (!b->slots[s].used || ctl[s].state == VX3H_SIM_RUNNING) continue;
+        finished.emplace_back();
+        vx3h_status st = pollOne(b, s, ctl[s], finished.back(), nullptr);
+        if (st) return st;
+    }
+    return VX3H_OK;
+}
+}  // namespace vx3
+
 extern "C" vx3h_status vx3h_batch_poll(vx3h_batch* b, vx3h_sim_status* out, int cap, int* n) {
     if (!b || !n) FAILV(VX3H_EINVAL, "null argument");
     HIPCHK(hipSetDevice(b->device));

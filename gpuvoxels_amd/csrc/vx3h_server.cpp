@@ -172,13 +172,10 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
     if (!sv->running.empty()) {
         vx3h_status st = vx3h_batch_step(sv->batch, sv->cfg.quantum_steps);
         if (st != VX3H_OK) return st;
-        int n = 0;
-        if ((st = vx3h_batch_poll(sv->batch, nullptr, 0, &n)) != VX3H_OK) return st;
-        std::vector<vx3h_sim_status> sts(n > 0 ? n : 1);
-        if ((st = vx3h_batch_poll(sv->batch, sts.data(), n, &n)) != VX3H_OK) return st;
+        std::vector<vx3h_sim_status> sts;   // results only of the simulations that stopped
+        if ((st = vx3::pollFinished(sv->batch, sts)) != VX3H_OK) return st;
         const double t = sv->now();
-        for (int i = 0; i < n; i++) {
-            if (sts[i].state == VX3H_SIM_RUNNING) continue;
+        for (size_t i = 0; i < sts.size(); i++) {
             auto it = sv->running.find(sts[i].slot);
             if (it == sv->running.end()) continue;
             Request* r = it->second;

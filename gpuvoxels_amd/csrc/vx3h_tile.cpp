@@ -54,12 +54,14 @@ struct Planner {
         return (int)part.size() + nh <= vmax && per[0] <= lax && per[1] <= lax && per[2] <= lax;
     }
 
-    // k: the number of blocks this part is meant to become
+    // k: the number of blocks this part is meant to become.  strict: a block that does not
+    // fit fails the plan (the caller tries more blocks); else it is halved.
+    bool strict = false;
     void cut(std::vector<int>& part, int k) {
         if (failed || part.empty()) return;
         if (k <= 1) {
             if (fits(part)) { parts.push_back(part); return; }
-            if (part.size() == 1) { failed = true; return; }
+            if (strict || part.size() == 1) { failed = true; return; }
             k = 2;
         }
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
@@ -67,13 +69,14 @@ struct Planner {
             for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], ijk[3 * v + a]); hi[a] = std::max(hi[a], ijk[3 * v + a]); }
         int ax = 0;
         for (int a = 1; a < 3; a++) if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
-        std::sort(part.begin(), part.end(), [&](int x, int y) {
-            const int cx = ijk[3 * x + ax], cy = ijk[3 * y + ax];
-            return cx != cy ? cx < cy : x < y;
-        });
         const int kl = k / 2;
         size_t n = (size_t)((double)part.size() * kl / k + 0.5);
         n = std::max<size_t>(1, std::min(n, part.size() - 1));
+        // the n lowest along the axis (ties by voxel index): a selection, not a sort
+        std::nth_element(part.begin(), part.begin() + n, part.end(), [&](int x, int y) {
+            const int cx = ijk[3 * x + ax], cy = ijk[3 * y + ax];
+            return cx != cy ? cx < cy : x < y;
+        });
         std::vector<int> left(part.begin(), part.begin() + n), right(part.begin() + n, part.end());
         cut(left, kl);
         cut(right, k - kl);
@@ -88,17 +91,29 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
     Planner p;
     p.nv = nv; p.nl = nl; p.ijk = ijk; p.lneg = lneg; p.lpos = lpos; p.laxis = laxis; p.vmax = vmax; p.lax = lax;
     p.index();
+    // Equal blocks: every tile's step takes as long as the largest block's, so the fewest
+    // blocks of equal size that all fit, found by raising their number a few percent at a
+    // time from a fill of 85% of the slots; a shape no equal split fits falls back to
+    // halving the blocks that do not.
     std::vector<std::vector<int>> best;
-    for (double frac : {0.8, 0.72, 0.64, 0.56, 0.48, 0.4}) {
-        const int target = std::max(1, (int)(frac * vmax));
+    auto attempt = [&](int k, bool strict) {
         std::vector<int> all(nv);
         for (int v = 0; v < nv; v++) all[v] = v;
         p.parts.clear();
         p.failed = false;
-        p.cut(all, (nv + target - 1) / target);
-        if (p.failed) continue;
-        if (best.empty() || p.parts.size() < best.size()) best = p.parts;
+        p.strict = strict;
+        p.cut(all, k);
+        return !p.failed;
+    };
+    for (int k = std::max(1, (int)((nv + 0.85 * vmax - 1) / (0.85 * vmax))); k <= maxTiles;
+         k = std::max(k + 1, (int)(k * 1.03))) {
+        if (attempt(k, true)) { best = p.parts; break; }
     }
+    if (best.empty())
+        for (double frac : {0.8, 0.6}) {
+            if (!attempt(std::max(1, (int)((nv + frac * vmax - 1) / (frac * vmax))), false)) continue;
+            if (best.empty() || p.parts.size() < best.size()) best = p.parts;
+        }
     if (best.empty() || (int)best.size() > maxTiles) return false;
 
     const int T = (int)best.size();
@@ -112,7 +127,15 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
     std::vector<int> rec(nv, -1);   // exchange record of each voxel another tile reads
     int nPub = 0;
     for (int t = 0; t < T; t++) {
-        const std::vector<int>& own = best[t];
+        // owned voxels another tile reads take the lowest slots (each thread's first voxels:
+        // their records are written first and drain while the rest integrate)
+        std::vector<int> own;
+        for (int pass = 0; pass < 2; pass++)
+            for (int v : best[t]) {
+                bool shared = false;
+                for (int j = p.inc0[v]; j < p.inc0[v + 1]; j++) shared |= owner[p.other(p.inc[j], v)] != t;
+                if (shared == (pass == 0)) own.push_back(v);
+            }
         std::vector<int> halo, links;
         for (int v : own)
             for (int i = p.inc0[v]; i < p.inc0[v + 1]; i++) {

@@ -8,8 +8,8 @@ namespace vx3 {
 
 // The tiles of one simulation.  Per tile (TileDesc, sim-local offsets into the arrays
 // below; sim, xOff, cOff and pOff are the caller's to fill):
-//   vox[vOff ..)   owned voxels ascending (bit 31: a voxel of another tile links to it),
-//                  then the halo voxels ascending;
+//   vox[vOff ..)   owned voxels (bit 31: a voxel of another tile links to it), those with
+//                  bit 31 first, each group ascending; then the halo voxels ascending;
 //   link[lOff ..)  every link with an owned end, ascending (bit 31: this tile owns the
 //                  link's negative end and writes the link back);
 //   ends[lOff ..)  the LDS slots of each such link's ends, negative | positive << 16;

@@ -116,16 +116,16 @@ def test_tiled_stop_condition_and_step_budget():
 
 
 def test_several_launch_groups_and_a_mixed_batch():
-    """Four 50k-voxel robots need more tiles than one cooperative launch holds; beside
+    """Five 50k-voxel robots need more tiles than one cooperative launch holds; beside
     them a one-workgroup robot and a slab with collisions on the per-step kernels."""
-    big = [robot(20 + i, 47) for i in range(4)]
+    big = [robot(20 + i, 47) for i in range(5)]
     small = robot(30, 9)
     slab = workloads.slab(24, 24, 4, steps=0)
     models = big + [small, slab]
     bt, st = run(models, _abi.PATH_AUTO, 300, spl=150)
     wg = [bt.workgroups(s) for s in st]
     print("workgroups", wg)
-    assert sum(wg[:4]) > 256 and wg[4] == 1 and wg[5] == 0
+    assert sum(wg[:5]) > 256 and wg[5] == 1 and wg[6] == 0
     bs, ss = run(models, _abi.PATH_STREAM, 300)
     assert_same(bt, st, bs, ss, 1e-14, 1e-13)
     bt.close()

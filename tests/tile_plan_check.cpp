@@ -31,8 +31,12 @@ int main() {
         if (d.tile != t || d.nTiles != T) BAD("tile numbering");
         if (d.nOwn < 1 || d.nOwn + d.nHalo > vmax) BAD("tile %d holds %d + %d voxels", t, d.nOwn, d.nHalo);
         for (int i = 0; i < d.nOwn; i++) {
-            const int v = p.vox[d.vOff + i] & 0x7fffffff;
-            if (i && v <= (p.vox[d.vOff + i - 1] & 0x7fffffff)) BAD("owned voxels not ascending");
+            const int raw = p.vox[d.vOff + i], v = raw & 0x7fffffff;
+            if (i) {   // voxels other tiles read first, each group ascending
+                const int pr = p.vox[d.vOff + i - 1];
+                if ((raw & 0x80000000) && !(pr & 0x80000000)) BAD("a published voxel after an unpublished one");
+                if (((raw ^ pr) & 0x80000000) == 0 && v <= (pr & 0x7fffffff)) BAD("owned voxels not ascending");
+            }
             if (owner[v] >= 0) BAD("voxel %d owned twice", v);
             owner[v] = t;
         }
