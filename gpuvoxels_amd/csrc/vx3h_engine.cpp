@@ -109,7 +109,8 @@ struct SlotHost {
     int nlSeen = 0;              // links known to the host shadows below (nl + attached ones read back)
     int shape = -1;              // resident kernel shape, -1 = stream path (or tiled)
     int tiles = 0;               // > 0: runs on the tiled resident kernel as this many workgroups
-    TilePlan tplan;              // its tiles (sim-local)
+    TilePlan tplan;              // its tiles (sim-local; the tables themselves live on the device)
+    int tv0 = 0, tvN = 0, tl0 = 0, tlN = 0, tx0 = 0, txN = 0, tp0 = 0;   // its ranges in the tile pools
     std::vector<int> ijk;
     std::vector<int> matid;
     std::vector<int> measured;   // per voxel
@@ -202,13 +203,14 @@ struct vx3h_batch {
     struct TileGroup { int first, count, sync0, nSync; };
     std::vector<TileGroup> tileGroups;
     TileDesc* dTiles = nullptr; size_t capTiles = 0;
-    int* dTileVox = nullptr; size_t capTileVox = 0;
-    int* dTileLink = nullptr; size_t capTileLink = 0;
-    int* dTileEnds = nullptr; size_t capTileEnds = 0;
-    int* dTileXidx = nullptr; size_t capTileXidx = 0;
-    void* dXch = nullptr; size_t capXch = 0;     // bytes
     int* dSync = nullptr; size_t capSync = 0;
-    double* dPart = nullptr; size_t capPart = 0;
+    // per simulation, uploaded once at admission: tile slots (voxel, exchange record), tile
+    // links (link, end slots), exchange records (two step parities), partial sums per tile
+    DevPool tileSlotPool, tileLinkPool, tileXchPool, tilePartPool;
+    int* dTileVox = nullptr; int* dTileXidx = nullptr;
+    int* dTileLink = nullptr; int* dTileEnds = nullptr;
+    void* dXch = nullptr;
+    double* dPart = nullptr;
     hipStream_t tileStream = nullptr;
     hipEvent_t tileJoin = nullptr;
     bool tilePending = false;
@@ -275,6 +277,11 @@ void initSpecs(vx3h_batch* b) {
     b->attPool.arrs = {{(void**)&d.attCand, 4}};
     b->pairPool.arrs = {{(void**)&d.attPair, 8}};
     b->nearPool.arrs = {{(void**)&d.nearList, 4}};
+    b->tileSlotPool.arrs = {{(void**)&b->dTileVox, 4}, {(void**)&b->dTileXidx, 4}};
+    b->tileLinkPool.arrs = {{(void**)&b->dTileLink, 4}, {(void**)&b->dTileEnds, 4}};
+    b->tileXchPool.arrs = {{(void**)&b->dXch, 2 * VX3_XCH_STRIDE * R}};   // one record, both parities
+    b->tilePartPool.arrs = {{(void**)&b->dPart, 8 * sizeof(double)}};    // one tile, both parities
+    b->tileXchPool.minCap = b->tilePartPool.minCap = 256;
     L(&d.lneg, 4, 1, false); L(&d.lpos, 4, 1, false); L(&d.lmeta, 4, 1, false); L(&d.lsim, 4, 1, false); L(&d.lflags, 4, 1, false);
     L(&d.q2x, R, 1, false); L(&d.q2y, R, 1, false); L(&d.q2z, R, 1, false);
     L(&d.a1x, R, 1, false); L(&d.a1y, R, 1, false); L(&d.a1z, R, 1, false);
@@ -456,12 +463,11 @@ template <typename T> vx3h_status growDev(T** p, size_t* cap, size_t n) {
     return VX3H_OK;
 }
 
-// The tiled simulations' tables on the device, after a simulation joined or left: whole
-// simulations packed into launch groups of at most tileCap tiles (first fit, largest
-// first: a launch lasts as long per step whether full or not); per simulation an
-// exchange area (two step parities of VX3_XCH_STRIDE reals per record), two sync words
-// (per group, from the group's base) and the partial sums.  None of it carries state between
-// launches.
+// The tiled launch groups, after a simulation joined or left: whole simulations packed
+// into groups of at most tileCap tiles (first fit, largest first: a launch lasts as long
+// per step whether full or not), and their tile descriptors pointing at each
+// simulation's ranges in the tile pools (filled at admission), with two sync words per
+// simulation from its group's base.
 vx3h_status syncTiles(vx3h_batch* b) {
     if (!b->tileDirty) return VX3H_OK;
     std::vector<int> order = b->tiledSims;
@@ -476,56 +482,31 @@ vx3h_status syncTiles(vx3h_batch* b) {
         fill[i] += b->slots[s].tiles;
     }
     std::vector<TileDesc> T;
-    std::vector<int> vox, link, ends, xidx;
     b->tileGroups.clear();
-    size_t xoff = 0, poff = 0;
     int syncTot = 0;
     for (const auto& bin : bins) {
         vx3h_batch::TileGroup g{(int)T.size(), 0, syncTot, 0};
         for (int s : bin) {
             const SlotHost& sh = b->slots[s];
-            const int n = sh.tiles;
-            const int vb = (int)vox.size(), lb = (int)link.size();
             for (TileDesc d : sh.tplan.tiles) {
                 d.sim = s;
-                d.vOff += vb;
-                d.lOff += lb;
-                d.xOff = (int)xoff;
+                d.vOff += sh.tv0;
+                d.lOff += sh.tl0;
+                d.xOff = sh.tx0 * 2 * VX3_XCH_STRIDE;
                 d.xPar = VX3_XCH_STRIDE * sh.tplan.nPub;
                 d.cOff = g.nSync;
-                d.pOff = (int)poff;
+                d.pOff = sh.tp0 * 8;
                 T.push_back(d);
             }
-            vox.insert(vox.end(), sh.tplan.vox.begin(), sh.tplan.vox.end());
-            link.insert(link.end(), sh.tplan.link.begin(), sh.tplan.link.end());
-            ends.insert(ends.end(), sh.tplan.ends.begin(), sh.tplan.ends.end());
-            xidx.insert(xidx.end(), sh.tplan.xidx.begin(), sh.tplan.xidx.end());
-            xoff += (size_t)2 * VX3_XCH_STRIDE * sh.tplan.nPub;
-            poff += (size_t)8 * n;
-            g.count += n;
+            g.count += sh.tiles;
             g.nSync += 2;
             syncTot += 2;
         }
         b->tileGroups.push_back(g);
     }
-    if (xoff > (size_t)INT32_MAX) FAILV(VX3H_ENOMEM, "tiled simulations' exchange areas exceed 2^31 values");
     vx3h_status st;
-    if ((st = growDev(&b->dTiles, &b->capTiles, T.size())) || (st = growDev(&b->dTileVox, &b->capTileVox, vox.size())) ||
-        (st = growDev(&b->dTileLink, &b->capTileLink, link.size())) ||
-        (st = growDev(&b->dTileEnds, &b->capTileEnds, ends.size())) ||
-        (st = growDev(&b->dTileXidx, &b->capTileXidx, xidx.size())) ||
-        (st = growDev((unsigned char**)&b->dXch, &b->capXch, xoff * b->realSize)) ||
-        (st = growDev(&b->dSync, &b->capSync, (size_t)syncTot)) || (st = growDev(&b->dPart, &b->capPart, poff)))
-        return st;
+    if ((st = growDev(&b->dTiles, &b->capTiles, T.size())) || (st = growDev(&b->dSync, &b->capSync, (size_t)syncTot))) return st;
     if (!T.empty()) HIPCHK(hipMemcpy(b->dTiles, T.data(), T.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
-    if (!vox.empty()) {
-        HIPCHK(hipMemcpy(b->dTileVox, vox.data(), vox.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->dTileXidx, xidx.data(), xidx.size() * 4, hipMemcpyHostToDevice));
-    }
-    if (!link.empty()) {
-        HIPCHK(hipMemcpy(b->dTileLink, link.data(), link.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->dTileEnds, ends.data(), ends.size() * 4, hipMemcpyHostToDevice));
-    }
     b->tileDirty = false;
     return VX3H_OK;
 }
@@ -707,9 +688,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
     }
     if (b->tileStream) { hipStreamSynchronize(b->tileStream); hipStreamDestroy(b->tileStream); }
     if (b->tileJoin) hipEventDestroy(b->tileJoin);
-    for (void* p : {(void*)b->dTiles, (void*)b->dTileVox, (void*)b->dTileLink, (void*)b->dTileEnds, (void*)b->dTileXidx,
-                    b->dXch, (void*)b->dSync, (void*)b->dPart})
-        if (p) hipFree(p);
+    for (void* p : {(void*)b->dTiles, (void*)b->dSync}) if (p) hipFree(p);
     if (b->forkEv) hipEventDestroy(b->forkEv);
     if (b->graphExec) hipGraphExecDestroy(b->graphExec);
     if (b->graph) hipGraphDestroy(b->graph);
@@ -718,7 +697,8 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
     for (void* p : {b->dVmat, b->dLmat, b->dExt, b->dTok, b->dSimMats, (void*)b->db.sims, (void*)b->db.hist}) if (p) hipFree(p);
     for (int* p : b->dResLists) if (p) hipFree(p);
     if (b->db.colAny) hipFree(b->db.colAny);
-    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool, &b->cellPool, &b->tgtPool, &b->attPool, &b->pairPool})
+    for (DevPool* pl : {&b->nbPool, &b->hashPool, &b->nearPool, &b->cellPool, &b->tgtPool, &b->attPool, &b->pairPool,
+                        &b->tileSlotPool, &b->tileLinkPool, &b->tileXchPool, &b->tilePartPool})
         for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
     for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
     for (hipEvent_t e : b->evPool) hipEventDestroy(e);
@@ -1161,9 +1141,23 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN; sh.c0 = c0;
     sh.tg0 = tg0; sh.nTg = nTg;
     sh.at0 = at0; sh.ap0 = ap0; sh.apN = apN;
-    if (tiles) {
+    if (tiles) {   // the tile tables into their pools, once
+        vx3h_status st;
+        sh.tvN = (int)tplan.vox.size();
+        sh.tlN = std::max(1, (int)tplan.link.size());
+        sh.txN = std::max(1, tplan.nPub);
+        if ((st = poolAlloc(b, b->tileSlotPool, sh.tvN, sh.tv0)) || (st = poolAlloc(b, b->tileLinkPool, sh.tlN, sh.tl0)) ||
+            (st = poolAlloc(b, b->tileXchPool, sh.txN, sh.tx0)) || (st = poolAlloc(b, b->tilePartPool, tiles, sh.tp0)))
+            return st;
+        HIPCHK(hipMemcpy(b->dTileVox + sh.tv0, tplan.vox.data(), tplan.vox.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->dTileXidx + sh.tv0, tplan.xidx.data(), tplan.xidx.size() * 4, hipMemcpyHostToDevice));
+        if (!tplan.link.empty()) {
+            HIPCHK(hipMemcpy(b->dTileLink + sh.tl0, tplan.link.data(), tplan.link.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(b->dTileEnds + sh.tl0, tplan.ends.data(), tplan.ends.size() * 4, hipMemcpyHostToDevice));
+        }
         sh.tiles = tiles;
-        sh.tplan = std::move(tplan);
+        sh.tplan.tiles = std::move(tplan.tiles);   // the descriptors stay on the host
+        sh.tplan.nPub = tplan.nPub;
         b->tiledSims.push_back(s);
         b->tileDirty = true;
     } else if (shape >= 0) {   // the kernel variant with contacts for a simulation with collisions
@@ -1786,6 +1780,10 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
     }
     if (sh.tiles) {
         b->tiledSims.erase(std::remove(b->tiledSims.begin(), b->tiledSims.end(), slot), b->tiledSims.end());
+        b->tileSlotPool.ra.release(sh.tv0, sh.tvN);
+        b->tileLinkPool.ra.release(sh.tl0, sh.tlN);
+        b->tileXchPool.ra.release(sh.tx0, sh.txN);
+        b->tilePartPool.ra.release(sh.tp0, sh.tiles);
         b->tileDirty = true;
     } else if (sh.shape >= 0) {
         auto& L = b->resLists[sh.shape];
