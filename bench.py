@@ -269,11 +269,13 @@ def config_leg(model, precision, steps, interval=0, kernel_steps=50, warmup=10):
     dt = time.perf_counter() - t0
     sts = b.poll()
     st = sts[s]
+    wg = [b.workgroups(x) for x in slots]
     kt = b.kernel_times(kernel_steps) if kernel_steps else {}
     b.close()
     out = {"value": V * steps / dt, "unit": "voxel-steps/s", "voxels": V, "links": L, "steps": steps,
            "sims": len(models), "ms_per_step": dt / steps * 1e3, "dtype": "f64" if f64 else "f32", "admit_s": admit_s,
            "state": int(st.state), "steps_done": int(min(sts[x].steps for x in slots)), "kernel_ms": kt,
+           "resident_workgroups": int(sum(wg)), "tiled_sims": int(sum(w > 1 for w in wg)),
            "alg_frac_per_step": bytes_per_step(V, L, f64) / (dt / steps) / 1e9 / HBM_PEAK_GBS}
     if "links" in kt:
         out["links_hbm_frac"] = (196 if f64 else 124) * L / (kt["links"] * 1e-3) / 1e9 / HBM_PEAK_GBS
@@ -285,6 +287,7 @@ def config_leg(model, precision, steps, interval=0, kernel_steps=50, warmup=10):
 
 def config_legs(args):
     """The configuration legs named in args.leg_names, each a batch of its own."""
+    import numpy as np
     from gpuvoxels_amd import _abi, workloads
     legs = {}
     names = set(args.leg_names.split(","))
@@ -305,6 +308,15 @@ def config_legs(args):
             legs["C3_arena_balls_f64"] = config_leg(m3, _abi.PREC_F64, args.leg_steps or 3000, interval=8)
         if "C3f32" in names:
             legs["C3_arena_balls_f32"] = config_leg(m3, _abi.PREC_F32, args.leg_steps or 3000, interval=8)
+    if "C4t" in names or "C4tf32" in names:
+        # the serving stream's large requests: 15 robots on 30^3 grids (about 13k voxels
+        # each), too large for one workgroup, on the tiled resident kernel
+        rng = np.random.default_rng(args.seed + 30)
+        mt = [workloads.robot(rng, grid=30, steps=0, name="big%02d" % i) for i in range(15)]
+        if "C4t" in names:
+            legs["C4_large_robots_tiled_f64"] = config_leg(mt, _abi.PREC_F64, args.leg_steps or 4000, kernel_steps=1000)
+        if "C4tf32" in names:
+            legs["C4_large_robots_tiled_f32"] = config_leg(mt, _abi.PREC_F32, args.leg_steps or 4000, kernel_steps=1000)
     for k, v in legs.items():
         log("leg %s: %.3e voxel-steps/s, %.4f ms/step, kernels %s" % (k, v["value"], v["ms_per_step"], v["kernel_ms"]))
     return legs
@@ -422,9 +434,10 @@ def main():
                          "engine, FP64 under tiny initial-velocity perturbations) on the robot batch")
     ap.add_argument("--no-legs", dest="legs", action="store_false",
                     help="skip the single-cube, slab and arena-balls configuration legs")
-    ap.add_argument("--legs", dest="leg_names", default="C0,C2,C1c,C3,C3f32",
+    ap.add_argument("--legs", dest="leg_names", default="C0,C2,C1c,C3,C3f32,C4t,C4tf32",
                     help="comma-separated configuration legs to run: C0 (single cube), C2 (slab), C1c (the robots "
-                         "with self-collision), C3 (arena balls, FP64), C3f32 (arena balls, FP32)")
+                         "with self-collision), C3 (arena balls, FP64), C3f32 (arena balls, FP32), C4t / C4tf32 (15 "
+                         "robots of about 13k voxels on the tiled resident kernel, FP64 / FP32)")
     ap.add_argument("--legs-only", action="store_true",
                     help="run only the configuration legs named by --legs (one JSON object of legs, no headline): "
                          "one batch per leg, for profiling a leg in a process of its own")
