@@ -14,9 +14,9 @@ friction stick/slip amplifies rounding differences, and only FP64 state keeps
 every robot's fitness within 0.5% of the CPU engine over the 20,000-step
 horizon.  The FP32 state path is timed beside it (`f32`).
 
-Unless --block/--steps-per-launch are given, the batch configuration (preferred
-workgroup size, steps per resident launch) is the autotuner's choice
-(vx3h_autotune) on this same workload, reported under `autotune`.
+Unless --block is given, the preferred workgroup size is the autotuner's choice
+(vx3h_autotune) on this same workload, reported under `autotune` with the whole
+trial table; resident launches run 1000 steps unless --steps-per-launch is given.
 
 Printed: one JSON line with voxel-steps/s (value), the roofline of the dominant
 kernel (HIP events on the engine's stream; for the resident kernel also its FP64
@@ -410,7 +410,7 @@ def main():
     ap.add_argument("--block", type=int, default=None,
                     help="preferred workgroup size (0 = engine order); default: chosen by the autotuner")
     ap.add_argument("--steps-per-launch", type=int, default=None,
-                    help="steps of one resident kernel launch; default: chosen by the autotuner")
+                    help="steps of one resident kernel launch; default 1000")
     ap.add_argument("--no-autotune", action="store_true", help="block 0 and 1000 steps per launch unless given")
     ap.add_argument("--tune-steps", type=int, default=2000, help="timed steps per autotuner candidate")
     ap.add_argument("--tune-warmup", type=int, default=4000,
@@ -498,7 +498,11 @@ def main():
                           for b_, s_, _, v_ in trials]}
         log("autotune (%s): %s in %.1f s" % ("f64" if p == _abi.PREC_F64 else "f32", rec["chosen"], rec["seconds"]))
         block = args.block if args.block is not None else cfg["block_size"]
-        spl_ = args.steps_per_launch if args.steps_per_launch is not None else cfg["steps_per_launch"]
+        # launches of 1000 steps unless given: the tuner's launch lengths differ by a few
+        # percent at most, and a fixed length keeps the launch durations comparable with
+        # the committed kernel profiles (profiles/kernel_trace_resident_512robots.txt)
+        spl_ = args.steps_per_launch if args.steps_per_launch is not None else 1000
+        rec["used"] = {"block_size": block, "steps_per_launch": spl_}
         return block, spl_, rec
 
     block, spl_cfg, tune_rec = tuned(prec)
