@@ -46,7 +46,9 @@
 // the end of a launch), and a repair launch right after runs those simulations again
 // from the same start to the link pass of that step, where the one-workgroup kernel
 // stops.  A tile that waits at a barrier longer than TileArgs::spin wall-clock ticks
-// fails the simulation instead of waiting on.
+// fails the simulation instead of waiting on.  A simulation of one tile (a one-workgroup
+// simulation placed in a tiled launch) has no barrier or exchange and steps exactly as
+// the one-workgroup kernel does.
 //
 // Included by the per-precision kernel units after vx3_kernels.inl (VX_NS, real).
 
@@ -212,6 +214,9 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     const int s = TILED ? td.sim : simList[blockIdx.x];
     SimCtl& scg = b.sims[s];
     const int tid = threadIdx.x;
+    // TILED: a simulation of one tile (a one-workgroup simulation run inside a tiled launch)
+    // has no barrier and no exchange: it steps as the one-workgroup kernel does
+    const bool multi = TILED && td.nTiles > 1;
     int divAt = -1;      // TILED: the launch's step at which a link diverged (a repair launch stops there)
     unsigned gen = 0;    // TILED: barriers passed in this launch
     if (scg.state != 0 || (scg.flags & SC_COL_GROW)) return;   // finished, or waiting to move to the per-step kernels
@@ -738,7 +743,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
         }
         // written before the last link-pass barrier; a tile learns of its simulation's
         // divergence only at the step's barrier and stops at that step on a second run
-        const bool diverged = TILED ? it == divAt : sDiv != 0;
+        const bool diverged = multi ? it == divAt : sDiv != 0;
         steps++;
         const bool sample = !diverged && cycleStep > 0 && steps % cycleStep == 0;
         double psx = 0, psy = 0, psz = 0, psm = 0;
@@ -831,7 +836,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             }
             if ((tid & 63) == 0) { sRed[0][tid >> 6] = psx; sRed[1][tid >> 6] = psy; sRed[2][tid >> 6] = psz; sRed[3][tid >> 6] = psm; }
         }
-        if (TILED && !diverged) {
+        if (multi && !diverged) {
             // ---- arrival at the simulation's barrier (TILED): the records are out; the wait
             // comes in the next step's link pass, or now on a sampling step (the centre of
             // mass needs every tile's partial sum)
@@ -856,7 +861,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             angleSamples++;
             if (tid == 0) {
                 double tot[4] = {0, 0, 0, 0};
-                if (TILED) {
+                if (multi) {
                     for (int q = 0; q < 4; q++) tot[q] = sRed[q][0];
                 } else {
                     for (int w = 0; w < BS / 64; w++)

@@ -111,6 +111,8 @@ struct SlotHost {
     int tiles = 0;               // > 0: runs on the tiled resident kernel as this many workgroups
     TilePlan tplan;              // its tiles (sim-local; the tables themselves live on the device)
     int tv0 = 0, tvN = 0, tl0 = 0, tlN = 0, tx0 = 0, txN = 0, tp0 = 0;   // its ranges in the tile pools
+    bool oneTile = false;        // a one-workgroup simulation with one-tile tables in the pools
+    int axisLinks = 0;           // links on its busiest axis
     std::vector<int> ijk;
     std::vector<int> matid;
     std::vector<int> measured;   // per voxel
@@ -214,6 +216,10 @@ struct vx3h_batch {
     hipStream_t tileStream = nullptr;
     hipEvent_t tileJoin = nullptr;
     bool tilePending = false;
+    // while tiled simulations run, the one-workgroup simulations without collisions run in
+    // the same cooperative launches as one-tile entries (a cooperative launch can only
+    // start once every CU it needs is free: beside them it would wait for theirs)
+    bool absorbed = false;
     // one stream per resident shape: each shape's launches run beside the stream-path
     // steps of the main stream (disjoint simulations) and beside the other shapes;
     // fork/join events order them with the main stream around every chunk of steps
@@ -468,18 +474,68 @@ template <typename T> vx3h_status growDev(T** p, size_t* cap, size_t n) {
 // per step whether full or not), and their tile descriptors pointing at each
 // simulation's ranges in the tile pools (filled at admission), with two sync words per
 // simulation from its group's base.
+// One-tile tables of a one-workgroup simulation (its voxels and links in order, no halo,
+// no exchange records), uploaded once.
+vx3h_status ensureOneTile(vx3h_batch* b, int s) {
+    SlotHost& sh = b->slots[s];
+    if (sh.oneTile) return VX3H_OK;
+    TileDesc d{};
+    d.tile = 0; d.nTiles = 1; d.vOff = 0; d.nOwn = sh.nv; d.nHalo = 0; d.lOff = 0; d.nLinks = sh.nl;
+    std::vector<int> vox(sh.nv), xidx(sh.nv, -1), link(sh.nl), ends(sh.nl);
+    for (int i = 0; i < sh.nv; i++) vox[i] = i;
+    for (int l = 0; l < sh.nl; l++) {
+        link[l] = l | (int)0x80000000u;
+        ends[l] = sh.lneg[l] | (sh.lpos[l] << 16);
+    }
+    sh.tvN = std::max(1, sh.nv);
+    sh.tlN = std::max(1, sh.nl);
+    sh.txN = 1;
+    vx3h_status st;
+    if ((st = poolAlloc(b, b->tileSlotPool, sh.tvN, sh.tv0)) || (st = poolAlloc(b, b->tileLinkPool, sh.tlN, sh.tl0)) ||
+        (st = poolAlloc(b, b->tileXchPool, sh.txN, sh.tx0)) || (st = poolAlloc(b, b->tilePartPool, 1, sh.tp0)))
+        return st;
+    if (sh.nv) {
+        HIPCHK(hipMemcpy(b->dTileVox + sh.tv0, vox.data(), vox.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->dTileXidx + sh.tv0, xidx.data(), xidx.size() * 4, hipMemcpyHostToDevice));
+    }
+    if (sh.nl) {
+        HIPCHK(hipMemcpy(b->dTileLink + sh.tl0, link.data(), link.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->dTileEnds + sh.tl0, ends.data(), ends.size() * 4, hipMemcpyHostToDevice));
+    }
+    sh.tplan.tiles.assign(1, d);
+    sh.tplan.nPub = 0;
+    sh.oneTile = true;
+    return VX3H_OK;
+}
+
 vx3h_status syncTiles(vx3h_batch* b) {
     if (!b->tileDirty) return VX3H_OK;
     std::vector<int> order = b->tiledSims;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->slots[x].tiles > b->slots[y].tiles; });
+    // with tiled simulations running, the one-workgroup simulations without collisions
+    // (lists [0, shapes)) fill the launches' free slots as one-tile entries, when every one
+    // of them fits the tiled kernel's shape
+    b->absorbed = !order.empty();
+    const ResidentShapeInfo& ts = b->shapes[b->tiledShape];
+    for (size_t i = 0; b->absorbed && i < b->shapes.size(); i++)
+        for (int s : b->resLists[i])
+            if (b->slots[s].nv > ts.block * ts.vpt || b->slots[s].axisLinks > ts.block * ts.lpa) b->absorbed = false;
+    if (b->absorbed)
+        for (size_t i = 0; i < b->shapes.size(); i++)
+            for (int s : b->resLists[i]) {
+                vx3h_status st = ensureOneTile(b, s);
+                if (st) return st;
+                order.push_back(s);
+            }
+    auto tilesOf = [&](int s) { return b->slots[s].tiles ? b->slots[s].tiles : 1; };
     std::vector<std::vector<int>> bins;
     std::vector<int> fill;
     for (int s : order) {
         size_t i = 0;
-        while (i < bins.size() && fill[i] + b->slots[s].tiles > b->tileCap) i++;
+        while (i < bins.size() && fill[i] + tilesOf(s) > b->tileCap) i++;
         if (i == bins.size()) { bins.emplace_back(); fill.push_back(0); }
         bins[i].push_back(s);
-        fill[i] += b->slots[s].tiles;
+        fill[i] += tilesOf(s);
     }
     std::vector<TileDesc> T;
     b->tileGroups.clear();
@@ -498,7 +554,7 @@ vx3h_status syncTiles(vx3h_batch* b) {
                 d.pOff = sh.tp0 * 8;
                 T.push_back(d);
             }
-            g.count += sh.tiles;
+            g.count += tilesOf(s);
             g.nSync += 2;
             syncTot += 2;
         }
@@ -526,7 +582,7 @@ vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
     bool forked = false;
     for (size_t i = 0; i < b->resLists.size(); i++) {
         const int n = (int)b->resLists[i].size();
-        if (!n) continue;
+        if (!n || (b->absorbed && i < b->shapes.size())) continue;   // absorbed: they run in the tiled launches
         if (!forked) {
             if (!b->forkEv) HIPCHK(hipEventCreateWithFlags(&b->forkEv, hipEventDisableTiming));
             HIPCHK(hipEventRecord(b->forkEv, b->stream));
@@ -1138,6 +1194,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     sh.fit = H.fitTok;
     sh.bytes = need;
     sh.shape = shape;
+    sh.axisLinks = std::max(perAxis[0], std::max(perAxis[1], perAxis[2]));
     sh.nb0 = nb0; sh.nbN = nbN; sh.h0 = h0; sh.hN = hN; sh.near0 = near0; sh.nearN = nearN; sh.c0 = c0;
     sh.tg0 = tg0; sh.nTg = nTg;
     sh.at0 = at0; sh.ap0 = ap0; sh.apN = apN;
@@ -1164,6 +1221,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
         sh.shape = shape + (H.collisions ? (int)b->shapes.size() : 0);
         b->resLists[sh.shape].push_back(s);
         b->resDirty = true;
+        b->tileDirty = true;
     }
     else b->nStreamSims++;
     sh.vlinks = H.voxelLinks;
@@ -1446,7 +1504,7 @@ extern "C" vx3h_status vx3h_batch_kernel_times(vx3h_batch* b, int n_steps, doubl
     if (st || (st = joinResident(b))) return st;
     HIPCHK(hipStreamSynchronize(b->stream));
     for (size_t i = 0; i < nl; i++) {
-        if (i < b->resLists.size() && b->resLists[i].empty()) continue;
+        if (i < b->resLists.size() && (b->resLists[i].empty() || (b->absorbed && i < b->shapes.size()))) continue;
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, rev[2 * i], rev[2 * i + 1]));
         avg_ms[5] += ms;
@@ -1672,6 +1730,7 @@ vx3h_status growCollisionTables(vx3h_batch* b, std::vector<SimCtl>& ctl) {
             auto& L = b->resLists[sh.shape];
             L.erase(std::remove(L.begin(), L.end(), s), L.end());
             b->resDirty = true;
+            b->tileDirty = true;
             sh.shape = -1;
             b->nStreamSims++;
             c.flags &= ~SC_RESIDENT;
@@ -1789,7 +1848,14 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
         auto& L = b->resLists[sh.shape];
         L.erase(std::remove(L.begin(), L.end(), slot), L.end());
         b->resDirty = true;
+        b->tileDirty = true;
     } else b->nStreamSims--;
+    if (sh.oneTile) {
+        b->tileSlotPool.ra.release(sh.tv0, sh.tvN);
+        b->tileLinkPool.ra.release(sh.tl0, sh.tlN);
+        b->tileXchPool.ra.release(sh.tx0, sh.txN);
+        b->tilePartPool.ra.release(sh.tp0, 1);
+    }
     b->bytesUsed -= sh.bytes;
     sh = SlotHost();
     b->db.vHi = b->va.hi;

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 
 namespace vx3 {
 namespace {
@@ -36,13 +37,16 @@ struct Planner {
     int other(int l, int v) const { return lneg[l] == v ? lpos[l] : lneg[l]; }
 
     // owned + halo voxels within vmax, links with an owned end within lax per axis
-    bool fits(const std::vector<int>& part) {
-        if ((int)part.size() > vmax) return false;
+    int lastNeed = 0;   // slots (owned + halo) of the last block that did not fit
+    bool fits(const int* part, size_t n) {
+        lastNeed = (int)n;
+        if ((int)n > vmax) return false;
         ++epoch;
         const int own = 2 * epoch, halo = own + 1;
-        for (int v : part) vmark[v] = own;
+        for (size_t i = 0; i < n; i++) vmark[part[i]] = own;
         int nh = 0, per[3] = {0, 0, 0};
-        for (int v : part)
+        for (size_t j = 0; j < n; j++) {
+            const int v = part[j];
             for (int i = inc0[v]; i < inc0[v + 1]; i++) {
                 const int l = inc[i];
                 if (lmark[l] == epoch) continue;
@@ -51,35 +55,40 @@ struct Planner {
                 const int u = other(l, v);
                 if (vmark[u] != own && vmark[u] != halo) { vmark[u] = halo; nh++; }
             }
-        return (int)part.size() + nh <= vmax && per[0] <= lax && per[1] <= lax && per[2] <= lax;
+        }
+        lastNeed = (int)n + nh;
+        return (int)n + nh <= vmax && per[0] <= lax && per[1] <= lax && per[2] <= lax;
     }
 
-    // k: the number of blocks this part is meant to become.  strict: a block that does not
-    // fit fails the plan (the caller tries more blocks); else it is halved.
+    // k: the number of blocks the voxels part[0, n) are meant to become (the array is
+    // reordered in place).  strict: a block that does not fit fails the plan (the caller
+    // tries more blocks); else it is halved.
     bool strict = false;
-    void cut(std::vector<int>& part, int k) {
-        if (failed || part.empty()) return;
+    void cut(int* part, size_t n, int k) {
+        if (failed || n == 0) return;
         if (k <= 1) {
-            if (fits(part)) { parts.push_back(part); return; }
-            if (strict || part.size() == 1) { failed = true; return; }
+            if (fits(part, n)) { parts.emplace_back(part, part + n); return; }
+            if (strict || n == 1) { failed = true; return; }
             k = 2;
         }
         int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        for (int v : part)
-            for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], ijk[3 * v + a]); hi[a] = std::max(hi[a], ijk[3 * v + a]); }
+        for (size_t i = 0; i < n; i++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], ijk[3 * part[i] + a]);
+                hi[a] = std::max(hi[a], ijk[3 * part[i] + a]);
+            }
         int ax = 0;
         for (int a = 1; a < 3; a++) if (hi[a] - lo[a] > hi[ax] - lo[ax]) ax = a;
         const int kl = k / 2;
-        size_t n = (size_t)((double)part.size() * kl / k + 0.5);
-        n = std::max<size_t>(1, std::min(n, part.size() - 1));
-        // the n lowest along the axis (ties by voxel index): a selection, not a sort
-        std::nth_element(part.begin(), part.begin() + n, part.end(), [&](int x, int y) {
+        size_t m = (size_t)((double)n * kl / k + 0.5);
+        m = std::max<size_t>(1, std::min(m, n - 1));
+        // the m lowest along the axis (ties by voxel index): a selection, not a sort
+        std::nth_element(part, part + m, part + n, [&](int x, int y) {
             const int cx = ijk[3 * x + ax], cy = ijk[3 * y + ax];
             return cx != cy ? cx < cy : x < y;
         });
-        std::vector<int> left(part.begin(), part.begin() + n), right(part.begin() + n, part.end());
-        cut(left, kl);
-        cut(right, k - kl);
+        cut(part, m, kl);
+        cut(part + m, n - m, k - kl);
     }
 };
 
@@ -92,22 +101,23 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
     p.nv = nv; p.nl = nl; p.ijk = ijk; p.lneg = lneg; p.lpos = lpos; p.laxis = laxis; p.vmax = vmax; p.lax = lax;
     p.index();
     // Equal blocks: every tile's step takes as long as the largest block's, so the fewest
-    // blocks of equal size that all fit, found by raising their number a few percent at a
-    // time from a fill of 85% of the slots; a shape no equal split fits falls back to
-    // halving the blocks that do not.
+    // blocks of equal size that all fit, from a fill of 80% of the slots; a plan that fails
+    // tries next the count at which its overflowing block's owned + halo would have fit
+    // (blocks of n voxels need about n (1 + halo ratio) slots).  A shape no equal split
+    // fits falls back to halving the blocks that do not.
     std::vector<std::vector<int>> best;
+    std::vector<int> all(nv);
     auto attempt = [&](int k, bool strict) {
-        std::vector<int> all(nv);
         for (int v = 0; v < nv; v++) all[v] = v;
         p.parts.clear();
         p.failed = false;
         p.strict = strict;
-        p.cut(all, k);
+        p.cut(all.data(), all.size(), k);
         return !p.failed;
     };
-    for (int k = std::max(1, (int)((nv + 0.85 * vmax - 1) / (0.85 * vmax))); k <= maxTiles;
-         k = std::max(k + 1, (int)(k * 1.03))) {
+    for (int k = std::max(1, (int)((nv + 0.8 * vmax - 1) / (0.8 * vmax))); k <= maxTiles;) {
         if (attempt(k, true)) { best = p.parts; break; }
+        k = std::max(k + 1, (int)std::ceil((double)k * p.lastNeed / vmax));
     }
     if (best.empty())
         for (double frac : {0.8, 0.6}) {
@@ -118,36 +128,50 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
 
     const int T = (int)best.size();
     std::vector<int> owner(nv, -1);
-    for (int t = 0; t < T; t++) {
-        std::sort(best[t].begin(), best[t].end());
+    for (int t = 0; t < T; t++)
         for (int v : best[t]) owner[v] = t;
+    // per voxel: does a voxel of another tile link to it
+    std::vector<char> shared(nv, 0);
+    for (int l = 0; l < nl; l++)
+        if (owner[lneg[l]] != owner[lpos[l]]) shared[lneg[l]] = shared[lpos[l]] = 1;
+    // the links of each tile (those with an owned end), bucketed in link order
+    std::vector<int> lcount(T + 1, 0);
+    for (int l = 0; l < nl; l++) {
+        lcount[owner[lneg[l]] + 1]++;
+        if (owner[lpos[l]] != owner[lneg[l]]) lcount[owner[lpos[l]] + 1]++;
+    }
+    for (int t = 0; t < T; t++) lcount[t + 1] += lcount[t];
+    std::vector<int> tlinks(lcount[T]);
+    {
+        std::vector<int> at(lcount.begin(), lcount.end() - 1);
+        for (int l = 0; l < nl; l++) {
+            tlinks[at[owner[lneg[l]]]++] = l;
+            if (owner[lpos[l]] != owner[lneg[l]]) tlinks[at[owner[lpos[l]]]++] = l;
+        }
     }
     out->tiles.clear(); out->vox.clear(); out->link.clear(); out->ends.clear(); out->xidx.clear();
+    out->vox.reserve(nv + nv / 2); out->xidx.reserve(nv + nv / 2);
+    out->link.reserve(tlinks.size()); out->ends.reserve(tlinks.size());
     std::vector<int> slot(nv, -1);
     std::vector<int> rec(nv, -1);   // exchange record of each voxel another tile reads
     int nPub = 0;
+    std::vector<int> own, halo;
     for (int t = 0; t < T; t++) {
+        std::vector<int>& part = best[t];
+        std::sort(part.begin(), part.end());
         // owned voxels another tile reads take the lowest slots (each thread's first voxels:
         // their records are written first and drain while the rest integrate)
-        std::vector<int> own;
-        for (int pass = 0; pass < 2; pass++)
-            for (int v : best[t]) {
-                bool shared = false;
-                for (int j = p.inc0[v]; j < p.inc0[v + 1]; j++) shared |= owner[p.other(p.inc[j], v)] != t;
-                if (shared == (pass == 0)) own.push_back(v);
-            }
-        std::vector<int> halo, links;
-        for (int v : own)
-            for (int i = p.inc0[v]; i < p.inc0[v + 1]; i++) {
-                const int l = p.inc[i];
-                links.push_back(l);
-                const int u = p.other(l, v);
-                if (owner[u] != t) halo.push_back(u);
-            }
+        own.clear();
+        for (int v : part) if (shared[v]) own.push_back(v);
+        for (int v : part) if (!shared[v]) own.push_back(v);
+        halo.clear();
+        for (int i = lcount[t]; i < lcount[t + 1]; i++) {
+            const int l = tlinks[i];
+            if (owner[lneg[l]] != t) halo.push_back(lneg[l]);
+            if (owner[lpos[l]] != t) halo.push_back(lpos[l]);
+        }
         std::sort(halo.begin(), halo.end());
         halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
-        std::sort(links.begin(), links.end());
-        links.erase(std::unique(links.begin(), links.end()), links.end());
         TileDesc d{};
         d.tile = t;
         d.nTiles = T;
@@ -155,14 +179,12 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
         d.nOwn = (int)own.size();
         d.nHalo = (int)halo.size();
         d.lOff = (int)out->link.size();
-        d.nLinks = (int)links.size();
+        d.nLinks = lcount[t + 1] - lcount[t];
         for (int i = 0; i < d.nOwn; i++) {
             const int v = own[i];
-            bool shared = false;
-            for (int j = p.inc0[v]; j < p.inc0[v + 1]; j++) shared |= owner[p.other(p.inc[j], v)] != t;
-            out->vox.push_back(v | (shared ? (int)0x80000000u : 0));
-            if (shared) rec[v] = nPub++;
-            out->xidx.push_back(shared ? rec[v] : -1);
+            out->vox.push_back(v | (shared[v] ? (int)0x80000000u : 0));
+            if (shared[v]) rec[v] = nPub++;
+            out->xidx.push_back(shared[v] ? rec[v] : -1);
             slot[v] = i;
         }
         for (int i = 0; i < d.nHalo; i++) {
@@ -170,7 +192,8 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
             out->xidx.push_back(-2 - halo[i]);   // the owner's record, filled in below
             slot[halo[i]] = d.nOwn + i;
         }
-        for (int l : links) {
+        for (int i = lcount[t]; i < lcount[t + 1]; i++) {
+            const int l = tlinks[i];
             out->link.push_back(l | (owner[lneg[l]] == t ? (int)0x80000000u : 0));
             out->ends.push_back(slot[lneg[l]] | (slot[lpos[l]] << 16));
         }

@@ -130,3 +130,25 @@ def test_several_launch_groups_and_a_mixed_batch():
     assert_same(bt, st, bs, ss, 1e-14, 1e-13)
     bt.close()
     bs.close()
+
+
+def test_one_workgroup_sims_inside_tiled_launches_equal_their_own_kernel():
+    """While tiled simulations run, one-workgroup simulations run inside the same
+    cooperative launches as one-tile entries: the same results, bit for bit, as on their
+    own kernel."""
+    small = [robot(40 + i, 9) for i in range(3)]
+    small[1].stop_tokens = [(MT["mtCONST"], 0.01), (MT["mtVAR"], 4.0), (MT["mtSUB"], 0.0), (MT["mtEND"], 0.0)]
+    big = robot(50, 22)
+    bm, sm = run(small + [big], _abi.PATH_AUTO, 1200, spl=400)
+    assert [bm.workgroups(s) for s in sm[:3]] == [1, 1, 1] and bm.workgroups(sm[3]) > 1
+    ba, sa = run(small, _abi.PATH_AUTO, 1200, spl=400)
+    pm, pa = bm.poll(), ba.poll()
+    for a, c in zip(sm[:3], sa):
+        ga, gc = bm.read_state(a), ba.read_state(c)
+        assert np.array_equal(ga["pos"], gc["pos"]) and np.array_equal(ga["orient"], gc["orient"])
+        assert np.array_equal(ga["flags"], gc["flags"])
+        assert pm[a].steps == pa[c].steps and pm[a].state == pa[c].state
+        assert pm[a].fitness == pa[c].fitness and pm[a].current_time == pa[c].current_time
+    assert pm[sm[1]].state == _abi.SIM_DONE and pm[sm[1]].steps < 1200
+    bm.close()
+    ba.close()
