@@ -152,3 +152,26 @@ def test_one_workgroup_sims_inside_tiled_launches_equal_their_own_kernel():
     assert pm[sm[1]].state == _abi.SIM_DONE and pm[sm[1]].steps < 1200
     bm.close()
     ba.close()
+
+
+def test_single_steps_and_the_no_allocation_step_on_tiled_simulations():
+    """doTimeStep (vx3h_sim_step) on a tiled simulation between tiled launches, and the
+    no-allocation entry point (every simulation on the per-step kernels) on a batch with
+    a tiled simulation: the same states as the per-step kernels throughout."""
+    m = robot(60, 16)
+    bt = engine.Batch(precision=_abi.PREC_F64, path=_abi.PATH_AUTO, steps_per_launch=200)
+    st = bt.admit(m)
+    assert bt.workgroups(st) > 1
+    bs = engine.Batch(precision=_abi.PREC_F64, path=_abi.PATH_STREAM)
+    ss = bs.admit(m)
+    bt.step(300)
+    for _ in range(5):
+        assert bt.do_time_step(st)
+    bt.step(200)
+    bt.step_noalloc(50)
+    bt.step(250)
+    bs.step(805)
+    assert_same(bt, [st], bs, [ss], 1e-14, 1e-13)
+    assert bt.poll()[st].steps == 805
+    bt.close()
+    bs.close()
