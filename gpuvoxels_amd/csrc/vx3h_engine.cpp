@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -773,13 +774,57 @@ extern "C" vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* d, int precision,
     return VX3H_OK;
 }
 
+namespace vx3 {
+struct PreparedSim {
+    HostModel H;
+    bool planned = false;   // the plan below is the tiled shape's (or none fits one launch)
+    int vmax = 0, lax = 0, cap = 0;
+    TilePlan plan;
+    bool planOk = false;
+};
+
+// the tile plan a simulation of this model gets if it is too large for one workgroup
+// (planned for any model over the tiled shape's one-workgroup capacity without collisions;
+// admission decides whether to use it)
+vx3h_status prepareSim(const vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim** out) {
+    *out = nullptr;
+    if (!b || !desc) FAILV(VX3H_EINVAL, "null argument");
+    PreparedSim* p = new PreparedSim();
+    if (!buildModel(desc, p->H, g_err)) { delete p; return VX3H_EINVAL; }
+    const HostModel& H = p->H;
+    if (b->tiledShape >= 0 && b->tileCap > 1 && !H.collisions && H.nv > 0) {
+        const ResidentShapeInfo& r = b->shapes[b->tiledShape];
+        int perAxis[3] = {0, 0, 0};
+        for (int i = 0; i < H.nl; i++) perAxis[H.laxis[i]]++;
+        const int lax = r.block * r.lpa, vmax = r.block * r.vpt;
+        if (H.nv > vmax || perAxis[0] > lax || perAxis[1] > lax || perAxis[2] > lax) {
+            p->planned = true;
+            p->vmax = vmax; p->lax = lax; p->cap = b->tileCap;
+            p->planOk = planTiles(H.nv, H.ijk.data(), H.nl, H.lneg.data(), H.lpos.data(), H.laxis.data(), vmax, lax,
+                                  b->tileCap, &p->plan);
+        }
+    }
+    *out = p;
+    return VX3H_OK;
+}
+
+void freePrepared(PreparedSim* p) { delete p; }
+}  // namespace vx3
+
 extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, int* slot) {
+    return vx3::admitPrepared(b, desc, nullptr, slot);
+}
+
+namespace vx3 {
+vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim* pre, int* slot) {
+    std::unique_ptr<PreparedSim> own(pre);
     if (!b || !desc || !slot) FAILV(VX3H_EINVAL, "null argument");
     if (desc->stop_condition_type < VX3H_SC_NONE || desc->stop_condition_type > VX3H_SC_MIN_MAXMOVE)
         FAILV(VX3H_EINVAL, "unknown stop condition type");
     HIPCHK(hipSetDevice(b->device));
-    HostModel H;
-    if (!buildModel(desc, H, g_err)) return VX3H_EINVAL;
+    HostModel Hbuilt;
+    if (!pre && !buildModel(desc, Hbuilt, g_err)) return VX3H_EINVAL;
+    HostModel& H = pre ? pre->H : Hbuilt;
     if (H.stopTok.size() > 64) FAILV(VX3H_EUNSUPPORTED, "stop condition longer than 64 tokens");
     for (int k = 0; k < 3; k++)
         if (H.ffTok[k].size() > 64) FAILV(VX3H_EUNSUPPORTED, "force-field program longer than 64 tokens");
@@ -965,10 +1010,14 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     if (b->cfg.path != VX3H_PATH_STREAM && NV > 0 && resOk && shape < 0 && !H.collisions && b->tiledShape >= 0 &&
         b->tileCap > 1) {
         const ResidentShapeInfo& r = b->shapes[b->tiledShape];
-        if (planTiles(NV, H.ijk.data(), NL, H.lneg.data(), H.lpos.data(), H.laxis.data(), r.block * r.vpt, r.block * r.lpa,
-                      b->tileCap, &tplan) &&
-            tplan.tiles.size() > 1)
-            tiles = (int)tplan.tiles.size();
+        bool ok;
+        if (pre && pre->planned && pre->vmax == r.block * r.vpt && pre->lax == r.block * r.lpa && pre->cap == b->tileCap) {
+            ok = pre->planOk;   // planned ahead for this shape
+            tplan = std::move(pre->plan);
+        } else
+            ok = planTiles(NV, H.ijk.data(), NL, H.lneg.data(), H.lpos.data(), H.laxis.data(), r.block * r.vpt, r.block * r.lpa,
+                           b->tileCap, &tplan);
+        if (ok && tplan.tiles.size() > 1) tiles = (int)tplan.tiles.size();
     }
     if (b->cfg.path == VX3H_PATH_RESIDENT && NV > 0 && resOk && shape < 0 && !tiles)
         FAILV(VX3H_EUNSUPPORTED, H.collisions ? "simulation with collisions too large for one workgroup of the resident kernel"
@@ -1254,6 +1303,7 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
     *slot = s;
     return VX3H_OK;
 }
+}  // namespace vx3
 
 extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
     if (!b || n_steps < 0) FAILV(VX3H_EINVAL, "bad argument");

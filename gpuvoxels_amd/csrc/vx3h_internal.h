@@ -10,5 +10,13 @@ extern thread_local std::string g_err;   // last error of the calling thread (vx
 // status (centre of mass, fitness, distances: positions read back) of each simulation
 // not running, after one read of the control blocks for all of them
 vx3h_status pollFinished(vx3h_batch* b, std::vector<vx3h_sim_status>& finished);
+// vx3h_sim_admit in two parts: the host model and (for a simulation too large for one
+// workgroup) its tile plan, built on the calling thread with nothing shared written
+// (several may run at once, on different threads); then the admission itself.  A
+// prepared simulation is for the batch it was prepared for; admitPrepared frees it.
+struct PreparedSim;
+vx3h_status prepareSim(const vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim** out);
+vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim* p, int* slot);
+void freePrepared(PreparedSim* p);
 }
 #endif

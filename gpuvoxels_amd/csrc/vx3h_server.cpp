@@ -9,12 +9,14 @@
 // freeing its ranges for the next admissions.  Admission, retirement and result
 // reads happen between rounds of quantum_steps steps.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vx3h.h"
@@ -133,23 +135,46 @@ extern "C" vx3h_status vx3h_server_submit(vx3h_server* sv, const vx3h_sim_desc* 
 
 extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
     if (!sv) { g_err = "null server"; return VX3H_EINVAL; }
-    // admit in arrival order while the footprint fits
-    for (;;) {
-        Request* r = nullptr;
-        {
-            std::lock_guard<std::mutex> lk(sv->mu);
-            if (sv->queue.empty()) break;
+    // admit in arrival order while the footprint fits: the host models and tile plans of
+    // this round's admissions are prepared on several threads, then admitted in order
+    std::vector<Request*> take;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        size_t bytes = 0;
+        while (!sv->queue.empty()) {
             Request* h = sv->queue.front();
-            if (sv->cfg.max_sims > 0 && (int)sv->running.size() >= sv->cfg.max_sims) break;
-            if (sv->budget && sv->used + h->bytes > sv->budget) break;
+            if (sv->cfg.max_sims > 0 && (int)(sv->running.size() + take.size()) >= sv->cfg.max_sims) break;
+            if (sv->budget && sv->used + bytes + h->bytes > sv->budget) break;
             sv->queue.pop_front();
-            r = h;
+            take.push_back(h);
+            bytes += h->bytes;
         }
+    }
+    std::vector<vx3::PreparedSim*> prep(take.size(), nullptr);
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>(take.size(), std::min(16u, hw));
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t i; (i = next.fetch_add(1)) < take.size();)
+                if (vx3::prepareSim(sv->batch, &take[i]->desc.d, &prep[i]) != VX3H_OK) prep[i] = nullptr;
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+    }
+    for (size_t i = 0; i < take.size(); i++) {
+        Request* r = take[i];
         int slot = -1;
-        vx3h_status st = vx3h_sim_admit(sv->batch, &r->desc.d, &slot);
+        vx3h_status st = vx3::admitPrepared(sv->batch, &r->desc.d, prep[i], &slot);   // frees prep[i]
+        prep[i] = nullptr;
         if (st == VX3H_ENOMEM && !sv->running.empty()) {   // the engine's own budget: wait for retirements
             std::lock_guard<std::mutex> lk(sv->mu);
-            sv->queue.push_front(r);
+            for (size_t j = take.size(); j-- > i;) {
+                if (j > i) { vx3::freePrepared(prep[j]); prep[j] = nullptr; }
+                sv->queue.push_front(take[j]);
+            }
             break;
         }
         if (st != VX3H_OK) {
