@@ -777,6 +777,7 @@ extern "C" vx3h_status vx3h_sim_footprint(const vx3h_sim_desc* d, int precision,
 namespace vx3 {
 struct PreparedSim {
     HostModel H;
+    long long nv = 0, nlCap = 0;   // voxels and link slots (room for attached links included)
     bool planned = false;   // the plan below is the tiled shape's (or none fits one launch)
     int vmax = 0, lax = 0, cap = 0;
     TilePlan plan;
@@ -792,6 +793,8 @@ vx3h_status prepareSim(const vx3h_batch* b, const vx3h_sim_desc* desc, PreparedS
     PreparedSim* p = new PreparedSim();
     if (!buildModel(desc, p->H, g_err)) { delete p; return VX3H_EINVAL; }
     const HostModel& H = p->H;
+    p->nv = H.nv;
+    p->nlCap = desc->enable_attach && H.nv > 0 ? std::max(H.nl, 3 * H.nv) : H.nl;
     if (b->tiledShape >= 0 && b->tileCap > 1 && !H.collisions && H.nv > 0) {
         const ResidentShapeInfo& r = b->shapes[b->tiledShape];
         int perAxis[3] = {0, 0, 0};
@@ -809,6 +812,22 @@ vx3h_status prepareSim(const vx3h_batch* b, const vx3h_sim_desc* desc, PreparedS
 }
 
 void freePrepared(PreparedSim* p) { delete p; }
+
+void preparedSize(const PreparedSim* p, long long* nv, long long* nl) {
+    *nv = p ? p->nv : 0;
+    *nl = p ? p->nlCap : 0;
+}
+
+vx3h_status reserveBatch(vx3h_batch* b, long long nv, long long nl) {
+    if (!b) FAILV(VX3H_EINVAL, "null batch");
+    HIPCHK(hipSetDevice(b->device));
+    vx3h_status st;
+    const long long vneed = b->va.hi + nv, lneed = b->la.hi + nl;
+    if (vneed > INT32_MAX || lneed > INT32_MAX) FAILV(VX3H_ENOMEM, "batch beyond 2^31 voxels or links");
+    if (vneed > b->db.Vcap && (st = growArrays(b, b->vArrs, b->db.Vcap, (int)vneed, b->va.hi, true))) return st;
+    if (lneed > b->db.Lcap && (st = growArrays(b, b->lArrs, b->db.Lcap, (int)lneed, b->la.hi, false))) return st;
+    return VX3H_OK;
+}
 }  // namespace vx3
 
 extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, int* slot) {

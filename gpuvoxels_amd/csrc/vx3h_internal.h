@@ -18,5 +18,10 @@ struct PreparedSim;
 vx3h_status prepareSim(const vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim** out);
 vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim* p, int* slot);
 void freePrepared(PreparedSim* p);
+// voxels and link slots a prepared simulation takes
+void preparedSize(const PreparedSim* p, long long* nv, long long* nl);
+// room for nv more voxels and nl more links past the batch arrays' high-water marks, in
+// one growth (admissions that follow then grow nothing)
+vx3h_status reserveBatch(vx3h_batch* b, long long nv, long long nl);
 }
 #endif

@@ -164,6 +164,16 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
         work();
         for (auto& th : pool) th.join();
     }
+    if (take.size() > 1) {   // one growth of the batch arrays for the whole round
+        long long nv = 0, nl = 0;
+        for (vx3::PreparedSim* p : prep) {
+            long long a, c;
+            vx3::preparedSize(p, &a, &c);
+            nv += a;
+            nl += c;
+        }
+        vx3::reserveBatch(sv->batch, nv, nl);   // best effort: admission grows what is still short
+    }
     for (size_t i = 0; i < take.size(); i++) {
         Request* r = take[i];
         int slot = -1;
