@@ -247,6 +247,11 @@ struct vx3h_batch {
     size_t loadsFs = 0;   // slot stride of that buffer at that step
     int loadsNv = 0;      // voxels [0, loadsNv) it holds
     bool resumePending = false;
+    // host-to-device uploads of admissions: staged in pinned host memory and copied on the
+    // main stream, so an admission waits for the device once (at the next admission's
+    // start) rather than once per array; a region is reused only after the stream has run
+    char* stage = nullptr;
+    size_t stageCap = 0, stageUsed = 0;
 };
 
 namespace {
@@ -296,6 +301,43 @@ void initSpecs(vx3h_batch* b) {
     L(&d.strain, 4, 1, false); L(&d.maxStrain, 4, 1, false); L(&d.strainOffset, 4, 1, false); L(&d.lnew, 4, 1, false);
 }
 
+// room for bytes in the staging buffer: waits for the main stream (every copy from the
+// buffer has run) when the buffer is full, and grows it when it is too small
+vx3h_status stageAlloc(vx3h_batch* b, size_t bytes, char** out) {
+    const size_t r = (bytes + 255) & ~(size_t)255;
+    if (b->stageUsed + r > b->stageCap) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->stageUsed = 0;
+        if (r > b->stageCap) {
+            if (b->stage) HIPCHK(hipHostFree(b->stage));
+            b->stage = nullptr;
+            b->stageCap = 0;
+            const size_t c = std::max(r, std::max(2 * b->stageCap, (size_t)16 << 20));
+            HIPCHK(hipHostMalloc((void**)&b->stage, c, hipHostMallocDefault));
+            b->stageCap = c;
+        }
+    }
+    *out = b->stage + b->stageUsed;
+    b->stageUsed += r;
+    return VX3H_OK;
+}
+
+// host-to-device copy on the main stream through the staging buffer (the source may be
+// freed on return)
+vx3h_status upload(vx3h_batch* b, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return VX3H_OK;
+    char* p;
+    if (vx3h_status st = stageAlloc(b, bytes, &p)) return st;
+    memcpy(p, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, b->stream));
+    return VX3H_OK;
+}
+
+vx3h_status fillBytes(vx3h_batch* b, void* dst, int value, size_t bytes) {
+    if (bytes) HIPCHK(hipMemsetAsync(dst, value, bytes, b->stream));
+    return VX3H_OK;
+}
+
 vx3h_status growArrays(vx3h_batch* b, std::vector<ArrSpec>& arrs, int& cap, int need, int used, bool isVoxel) {
     if (need <= cap) return VX3H_OK;
     int ncap = std::max(need, std::max(2 * cap, 1024));
@@ -335,8 +377,10 @@ vx3h_status poolAlloc(vx3h_batch* b, DevPool& p, int n, int& start) {
     for (auto& a : p.arrs) {
         void* np = nullptr;
         HIPCHK(hipMalloc(&np, (size_t)ncap * a.second));
-        HIPCHK(hipMemset(np, 0xFF, (size_t)ncap * a.second));   // hashSim: no owner
-        if (*a.first && p.cap) HIPCHK(hipMemcpy(np, *a.first, (size_t)p.cap * a.second, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemsetAsync(np, 0xFF, (size_t)ncap * a.second, b->stream));   // hashSim: no owner
+        if (*a.first && p.cap)
+            HIPCHK(hipMemcpyAsync(np, *a.first, (size_t)p.cap * a.second, hipMemcpyDeviceToDevice, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
         if (*a.first) HIPCHK(hipFree(*a.first));
         *a.first = np;
     }
@@ -357,8 +401,7 @@ template <typename E> vx3h_status uploadTable(vx3h_batch* b, void*& dptr, size_t
         cap = nc;
         b->layout++;
     }
-    HIPCHK(hipMemcpy(dptr, v.data(), v.size() * sizeof(E), hipMemcpyHostToDevice));
-    return VX3H_OK;
+    return upload(b, dptr, v.data(), v.size() * sizeof(E));
 }
 
 template <typename T> void putReal(std::vector<char>& out, const std::vector<double>& src) {
@@ -367,11 +410,17 @@ template <typename T> void putReal(std::vector<char>& out, const std::vector<dou
     for (size_t i = 0; i < src.size(); i++) p[i] = (T)src[i];
 }
 
+// reals in the state precision, converted straight into the staging buffer
 vx3h_status copyReal(vx3h_batch* b, void* dst, const std::vector<double>& src) {
-    std::vector<char> tmp;
-    if (b->realSize == 4) putReal<float>(tmp, src);
-    else putReal<double>(tmp, src);
-    HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    const size_t bytes = src.size() * b->realSize;
+    if (!bytes) return VX3H_OK;
+    char* p;
+    if (vx3h_status st = stageAlloc(b, bytes, &p)) return st;
+    if (b->realSize == 4)
+        for (size_t i = 0; i < src.size(); i++) ((float*)p)[i] = (float)src[i];
+    else
+        memcpy(p, src.data(), bytes);
+    HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, b->stream));
     return VX3H_OK;
 }
 
@@ -408,8 +457,8 @@ vx3h_status uploadNearMasks(vx3h_batch* b, int v0, int nv, const int* ijk, const
         }
     if (nv > 0)
         for (int w = 0; w < VX3_NEAR_WORDS; w++)
-            HIPCHK(hipMemcpy(b->db.nearMask + (size_t)w * b->db.Vcap + v0, m.data() + (size_t)w * nv, (size_t)nv * 4,
-                             hipMemcpyHostToDevice));
+            if (vx3h_status st = upload(b, b->db.nearMask + (size_t)w * b->db.Vcap + v0, m.data() + (size_t)w * nv, (size_t)nv * 4))
+                return st;
     return VX3H_OK;
 }
 
@@ -495,14 +544,11 @@ vx3h_status ensureOneTile(vx3h_batch* b, int s) {
     if ((st = poolAlloc(b, b->tileSlotPool, sh.tvN, sh.tv0)) || (st = poolAlloc(b, b->tileLinkPool, sh.tlN, sh.tl0)) ||
         (st = poolAlloc(b, b->tileXchPool, sh.txN, sh.tx0)) || (st = poolAlloc(b, b->tilePartPool, 1, sh.tp0)))
         return st;
-    if (sh.nv) {
-        HIPCHK(hipMemcpy(b->dTileVox + sh.tv0, vox.data(), vox.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->dTileXidx + sh.tv0, xidx.data(), xidx.size() * 4, hipMemcpyHostToDevice));
-    }
-    if (sh.nl) {
-        HIPCHK(hipMemcpy(b->dTileLink + sh.tl0, link.data(), link.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->dTileEnds + sh.tl0, ends.data(), ends.size() * 4, hipMemcpyHostToDevice));
-    }
+    if ((st = upload(b, b->dTileVox + sh.tv0, vox.data(), vox.size() * 4)) ||
+        (st = upload(b, b->dTileXidx + sh.tv0, xidx.data(), xidx.size() * 4)) ||
+        (st = upload(b, b->dTileLink + sh.tl0, link.data(), link.size() * 4)) ||
+        (st = upload(b, b->dTileEnds + sh.tl0, ends.data(), ends.size() * 4)))
+        return st;
     sh.tplan.tiles.assign(1, d);
     sh.tplan.nPub = 0;
     sh.oneTile = true;
@@ -759,6 +805,7 @@ extern "C" vx3h_status vx3h_batch_destroy(vx3h_batch* b) {
         for (auto& a : pl->arrs) if (*a.first) hipFree(*a.first);
     for (auto& pr : b->evUsed) { b->evPool.push_back(pr.first); b->evPool.push_back(pr.second); }
     for (hipEvent_t e : b->evPool) hipEventDestroy(e);
+    if (b->stage) hipHostFree(b->stage);
     hipStreamDestroy(b->stream);
     delete b;
     return VX3H_OK;
@@ -835,6 +882,10 @@ extern "C" vx3h_status vx3h_sim_admit(vx3h_batch* b, const vx3h_sim_desc* desc, 
 }
 
 namespace vx3 {
+// the admission's copies and fills, on the main stream
+#define UPLOAD(...) do { if (vx3h_status st_ = upload(b, __VA_ARGS__)) return st_; } while (0)
+#define FILL(...) do { if (vx3h_status st_ = fillBytes(b, __VA_ARGS__)) return st_; } while (0)
+
 vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim* pre, int* slot) {
     std::unique_ptr<PreparedSim> own(pre);
     if (!b || !desc || !slot) FAILV(VX3H_EINVAL, "null argument");
@@ -856,6 +907,7 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
     const size_t need = (size_t)H.nv * voxelBytes(rs) + (size_t)NLcap * linkBytes(rs) + sizeof(SimCtl) + H.exts.size() * sizeof(ExtDev);
     if (b->bytesUsed + need > b->budget) FAILV(VX3H_ENOMEM, "simulation does not fit the batch's device memory budget");
     HIPCHK(hipStreamSynchronize(b->stream));
+    b->stageUsed = 0;   // the stream has run every staged copy
     // slot
     int s = -1;
     for (int i = 0; i < (int)b->slots.size(); i++) if (!b->slots[i].used) { s = i; break; }
@@ -864,13 +916,14 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
         int nc = std::max((int)b->slots.size(), 2 * b->simCap + 8);
         SimCtl* np = nullptr;
         HIPCHK(hipMalloc(&np, nc * sizeof(SimCtl)));
-        HIPCHK(hipMemset(np, 0, nc * sizeof(SimCtl)));
+        FILL(np, 0, nc * sizeof(SimCtl));
         double* nh = nullptr;
         const size_t hs = 3 * VX3_HIST * sizeof(double);
         HIPCHK(hipMalloc(&nh, nc * hs));
         if (b->db.sims) {
-            HIPCHK(hipMemcpy(np, b->db.sims, b->simCap * sizeof(SimCtl), hipMemcpyDeviceToDevice));
-            HIPCHK(hipMemcpy(nh, b->db.hist, b->simCap * hs, hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpyAsync(np, b->db.sims, b->simCap * sizeof(SimCtl), hipMemcpyDeviceToDevice, b->stream));
+            HIPCHK(hipMemcpyAsync(nh, b->db.hist, b->simCap * hs, hipMemcpyDeviceToDevice, b->stream));
+            HIPCHK(hipStreamSynchronize(b->stream));
             HIPCHK(hipFree(b->db.sims));
             HIPCHK(hipFree(b->db.hist));
         }
@@ -954,23 +1007,23 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
             if (desc->materials[desc->material[i]].is_target) vf[i] |= VF_TARGET;
         if (attach)   // VX3_Voxel::enableAttach starts true (VX3_Voxel.h:302)
             for (int i = 0; i < NV; i++) vf[i] |= VF_ATTACH;
-        HIPCHK(hipMemcpy(d.vflags + v0, vf.data(), NV * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.tfac + v0, H.tfac0.data(), NV * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.phase + v0, H.phase.data(), NV * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.vmat + v0, vm.data(), NV * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.vext + v0, ve.data(), NV * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.vijk + 3 * (size_t)v0, H.ijk.data(), NV * 12, hipMemcpyHostToDevice));
-        HIPCHK(hipMemset(d.psx + v0, 0, NV * 4));
-        HIPCHK(hipMemset(d.psy + v0, 0, NV * 4));
-        HIPCHK(hipMemset(d.psz + v0, 0, NV * 4));
-        for (int k = 0; k < 6; k++) HIPCHK(hipMemset(d.es + (size_t)k * d.Vcap + v0, 0, NV * 4));
-        HIPCHK(hipMemcpy(d.vsim + v0, vs.data(), NV * 4, hipMemcpyHostToDevice));
+        UPLOAD(d.vflags + v0, vf.data(), NV * 4);
+        UPLOAD(d.tfac + v0, H.tfac0.data(), NV * 4);
+        UPLOAD(d.phase + v0, H.phase.data(), NV * 8);
+        UPLOAD(d.vmat + v0, vm.data(), NV * 4);
+        UPLOAD(d.vext + v0, ve.data(), NV * 4);
+        UPLOAD(d.vijk + 3 * (size_t)v0, H.ijk.data(), NV * 12);
+        FILL(d.psx + v0, 0, NV * 4);
+        FILL(d.psy + v0, 0, NV * 4);
+        FILL(d.psz + v0, 0, NV * 4);
+        for (int k = 0; k < 6; k++) FILL(d.es + (size_t)k * d.Vcap + v0, 0, NV * 4);
+        UPLOAD(d.vsim + v0, vs.data(), NV * 4);
         if (!H.cilia.empty()) {   // per voxel: base, shift, the material's Cilia scale and switch-on time
             std::vector<double> col(NV);
             for (int k = 0; k < 8; k++) {
                 for (int i = 0; i < NV; i++)
                     col[i] = k < 6 ? H.cilia[6 * (size_t)i + k] : k == 6 ? H.vmCilia[H.vmat[i]] : H.vmCiliaOn[H.vmat[i]];
-                HIPCHK(hipMemcpy(d.cil + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+                UPLOAD(d.cil + (size_t)k * d.Vcap + v0, col.data(), NV * 8);
             }
         }
     }
@@ -980,23 +1033,23 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
             ln[i] = v0 + H.lneg[i]; lp[i] = v0 + H.lpos[i];
             lm[i] = H.laxis[i] | (lmIdx[H.lmat[i]] << 2);
         }
-        HIPCHK(hipMemcpy(d.lneg + l0, ln.data(), NL * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.lpos + l0, lp.data(), NL * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.lmeta + l0, lm.data(), NL * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d.lflags + l0, lf.data(), NL * 4, hipMemcpyHostToDevice));
+        UPLOAD(d.lneg + l0, ln.data(), NL * 4);
+        UPLOAD(d.lpos + l0, lp.data(), NL * 4);
+        UPLOAD(d.lmeta + l0, lm.data(), NL * 4);
+        UPLOAD(d.lflags + l0, lf.data(), NL * 4);
         std::vector<double> zero(NL, 0.0);
         vx3h_status st;
         for (void* p : {(void*)d.q2x, (void*)d.q2y, (void*)d.q2z, (void*)d.a1x, (void*)d.a1y, (void*)d.a1z, (void*)d.a2x, (void*)d.a2y, (void*)d.a2z})
             if ((st = copyReal(b, at(p, rs, l0), zero))) return st;
-        HIPCHK(hipMemset(d.strain + l0, 0, NL * 4));
-        HIPCHK(hipMemset(d.maxStrain + l0, 0, NL * 4));
-        HIPCHK(hipMemset(d.strainOffset + l0, 0, NL * 4));
-        HIPCHK(hipMemcpy(d.lsim + l0, ls.data(), NL * 4, hipMemcpyHostToDevice));
+        FILL(d.strain + l0, 0, NL * 4);
+        FILL(d.maxStrain + l0, 0, NL * 4);
+        FILL(d.strainOffset + l0, 0, NL * 4);
+        UPLOAD(d.lsim + l0, ls.data(), NL * 4);
     }
-    if (NLcap) HIPCHK(hipMemset(d.lnew + l0, 0, (size_t)NLcap * 4));
+    if (NLcap) FILL(d.lnew + l0, 0, (size_t)NLcap * 4);
     if (NLcap > NL) {   // room for attached links: not owned until made
-        HIPCHK(hipMemset(d.lsim + l0 + NL, 0xFF, (size_t)(NLcap - NL) * 4));
-        HIPCHK(hipMemset(d.lflags + l0 + NL, 0, (size_t)(NLcap - NL) * 4));
+        FILL(d.lsim + l0 + NL, 0xFF, (size_t)(NLcap - NL) * 4);
+        FILL(d.lflags + l0 + NL, 0, (size_t)(NLcap - NL) * 4);
     }
     // execution path: resident when the sim fits one of the on-chip kernel shapes
     int shape = -1;
@@ -1112,14 +1165,14 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
                 const int l = H.voxelLinks[6 * (size_t)i + k];
                 nb[i] = l < 0 ? -1 : v0 + (H.lneg[l] == i ? H.lpos[l] : H.lneg[l]);
             }
-            HIPCHK(hipMemcpy(d.nbr + (size_t)k * d.Vcap + v0, nb.data(), NV * 4, hipMemcpyHostToDevice));
+            UPLOAD(d.nbr + (size_t)k * d.Vcap + v0, nb.data(), NV * 4);
         }
     }
     if (desc->enable_signals && NV) {
         c.flags |= SC_SIGNALS | SC_EXTRAS;
         b->anyExtras = b->anySignals = true;
         const std::vector<double> zero(NV, 0.0);
-        for (int k = 0; k < 8; k++) HIPCHK(hipMemcpy(d.sg + (size_t)k * d.Vcap + v0, zero.data(), NV * 8, hipMemcpyHostToDevice));
+        for (int k = 0; k < 8; k++) UPLOAD(d.sg + (size_t)k * d.Vcap + v0, zero.data(), NV * 8);
         std::vector<double> col(NV);
         for (int k = 0; k < 5; k++) {
             for (int i = 0; i < NV; i++) {
@@ -1127,7 +1180,7 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
                 col[i] = k == 0 ? (double)(md.is_pacemaker != 0) : k == 1 ? md.pacemaker_period : k == 2 ? md.signal_value_decay
                          : k == 3 ? md.signal_time_delay : md.inactive_period;
             }
-            HIPCHK(hipMemcpy(d.sp + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+            UPLOAD(d.sp + (size_t)k * d.Vcap + v0, col.data(), NV * 8);
         }
     }
     if (desc->secondary_experiment) {
@@ -1138,14 +1191,14 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
         std::vector<double> col(NV);
         for (int k = 0; k < 3; k++) {   // saveInitialPosition at creation: the lattice positions
             for (int i = 0; i < NV; i++) col[i] = H.ijk[3 * i + k] * H.voxSize;
-            if (NV) HIPCHK(hipMemcpy(d.ip + (size_t)k * d.Vcap + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+            if (NV) UPLOAD(d.ip + (size_t)k * d.Vcap + v0, col.data(), NV * 8);
         }
         for (int i = 0; i < NV; i++) col[i] = desc->materials[desc->material[i]].remove_after;
-        if (NV) HIPCHK(hipMemcpy(d.rt + v0, col.data(), NV * 8, hipMemcpyHostToDevice));
+        if (NV) UPLOAD(d.rt + v0, col.data(), NV * 8);
     }
     if (voxcad) {
         const std::vector<double> unset(3 * VX3_HIST, -1.0);   // ClearHistories
-        HIPCHK(hipMemcpy(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double), hipMemcpyHostToDevice));
+        UPLOAD(b->db.hist + (size_t)s * 3 * VX3_HIST, unset.data(), unset.size() * sizeof(double));
     }
     int tg0 = 0, nTg = 0;
     if (closePairs && NV > 0) {   // registerTargets (VX3_VoxelyzeKernel.cu:536-543): target voxels in voxel order
@@ -1156,7 +1209,7 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
         if (nTg) {
             vx3h_status st = poolAlloc(b, b->tgtPool, nTg, tg0);
             if (st) return st;
-            HIPCHK(hipMemcpy(b->db.tgtList + tg0, tg.data(), (size_t)nTg * 4, hipMemcpyHostToDevice));
+            UPLOAD(b->db.tgtList + tg0, tg.data(), (size_t)nTg * 4);
         }
         c.closeR = desc->max_dist_close_pairs * H.voxSize;   // MaxDistInVoxelLengthsToCountAsPair * voxSize
         c.tg0 = tg0;
@@ -1226,13 +1279,13 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
             c.flags |= SC_NEARMASK;
         }
         for (int& x : nearStart) x += near0;
-        if (!nearList.empty()) HIPCHK(hipMemcpy(b->db.nearList + near0, nearList.data(), nearList.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->db.nearStart + v0, nearStart.data(), NV * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->db.nearCount + v0, nearCount.data(), NV * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemset(b->db.nbCount + v0, 0, NV * 4));
-        HIPCHK(hipMemset(b->db.nbStart + v0, 0, NV * 4));
+        if (!nearList.empty()) UPLOAD(b->db.nearList + near0, nearList.data(), nearList.size() * 4);
+        UPLOAD(b->db.nearStart + v0, nearStart.data(), NV * 4);
+        UPLOAD(b->db.nearCount + v0, nearCount.data(), NV * 4);
+        FILL(b->db.nbCount + v0, 0, NV * 4);
+        FILL(b->db.nbStart + v0, 0, NV * 4);
         std::vector<int> own(hN, s);
-        HIPCHK(hipMemcpy(b->db.hashSim + h0, own.data(), hN * 4, hipMemcpyHostToDevice));
+        UPLOAD(b->db.hashSim + h0, own.data(), hN * 4);
         c.nb0 = nb0;
         c.nbCap = nbN;
         c.h0 = h0;
@@ -1249,7 +1302,7 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
         b->db.Ccap = b->nbPool.cap;
         b->db.hashTot = b->hashPool.ra.hi * VX3_COL_HBLOCK;
     }
-    HIPCHK(hipMemcpy(b->db.sims + s, &c, sizeof(c), hipMemcpyHostToDevice));
+    UPLOAD(b->db.sims + s, &c, sizeof(c));
 
     SlotHost& sh = b->slots[s];
     sh = SlotHost();
@@ -1274,11 +1327,11 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
         if ((st = poolAlloc(b, b->tileSlotPool, sh.tvN, sh.tv0)) || (st = poolAlloc(b, b->tileLinkPool, sh.tlN, sh.tl0)) ||
             (st = poolAlloc(b, b->tileXchPool, sh.txN, sh.tx0)) || (st = poolAlloc(b, b->tilePartPool, tiles, sh.tp0)))
             return st;
-        HIPCHK(hipMemcpy(b->dTileVox + sh.tv0, tplan.vox.data(), tplan.vox.size() * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(b->dTileXidx + sh.tv0, tplan.xidx.data(), tplan.xidx.size() * 4, hipMemcpyHostToDevice));
+        UPLOAD(b->dTileVox + sh.tv0, tplan.vox.data(), tplan.vox.size() * 4);
+        UPLOAD(b->dTileXidx + sh.tv0, tplan.xidx.data(), tplan.xidx.size() * 4);
         if (!tplan.link.empty()) {
-            HIPCHK(hipMemcpy(b->dTileLink + sh.tl0, tplan.link.data(), tplan.link.size() * 4, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(b->dTileEnds + sh.tl0, tplan.ends.data(), tplan.ends.size() * 4, hipMemcpyHostToDevice));
+            UPLOAD(b->dTileLink + sh.tl0, tplan.link.data(), tplan.link.size() * 4);
+            UPLOAD(b->dTileEnds + sh.tl0, tplan.ends.data(), tplan.ends.size() * 4);
         }
         sh.tiles = tiles;
         sh.tplan.tiles = std::move(tplan.tiles);   // the descriptors stay on the host
@@ -1322,6 +1375,8 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
     *slot = s;
     return VX3H_OK;
 }
+#undef UPLOAD
+#undef FILL
 }  // namespace vx3
 
 extern "C" vx3h_status vx3h_batch_step(vx3h_batch* b, int n_steps) {
