@@ -100,11 +100,13 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
     Planner p;
     p.nv = nv; p.nl = nl; p.ijk = ijk; p.lneg = lneg; p.lpos = lpos; p.laxis = laxis; p.vmax = vmax; p.lax = lax;
     p.index();
-    // Equal blocks: every tile's step takes as long as the largest block's, so the fewest
-    // blocks of equal size that all fit, from a fill of 80% of the slots; a plan that fails
-    // tries next the count at which its overflowing block's owned + halo would have fit
-    // (blocks of n voxels need about n (1 + halo ratio) slots).  A shape no equal split
-    // fits falls back to halving the blocks that do not.
+    // Equal blocks: every tile's step takes as long as the largest block's, and each more
+    // tile adds halo, exchange records and barrier arrivals, so the fewest blocks of equal
+    // size that all fit.  From the count at which the blocks would fill every slot, a plan
+    // that fails tries next the count at which its overflowing block's owned + halo would
+    // have fit (blocks of n voxels need about n (1 + halo ratio) slots); the first count
+    // that fits is then lowered one at a time towards the last that failed.  A shape no
+    // equal split fits falls back to halving the blocks that do not.
     std::vector<std::vector<int>> best;
     std::vector<int> all(nv);
     auto attempt = [&](int k, bool strict) {
@@ -115,8 +117,14 @@ bool planTiles(int nv, const int* ijk, int nl, const int* lneg, const int* lpos,
         p.cut(all.data(), all.size(), k);
         return !p.failed;
     };
-    for (int k = std::max(1, (int)((nv + 0.8 * vmax - 1) / (0.8 * vmax))); k <= maxTiles;) {
-        if (attempt(k, true)) { best = p.parts; break; }
+    int failedK = 0;
+    for (int k = std::max(1, (nv + vmax - 1) / vmax); k <= maxTiles;) {
+        if (attempt(k, true)) {
+            best = p.parts;
+            for (int j = k - 1; j > failedK && attempt(j, true); j--) best = p.parts;
+            break;
+        }
+        failedK = k;
         k = std::max(k + 1, (int)std::ceil((double)k * p.lastNeed / vmax));
     }
     if (best.empty())
