@@ -556,11 +556,24 @@ __global__ void __launch_bounds__(256) k_col_place(DevBatch<real> b) {
 }
 
 // visits every watched partner u of surface voxel v once (any order)
+//
+// Before the exact test, a box test on the float positions of this rebuild (lastWatch:
+// k_col_insert writes them for every voxel it puts in the buckets, earlier in the same
+// step), one 12-byte read per candidate instead of three reals from three arrays.  It
+// only drops candidates further than the watch radius along one axis by more than float
+// rounding can explain: with a = the voxel's coordinate, b = the candidate's and both
+// rounded to float, the float difference is within 2^-24 (|a| + |b| + |difference|) of
+// a - b, which for a watched pair (|a - b| <= r) is below 2^-22 (|a| + r); the bound
+// used is twice that.  Candidates are read four at a time so their loads overlap.
 template <typename F>
 __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const SimCtl& sc, int v, F f) {
     int c[3];
     colCell(b, v, sc.colCell, c);
     const unsigned mask = (1u << sc.hBits) - 1;
+    const float* lw = b.lastWatch;
+    const float vx = lw[3 * v], vy = lw[3 * v + 1], vz = lw[3 * v + 2];
+    const float r = sqrtf(sc.colThreshSq) * (1.0f + 0x1p-20f);
+    const float rx = r + (fabsf(vx) + r) * 0x1p-21f, ry = r + (fabsf(vy) + r) * 0x1p-21f, rz = r + (fabsf(vz) + r) * 0x1p-21f;
     int hs[27];
 #pragma unroll
     for (int k = 0; k < 27; k++)
@@ -573,9 +586,21 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
         const int n = b.hashCnt[h];
         if (!n) continue;
         const int* cv = b.cellVox + colBucketStart(b, h);
-        for (int i = 0; i < n; i++) {
-            const int u = cv[i];
-            if (u != v && colWatched(b, sc, v, u)) f(u);
+        for (int i0 = 0; i0 < n; i0 += 4) {
+            int u[4];
+            float px[4], py[4], pz[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) u[q] = i0 + q < n ? cv[i0 + q] : -1;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int w = u[q] < 0 ? v : u[q];
+                px[q] = lw[3 * w]; py[q] = lw[3 * w + 1]; pz[q] = lw[3 * w + 2];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (u[q] >= 0 && u[q] != v && fabsf(vx - px[q]) <= rx && fabsf(vy - py[q]) <= ry && fabsf(vz - pz[q]) <= rz &&
+                    colWatched(b, sc, v, u[q]))
+                    f(u[q]);
         }
     }
 }
