@@ -610,7 +610,8 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
 // VX3_COL_STAGE; beyond that the cells are walked a second time straight into the table),
 // the voxel receives a range of the neighbour table, and the list is sorted there.  A
 // table too small stops the simulation after its link pass until the host grows it.
-#define VX3_COL_STAGE 48
+// 32 entries (32 KB per workgroup) keep five workgroups on a CU, the register limit
+#define VX3_COL_STAGE 32
 __global__ void __launch_bounds__(256) k_col_list(DevBatch<real> b) {
     if (!*b.colAny) return;
     __shared__ int stage[VX3_COL_STAGE][256];   // [entry][thread]: a thread's column, conflict-free
