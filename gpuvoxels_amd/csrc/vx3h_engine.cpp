@@ -342,6 +342,18 @@ vx3h_status growArrays(vx3h_batch* b, std::vector<ArrSpec>& arrs, int& cap, int 
     if (need <= cap) return VX3H_OK;
     int ncap = std::max(need, std::max(2 * cap, 1024));
     HIPCHK(hipStreamSynchronize(b->stream));
+    // The arrays are replaced one at a time (the old one freed after its copy), so the
+    // device needs the growth of all of them plus the largest new one.  Checked first: an
+    // allocation failing half way would leave arrays of two capacities (and two strides).
+    {
+        size_t grow = 0, largest = 0, freeB = 0, totalB = 0;
+        for (const ArrSpec& a : arrs) {
+            grow += (size_t)(ncap - cap) * a.per * a.elem;
+            largest = std::max(largest, (size_t)ncap * a.per * a.elem);
+        }
+        HIPCHK(hipMemGetInfo(&freeB, &totalB));
+        if (grow + largest > freeB) FAILV(VX3H_ENOMEM, "device memory too small to grow the batch arrays");
+    }
     for (ArrSpec& a : arrs) {
         void* np = nullptr;
         size_t bytes = (size_t)ncap * a.per * a.elem;
