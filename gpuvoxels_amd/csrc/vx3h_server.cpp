@@ -7,10 +7,13 @@
 // once its footprint (vx3h_sim_footprint) fits the budget left by the running
 // ones, stepped with everything else on the device, and retired when it finishes,
 // freeing its ranges for the next admissions.  Admission, retirement and result
-// reads happen between rounds of quantum_steps steps.
+// reads happen between rounds of quantum_steps steps.  Each request's host model and
+// tile plan are built by the server's preparation threads from its submission on, while
+// the device runs the current round, so an admission only uploads.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -75,6 +78,11 @@ struct Request {
     size_t bytes = 0;
     double tSubmit = 0, tAdmit = 0;
     int slot = -1;
+    // host model and tile plan (vx3::prepareSim), made by a preparation thread or, if none
+    // has started on it, by the round that admits the request; state guarded by the
+    // server's mutex
+    enum { PREP_WAITING, PREP_RUNNING, PREP_DONE } prepState = PREP_WAITING;
+    vx3::PreparedSim* prep = nullptr;   // null when done: the preparation failed (admission reports why)
 };
 
 }  // namespace
@@ -84,13 +92,40 @@ struct vx3h_server {
     vx3h_batch* batch = nullptr;
     size_t budget = 0, used = 0;
     std::chrono::steady_clock::time_point t0;
-    std::mutex mu;                      // guards queue and nextTicket
+    std::mutex mu;                      // guards queue, nextTicket, prepQueue, stopping and Request::prepState
     std::deque<Request*> queue;
+    std::deque<Request*> prepQueue;     // submitted, not yet prepared, in submission order
+    std::condition_variable prepWork;   // a request to prepare, or stopping
+    std::condition_variable prepDone;   // a preparation finished
+    std::vector<std::thread> prepThreads;
+    bool stopping = false;
     long long nextTicket = 1;
     std::map<int, Request*> running;    // slot -> request
     std::vector<vx3h_server_result> done;
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
+
+namespace {
+
+void prepWorker(vx3h_server* sv) {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+        sv->prepWork.wait(lk, [&] { return sv->stopping || !sv->prepQueue.empty(); });
+        if (sv->stopping) return;
+        Request* r = sv->prepQueue.front();
+        sv->prepQueue.pop_front();
+        r->prepState = Request::PREP_RUNNING;
+        lk.unlock();
+        vx3::PreparedSim* p = nullptr;
+        if (vx3::prepareSim(sv->batch, &r->desc.d, &p) != VX3H_OK) p = nullptr;
+        lk.lock();
+        r->prep = p;
+        r->prepState = Request::PREP_DONE;
+        sv->prepDone.notify_all();
+    }
+}
+
+}  // namespace
 
 extern "C" double vx3h_server_clock(const vx3h_server* sv) { return sv ? sv->now() : 0.0; }
 
@@ -104,14 +139,23 @@ extern "C" vx3h_status vx3h_server_create(int device, const vx3h_server_cfg* cfg
     if (st != VX3H_OK) { delete sv; return st; }
     sv->budget = sv->cfg.batch.max_device_bytes;
     sv->t0 = std::chrono::steady_clock::now();
+    // preparation threads: host work only, one core left to the thread that drives the device
+    const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
+    for (unsigned t = 0; t < std::min(15u, hw - 1); t++) sv->prepThreads.emplace_back(prepWorker, sv);
     *out = sv;
     return VX3H_OK;
 }
 
 extern "C" vx3h_status vx3h_server_destroy(vx3h_server* sv) {
     if (!sv) return VX3H_OK;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stopping = true;
+    }
+    sv->prepWork.notify_all();
+    for (auto& th : sv->prepThreads) th.join();
     vx3h_batch_destroy(sv->batch);
-    for (Request* r : sv->queue) delete r;
+    for (Request* r : sv->queue) { vx3::freePrepared(r->prep); delete r; }
     for (auto& kv : sv->running) delete kv.second;
     delete sv;
     return VX3H_OK;
@@ -129,15 +173,21 @@ extern "C" vx3h_status vx3h_server_submit(vx3h_server* sv, const vx3h_sim_desc* 
     r->ticket = sv->nextTicket++;
     r->tSubmit = sv->now();
     sv->queue.push_back(r);
+    if (!sv->prepThreads.empty()) {
+        sv->prepQueue.push_back(r);
+        sv->prepWork.notify_one();
+    }
     if (ticket) *ticket = r->ticket;
     return VX3H_OK;
 }
 
 extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
     if (!sv) { g_err = "null server"; return VX3H_EINVAL; }
-    // admit in arrival order while the footprint fits: the host models and tile plans of
-    // this round's admissions are prepared on several threads, then admitted in order
-    std::vector<Request*> take;
+    // admit in arrival order while the footprint fits, each with its preparation: those
+    // no preparation thread has started are prepared here on several threads, then the
+    // round waits for the ones in progress
+    std::vector<Request*> take, mine;
+    std::vector<vx3::PreparedSim*> prep;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         size_t bytes = 0;
@@ -148,21 +198,39 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
             sv->queue.pop_front();
             take.push_back(h);
             bytes += h->bytes;
+            if (h->prepState == Request::PREP_WAITING) {
+                h->prepState = Request::PREP_RUNNING;
+                mine.push_back(h);
+            }
         }
+        if (!mine.empty())   // no longer for the preparation threads
+            sv->prepQueue.erase(std::remove_if(sv->prepQueue.begin(), sv->prepQueue.end(),
+                                               [](Request* r) { return r->prepState != Request::PREP_WAITING; }),
+                                sv->prepQueue.end());
     }
-    std::vector<vx3::PreparedSim*> prep(take.size(), nullptr);
     {
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        const size_t nt = std::min<size_t>(take.size(), std::min(16u, hw));
+        const size_t nt = std::min<size_t>(mine.size(), std::min(16u, hw));
         std::atomic<size_t> next{0};
         auto work = [&]() {
-            for (size_t i; (i = next.fetch_add(1)) < take.size();)
-                if (vx3::prepareSim(sv->batch, &take[i]->desc.d, &prep[i]) != VX3H_OK) prep[i] = nullptr;
+            for (size_t i; (i = next.fetch_add(1)) < mine.size();)
+                if (vx3::prepareSim(sv->batch, &mine[i]->desc.d, &mine[i]->prep) != VX3H_OK) mine[i]->prep = nullptr;
         };
         std::vector<std::thread> pool;
         for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
         work();
         for (auto& th : pool) th.join();
+        std::unique_lock<std::mutex> lk(sv->mu);
+        for (Request* r : mine) r->prepState = Request::PREP_DONE;
+        sv->prepDone.wait(lk, [&] {
+            for (Request* r : take)
+                if (r->prepState != Request::PREP_DONE) return false;
+            return true;
+        });
+        for (Request* r : take) {
+            prep.push_back(r->prep);
+            r->prep = nullptr;
+        }
     }
     if (take.size() > 1) {   // one growth of the batch arrays for the whole round
         long long nv = 0, nl = 0;
@@ -182,7 +250,8 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
         if (st == VX3H_ENOMEM && !sv->running.empty()) {   // the engine's own budget: wait for retirements
             std::lock_guard<std::mutex> lk(sv->mu);
             for (size_t j = take.size(); j-- > i;) {
-                if (j > i) { vx3::freePrepared(prep[j]); prep[j] = nullptr; }
+                if (j > i) { take[j]->prep = prep[j]; prep[j] = nullptr; }   // kept for the next round
+                else take[j]->prepState = Request::PREP_WAITING;             // its preparation went with the admission
                 sv->queue.push_front(take[j]);
             }
             break;
