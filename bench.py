@@ -443,6 +443,8 @@ def main():
     ap.add_argument("--legs-only", action="store_true",
                     help="run only the configuration legs named by --legs (one JSON object of legs, no headline): "
                          "one batch per leg, for profiling a leg in a process of its own")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving legs (one JSON object), for profiling them in a process of their own")
     ap.add_argument("--leg-steps", type=int, default=0,
                     help="timed steps of every configuration leg (0 = each leg's own: 10000, 5000, 2000, 3000)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -470,6 +472,10 @@ def main():
     if args.legs_only:
         legs = config_legs(args)
         print(json.dumps({"config_legs": legs}))
+        return
+    if args.serving_only:
+        prec = _abi.PREC_F64 if args.precision == "f64" else _abi.PREC_F32
+        print(json.dumps({"serving": serving_legs(prec, args.serving_seconds, args.serving_burst)}))
         return
     models = workloads.robots(args.robots, seed=args.seed, steps=0)
     if args.precision_horizon:

@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -141,7 +142,10 @@ extern "C" vx3h_status vx3h_server_create(int device, const vx3h_server_cfg* cfg
     sv->t0 = std::chrono::steady_clock::now();
     // preparation threads: host work only, one core left to the thread that drives the device
     const unsigned hw = std::max(2u, std::thread::hardware_concurrency());
-    for (unsigned t = 0; t < std::min(15u, hw - 1); t++) sv->prepThreads.emplace_back(prepWorker, sv);
+    try {
+        for (unsigned t = 0; t < std::min(15u, hw - 1); t++) sv->prepThreads.emplace_back(prepWorker, sv);
+    } catch (const std::system_error&) {   // fewer threads (none: each round prepares its own admissions)
+    }
     *out = sv;
     return VX3H_OK;
 }
@@ -217,7 +221,10 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
                 if (vx3::prepareSim(sv->batch, &mine[i]->desc.d, &mine[i]->prep) != VX3H_OK) mine[i]->prep = nullptr;
         };
         std::vector<std::thread> pool;
-        for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+        try {
+            for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+        } catch (const std::system_error&) {   // this thread and those started do the rest
+        }
         work();
         for (auto& th : pool) th.join();
         std::unique_lock<std::mutex> lk(sv->mu);
