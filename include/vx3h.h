@@ -464,7 +464,9 @@ typedef struct vx3h_server_result {
 } vx3h_server_result;
 vx3h_status vx3h_server_create(int device, const vx3h_server_cfg* cfg, vx3h_server** out);
 vx3h_status vx3h_server_destroy(vx3h_server* sv);
-/* Copies desc; max_steps > 0 overrides the description's step budget.  Thread-safe. */
+/* Copies desc; max_steps > 0 overrides the description's step budget.  Thread-safe.  The
+ * request's host model and tile plan are then built on the server's own threads (host
+ * work only) while rounds run; round, results and destroy belong to one thread. */
 vx3h_status vx3h_server_submit(vx3h_server* sv, const vx3h_sim_desc* desc, int max_steps, long long* ticket);
 /* One round: admit what fits, step every running simulation quantum_steps, retire the
  * finished ones.  *active receives running + queued requests after the round. */
