@@ -77,6 +77,7 @@ template <typename T> struct DevBatch {
     const int* simMats;       // per sim: global voxel-material then link-material table indices it uses
     int Vcap, Lcap, Ccap;     // Ccap: nbList capacity
     int hashTot;              // buckets in use (high-water mark, whole blocks)
+    int cellHi;               // cellVox entries in use (high-water mark)
     int vHi, lHi, nSims;      // high-water marks: kernels cover [0, vHi) / [0, lHi) / [0, nSims)
     int solo;                 // >= 0: only this simulation takes the step (on the per-step kernels,
                               // whichever path it was admitted to); -1: every stream-path simulation;

@@ -612,13 +612,20 @@ __device__ __forceinline__ void colForPartners(const DevBatch<real>& b, const Si
 // table too small stops the simulation after its link pass until the host grows it.
 // 32 entries (32 KB per workgroup) keep five workgroups on a CU, the register limit
 #define VX3_COL_STAGE 32
+//
+// One thread per cellVox entry rather than per voxel: every lane then holds a surface
+// voxel (interior voxels have no list), and lanes of one cell share their 27 buckets.
+// An entry is live when k_col_place put its voxel there in this step's rebuild; entries
+// past a simulation's current count keep voxels of an older rebuild (or -1).
 __global__ void __launch_bounds__(256) k_col_list(DevBatch<real> b) {
     if (!*b.colAny) return;
     __shared__ int stage[VX3_COL_STAGE][256];   // [entry][thread]: a thread's column, conflict-free
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = threadIdx.x;
     int s = -1;
-    const bool act = v < b.vHi && colVoxel(b, v, s) && (b.sims[s].flags & SC_REBUILD);
+    const int v = e < b.cellHi ? b.cellVox[e] : -1;
+    const bool act = v >= 0 && v < b.vHi && colVoxel(b, v, s) && (b.sims[s].flags & SC_REBUILD) &&
+                     colBucketStart(b, b.vbk[v]) + b.vrank[v] == e;
     int n = 0;
     if (act)
         colForPartners(b, b.sims[s], v, [&](int u) {

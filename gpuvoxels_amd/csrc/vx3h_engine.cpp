@@ -1313,6 +1313,7 @@ vx3h_status admitPrepared(vx3h_batch* b, const vx3h_sim_desc* desc, PreparedSim*
         b->anyColl = true;
         b->db.Ccap = b->nbPool.cap;
         b->db.hashTot = b->hashPool.ra.hi * VX3_COL_HBLOCK;
+        b->db.cellHi = b->cellPool.ra.hi;
     }
     UPLOAD(b->db.sims + s, &c, sizeof(c));
 
@@ -1967,6 +1968,7 @@ extern "C" vx3h_status vx3h_sim_retire(vx3h_batch* b, int slot) {
         b->nearPool.ra.release(sh.near0, sh.nearN);
         b->cellPool.ra.release(sh.c0, sh.nv);
         b->db.hashTot = b->hashPool.ra.hi * VX3_COL_HBLOCK;
+        b->db.cellHi = b->cellPool.ra.hi;
     }
     if (sh.nTg) b->tgtPool.ra.release(sh.tg0, sh.nTg);
     if (sh.apN) {
