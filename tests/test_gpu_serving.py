@@ -119,3 +119,45 @@ def test_full_size_requests_match_oracle_f64():
         print("%d voxels, %d steps: relative fitness error %.3e" % (m.n_voxels, h, err))
         assert err < 1e-6
         assert np.max(np.abs(np.array(g.current_com) - np.array(o.current_com))) < 1e-9
+
+
+def test_submissions_from_another_thread_while_rounds_run():
+    """Requests submitted from a second thread while the driving thread runs rounds (the
+    server prepares them on its own threads meanwhile) get the results they get alone."""
+    import threading
+    import time
+    rng = np.random.default_rng(11)
+    reqs = [(workloads.robot(rng, grid=int(g), steps=0, name="thr%d" % i), int(h))
+            for i, (g, h) in enumerate(zip(rng.integers(5, 13, size=12), rng.integers(200, 900, size=12)))]
+    sv = engine.Server(precision=_abi.PREC_F64, quantum_steps=150, max_sims=5)
+    tickets, lock = {}, threading.Lock()
+
+    def submitter():
+        for i, (m, h) in enumerate(reqs):
+            t = sv.submit(m, h)
+            with lock:
+                tickets[t] = i
+            time.sleep(0.005)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    results, t0 = {}, time.time()
+    while True:
+        active = sv.round()
+        for r in sv.results():
+            results[r.ticket] = r
+        if not th.is_alive() and active == 0 and len(results) == len(reqs):
+            break
+        if active == 0:
+            time.sleep(0.002)   # nothing queued yet: wait for the submitter
+        assert time.time() - t0 < 120
+    th.join()
+    sv.close()
+    assert sorted(results) == sorted(tickets)
+    for t, r in results.items():
+        m, h = reqs[tickets[t]]
+        assert r.status.state == _abi.SIM_DONE and r.status.steps == h
+        ref = solo(m, h, _abi.PREC_F64)
+        assert r.status.current_time == ref.current_time
+        assert r.status.fitness == ref.fitness
+        assert list(r.status.current_com) == list(ref.current_com)
