@@ -93,11 +93,13 @@ struct TileArgs {
     const int* ends;         // per tile link: LDS slots of its ends, negative | positive << 16
     const int* xidx;         // per tile slot: its exchange record (written if owned, read if halo), -1 = none
     void* xch;               // exchange areas, in the state precision
-    int* sync;               // per simulation: arrivals, divergence word (zero at launch start)
+    int* sync;               // per simulation its divergence word, then one arrival flag per
+                             // workgroup from sync[flag0] (all zero at launch start)
     double* part;            // per simulation: centre-of-mass partial sums
     long long spin;          // wall-clock ticks a tile waits at a barrier before failing the simulation
     int repair;              // 1: run only the simulations whose last launch diverged (divergence word
                              // = 1 + the launch step), up to that step's link pass
+    int flag0;               // index in sync of workgroup 0's arrival flag
 };
 
 }  // namespace vx3

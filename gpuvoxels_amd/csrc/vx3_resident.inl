@@ -151,7 +151,7 @@ __device__ __forceinline__ void ldsCopyWords(int* dst, const int* src, int words
 // compiler issues coherent across the device's L2 caches; so no fence has to write back
 // or invalidate a whole L2 per step.  Ordering comes from waiting for every outstanding
 // memory operation (s_waitcnt 0: the stores have reached their point of coherence)
-// before the arrival is counted.
+// before the arrival flag is written.
 template <typename T> __device__ __forceinline__ void xput(T* p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -159,22 +159,29 @@ template <typename T> __device__ __forceinline__ T xget(const T* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The simulation's barrier, in two halves.  One thread of each tile arrives, after every
-// thread of the tile has written its exchange data and waited for it (s_waitcnt 0) and
-// the workgroup has synchronized; later it waits until all n tiles have arrived at
-// barrier number `gen` (0-based).  False when the wait exceeded `spin` ticks.
-__device__ __forceinline__ void tileArrive(int* ctr) {
+// The simulation's barrier, in two halves.  Each tile has an arrival flag: one thread
+// writes it, after every thread of the tile has written its exchange data and waited for
+// it (s_waitcnt 0) and the workgroup has synchronized; later the first wave waits, one
+// flag per lane, until every tile's flag shows barrier number `gen` (0-based).  Flags
+// rather than one counter: no read-modify-write queue on one address, and the wave's
+// loads of all flags take one memory latency.  A repair launch counts from 1 << 24, so
+// the flags its launch left need no clearing.  False when the wait exceeded `spin` ticks.
+__device__ __forceinline__ int tileFlag(unsigned gen, int repair) { return (int)(gen + 1) + (repair ? (1 << 24) : 0); }
+__device__ __forceinline__ void tileArrive(int* flag, unsigned gen, int repair) {
     __builtin_amdgcn_s_waitcnt(0);
-    __hip_atomic_fetch_add(ctr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    xput(flag, tileFlag(gen, repair));
 }
-__device__ __forceinline__ bool tileWait(int* ctr, int n, unsigned gen, long long spin) {
-    const int target = n * (int)(gen + 1);
+// called by the 64 lanes of one wave; the same result in each
+__device__ __forceinline__ bool tileWait(const int* flags, int n, unsigned gen, int repair, long long spin) {
+    const int target = tileFlag(gen, repair);
     const long long t0 = wall_clock64();
-    while (xget(ctr) < target) {
+    for (;;) {
+        bool ready = true;
+        for (int q = (int)(threadIdx.x & 63); q < n; q += 64) ready = ready && xget(flags + q) >= target;
+        if (__all(ready)) return true;
         if (wall_clock64() - t0 > spin) return false;
         __builtin_amdgcn_s_sleep(1);
     }
-    return true;
 }
 
 template <int BS, int VPT, int LPA, bool COL, bool TILED = false>
@@ -221,7 +228,7 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     unsigned gen = 0;    // TILED: barriers passed in this launch
     if (scg.state != 0 || (scg.flags & SC_COL_GROW)) return;   // finished, or waiting to move to the per-step kernels
     if (TILED && ta.repair) {   // only simulations whose previous launch saw a divergence run again
-        const int dw = ta.sync[td.cOff + 1];
+        const int dw = ta.sync[td.cOff];
         if (!dw) return;
         divAt = dw - 1;
     }
@@ -507,12 +514,11 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
     // simulation fails).
     bool pend = false;
     auto finishBarrier = [&](bool loadHalo, bool sampleNow) -> int {
-        if (tid == 0) {
-            int* sync = ta.sync + td.cOff;
+        if (tid < 64) {   // the first wave; tile 0 of the simulation is workgroup blockIdx.x - td.tile
             int flag = 0;
-            if (!tileWait(sync, td.nTiles, gen, ta.spin)) flag = 2;
-            else if (divAt < 0 && xget(sync + 1)) flag = 1;
-            sTile = flag;
+            if (!tileWait(ta.sync + ta.flag0 + (blockIdx.x - td.tile), td.nTiles, gen, ta.repair, ta.spin)) flag = 2;
+            else if (divAt < 0 && xget(ta.sync + td.cOff)) flag = 1;
+            if (tid == 0) sTile = flag;
         }
         const int par = gen & 1;
         gen++;
@@ -843,7 +849,6 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
             __builtin_amdgcn_s_waitcnt(0);   // this thread's exchange stores are complete
             __syncthreads();
             if (tid == 0) {
-                int* sync = ta.sync + td.cOff;
                 if (sample) {
                     double* P = ta.part + td.pOff + (size_t)(gen & 1) * td.nTiles * 4;
                     double tot[4] = {0, 0, 0, 0};
@@ -851,8 +856,8 @@ __global__ void __launch_bounds__(BS) k_resident(DevBatch<real> b, const int* si
                         for (int q = 0; q < 4; q++) tot[q] += sRed[q][w];
                     for (int q = 0; q < 4; q++) xput(P + 4 * td.tile + q, tot[q]);
                 }
-                if (sDiv) __hip_atomic_fetch_max(sync + 1, it + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                tileArrive(sync);
+                if (sDiv) __hip_atomic_fetch_max(ta.sync + td.cOff, it + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                tileArrive(ta.sync + ta.flag0 + blockIdx.x, gen, ta.repair);
             }
             pend = true;
             if (sample && finishBarrier(true, true)) return;

@@ -38,7 +38,8 @@ hipError_t launchResidentAs(const DevBatch<VX_REAL>& b, hipStream_t s, int shape
 
 // Tiled launches: one workgroup per tile of the simulations in ta.tiles[0, nTiles), all
 // resident at once (a cooperative launch), after the simulations' sync words
-// ta.sync[0, nSync) are cleared.
+// ta.sync[0, nSync) (divergence words and arrival flags) are cleared; the repair launch
+// keeps them (it reads the divergence words, and its flags count from their own base).
 hipError_t launchTiled(const DevBatch<VX_REAL>& b, hipStream_t s, int shape, const TileArgs& ta, int nTiles, int nSync,
                        int nSteps, hipEvent_t* ev) {
     if (nTiles <= 0 || nSteps <= 0) return hipSuccess;
@@ -56,9 +57,8 @@ hipError_t launchTiled(const DevBatch<VX_REAL>& b, hipStream_t s, int shape, con
         void* args[] = {(void*)&bb, (void*)&none, (void*)&steps, (void*)&t};                                     \
         for (int pass = 0; pass < 2; pass++) {   /* the launch, then its repair launch */                        \
             t.repair = pass;                                                                                     \
-            /* arrivals (and, before the launch, divergence words) start at zero */                             \
-            hipLaunchKernelGGL(VX_NS::k_fill_ints, dim3((nSync + 255) / 256), dim3(256), 0, s, ta.sync, nSync,   \
-                               pass ? 2 : 1, 0);                                                                 \
+            if (!pass)                                                                                           \
+                hipLaunchKernelGGL(VX_NS::k_fill_ints, dim3((nSync + 255) / 256), dim3(256), 0, s, ta.sync, nSync, 1, 0); \
             if (ev && !pass) hipEventRecord(ev[0], s);                                                           \
             e = hipLaunchCooperativeKernel((const void*)kern, dim3(nTiles), dim3(B), args, (unsigned)lds, s);    \
             if (ev && !pass) hipEventRecord(ev[1], s);                                                           \

@@ -182,7 +182,7 @@ struct TileDesc {
     int lOff, nLinks;                // tileLink / tileEnds[lOff .. lOff + nLinks)
     int xOff, xPar;                  // exchange area from xOff: two step parities of xPar reals, each
                                      // VX3_XCH_STRIDE reals per voxel another tile reads
-    int cOff;                        // sync words [cOff, cOff + 2): arrivals, divergence
+    int cOff;                        // sync word cOff: the simulation's divergence word
     int pOff;                        // centre-of-mass partial sums: [2][nTiles][4] doubles from pOff
 };
 #define VX3_XCH_STRIDE 12            // position (3), orientation (4), thermal factor, Poisson strains (3), pad

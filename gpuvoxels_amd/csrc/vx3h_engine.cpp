@@ -614,9 +614,10 @@ vx3h_status syncTiles(vx3h_batch* b) {
                 T.push_back(d);
             }
             g.count += tilesOf(s);
-            g.nSync += 2;
-            syncTot += 2;
+            g.nSync++;
+            syncTot++;
         }
+        syncTot += g.count;   // the arrival flags, one per workgroup, after the divergence words
         b->tileGroups.push_back(g);
     }
     vx3h_status st;
@@ -692,6 +693,7 @@ vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
             ta.xidx = b->dTileXidx;
             ta.xch = b->dXch;
             ta.sync = b->dSync + g.sync0;
+            ta.flag0 = g.nSync;
             ta.part = b->dPart;
             ta.spin = b->spinTicks;
             hipEvent_t pair[2];
@@ -703,9 +705,9 @@ vx3h_status launchResidentAll(vx3h_batch* b, int nSteps, hipEvent_t* ev) {
                 evp = pair;
             }
             const hipError_t e = b->realSize == 4
-                ? launchTiled(b->db, b->tileStream, b->tiledShape, ta, g.count, g.nSync, nSteps, evp)
-                : launchTiled(*reinterpret_cast<DevBatch<double>*>(&b->db), b->tileStream, b->tiledShape, ta, g.count, g.nSync,
-                              nSteps, evp);
+                ? launchTiled(b->db, b->tileStream, b->tiledShape, ta, g.count, g.nSync + g.count, nSteps, evp)
+                : launchTiled(*reinterpret_cast<DevBatch<double>*>(&b->db), b->tileStream, b->tiledShape, ta, g.count,
+                              g.nSync + g.count, nSteps, evp);
             if (e != hipSuccess) FAILV(VX3H_EDEVICE, std::string("tiled resident launch failed: ") + hipGetErrorString(e));
         }
     }
