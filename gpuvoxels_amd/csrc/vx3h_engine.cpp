@@ -531,11 +531,6 @@ template <typename T> vx3h_status growDev(T** p, size_t* cap, size_t n) {
     return VX3H_OK;
 }
 
-// The tiled launch groups, after a simulation joined or left: whole simulations packed
-// into groups of at most tileCap tiles (first fit, largest first: a launch lasts as long
-// per step whether full or not), and their tile descriptors pointing at each
-// simulation's ranges in the tile pools (filled at admission), with two sync words per
-// simulation from its group's base.
 // One-tile tables of a one-workgroup simulation (its voxels and links in order, no halo,
 // no exchange records), uploaded once.
 vx3h_status ensureOneTile(vx3h_batch* b, int s) {
@@ -567,6 +562,11 @@ vx3h_status ensureOneTile(vx3h_batch* b, int s) {
     return VX3H_OK;
 }
 
+// The tiled launch groups, after a simulation joined or left: whole simulations packed
+// into groups of at most tileCap tiles (first fit, largest first: a launch lasts as long
+// per step whether full or not), and their tile descriptors pointing at each
+// simulation's ranges in the tile pools (filled at admission).  A group's sync area
+// holds one divergence word per simulation, then one arrival flag per workgroup.
 vx3h_status syncTiles(vx3h_batch* b) {
     if (!b->tileDirty) return VX3H_OK;
     std::vector<int> order = b->tiledSims;
