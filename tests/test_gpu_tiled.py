@@ -51,6 +51,19 @@ def test_tiled_f64_equals_stream():
     bs.close()
 
 
+def test_more_tiles_than_a_wave_has_lanes():
+    """A simulation of more than 64 tiles: a waiting tile's first wave checks several
+    arrival flags per lane."""
+    m = robot(6, 56)
+    bt, st = run([m], _abi.PATH_AUTO, 400)
+    print("voxels", m.n_voxels, "tiles", bt.workgroups(st[0]))
+    assert bt.workgroups(st[0]) > 64
+    bs, ss = run([m], _abi.PATH_STREAM, 400)
+    assert_same(bt, st, bs, ss, 1e-14, 1e-13)
+    bt.close()
+    bs.close()
+
+
 def test_tiled_f32_equals_stream():
     models = [robot(4, 14), robot(5, 20)]
     bt, st = run(models, _abi.PATH_AUTO, 1000, precision=_abi.PREC_F32)
