@@ -100,8 +100,7 @@ __device__ __forceinline__ void poissonStrain(const DevBatch<real>& b, int v) {
     b.psx[v] = st[0]; b.psy[v] = st[1]; b.psz[v] = st[2];
 }
 
-__global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void linkPass(const DevBatch<real>& b, int l) {
     if (l >= b.lHi) return;
     // The loads come in three rounds of independent requests, so that a link waits on
     // memory three times rather than once per table it visits: the link's own indices,
@@ -199,6 +198,8 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) {
         atomicAnd(&b.vflags[vp], ~(1 << (VF_LINKMASK_SHIFT + dp)));
     }
 }
+
+__global__ void __launch_bounds__(256) k_links(DevBatch<real> b) { linkPass(b, blockIdx.x * blockDim.x + threadIdx.x); }
 
 // ---- collisions (CVoxelyze::updateCollisions / regenerateCollisions, Voxelyze.cpp:671-751)
 //
@@ -412,8 +413,7 @@ __global__ void __launch_bounds__(VX3_NEAR_BS) k_near_emit(DevBatch<real> b) {
 // whether its lists are rebuilt this step: the first step, every colInterval steps,
 // after new nearby lists, or once a surface voxel moved more than recalcDist since the
 // last rebuild
-__global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void colStalePass(const DevBatch<real>& b, int v) {
     int s = -1;
     bool need = false, rebuilding = false;
     if (v < b.vHi && colVoxel(b, v, s)) {
@@ -446,6 +446,16 @@ __global__ void __launch_bounds__(256) k_col_stale(DevBatch<real> b) {
         if ((int)(threadIdx.x & 63) == lead) atomicOr(&b.sims[sl].flags, SC_REBUILD);
         pending &= ~__ballot(need && s == sl);
     }
+}
+
+// The staleness pass and the link pass in one launch (one thread per voxel and per link):
+// they are independent -- the link pass moves no voxel and changes no thermal factor,
+// and the staleness pass reads only voxel state and the control blocks' collision
+// bits -- and an idle launch costs about as much as the staleness pass itself.
+__global__ void __launch_bounds__(256) k_links_stale(DevBatch<real> b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    colStalePass(b, i);   // every lane of the wave: it votes
+    linkPass(b, i);
 }
 
 __device__ __forceinline__ bool colRebuilding(const DevBatch<real>& b, int s) {

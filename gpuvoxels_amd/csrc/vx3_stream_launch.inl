@@ -11,7 +11,7 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
         hipLaunchKernelGGL(VX_NS::k_near_count, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
         hipLaunchKernelGGL(VX_NS::k_near_emit, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
     }
-    hipLaunchKernelGGL(VX_NS::k_col_stale, dim3(gv), dim3(bs), 0, s, b);
+    // the staleness pass (colStalePass) ran with the link pass (k_links_stale)
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_clear, dim3((b.hashTot + 255) / 256), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_insert, dim3(gv), dim3(bs), 0, s, b);
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_bscan, dim3(b.hashTot / VX3_COL_HBLOCK), dim3(256), 0, s, b);
@@ -38,7 +38,10 @@ void launchStep(const DevBatch<VX_REAL>& b, hipStream_t s, const StepOpts& o, hi
     // nearby lists regenerated after a topology change follow the links this step's link
     // pass left (its detaches included).  A simulation whose table proves too small stands
     // still after its link pass (SC_LINKS_DONE) and resumes there once the table is grown.
-    if (b.lHi > 0) VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
+    if (o.contacts)   // with the collisions' staleness pass, one thread per link and per voxel
+        VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links_stale, dim3(gl > gv ? gl : gv), dim3(bs), 0, s, b));
+    else if (b.lHi > 0)
+        VX3_EV(PASS_LINKS, hipLaunchKernelGGL(VX_NS::k_links, dim3(gl), dim3(bs), 0, s, b));
     if (o.contacts) VX3_EV(PASS_CONTACTS, launchCollisions(b, s, bs, o.topo));
     if (o.contacts && o.extras) hipLaunchKernelGGL(VX_NS::k_hits, dim3(gv), dim3(bs), 0, s, b);
     if (o.extras) hipLaunchKernelGGL(VX_NS::k_prestep, dim3(gv), dim3(bs), 0, s, b, 1);
