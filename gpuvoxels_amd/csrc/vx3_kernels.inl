@@ -215,7 +215,8 @@ __global__ void __launch_bounds__(256) k_links(DevBatch<real> b) { linkPass(b, b
 //
 // Sort: k_col_insert counts the voxels of each hash bucket (and each voxel's rank in
 // its bucket), k_col_bscan turns the counts into offsets within blocks of
-// VX3_COL_HBLOCK buckets, k_col_boff into per-simulation offsets of those blocks, and
+// VX3_COL_HBLOCK buckets and, in the simulation's last block to finish, into
+// per-simulation offsets of those blocks, and
 // k_col_place scatters the voxels into the simulation's cell-ordered range of cellVox.
 // No bucket has a capacity.  The partner lists are counted, scanned and emitted; if a
 // simulation's partners exceed its neighbour table, it sets SC_COL_GROW and stands
@@ -513,40 +514,47 @@ __global__ void __launch_bounds__(256) k_col_bscan(DevBatch<real> b) {
     for (int i = 0; i < w; i++) before += part[i];
 #pragma unroll
     for (int k = 0; k < PER; k++) { b.hashStart[h0 + tid * PER + k] = before; before += x[k]; }
-    if (tid == 255) b.hashBlk[B] = before;
-}
-
-// per rebuilt simulation: offsets of its bucket blocks in its cellVox range
-__global__ void __launch_bounds__(256) k_col_boff(DevBatch<real> b) {
-    if (!*b.colAny) return;
-    const int s = blockIdx.x;
-    if (s >= b.nSims || !colRebuilding(b, s)) return;
+    // The block's total, then a count of the simulation's scanned blocks; the block that
+    // counts last turns the totals into offsets.  The total is an agent-scope store that
+    // has completed (s_waitcnt 0) before the count, and the last block reads the totals
+    // with agent-scope loads, so no cache writeback is needed between them.
+    if (tid == 255) {
+        __hip_atomic_store(&b.hashBlk[B], before, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    const int s = b.hashSim[h0];
     const SimCtl& sc = b.sims[s];
-    __shared__ int part[4];
+    const int nB = (1 << sc.hBits) / VX3_COL_HBLOCK;
+    __shared__ int last;
+    __syncthreads();
+    if (tid == 0) last = __hip_atomic_fetch_add(&b.sims[s].scanDone, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == nB - 1;
+    __syncthreads();
+    if (!last) return;
+    // per rebuilt simulation: offsets of its bucket blocks in its cellVox range
     __shared__ int carry;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int B0 = sc.h0 / VX3_COL_HBLOCK, nB = (1 << sc.hBits) / VX3_COL_HBLOCK;
+    const int B0 = sc.h0 / VX3_COL_HBLOCK;
     if (tid == 0) {
         carry = sc.c0;
-        b.sims[s].nbUsed = 0;   // the partner ranges of this rebuild are handed out from 0
+        b.sims[s].scanDone = 0;   // for the next rebuild (a later launch)
+        b.sims[s].nbUsed = 0;     // the partner ranges of this rebuild are handed out from 0
         b.sims[s].rebuilds++;
     }
     __syncthreads();
     for (int base = 0; base < nB; base += 256) {
         const int i = base + tid;
-        const int x = i < nB ? b.hashBlk[B0 + i] : 0;
-        int incl = x;
+        const int tot = i < nB ? __hip_atomic_load(&b.hashBlk[B0 + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+        int run = tot;
         for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
+            const int y = __shfl_up(run, o, 64);
+            if (lane >= o) run += y;
         }
-        if (lane == 63) part[w] = incl;
+        if (lane == 63) part[w] = run;
         __syncthreads();
-        int before = carry + incl - x;
-        for (int k = 0; k < w; k++) before += part[k];
-        if (i < nB) b.hashBlkOff[B0 + i] = before;
+        int off = carry + run - tot;
+        for (int q = 0; q < w; q++) off += part[q];
+        if (i < nB) b.hashBlkOff[B0 + i] = off;
         __syncthreads();
-        if (tid == 255) carry = before + x;
+        if (tid == 255) carry = off + tot;
         __syncthreads();
     }
 }

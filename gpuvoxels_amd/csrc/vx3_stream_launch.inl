@@ -15,7 +15,6 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_clear, dim3((b.hashTot + 255) / 256), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_insert, dim3(gv), dim3(bs), 0, s, b);
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_bscan, dim3(b.hashTot / VX3_COL_HBLOCK), dim3(256), 0, s, b);
-    hipLaunchKernelGGL(VX_NS::k_col_boff, dim3(b.nSims), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_place, dim3(gv), dim3(bs), 0, s, b);
     if (b.cellHi > 0) hipLaunchKernelGGL(VX_NS::k_col_list, dim3((b.cellHi + 255) / 256), dim3(256), 0, s, b);
 }

@@ -168,6 +168,7 @@ struct SimCtl {                     // one running simulation (VX3_VoxelyzeKerne
     // attach-condition programs in the token pool
     int nlMax, safety;
     int atOff[5], atLen[5];
+    int scanDone;                    // bucket blocks of the current rebuild scanned so far (k_col_bscan)
 };
 
 // One tile of a simulation that the resident kernel runs on several workgroups (tiled
