@@ -452,10 +452,13 @@ __device__ __forceinline__ void colStalePass(const DevBatch<real>& b, int v) {
 // The staleness pass and the link pass in one launch (one thread per voxel and per link):
 // they are independent -- the link pass moves no voxel and changes no thermal factor,
 // and the staleness pass reads only voxel state and the control blocks' collision
-// bits -- and an idle launch costs about as much as the staleness pass itself.
+// bits -- and an idle launch costs about as much as the staleness pass itself.  The
+// same launch zeroes every bucket count: only a rebuild reads them, and a rebuild
+// counts from zero in this step's later passes.
 __global__ void __launch_bounds__(256) k_links_stale(DevBatch<real> b) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     colStalePass(b, i);   // every lane of the wave: it votes
+    for (int h = i; h < b.hashTot; h += gridDim.x * blockDim.x) b.hashCnt[h] = 0;
     linkPass(b, i);
 }
 
@@ -463,13 +466,6 @@ __device__ __forceinline__ bool colRebuilding(const DevBatch<real>& b, int s) {
     if (s < 0 || !simRunning(b, s)) return false;
     const int f = b.sims[s].flags;
     return (f & SC_REBUILD) && (f & SC_COLLISIONS) && !(f & (SC_DIVERGED_PENDING | SC_COL_GROW));
-}
-
-__global__ void __launch_bounds__(256) k_col_clear(DevBatch<real> b) {
-    if (!*b.colAny) return;
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h >= b.hashTot) return;
-    if (colRebuilding(b, b.hashSim[h])) b.hashCnt[h] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_col_insert(DevBatch<real> b) {

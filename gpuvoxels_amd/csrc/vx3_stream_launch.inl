@@ -11,8 +11,8 @@ void launchCollisions(const DevBatch<VX_REAL>& b, hipStream_t s, int bs, bool to
         hipLaunchKernelGGL(VX_NS::k_near_count, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
         hipLaunchKernelGGL(VX_NS::k_near_emit, dim3(gn), dim3(VX3_NEAR_BS), 0, s, b);
     }
-    // the staleness pass (colStalePass) ran with the link pass (k_links_stale)
-    if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_clear, dim3((b.hashTot + 255) / 256), dim3(256), 0, s, b);
+    // the staleness pass (colStalePass) and the clearing of the bucket counts ran with the
+    // link pass (k_links_stale)
     hipLaunchKernelGGL(VX_NS::k_col_insert, dim3(gv), dim3(bs), 0, s, b);
     if (b.hashTot > 0) hipLaunchKernelGGL(VX_NS::k_col_bscan, dim3(b.hashTot / VX3_COL_HBLOCK), dim3(256), 0, s, b);
     hipLaunchKernelGGL(VX_NS::k_col_place, dim3(gv), dim3(bs), 0, s, b);
