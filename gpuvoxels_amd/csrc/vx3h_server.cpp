@@ -83,6 +83,7 @@ struct Request {
     // has started on it, by the round that admits the request; state guarded by the
     // server's mutex
     enum { PREP_WAITING, PREP_RUNNING, PREP_DONE } prepState = PREP_WAITING;
+    bool requeued = false;              // put back by a round (no longer counted ahead)
     vx3::PreparedSim* prep = nullptr;   // null when done: the preparation failed (admission reports why)
 };
 
@@ -100,6 +101,7 @@ struct vx3h_server {
     std::condition_variable prepDone;   // a preparation finished
     std::vector<std::thread> prepThreads;
     bool stopping = false;
+    int ahead = 0;                      // requests the threads took and no round has yet (host memory bound)
     long long nextTicket = 1;
     std::map<int, Request*> running;    // slot -> request
     std::vector<vx3h_server_result> done;
@@ -108,13 +110,17 @@ struct vx3h_server {
 
 namespace {
 
+// at most this many prepared requests wait for admission (each holds its host model)
+constexpr int kPrepAhead = 256;
+
 void prepWorker(vx3h_server* sv) {
     std::unique_lock<std::mutex> lk(sv->mu);
     for (;;) {
-        sv->prepWork.wait(lk, [&] { return sv->stopping || !sv->prepQueue.empty(); });
+        sv->prepWork.wait(lk, [&] { return sv->stopping || (!sv->prepQueue.empty() && sv->ahead < kPrepAhead); });
         if (sv->stopping) return;
         Request* r = sv->prepQueue.front();
         sv->prepQueue.pop_front();
+        sv->ahead++;
         r->prepState = Request::PREP_RUNNING;
         lk.unlock();
         vx3::PreparedSim* p = nullptr;
@@ -192,6 +198,7 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
     // round waits for the ones in progress
     std::vector<Request*> take, mine;
     std::vector<vx3::PreparedSim*> prep;
+    bool freed = false;
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         size_t bytes = 0;
@@ -205,6 +212,9 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
             if (h->prepState == Request::PREP_WAITING) {
                 h->prepState = Request::PREP_RUNNING;
                 mine.push_back(h);
+            } else if (!h->requeued) {
+                sv->ahead--;   // a preparation thread's: room for the next
+                freed = true;
             }
         }
         if (!mine.empty())   // no longer for the preparation threads
@@ -212,6 +222,7 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
                                                [](Request* r) { return r->prepState != Request::PREP_WAITING; }),
                                 sv->prepQueue.end());
     }
+    if (freed) sv->prepWork.notify_all();
     {
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
         const size_t nt = std::min<size_t>(mine.size(), std::min(16u, hw));
@@ -259,6 +270,7 @@ extern "C" vx3h_status vx3h_server_round(vx3h_server* sv, int* active) {
             for (size_t j = take.size(); j-- > i;) {
                 if (j > i) { take[j]->prep = prep[j]; prep[j] = nullptr; }   // kept for the next round
                 else take[j]->prepState = Request::PREP_WAITING;             // its preparation went with the admission
+                take[j]->requeued = true;
                 sv->queue.push_front(take[j]);
             }
             break;
